@@ -1,0 +1,169 @@
+#!/usr/bin/env python3
+"""Benchmark of the plain XPBD step on one MI355X.
+
+Workload (one "step" = one frame = solver.step(1/60, 8, 1), the reference's
+timed "physics ms" call): 1,048,576 particles (128 x 128 x 64 lattice,
+r = 0.005 m) falling onto a watertight 81,920-triangle icosphere plus the
+ground plane.  Inputs are resident on the GPU before timing starts; warm-up
+frames run first and the particle state is then reset to the initial lattice,
+so the K timed frames are frames 1..K of the configuration.
+
+Prints ONE JSON line: particle-substeps/s (whole job), the dominant kernel's
+achieved HBM bandwidth against the 8 TB/s peak (per-kernel durations from HIP
+events recorded on the solver's stream over a second, profiled pass of the
+same frames), and the oracle's single-thread CPU rate on a bounded sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "newton-xpbd-vulkan_amd"))
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+SUBSTEPS = 8
+DT = 1.0 / 60.0
+
+
+def stage_bytes(stage, N, C, Cm, U, tiles, mesh_bytes):
+    """Algorithmic HBM bytes of ONE launch of a stage: every array the kernel
+    must touch, counted once (neighbour re-reads are cache hits)."""
+    hist = tiles * 128 * 4
+    if stage == "integrate":      # P,V in; P,V out; flags; key out; hist; plane contacts
+        return N * (16 + 16 + 4 + 16 + 16 + 4) + hist + 56 * (C - Cm)
+    if stage == "sort_upsweep":
+        return 4 * N + hist
+    if stage == "sort_binscan":
+        return 2 * hist
+    if stage == "sort_scatter":   # mean over the 3 passes: pass 0 reads keys only
+        return (12 * N + 16 * N + 16 * N) / 3.0 + hist
+    if stage == "cell_ranges":
+        return 4 * N + 8 * U
+    if stage == "shape_solve":    # contact record, particle P,V gather, delta RMW
+        return C * (56 + 32 + 24)
+    if stage == "pp_solve_apply":  # vals, keys, flags, dshape rw, Pp, Vp, Ps, delta, P/V out; cell ranges
+        return N * (4 + 4 + 4 + 24 + 16 + 16 + 16 + 12 + 32) + 8 * U
+    if stage == "mesh_contacts":
+        return N * (16 + 4) + 56 * Cm + mesh_bytes
+    return 0
+
+
+def reset(s, sc):
+    p = sc["particles"]
+    s.set_particles(p["q"], p["qd"], p["inv_mass"], p["radii"], p["flags"])
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=60)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-profile", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU sharding is a design, not a run)", file=sys.stderr)
+        sys.exit(2)
+
+    import vkxpbd
+    from vkxpbd import scenes
+
+    sc = scenes.config1()
+    N = sc["particles"]["q"].shape[0]
+    verts, tris = sc["meshes"][0]
+    s = scenes.build_solver(vkxpbd, sc)
+    for _ in range(args.warmup):
+        s.step(DT, SUBSTEPS, 1)
+    reset(s, sc)
+    contacts = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s.step(DT, SUBSTEPS, 1)          # synchronous: returns after the GPU finished
+        contacts.append(s.last_contact_count)
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    value = N * SUBSTEPS * args.steps / elapsed
+    keys = s.get_sorted_cells()
+    U = int(np.count_nonzero(keys[1:] != keys[:-1]) + 1)
+    del s
+
+    out = {
+        "metric": "particle-substeps/s @1M (plain HIP step)",
+        "value": value,
+        "unit": "particle-substeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded lattice + icosphere, config 1)",
+        "config": {"workload": sc["name"], "particles": N, "triangles": int(tris.shape[0]), "substeps": SUBSTEPS,
+                   "iterations": 1, "frames_timed": args.steps, "dt": DT, "parallelism": "single-gpu"},
+    }
+
+    if not args.no_profile:
+        sp = scenes.build_solver(vkxpbd, sc, enable_profile=True)
+        for _ in range(args.warmup):
+            sp.step(DT, SUBSTEPS, 1)
+        reset(sp, sc)
+        sp.profile_reset()
+        cm = []
+        for _ in range(args.steps):
+            sp.step(DT, SUBSTEPS, 1)
+        shapes = sp.get_contact_shapes()
+        cm.append(int(np.count_nonzero(shapes > 0)))
+        prof = sp.profile_stages()
+        del sp
+        C = float(np.mean(contacts))
+        Cm = float(cm[-1]) * (C / max(contacts[-1], 1))
+        tiles = (N + 1023) // 1024
+        mesh_bytes = 2 * tris.shape[0] * 32 + tris.shape[0] * 16 + verts.shape[0] * 24
+        kern = {}
+        for name, (ms, launches) in prof.items():
+            if launches == 0 or name == "readback":
+                continue
+            avg_ms = ms / launches
+            b = stage_bytes(name, N, C, Cm, U, tiles, mesh_bytes)
+            kern[name] = {"avg_us": 1e3 * avg_ms, "launches": int(launches), "alg_bytes": b,
+                          "gbs": b / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0}
+        dom = max(kern, key=lambda k: kern[k]["avg_us"] * kern[k]["launches"])
+        d = kern[dom]
+        out["roofline"] = {"bound": "hbm", "kernel": dom, "achieved": d["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": d["gbs"] / HBM_PEAK_GBS, "traffic": None,
+                           "alg_bytes_per_launch": d["alg_bytes"], "avg_us": d["avg_us"]}
+        out["kernels"] = {k: {"avg_us": round(v["avg_us"], 3), "launches": v["launches"],
+                              "GBs": round(v["gbs"], 1)} for k, v in kern.items()}
+        out["gpu_ms_per_frame_profiled"] = sum(v[0] for v in prof.values()) / args.steps
+        out["contacts_mean"] = C
+        out["occupied_cells"] = U
+
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib
+        cfg = scenes.solver_config(vkxpbd, sc["r"])
+        o = oracle_lib.OracleSolver()
+        o.init(cfg, N)
+        o.add_mesh(verts, tris)
+        o.finalize_meshes()
+        p = sc["particles"]
+        o.set_particles(p["q"], p["qd"], p["inv_mass"], p["radii"], p["flags"])
+        c0 = time.perf_counter()
+        o.step(DT, SUBSTEPS, 1)
+        c1 = time.perf_counter()
+        out["cpu_baseline"] = {"value": N * SUBSTEPS / (c1 - c0), "unit": "particle-substeps/s", "cores": 1,
+                               "kind": "port",
+                               "sample": "frame 1 of the same workload (8 substeps from the initial state), "
+                                         "fp32 CPU restatement, 1 thread", "seconds": c1 - c0}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,197 @@
+/*
+ * xpbd_capi.h -- C-ABI of the MI355X XPBD particle solver (libvkxpbd.so).
+ *
+ * This is the drop-in boundary for the reference's GPU step: every entry point
+ * replaces one method of vkx::XpbdSolver (src/scene.h:74-227) or one free
+ * function of the vkxpbd pybind11 module (src/bindings.cpp:8-301).  The
+ * Python module `vkxpbd` in this repository binds exactly these symbols.
+ *
+ * Conventions
+ *   - Plain pointers and sizes; no C++ or torch types cross the ABI.
+ *   - Every call returns int status: 0 = ok, nonzero = error.  The message is
+ *     available from xpbd_last_error(h) and carries the reference's text where
+ *     the reference throws (e.g. "Particle count mismatch").
+ *   - Host arrays are copied in before the call returns (the reference's
+ *     synchronous staging, src/buffer.h:80-108); getters copy out into
+ *     caller-provided buffers whose capacity (in elements) is passed.
+ *   - Vectors are packed xyz float32 (3 per element) on the host side.
+ *   - Calls on one handle are not thread-safe (as the reference,
+ *     src/buffer.h:110-139).  Distinct handles are independent.
+ */
+#ifndef XPBD_CAPI_H
+#define XPBD_CAPI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define XPBD_NUM_CELLS 2097152 /* 128^3 hash-grid cells, src/scene.cpp:17-19 */
+#define XPBD_DIAG_WORDS 32     /* diagnostic word buffers, src/scene.cpp:120-128 */
+
+/* Mirrors vkx::SolverConfig (src/scene.h:14-41) field for field, minus the
+ * std::string shader_dir, which the Python layer keeps (no shaders here). */
+typedef struct xpbd_config {
+    float dt;
+    int32_t substeps;
+    int32_t iterations;
+    float particle_v_max;
+    float soft_contact_mu;
+    float shape_material_mu;
+    float soft_contact_relaxation;
+    float particle_adhesion;
+    float particle_mu;
+    float particle_cohesion;
+    float particle_max_radius;
+    float search_radius;
+    float soft_contact_margin;
+    int32_t enable_restitution;
+    float soft_contact_restitution;
+    int32_t has_ground_plane;
+    float ground_plane_altitude;
+    float gravity[3];
+    int32_t device_index;
+    int32_t compat_mode;
+    int32_t enable_validation;
+    int32_t enable_diagnostics;
+    int32_t enable_profile;
+    uint32_t max_diag_pairs;
+    int32_t substep_batch_size;
+} xpbd_config;
+
+/* One particle-particle diagnostic record, 80 bytes: vkx::GPUPairRecord
+ * (src/scene.h:43-64). */
+typedef struct xpbd_pair_record {
+    uint32_t i, j;
+    float dist, err;
+    float normal[3];
+    float vrel[3];
+    float vn, vt_len, denom, lambda_n, lambda_f;
+    float delta_i[3];
+    uint32_t iter_index, pad0;
+} xpbd_pair_record;
+
+typedef struct xpbd_solver xpbd_solver;
+
+/* Default values of vkx::SolverConfig (src/scene.h:14-41). */
+void xpbd_config_default(xpbd_config* cfg);
+
+/* enumerate_devices() (src/bindings.cpp:48; src/context.h:38-69): number of
+ * HIP devices and the name of one of them. */
+int xpbd_device_count(int* count);
+int xpbd_device_name(int index, char* buf, size_t buflen);
+
+/* vkx::XpbdSolver::XpbdSolver / ~XpbdSolver (src/scene.cpp:27-45). */
+int xpbd_create(xpbd_solver** out);
+void xpbd_destroy(xpbd_solver* h);
+/* Message of the last failed call on h (empty if none).  h may be NULL, then
+ * the message of the last failed xpbd_create/xpbd_device_* call is returned. */
+const char* xpbd_last_error(const xpbd_solver* h);
+
+/* XpbdSolver::init (src/scene.cpp:65-152).  max_contacts 0 means 2*n. */
+int xpbd_init(xpbd_solver* h, const xpbd_config* cfg, size_t num_particles, size_t max_contacts);
+
+/* XpbdSolver::add_mesh (src/scene.cpp:154-202).  rot_xyzw may be NULL
+ * (identity); watertight_override -1 = auto-detect, 0 = false, 1 = true. */
+int xpbd_add_mesh(xpbd_solver* h, const float* verts, size_t num_verts, const uint32_t* tris, size_t num_tris,
+                  float shape_margin, const float* rot_xyzw, int watertight_override);
+
+/* XpbdSolver::finalize_meshes (src/scene.cpp:204-390). */
+int xpbd_finalize_meshes(xpbd_solver* h);
+
+/* XpbdSolver::set_particles (src/scene.cpp:592-615).  qd NULL uploads zeros;
+ * inv_mass / radii / flags NULL keep the previous device contents. */
+int xpbd_set_particles(xpbd_solver* h, const float* q, const float* qd, const float* inv_mass, const float* radii,
+                       const int32_t* flags, size_t n);
+
+/* XpbdSolver::update_mesh (src/scene.cpp:617-696).  vels may be NULL (zero). */
+int xpbd_update_mesh(xpbd_solver* h, size_t mesh_index, const float* verts, size_t num_verts, const float* vels);
+
+/* XpbdSolver::step (src/scene.cpp:698-1092). */
+int xpbd_step(xpbd_solver* h, float dt, int substeps, int iterations);
+/* XpbdSolver::collide_only (src/scene.cpp:1229-1245). */
+int xpbd_collide_only(xpbd_solver* h);
+/* XpbdSolver::step_solve_only (src/scene.cpp:1094-1227). */
+int xpbd_step_solve_only(xpbd_solver* h, float dt, int substeps, int iterations);
+
+/* XpbdSolver::get_positions / get_velocities (src/scene.cpp:1247-1267):
+ * out holds n*3 floats; n is clamped to the particle count. */
+int xpbd_get_positions(xpbd_solver* h, float* out, size_t n);
+int xpbd_get_velocities(xpbd_solver* h, float* out, size_t n);
+
+/* particle_count / mesh_count / last_contact_count (src/scene.h:101-103). */
+int xpbd_particle_count(const xpbd_solver* h, size_t* out);
+int xpbd_mesh_count(const xpbd_solver* h, size_t* out);
+int xpbd_last_contact_count(const xpbd_solver* h, int* out);
+
+/* get_contact_* (src/scene.cpp:1269-1302): the first last_contact_count
+ * entries; cap is the capacity of out in contacts. */
+int xpbd_get_contact_particles(xpbd_solver* h, int32_t* out, size_t cap);
+int xpbd_get_contact_shapes(xpbd_solver* h, int32_t* out, size_t cap);
+int xpbd_get_contact_normals(xpbd_solver* h, float* out, size_t cap);
+int xpbd_get_contact_body_positions(xpbd_solver* h, float* out, size_t cap);
+int xpbd_get_contact_body_velocities(xpbd_solver* h, float* out, size_t cap);
+
+/* XpbdSolver::set_contact_data (src/scene.cpp:1304-1315).  Body velocities
+ * are not injected, as in the reference. */
+int xpbd_set_contact_data(xpbd_solver* h, const int32_t* particles, const int32_t* shapes, const float* body_positions,
+                          const float* normals, int contact_count);
+
+/* Grid debug getters (src/scene.cpp:1317-1348).  sorted cells / ids / deltas
+ * hold n (n*3 for deltas) elements; cell tables hold XPBD_NUM_CELLS. */
+int xpbd_get_sorted_cells(xpbd_solver* h, uint32_t* out, size_t n);
+int xpbd_get_sorted_point_ids(xpbd_solver* h, uint32_t* out, size_t n);
+int xpbd_get_cell_starts(xpbd_solver* h, int32_t* out, size_t n);
+int xpbd_get_cell_ends(xpbd_solver* h, int32_t* out, size_t n);
+int xpbd_get_deltas(xpbd_solver* h, float* out, size_t n);
+
+/* Diagnostics (src/scene.cpp:1350-1414). */
+int xpbd_get_diag_pair_count(xpbd_solver* h, uint32_t* out);
+int xpbd_get_diag_overflow_count(xpbd_solver* h, uint32_t* out);
+/* Copies min(count, max_diag_pairs, cap) records; *written receives the number. */
+int xpbd_get_diag_pairs(xpbd_solver* h, xpbd_pair_record* out, size_t cap, size_t* written);
+int xpbd_reset_diag_buffers(xpbd_solver* h);
+/* 32-word diagnostic buffers (get_p11_diag, get_p11_apply_diag,
+ * get_mesh_candidate_diag). */
+int xpbd_get_p11_diag(xpbd_solver* h, uint32_t* out32);
+int xpbd_get_p11_apply_diag(xpbd_solver* h, uint32_t* out32);
+int xpbd_get_mesh_candidate_diag(xpbd_solver* h, uint32_t* out32);
+/* get_shape_contact_diag: *len receives the float count; out may be NULL to
+ * query the length only. */
+int xpbd_get_shape_contact_diag(xpbd_solver* h, float* out, size_t cap, size_t* len);
+
+/* get_mesh_bvh_lowers / uppers (src/scene.cpp:1416-1436): nodes*4 floats,
+ * w = packed u32 bits.  *nodes receives the node count (0 for a bad index). */
+int xpbd_get_mesh_bvh_node_count(xpbd_solver* h, size_t mesh_index, size_t* nodes);
+int xpbd_get_mesh_bvh_lowers(xpbd_solver* h, size_t mesh_index, float* out, size_t cap_nodes);
+int xpbd_get_mesh_bvh_uppers(xpbd_solver* h, size_t mesh_index, float* out, size_t cap_nodes);
+
+/* XpbdSolver::print_profiler_summary (src/scene.cpp:1438-1515). */
+int xpbd_print_profiler_summary(xpbd_solver* h);
+
+/* Per-stage GPU time accumulated by the profiler (HIP events on the solver's
+ * stream; enable_profile or VKXPBD_PROFILE=1).  No reference counterpart as a
+ * separate call: it exposes the numbers print_profiler_summary prints.
+ * names receives up to cap NUL-separated stage names (may be NULL), ms and
+ * launches receive per-stage totals; *count receives the stage count. */
+int xpbd_profile_stages(xpbd_solver* h, char* names, size_t names_len, double* ms, uint64_t* launches, size_t cap,
+                        size_t* count);
+int xpbd_profile_reset(xpbd_solver* h);
+
+/* Host-only LBVH build of one mesh, the builder finalize_meshes uses
+ * (MeshBvh::build, src/mesh_bvh.h:40-69 -> src/mesh_lbvh.h:49-223).  Needs no
+ * GPU.  lowers/uppers receive 2*num_tris packed nodes (4 floats each, w =
+ * packed u32 bits), prim num_tris primitive ids, *root the root node. */
+int xpbd_lbvh_build(const float* verts, size_t num_verts, const uint32_t* tris, size_t num_tris, float* lowers,
+                    float* uppers, uint32_t* prim, uint32_t* root);
+/* Build over verts, then refit to new_verts (MeshBvh::refit,
+ * src/mesh_bvh.h:142-181).  Host only. */
+int xpbd_lbvh_refit(const float* verts, const float* new_verts, size_t num_verts, const uint32_t* tris,
+                    size_t num_tris, float* lowers, float* uppers);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* XPBD_CAPI_H */

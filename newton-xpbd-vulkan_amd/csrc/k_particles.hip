@@ -1,0 +1,398 @@
+// k_particles.hip -- per-particle and per-contact kernels of the XPBD substep.
+//
+// Layout in HBM (one solver): positions as float4 (x, y, z, radius) and
+// velocities as float4 (vx, vy, vz, inverse mass), three rotating buffers each
+// (state / predicted / output), so a neighbour read is one 16-byte load and no
+// copy-back is ever needed.  Flags are a separate int32 array.
+//
+// Kernel map (reference shader -> here):
+//   generate_plane_contacts + integrate_particles + grid_cell_indices +
+//   sort_histogram (pass 0)                 -> k_integrate
+//   generate_plane_contacts (collide_only)  -> k_plane_contacts
+//   grid_offsets                            -> k_cell_ranges (no table clear)
+//   solve_particle_shape_contacts           -> k_shape_solve
+//   solve_particle_particle_contacts +
+//   apply_particle_deltas                   -> k_pp_apply
+//   apply_particle_shape_restitution        -> k_restitution
+#include "xpbd_kernels.h"
+
+namespace xpbd {
+
+// Warp-striped tile mapping shared with the radix-sort kernels: wave w of a
+// 256-thread block owns keys [w*256, w*256+256) of the tile, 64 per step, so
+// item order within a tile is (wave, step, lane) = input order.
+__device__ __forceinline__ int tile_item(int tile, int it) {
+    int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    return tile * kSortTile + w * (kSortItems * kWave) + it * kWave + lane;
+}
+
+// ---------------------------------------------------------------------------
+// integrate_particles.comp:26-60 fused with generate_plane_contacts.comp:27-54
+// (evaluated on the pre-integration position, as the reference orders them:
+// scene.cpp:853-871), grid_cell_indices.comp:17-32 on the predicted position,
+// and the per-tile digit-0 histogram of the first radix pass.
+// The force buffer and world index of the reference have no setter and are
+// always zero (scene.cpp:82-85), so v1 = v0 + (0*w + g*[w>0])*dt.
+__global__ __launch_bounds__(256) void k_integrate(IntegrateArgs a) {
+    __shared__ uint32_t h[kSortBins];
+    const int tile = blockIdx.x;
+    if (threadIdx.x < kSortBins) h[threadIdx.x] = 0;
+    if (a.cc_next != nullptr && blockIdx.x == 0 && threadIdx.x == 0) *a.cc_next = 0;
+    __syncthreads();
+    for (int it = 0; it < kSortItems; it++) {
+        int idx = tile_item(tile, it);
+        bool valid = idx < a.n;
+        float4 P = make_float4(0.f, 0.f, 0.f, 0.f), V = P;
+        int f = 0;
+        if (valid) { P = a.Ps[idx]; V = a.Vs[idx]; f = a.flags[idx]; }
+        bool want = false;
+        if (a.has_plane && valid && (f & 1)) {
+            float dist = P.z - a.altitude;
+            want = dist < a.margin + P.w;
+        }
+        if (a.cc != nullptr) {
+            int slot = wave_reserve(a.cc, want);
+            if (want && slot < a.cmax) {
+                a.cpart[slot] = idx;
+                a.cshape[slot] = 0;
+                a.cbpos[slot] = make_float4(P.x, P.y, a.altitude, 0.f);
+                a.cbvel[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
+                a.cnorm[slot] = make_float4(0.f, 0.f, 1.f, 0.f);
+            }
+        }
+        if (!valid) continue;
+        float4 Pn = P, Vn = V;
+        if (f & 1) {
+            float w = V.w;
+            float gs = w > 0.0f ? 1.0f : 0.0f;
+            float fx = 0.0f * w, fy = 0.0f * w, fz = 0.0f * w;
+            f3 v1 = {V.x + (fx + a.gx * gs) * a.dt, V.y + (fy + a.gy * gs) * a.dt, V.z + (fz + a.gz * gs) * a.dt};
+            float mag = len3(v1);
+            if (mag > a.vmax) v1 = v1 * (a.vmax / mag);
+            Pn = make_float4(P.x + v1.x * a.dt, P.y + v1.y * a.dt, P.z + v1.z * a.dt, P.w);
+            Vn = make_float4(v1.x, v1.y, v1.z, V.w);
+        }
+        a.Pp[idx] = Pn;
+        a.Vp[idx] = Vn;
+        uint32_t key = cell_of(Pn.x, Pn.y, Pn.z, a.inv_cell);
+        a.keys[idx] = key;
+        atomicAdd(&h[key & (kSortBins - 1)], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x < kSortBins) a.hist[threadIdx.x * a.tiles + tile] = h[threadIdx.x];
+}
+
+// generate_plane_contacts.comp:27-54 on the current positions (collide_only).
+__global__ __launch_bounds__(256) void k_plane_contacts(PlaneArgs a) {
+    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    bool want = false;
+    float4 P = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (idx < a.n && (a.flags[idx] & 1)) {
+        P = a.P[idx];
+        want = (P.z - a.altitude) < a.margin + P.w;
+    }
+    int slot = wave_reserve(a.cc, want);
+    if (want && slot < a.cmax) {
+        a.cpart[slot] = idx;
+        a.cshape[slot] = 0;
+        a.cbpos[slot] = make_float4(P.x, P.y, a.altitude, 0.f);
+        a.cbvel[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
+        a.cnorm[slot] = make_float4(0.f, 0.f, 1.f, 0.f);
+    }
+}
+
+// grid_offsets.comp:16-35 without the per-substep 16 MiB table clear of
+// scene.cpp:918-923: every occupied cell's (start, end) is rewritten here, and
+// readers accept an entry only if keys[start] == cell, so stale entries of
+// cells emptied since an earlier substep are never used.
+__global__ __launch_bounds__(256) void k_cell_ranges(int n, const uint32_t* __restrict__ keys, int* __restrict__ range) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    uint32_t c = keys[t];
+    if (t == 0 || keys[t - 1] != c) range[2 * c] = t;
+    if (t == n - 1 || keys[t + 1] != c) range[2 * c + 1] = t + 1;
+}
+
+// Dense tables exactly as the reference leaves them after grid_offsets
+// (get_cell_starts / get_cell_ends); tables are zeroed by the caller.
+__global__ __launch_bounds__(256) void k_cell_tables(int n, const uint32_t* __restrict__ keys, int* __restrict__ starts,
+                                                     int* __restrict__ ends) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    int c = int(keys[t]);
+    if (t == 0) starts[c] = 0;
+    else {
+        int p = int(keys[t - 1]);
+        if (c != p) { starts[c] = t; ends[p] = t; }
+    }
+    if (t == n - 1) ends[c] = t + 1;
+}
+
+// Rigid transform of a shape's body; identity and zeros for body < 0 (the
+// ground plane), as solve_particle_shape_contacts.comp:123-131.
+struct BodyFrame { f3 pos; float4 rot; f3 com; f3 v; f3 w; };
+__device__ __forceinline__ BodyFrame load_body(const BodyArgs& b, int body) {
+    BodyFrame F{{0.f, 0.f, 0.f}, make_float4(0.f, 0.f, 0.f, 1.f), {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
+    if (body >= 0) {
+        F.pos = xyz(b.q[2 * body]);
+        F.rot = b.q[2 * body + 1];
+        F.com = {b.com[3 * body], b.com[3 * body + 1], b.com[3 * body + 2]};
+        F.v = xyz(b.qd[2 * body]);
+        F.w = xyz(b.qd[2 * body + 1]);
+    }
+    return F;
+}
+
+// solve_particle_shape_contacts.comp:104-200.  Accumulates into dshape, which
+// k_pp_apply consumes and re-zeroes.
+__global__ __launch_bounds__(256) void k_shape_solve(ShapeArgs a) {
+    const int count = min(a.cmax, *a.cc);
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < count; t += gridDim.x * blockDim.x) {
+        int s = a.cshape[t];
+        int body = a.body.shape_body[s];
+        int pi = a.cpart[t];
+        int pf = a.flags[pi];
+        if ((pf & 1) == 0) continue;
+        if (pf & 2) {
+            if (body < 0) continue;
+            if (a.body.flags[body] & 4) continue;
+            if (a.body.invm[body] == 0.0f) continue;
+        }
+        float4 P = a.P[pi], Vv = a.V[pi];
+        f3 px = xyz(P), pv = xyz(Vv);
+        BodyFrame F = load_body(a.body, body);
+        f3 bx = F.pos + quat_rotate_sc(F.rot, xyz(a.cbpos[t]));
+        f3 r = bx - (F.pos + quat_rotate_sc(F.rot, F.com));
+        f3 n = xyz(a.cnorm[t]);
+        f3 d = px - bx;
+        float c = fma_(n.z, d.z, fma_(n.x, d.x, n.y * d.y)) - P.w;
+        if (c > a.ka) continue;
+        float mu = 0.5f * (a.pmu + a.body.mat_mu[s]);
+        f3 bv = F.v + cross3(F.w, r) + quat_rotate_sc(F.rot, xyz(a.cbvel[t]));
+        f3 v = pv - bv;
+        float lambda_n = c;
+        f3 delta_n = n * lambda_n;
+        float vn = n.y * v.y;
+        vn = fma_(n.x, v.x, vn);
+        vn = fma_(n.z, v.z, vn);
+        f3 vt = {fma_(-n.x, vn, v.x), fma_(-n.y, vn, v.y), fma_(-n.z, vn, v.z)};
+        float w1 = Vv.w;
+        float w2 = 0.0f;
+        if (body >= 0) {
+            f3 ang = cross3(r, n);
+            f3 ra = quat_rotate_inv_sc(F.rot, ang);
+            const float* I = a.body.invI + 9 * body;   // row-major, as this shader reads it
+            f3 Ira = {I[0] * ra.x + I[1] * ra.y + I[2] * ra.z, I[3] * ra.x + I[4] * ra.y + I[5] * ra.z,
+                      I[6] * ra.x + I[7] * ra.y + I[8] * ra.z};
+            w2 = a.body.invm[body] + dot3(ra, Ira);
+        }
+        float denom = w1 + w2;
+        if (denom == 0.0f) continue;
+        float l2 = vt.y * vt.y;
+        l2 = fma_(vt.x, vt.x, l2);
+        l2 = fma_(vt.z, vt.z, l2);
+        float vt_len = exact_sqrt(l2);
+        float lambda_f = gmax(mu * lambda_n, -vt_len * a.dt);
+        f3 vt_dir = vt_len > 0.0f ? exact_div(vt, vt_len) : f3{0.f, 0.f, 0.f};
+        f3 diff = {fma_(vt_dir.x, lambda_f, -delta_n.x), fma_(vt_dir.y, lambda_f, -delta_n.y),
+                   fma_(vt_dir.z, lambda_f, -delta_n.z)};
+        f3 dtot = exact_div(diff, denom) * a.relax;
+        f3 fd = w1 * dtot;
+        atomicAdd(&a.dshape[3 * pi + 0], fd.x);
+        atomicAdd(&a.dshape[3 * pi + 1], fd.y);
+        atomicAdd(&a.dshape[3 * pi + 2], fd.z);
+        if (body >= 0 && a.body.invm[body] > 0.0f) {
+            f3 dv = {dtot.x / a.dt, dtot.y / a.dt, dtot.z / a.dt};
+            f3 dw = cross3(r, dv);
+            float* bd = a.body.delta + 8 * body;
+            atomicAdd(bd + 0, -dv.x); atomicAdd(bd + 1, -dv.y); atomicAdd(bd + 2, -dv.z);
+            atomicAdd(bd + 4, -dw.x); atomicAdd(bd + 5, -dw.y); atomicAdd(bd + 6, -dw.z);
+        }
+    }
+}
+
+// solve_particle_particle_contacts.comp:107-218 fused with
+// apply_particle_deltas.comp:38-58.  One thread per sorted slot t (the
+// reference's thread order), particle i = vals[t].  Particle i's delta is
+// (shape contributions) + (its own particle-particle sum): the reference
+// accumulates exactly these with float atomics into a zeroed buffer, and with
+// at most two shape contacts per particle the sum is order independent, so a
+// single thread can finish the particle and write the applied state directly.
+template <bool DIAG>
+__global__ __launch_bounds__(256) void k_pp_apply(PPArgs a) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.n) return;
+    const int i = int(a.vals[t]);
+    const int pf = a.flags[i];
+    f3 ds = {a.dshape[3 * i], a.dshape[3 * i + 1], a.dshape[3 * i + 2]};
+    a.dshape[3 * i] = 0.0f; a.dshape[3 * i + 1] = 0.0f; a.dshape[3 * i + 2] = 0.0f;
+    const float4 Pi = a.Pp[i], Vi = a.Vp[i];
+    if ((pf & 1) == 0) {   // inactive: passthrough (the reference leaves x_out unwritten)
+        a.delta[3 * i] = ds.x; a.delta[3 * i + 1] = ds.y; a.delta[3 * i + 2] = ds.z;
+        a.Po[i] = Pi;
+        a.Vo[i] = Vi;
+        return;
+    }
+    const int proxy = pf & 2;
+    const f3 x = xyz(Pi), v = xyz(Vi);
+    const float rad = Pi.w, w1 = Vi.w;
+    const float R = rad + a.max_r + a.coh;
+    const float inv = a.inv_cell;
+    const int xs = (int)floorf((x.x - R) * inv), ys = (int)floorf((x.y - R) * inv), zs = (int)floorf((x.z - R) * inv);
+    const int xe = min((int)floorf((x.x + R) * inv), xs + kGridDim - 1);
+    const int ye = min((int)floorf((x.y + R) * inv), ys + kGridDim - 1);
+    const int ze = min((int)floorf((x.z + R) * inv), zs + kGridDim - 1);
+    f3 acc = {0.f, 0.f, 0.f};
+    for (int cz = zs; cz <= ze; cz++)
+        for (int cy = ys; cy <= ye; cy++)
+            for (int cx = xs; cx <= xe; cx++) {
+                const int cell = cell_index(cx, cy, cz);
+                const int2 rg = reinterpret_cast<const int2*>(a.range)[cell];
+                if (rg.x >= a.n || a.keys[rg.x] != uint32_t(cell)) continue;   // stale entry: cell empty now
+                for (int k = rg.x; k < rg.y; k++) {
+                    const int j = int(a.vals[k]);
+                    if (j == i) continue;
+                    const int nf = a.flags[j];
+                    if ((nf & 1) == 0) continue;
+                    const float4 Pj = a.Pp[j];
+                    const float4 Vj = a.Vp[j];
+                    if (proxy != 0 && (((nf & 2) != 0) || (Vj.w <= 0.0f))) continue;
+                    const f3 rd = x - xyz(Pj);
+                    if (fabsf(rd.x) > R || fabsf(rd.y) > R || fabsf(rd.z) > R) continue;
+                    const float d = exact_length(rd);
+                    const float err = d - rad - Pj.w;
+                    if (err > a.coh || d <= 0.0f) continue;
+                    const float w2 = Vj.w;
+                    const float denom = w1 + w2;
+                    if (denom <= 0.0f) continue;
+                    const f3 nn = exact_div(rd, d);
+                    const f3 vrel = v - xyz(Vj);
+                    const float lambda_n = err;
+                    const float vn = exact_dot(nn, vrel);
+                    const f3 vt = {fma_(vn, -nn.x, vrel.x), fma_(vn, -nn.y, vrel.y), fma_(vn, -nn.z, vrel.z)};
+                    const float vt_len = exact_length(vt);
+                    const float lambda_f = gmax(a.mu * lambda_n, -vt_len * a.dt);
+                    const f3 vt_dir = vt_len > 0.0f ? exact_div(vt, vt_len) : f3{0.f, 0.f, 0.f};
+                    const f3 p2 = nn * err;
+                    const f3 df = {fma_(vt_dir.x, lambda_f, -p2.x), fma_(vt_dir.y, lambda_f, -p2.y),
+                                   fma_(vt_dir.z, lambda_f, -p2.z)};
+                    const f3 pair_d = exact_div(df, denom);
+                    acc = acc + pair_d;
+                    if (DIAG) {
+                        uint32_t slot = atomicAdd(a.diag_header, 1u);
+                        if (slot < a.max_diag_pairs) {
+                            f3 dw = {fma_(pair_d.x, w1, 0.f), fma_(pair_d.y, w1, 0.f), fma_(pair_d.z, w1, 0.f)};
+                            f3 di = {fma_(dw.x, a.relax, 0.f), fma_(dw.y, a.relax, 0.f), fma_(dw.z, a.relax, 0.f)};
+                            float* rec = a.diag_pairs + 20 * size_t(slot);
+                            reinterpret_cast<uint32_t*>(rec)[0] = uint32_t(i);
+                            reinterpret_cast<uint32_t*>(rec)[1] = uint32_t(j);
+                            rec[2] = d; rec[3] = err;
+                            rec[4] = nn.x; rec[5] = nn.y; rec[6] = nn.z;
+                            rec[7] = vrel.x; rec[8] = vrel.y; rec[9] = vrel.z;
+                            rec[10] = vn; rec[11] = vt_len; rec[12] = denom; rec[13] = lambda_n; rec[14] = lambda_f;
+                            rec[15] = di.x; rec[16] = di.y; rec[17] = di.z;
+                            reinterpret_cast<uint32_t*>(rec)[18] = a.iter_index;
+                            reinterpret_cast<uint32_t*>(rec)[19] = 0u;
+                        } else {
+                            atomicAdd(a.diag_header + 1, 1u);
+                        }
+                    }
+                }
+            }
+    const f3 dw = {fma_(acc.x, w1, 0.f), fma_(acc.y, w1, 0.f), fma_(acc.z, w1, 0.f)};
+    const f3 tot = {fma_(dw.x, a.relax, 0.f), fma_(dw.y, a.relax, 0.f), fma_(dw.z, a.relax, 0.f)};
+    const f3 dl = ds + tot;
+    a.delta[3 * i] = dl.x; a.delta[3 * i + 1] = dl.y; a.delta[3 * i + 2] = dl.z;
+    // apply_particle_deltas.comp:44-57
+    const f3 x0 = xyz(a.Ps[i]);
+    f3 xn = x + dl;
+    f3 vnew = markstein_div(xn - x0, a.dt);
+    const float mag = len3(vnew);
+    if (mag > a.vmax) {
+        vnew = vnew * (a.vmax / mag);
+        xn = x0 + vnew * a.dt;
+    }
+    a.Po[i] = make_float4(xn.x, xn.y, xn.z, rad);
+    a.Vo[i] = make_float4(vnew.x, vnew.y, vnew.z, w1);
+}
+
+// apply_particle_shape_restitution.comp:48-116.  Body previous pose/velocity
+// are the current ones (scene.cpp:570-574 binds the same buffers twice).
+__global__ __launch_bounds__(256) void k_restitution(RestArgs a) {
+    const int count = min(a.cmax, *a.cc);
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < count; t += gridDim.x * blockDim.x) {
+        int s = a.cshape[t];
+        int body = a.body.shape_body[s];
+        int pi = a.cpart[t];
+        if ((a.flags[pi] & 1) == 0) continue;
+        float4 Vn = a.V[pi];
+        f3 vnew = xyz(Vn);
+        float4 X0 = a.P0[pi];
+        f3 px = xyz(X0);
+        f3 vold = xyz(a.V0[pi]);
+        BodyFrame F = load_body(a.body, body);
+        f3 cb = xyz(a.cbpos[t]);
+        f3 bx = F.pos + quat_rotate_sc(F.rot, cb);
+        f3 n = xyz(a.cnorm[t]);
+        f3 d = px - bx;
+        float c = fma_(n.z, d.z, fma_(n.x, d.x, n.y * d.y)) - X0.w;
+        if (c > a.ka) continue;
+        f3 bxp = F.pos + quat_rotate_sc(F.rot, cb);
+        f3 r = bxp - (F.pos + quat_rotate_sc(F.rot, F.com));
+        f3 bvc = quat_rotate_sc(F.rot, xyz(a.cbvel[t]));
+        f3 bvo = bvc, bvn = bvc;
+        if (body >= 0) {
+            bvo = (F.v + cross3(F.w, r)) + bvc;
+            bvn = (F.v + cross3(F.w, r)) + bvc;
+        }
+        f3 dvo = vold - bvo;
+        float rvo = fma_(n.z, dvo.z, fma_(n.y, dvo.y, n.x * dvo.x));
+        f3 dvn = vnew - bvn;
+        float rvn = fma_(n.z, dvn.z, fma_(n.x, dvn.x, n.y * dvn.y));
+        if (rvo < 0.0f) {
+            float rt = gmax((-a.restitution) * rvo, 0.0f);
+            float sc = (-rvn) + rt;
+            float* vp = reinterpret_cast<float*>(a.V + pi);
+            atomicAdd(vp + 0, n.x * sc);
+            atomicAdd(vp + 1, n.y * sc);
+            atomicAdd(vp + 2, n.z * sc);
+        }
+    }
+}
+
+__global__ void k_zero_int(int* p, int count) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < count) p[t] = 0;
+}
+
+// ------------------------------------------------------------- launchers --
+static inline unsigned blocks_for(long long n, int bs) { return unsigned((n + bs - 1) / bs); }
+
+void launch_integrate(const IntegrateArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_integrate, dim3(a.tiles), dim3(256), 0, s, a);
+}
+void launch_plane_contacts(const PlaneArgs& a, hipStream_t s) {
+    if (a.n > 0) hipLaunchKernelGGL(k_plane_contacts, dim3(blocks_for(a.n, 256)), dim3(256), 0, s, a);
+}
+void launch_cell_ranges(int n, const uint32_t* keys, int* range, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_cell_ranges, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, keys, range);
+}
+void launch_cell_tables(int n, const uint32_t* keys, int* starts, int* ends, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_cell_tables, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, keys, starts, ends);
+}
+void launch_shape_solve(const ShapeArgs& a, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(k_shape_solve, dim3(grid), dim3(256), 0, s, a);
+}
+void launch_pp_apply(const PPArgs& a, bool diag, hipStream_t s) {
+    if (a.n <= 0) return;
+    if (diag) hipLaunchKernelGGL(k_pp_apply<true>, dim3(blocks_for(a.n, 256)), dim3(256), 0, s, a);
+    else hipLaunchKernelGGL(k_pp_apply<false>, dim3(blocks_for(a.n, 256)), dim3(256), 0, s, a);
+}
+void launch_restitution(const RestArgs& a, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(k_restitution, dim3(grid), dim3(256), 0, s, a);
+}
+void launch_zero_int(int* p, int count, hipStream_t s) {
+    if (count > 0) hipLaunchKernelGGL(k_zero_int, dim3(blocks_for(count, 256)), dim3(256), 0, s, p, count);
+}
+
+}  // namespace xpbd

@@ -1,0 +1,184 @@
+// lbvh.cpp -- host LBVH construction and refit for the mesh narrowphase.
+//
+// Construction is the agglomerative bottom-up LBVH (Apetrei 2014) that
+// src/mesh_lbvh.h:49-223 uses; the topology it yields depends only on the
+// sorted Morton codes, so it is computed here in three passes: (1) codes +
+// stable LSD radix sort, (2) parent links by climbing from every leaf with a
+// per-node arrival count, (3) boxes and leaf collapsing top-down.
+#include "lbvh.h"
+
+#include <algorithm>
+#include <cstring>
+
+namespace xpbd {
+
+namespace {
+
+inline uint32_t interleave3(uint32_t v) {   // 10 bits -> every third bit
+    v &= 0x3FFu;
+    v = (v | (v << 16)) & 0xFF0000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+inline uint32_t quantize1024(float t) {
+    int q = int(t * 1024.0f);
+    return uint32_t(q < 0 ? 0 : (q > 1023 ? 1023 : q));
+}
+
+inline int prefix_len(uint64_t a, uint64_t b) {
+    uint64_t x = a ^ b;
+    return x ? __builtin_clzll(x) : 64;
+}
+
+}  // namespace
+
+void lbvh_build(const float* pts, const uint32_t* tris, uint32_t nt, MeshBvhHost& out) {
+    out.lo.assign(size_t(2) * nt, NodeHalf{0.f, 0.f, 0.f, 0u});
+    out.hi.assign(size_t(2) * nt, NodeHalf{0.f, 0.f, 0.f, 0u});
+    out.prim.assign(nt, 0u);
+    out.root = 0;
+    if (nt == 0) return;
+
+    // triangle boxes and scene box
+    std::vector<float> bmin(size_t(3) * nt), bmax(size_t(3) * nt);
+    float smin[3] = {1e30f, 1e30f, 1e30f}, smax[3] = {-1e30f, -1e30f, -1e30f};
+    for (uint32_t t = 0; t < nt; t++) {
+        for (int ax = 0; ax < 3; ax++) {
+            float a = pts[size_t(tris[3 * t]) * 3 + ax], b = pts[size_t(tris[3 * t + 1]) * 3 + ax],
+                  c = pts[size_t(tris[3 * t + 2]) * 3 + ax];
+            float lo = std::min(std::min(a, b), c), hi = std::max(std::max(a, b), c);
+            bmin[3 * t + ax] = lo;
+            bmax[3 * t + ax] = hi;
+            smin[ax] = std::min(smin[ax], lo);
+            smax[ax] = std::max(smax[ax], hi);
+        }
+    }
+    float inv[3];
+    for (int ax = 0; ax < 3; ax++) inv[ax] = 1.0f / ((smax[ax] - smin[ax]) + 0.0001f);
+
+    // Morton codes, then a stable 3 x 10-bit LSD counting sort of (code, id)
+    std::vector<uint32_t> code(nt), order(nt), tmp(nt);
+    for (uint32_t t = 0; t < nt; t++) {
+        uint32_t q[3];
+        for (int ax = 0; ax < 3; ax++) {
+            float c = 0.5f * (bmin[3 * t + ax] + bmax[3 * t + ax]);
+            q[ax] = quantize1024((c - smin[ax]) * inv[ax]);
+        }
+        code[t] = (interleave3(q[2]) << 2) | (interleave3(q[1]) << 1) | interleave3(q[0]);
+        order[t] = t;
+    }
+    std::vector<uint32_t> cnt(1024);
+    for (int pass = 0; pass < 3; pass++) {
+        const int sh = 10 * pass;
+        std::fill(cnt.begin(), cnt.end(), 0u);
+        for (uint32_t k = 0; k < nt; k++) cnt[(code[order[k]] >> sh) & 1023u]++;
+        uint32_t run = 0;
+        for (auto& c : cnt) { uint32_t v = c; c = run; run += v; }
+        for (uint32_t k = 0; k < nt; k++) tmp[cnt[(code[order[k]] >> sh) & 1023u]++] = order[k];
+        order.swap(tmp);
+    }
+    std::vector<uint64_t> key(nt);
+    for (uint32_t k = 0; k < nt; k++) { key[k] = code[order[k]]; out.prim[k] = order[k]; }
+
+    // gap g separates sorted leaves g and g+1; its internal node is n + g
+    const int n = int(nt);
+    std::vector<int> gap(n > 1 ? n - 1 : 1, 0);
+    for (int g = 0; g + 1 < n; g++) gap[g] = prefix_len(key[g], key[g + 1]);
+
+    const int total = 2 * n;
+    std::vector<int> parent(total, -1), left(total, -1), right(total, -1), lo_rng(total, 0), hi_rng(total, 0),
+        arrivals(total, 0);
+    for (int i = 0; i < n; i++) { lo_rng[i] = hi_rng[i] = i; }
+    for (int i = 0; i < n; i++) {
+        int node = i;
+        while (true) {
+            const int l = lo_rng[node], r = hi_rng[node];
+            if (l == 0 && r == n - 1) { out.root = uint32_t(node); break; }
+            // merge towards the neighbouring gap with the longer common prefix;
+            // equal prefixes go by the parity of the two bordering primitive ids
+            bool merge_right;
+            if (l == 0) merge_right = true;
+            else if (r == n - 1) merge_right = false;
+            else if (gap[r] > gap[l - 1]) merge_right = true;
+            else if (gap[r] < gap[l - 1]) merge_right = false;
+            else merge_right = ((out.prim[l - 1] & 1u) != (out.prim[r] & 1u));
+            int p;
+            if (merge_right) { p = n + r; left[p] = node; lo_rng[p] = l; }
+            else { p = n + l - 1; right[p] = node; hi_rng[p] = r; }
+            parent[node] = p;
+            if (++arrivals[p] == 1) break;   // the sibling is still climbing
+            node = p;
+        }
+    }
+
+    // depths from the root, then boxes bottom-up in reverse BFS order
+    std::vector<int> depth(total, 0), bfs;
+    bfs.reserve(total);
+    bfs.push_back(int(out.root));
+    depth[out.root] = 1;
+    for (size_t k = 0; k < bfs.size(); k++) {
+        int v = bfs[k];
+        if (v < n) continue;
+        for (int c : {left[v], right[v]}) { depth[c] = depth[v] + 1; bfs.push_back(c); }
+    }
+    for (int i = 0; i < n; i++) {
+        const uint32_t t = out.prim[i];
+        out.lo[i] = NodeHalf{bmin[3 * t], bmin[3 * t + 1], bmin[3 * t + 2], uint32_t(i) | (1u << 30)};
+        out.hi[i] = NodeHalf{bmax[3 * t], bmax[3 * t + 1], bmax[3 * t + 2], uint32_t(i) + 1u};
+    }
+    for (size_t k = bfs.size(); k-- > 0;) {
+        const int v = bfs[k];
+        if (v < n) continue;
+        const NodeHalf &a = out.lo[left[v]], &b = out.lo[right[v]], &c = out.hi[left[v]], &d = out.hi[right[v]];
+        out.lo[v] = NodeHalf{std::min(a.x, b.x), std::min(a.y, b.y), std::min(a.z, b.z), uint32_t(left[v])};
+        out.hi[v] = NodeHalf{std::max(c.x, d.x), std::max(c.y, d.y), std::max(c.z, d.z), uint32_t(right[v])};
+    }
+    // collapse small or deep subtrees into packed leaves over [lo, hi)
+    for (int v : bfs) {
+        const int span = hi_rng[v] + 1 - lo_rng[v];
+        if (span <= 4 || depth[v] >= 32) {
+            out.lo[v].packed = uint32_t(lo_rng[v]) | (1u << 30);
+            out.hi[v].packed = uint32_t(hi_rng[v] + 1);
+        }
+    }
+}
+
+namespace {
+void refit_node(MeshBvhHost& b, const float* pts, const uint32_t* tris, uint32_t v) {
+    const uint32_t lp = b.lo[v].packed;
+    if ((lp >> 30) & 3u) {
+        float mn[3] = {1e30f, 1e30f, 1e30f}, mx[3] = {-1e30f, -1e30f, -1e30f};
+        const uint32_t e = b.hi[v].packed & 0x3FFFFFFFu;
+        for (uint32_t k = lp & 0x3FFFFFFFu; k < e; k++) {
+            const uint32_t t = b.prim[k];
+            for (int c = 0; c < 3; c++)
+                for (int ax = 0; ax < 3; ax++) {
+                    const float x = pts[size_t(tris[3 * t + c]) * 3 + ax];
+                    mn[ax] = std::min(mn[ax], x);
+                    mx[ax] = std::max(mx[ax], x);
+                }
+        }
+        b.lo[v].x = mn[0]; b.lo[v].y = mn[1]; b.lo[v].z = mn[2];
+        b.hi[v].x = mx[0]; b.hi[v].y = mx[1]; b.hi[v].z = mx[2];
+        return;
+    }
+    const uint32_t l = lp & 0x3FFFFFFFu, r = b.hi[v].packed;
+    refit_node(b, pts, tris, l);
+    refit_node(b, pts, tris, r);
+    b.lo[v].x = std::min(b.lo[l].x, b.lo[r].x);
+    b.lo[v].y = std::min(b.lo[l].y, b.lo[r].y);
+    b.lo[v].z = std::min(b.lo[l].z, b.lo[r].z);
+    b.hi[v].x = std::max(b.hi[l].x, b.hi[r].x);
+    b.hi[v].y = std::max(b.hi[l].y, b.hi[r].y);
+    b.hi[v].z = std::max(b.hi[l].z, b.hi[r].z);
+}
+}  // namespace
+
+void lbvh_refit(MeshBvhHost& b, const float* pts, const uint32_t* tris) {
+    if (!b.lo.empty()) refit_node(b, pts, tris, b.root);
+}
+
+}  // namespace xpbd

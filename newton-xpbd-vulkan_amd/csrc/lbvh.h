@@ -1,0 +1,35 @@
+// lbvh.h -- triangle-mesh LBVH in the packed node format the narrowphase
+// reads (src/mesh_bvh.h:14-20): per node two 16-byte halves, lower =
+// (min xyz, left child | leaf flag << 30), upper = (max xyz, right child);
+// a leaf stores the primitive range [lo, hi) in the two packed words.
+#pragma once
+#include <cstdint>
+#include <vector>
+
+namespace xpbd {
+
+struct NodeHalf {
+    float x, y, z;
+    uint32_t packed;
+};
+static_assert(sizeof(NodeHalf) == 16, "node half must be 16 bytes");
+
+struct MeshBvhHost {
+    std::vector<NodeHalf> lo, hi;   // 2 * num_tris nodes
+    std::vector<uint32_t> prim;     // leaf-order triangle ids (mesh local)
+    uint32_t root = 0;
+};
+
+// Builds the BVH of one mesh.  pts: vertex xyz; tris: 3 vertex indices per
+// triangle (indices into pts).  Same construction as the reference
+// (src/mesh_lbvh.h:49-223): 10-bit-per-axis Morton codes of triangle-box
+// centroids, stable order, agglomerative bottom-up pairing on the
+// common-prefix lengths, then subtrees spanning <= 4 primitives (or at depth
+// >= 32) collapse to packed leaves.
+void lbvh_build(const float* pts, const uint32_t* tris, uint32_t num_tris, MeshBvhHost& out);
+
+// Recomputes node boxes from the current vertex positions over the nodes
+// reachable from the root (src/mesh_bvh.h:142-181); topology is unchanged.
+void lbvh_refit(MeshBvhHost& b, const float* pts, const uint32_t* tris);
+
+}  // namespace xpbd

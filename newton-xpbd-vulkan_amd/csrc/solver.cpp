@@ -1,0 +1,696 @@
+// solver.cpp -- host side of the MI355X XPBD step (see solver.h).
+//
+// Substep pipeline on one HIP stream (K = iterations):
+//   [k_mesh_contacts]            particle/mesh narrowphase on x0      (if meshes)
+//   k_integrate                  plane contacts on x0, x0 -> x_pred, cell keys,
+//                                digit-0 tile histograms
+//   sort: binscan, scatter, (upsweep, binscan, scatter) x 2
+//   k_cell_ranges                (start, end) of every occupied cell
+//   K x { k_shape_solve, k_pp_apply }
+//   [k_restitution]
+// Compared with src/scene.cpp:805-992 this drops the contact-counter fill,
+// the 16 MiB cell-table fills, the delta fill and the odd-K copy-back, and
+// replaces 15 sort dispatches by 8.
+#include "solver.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <stdexcept>
+
+#define HIPCHECK(x)                                                                                          \
+    do {                                                                                                     \
+        hipError_t e_ = (x);                                                                                 \
+        if (e_ != hipSuccess)                                                                                \
+            throw std::runtime_error(std::string("HIP error ") + hipGetErrorString(e_) + " at " + __FILE__ + \
+                                     ":" + std::to_string(__LINE__));                                        \
+    } while (0)
+
+namespace xpbd {
+
+namespace {
+
+template <class T> void dalloc(DevBuf<T>& b, size_t n) {
+    if (b.p) { (void)hipFree(b.p); b.p = nullptr; b.n = 0; }
+    size_t m = n ? n : 1;
+    HIPCHECK(hipMalloc(reinterpret_cast<void**>(&b.p), m * sizeof(T)));
+    HIPCHECK(hipMemset(b.p, 0, m * sizeof(T)));
+    b.n = n;
+}
+template <class T> void dfree(DevBuf<T>& b) {
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.n = 0;
+}
+template <class T> void upload(DevBuf<T>& b, const T* src, size_t n) {
+    dalloc(b, n);
+    if (n) HIPCHECK(hipMemcpy(b.p, src, n * sizeof(T), hipMemcpyHostToDevice));
+}
+
+struct DeviceGuard {
+    explicit DeviceGuard(int dev) { (void)hipSetDevice(dev); }
+};
+
+constexpr const char* kStageNames[] = {"mesh_contacts", "integrate",  "sort_upsweep", "sort_binscan", "sort_scatter",
+                                       "cell_ranges",   "shape_solve", "pp_solve_apply", "restitution", "readback"};
+
+}  // namespace
+
+Solver::Solver() = default;
+
+Solver::~Solver() {
+    if (!initialized_) return;
+    DeviceGuard g(device_);
+    (void)hipStreamSynchronize(stream_);
+    if (profiling_) print_profiler_summary();
+    free_all();
+}
+
+void Solver::require_init() const {
+    if (!initialized_) throw std::runtime_error("XpbdSolver not initialized");
+}
+
+void Solver::free_all() {
+    for (int k = 0; k < 3; k++) { dfree(P_[k]); dfree(V_[k]); }
+    dfree(flags_); dfree(keysA_); dfree(valsA_); dfree(keysB_); dfree(valsB_); dfree(hist_); dfree(bintot_);
+    dfree(range_); dfree(tbl_); dfree(cpart_); dfree(cshape_); dfree(cc_); dfree(cbpos_); dfree(cbvel_); dfree(cnorm_);
+    dfree(dshape_); dfree(delta_); dfree(body_q_); dfree(body_qd_); dfree(body_com_); dfree(body_invI_);
+    dfree(body_invm_); dfree(body_delta_); dfree(mat_mu_); dfree(body_flags_); dfree(shape_body_);
+    dfree(verts_d_); dfree(vvel_d_); dfree(tris_d_); dfree(prim_d_); dfree(err_d_); dfree(nl_d_); dfree(nu_d_);
+    dfree(info_d_); dfree(diag_header_); dfree(diag_pairs_);
+    for (hipEvent_t e : ev_pool_) (void)hipEventDestroy(e);
+    ev_pool_.clear();
+    if (count_host_) { (void)hipHostFree(count_host_); count_host_ = nullptr; }
+    if (stream_) { (void)hipStreamDestroy(stream_); stream_ = nullptr; }
+}
+
+void Solver::alloc_all() {
+    for (int k = 0; k < 3; k++) { dalloc(P_[k], n_); dalloc(V_[k], n_); }
+    s_ = 0; p_ = 1; o_ = 2;
+    dalloc(flags_, n_);
+    radius_h_.assign(n_, 0.f);
+    invm_h_.assign(n_, 0.f);
+    dalloc(keysA_, n_); dalloc(valsA_, n_); dalloc(keysB_, n_); dalloc(valsB_, n_);
+    const size_t tiles = (n_ + kSortTile - 1) / kSortTile;
+    dalloc(hist_, size_t(kSortBins) * std::max<size_t>(tiles, 1));
+    dalloc(bintot_, kSortBins);
+    dalloc(range_, size_t(2) * kNumCells);
+    dalloc(cpart_, cmax_); dalloc(cshape_, cmax_);
+    dalloc(cbpos_, cmax_); dalloc(cbvel_, cmax_); dalloc(cnorm_, cmax_);
+    dalloc(cc_, 4);   // [slot 0, slot 1, error flags, spare]
+    dalloc(dshape_, 3 * n_); dalloc(delta_, 3 * n_);
+    dalloc(diag_header_, 2);
+    max_pairs_ = cfg_.max_diag_pairs ? cfg_.max_diag_pairs : 100000u;
+    dalloc(diag_pairs_, cfg_.enable_diagnostics ? size_t(20) * max_pairs_ : 20);
+    HIPCHECK(hipHostMalloc(reinterpret_cast<void**>(&count_host_), 4 * sizeof(int32_t), hipHostMallocDefault));
+}
+
+void Solver::init(const xpbd_config& cfg, size_t n, size_t max_contacts) {
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) throw std::runtime_error("No HIP devices found");
+    if (n > (size_t(1) << 28)) throw std::runtime_error("num_particles exceeds the supported maximum of 2^28");
+    if (initialized_) { DeviceGuard g(device_); (void)hipDeviceSynchronize(); free_all(); initialized_ = false; }
+    cfg_ = cfg;
+    n_ = n;
+    cmax_ = max_contacts ? max_contacts : n * 2;
+    if (cmax_ > size_t(0x7FFFFFFF)) throw std::runtime_error("max_contacts exceeds the int32 range");
+    device_ = (cfg.device_index >= 0 && cfg.device_index < count) ? cfg.device_index : 0;
+    DeviceGuard g(device_);
+    HIPCHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    alloc_all();
+    const char* env = std::getenv("VKXPBD_PROFILE");
+    bool env_on = env && (std::strcmp(env, "1") == 0 || std::strcmp(env, "true") == 0);
+    profiling_ = cfg.enable_profile || env_on;
+    if (profiling_) std::printf("[vkxpbd] GPU profiler enabled (HIP events)\n");
+    grid_valid_ = false;
+    contact_slot_ = 0;
+    last_count_ = 0;
+    initialized_ = true;
+    finalized_ = false;
+}
+
+void Solver::add_mesh(const float* v, size_t nv, const uint32_t* t, size_t nt, float margin, const float* rot, int wt) {
+    MeshHost m;
+    m.verts.assign(v, v + nv * 3);
+    m.tris.assign(t, t + nt * 3);
+    m.margin = margin;
+    if (rot) std::memcpy(m.rot, rot, sizeof(m.rot));
+    if (wt >= 0) m.watertight = wt != 0;
+    else if (nt == 0 || nv == 0) m.watertight = false;
+    else {   // every undirected edge shared by exactly two triangles (scene.cpp:170-198)
+        std::vector<uint64_t> e;
+        e.reserve(nt * 3);
+        for (size_t k = 0; k < nt; k++)
+            for (int s = 0; s < 3; s++) {
+                uint64_t a = t[k * 3 + s], b = t[k * 3 + (s + 1) % 3];
+                e.push_back(a < b ? (a << 32) | b : (b << 32) | a);
+            }
+        std::sort(e.begin(), e.end());
+        bool closed = true;
+        for (size_t i = 0; i < e.size() && closed;) {
+            size_t j = i + 1;
+            while (j < e.size() && e[j] == e[i]) j++;
+            closed = (j - i) == 2;
+            i = j;
+        }
+        m.watertight = closed;
+    }
+    meshes_.push_back(std::move(m));
+    finalized_ = false;
+}
+
+void Solver::upload_nodes() {
+    size_t total = 0;
+    for (auto& b : bvh_) total += b.lo.size();
+    std::vector<float4> nl(total), nu(total);
+    for (size_t c = 0; c < bvh_.size(); c++) {
+        const uint32_t nb = uint32_t(node_off_[c]), pb = uint32_t(prim_off_[c]);
+        for (size_t i = 0; i < bvh_[c].lo.size(); i++) {
+            const NodeHalf L = bvh_[c].lo[i], U = bvh_[c].hi[i];
+            uint32_t lw, uw;
+            if ((L.packed >> 30) & 3u) {   // leaf: primitive range, re-based (scene.cpp:299-313)
+                lw = ((L.packed & 0x3FFFFFFFu) + pb) | (1u << 30);
+                uw = (U.packed & 0x3FFFFFFFu) + pb;
+            } else {
+                lw = (L.packed & 0x3FFFFFFFu) + nb;
+                uw = (U.packed & 0x3FFFFFFFu) + nb;
+            }
+            float lf, uf;
+            std::memcpy(&lf, &lw, 4);
+            std::memcpy(&uf, &uw, 4);
+            nl[nb + i] = make_float4(L.x, L.y, L.z, lf);
+            nu[nb + i] = make_float4(U.x, U.y, U.z, uf);
+        }
+    }
+    upload(nl_d_, nl.data(), nl.size());
+    upload(nu_d_, nu.data(), nu.size());
+}
+
+void Solver::finalize_meshes() {
+    if (!initialized_) throw std::runtime_error("XpbdSolver::init must be called before finalize_meshes");
+    DeviceGuard g(device_);
+    HIPCHECK(hipStreamSynchronize(stream_));
+    const size_t M = meshes_.size();
+    std::vector<int32_t> shape_body(M + 1, -1);
+    for (size_t c = 0; c < M; c++) shape_body[c + 1] = int32_t(c);
+    std::vector<float> mat_mu(M + 1, cfg_.shape_material_mu);
+    upload(shape_body_, shape_body.data(), shape_body.size());
+    upload(mat_mu_, mat_mu.data(), mat_mu.size());
+    // one static body per mesh: identity pose, zero velocity / com / inverse mass and inertia
+    const size_t B = std::max<size_t>(M, 1);
+    std::vector<float4> bq(2 * B, make_float4(0.f, 0.f, 0.f, 0.f));
+    for (size_t b = 0; b < B; b++) bq[2 * b + 1] = make_float4(0.f, 0.f, 0.f, 1.f);
+    upload(body_q_, bq.data(), bq.size());
+    dalloc(body_qd_, 2 * B);
+    dalloc(body_com_, 3 * B);
+    dalloc(body_invI_, 9 * B);
+    dalloc(body_invm_, B);
+    dalloc(body_flags_, B);
+    dalloc(body_delta_, 8 * B);
+
+    all_verts_.clear(); all_tris_.clear(); vert_off_.clear(); tri_off_.clear(); node_off_.clear(); prim_off_.clear();
+    bvh_.assign(M, MeshBvhHost{});
+    info_h_.assign(M, MeshInfo{});
+    size_t nb = 0, pb = 0;
+    std::vector<uint32_t> prim_global;
+    for (size_t c = 0; c < M; c++) {
+        const uint32_t vo = uint32_t(all_verts_.size() / 3), to = uint32_t(all_tris_.size() / 3);
+        vert_off_.push_back(vo);
+        tri_off_.push_back(to);
+        all_verts_.insert(all_verts_.end(), meshes_[c].verts.begin(), meshes_[c].verts.end());
+        for (uint32_t k : meshes_[c].tris) all_tris_.push_back(k + vo);
+        const uint32_t nt = uint32_t(meshes_[c].tris.size() / 3);
+        lbvh_build(all_verts_.data(), all_tris_.data() + size_t(to) * 3, nt, bvh_[c]);
+        if (c == 0) {   // narrowphase epsilon from mesh 0's mean edge length (scene.cpp:274-281,325)
+            double s = 0.0;
+            const uint32_t* tt = all_tris_.data() + size_t(to) * 3;
+            for (uint32_t k = 0; k < nt; k++)
+                for (int e = 0; e < 3; e++) {
+                    const uint32_t i0 = tt[3 * k + e], i1 = tt[3 * k + (e + 1) % 3];
+                    const float dx = all_verts_[3 * i1] - all_verts_[3 * i0];
+                    const float dy = all_verts_[3 * i1 + 1] - all_verts_[3 * i0 + 1];
+                    const float dz = all_verts_[3 * i1 + 2] - all_verts_[3 * i0 + 2];
+                    s += std::sqrt(dx * dx + dy * dy + dz * dz);
+                }
+            avg_edge_ = float(s / double(size_t(nt) * 3));
+        }
+        node_off_.push_back(nb);
+        prim_off_.push_back(pb);
+        info_h_[c].root = uint32_t(nb + bvh_[c].root);
+        info_h_[c].margin = meshes_[c].margin;
+        info_h_[c].watertight = meshes_[c].watertight ? 1u : 0u;
+        info_h_[c].prim_off = uint32_t(pb);
+        info_h_[c].prim_cnt = uint32_t(bvh_[c].prim.size());
+        for (uint32_t p : bvh_[c].prim) prim_global.push_back(p + to);
+        nb += bvh_[c].lo.size();
+        pb += bvh_[c].prim.size();
+    }
+    if (M > 0) {
+        std::printf("[VKXPBD DIAG] finalize_meshes: n_cubes=%zu, total_nodes=%zu\n", M, nb);
+        for (size_t c = 0; c < M; c++)
+            std::printf("  Mesh %zu: root=%u, nodes=%zu, prims=%zu\n", c, info_h_[c].root, bvh_[c].lo.size(),
+                        bvh_[c].prim.size());
+        std::fflush(stdout);
+    }
+    all_vels_.assign(all_verts_.size(), 0.f);
+    upload(verts_d_, all_verts_.data(), all_verts_.size());
+    upload(vvel_d_, all_vels_.data(), all_vels_.size());
+    upload(tris_d_, all_tris_.data(), all_tris_.size());
+    upload(prim_d_, prim_global.data(), prim_global.size());
+    upload(info_d_, info_h_.data(), info_h_.size());
+    upload_nodes();
+    finalized_ = true;
+}
+
+void Solver::set_particles(const float* q, const float* qd, const float* w, const float* r, const int32_t* f, size_t n) {
+    require_init();
+    if (n != n_) throw std::runtime_error("Particle count mismatch");
+    DeviceGuard g(device_);
+    HIPCHECK(hipStreamSynchronize(stream_));
+    if (w) std::memcpy(invm_h_.data(), w, n * sizeof(float));
+    if (r) std::memcpy(radius_h_.data(), r, n * sizeof(float));
+    std::vector<float4> P(n), V(n);
+    for (size_t i = 0; i < n; i++) {
+        P[i] = make_float4(q[3 * i], q[3 * i + 1], q[3 * i + 2], radius_h_[i]);
+        V[i] = qd ? make_float4(qd[3 * i], qd[3 * i + 1], qd[3 * i + 2], invm_h_[i]) : make_float4(0.f, 0.f, 0.f, invm_h_[i]);
+    }
+    if (n) {
+        HIPCHECK(hipMemcpy(P_[s_].p, P.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(V_[s_].p, V.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+        if (f) HIPCHECK(hipMemcpy(flags_.p, f, n * sizeof(int32_t), hipMemcpyHostToDevice));
+    }
+}
+
+void Solver::update_mesh(size_t mi, const float* v, size_t nv, const float* vel) {
+    if (!finalized_) throw std::runtime_error("update_mesh: call finalize_meshes() first");
+    if (mi >= meshes_.size()) throw std::runtime_error("update_mesh: mesh_index out of range");
+    if (nv != meshes_[mi].verts.size() / 3) throw std::runtime_error("update_mesh: vertex count mismatch");
+    DeviceGuard g(device_);
+    HIPCHECK(hipStreamSynchronize(stream_));
+    const size_t vs = vert_off_[mi];
+    std::memcpy(all_verts_.data() + 3 * vs, v, nv * 3 * sizeof(float));
+    if (vel) std::memcpy(all_vels_.data() + 3 * vs, vel, nv * 3 * sizeof(float));
+    else std::fill(all_vels_.begin() + 3 * vs, all_vels_.begin() + 3 * (vs + nv), 0.f);
+    std::memcpy(meshes_[mi].verts.data(), v, nv * 3 * sizeof(float));
+    lbvh_refit(bvh_[mi], all_verts_.data(), all_tris_.data() + 3 * tri_off_[mi]);
+    HIPCHECK(hipMemcpy(verts_d_.p + 3 * vs, v, nv * 3 * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(vvel_d_.p + 3 * vs, all_vels_.data() + 3 * vs, nv * 3 * sizeof(float), hipMemcpyHostToDevice));
+    upload_nodes();
+}
+
+BodyArgs Solver::body_args() const {
+    BodyArgs b;
+    b.q = body_q_.p; b.qd = body_qd_.p; b.com = body_com_.p; b.invI = body_invI_.p; b.invm = body_invm_.p;
+    b.flags = body_flags_.p; b.shape_body = shape_body_.p; b.mat_mu = mat_mu_.p; b.delta = body_delta_.p;
+    return b;
+}
+
+// ------------------------------------------------------------- profiler ----
+void Solver::stage_begin(Stage) {
+    if (!profiling_) return;
+    if (ev_used_ + 2 > ev_pool_.size()) {
+        for (int k = 0; k < 64; k++) {
+            hipEvent_t e;
+            HIPCHECK(hipEventCreate(&e));
+            ev_pool_.push_back(e);
+        }
+    }
+    cur_a_ = ev_pool_[ev_used_++];
+    HIPCHECK(hipEventRecord(cur_a_, stream_));
+}
+void Solver::stage_end(Stage s) {
+    if (!profiling_) return;
+    hipEvent_t b = ev_pool_[ev_used_++];
+    HIPCHECK(hipEventRecord(b, stream_));
+    pending_.push_back(Pending{int(s), cur_a_, b});
+}
+void Solver::sync_profile() {
+    if (!profiling_) return;
+    HIPCHECK(hipStreamSynchronize(stream_));
+    for (const Pending& p : pending_) {
+        float ms = 0.f;
+        HIPCHECK(hipEventElapsedTime(&ms, p.a, p.b));
+        stage_ms_[p.stage] += ms;
+        stage_n_[p.stage] += 1;
+    }
+    pending_.clear();
+    ev_used_ = 0;
+}
+
+// ----------------------------------------------------------------- step ----
+void Solver::run_sort(int tiles) {
+    const int n = int(n_);
+    // pass 0: tile histograms came from k_integrate; ids are implicit
+    stage_begin(kSortScan);
+    launch_sort_binscan(hist_.p, bintot_.p, tiles, stream_);
+    stage_end(kSortScan);
+    stage_begin(kSortScatter);
+    launch_sort_scatter(n, keysB_.p, nullptr, keysA_.p, valsA_.p, 0, hist_.p, bintot_.p, tiles, stream_);
+    stage_end(kSortScatter);
+    for (int pass = 1; pass < kSortPasses; pass++) {
+        const int shift = pass * kSortBits;
+        uint32_t* kin = (pass == 1) ? keysA_.p : keysB_.p;
+        uint32_t* vin = (pass == 1) ? valsA_.p : valsB_.p;
+        uint32_t* kout = (pass == 1) ? keysB_.p : keysA_.p;
+        uint32_t* vout = (pass == 1) ? valsB_.p : valsA_.p;
+        stage_begin(kSortUp);
+        launch_sort_upsweep(n, kin, shift, hist_.p, tiles, stream_);
+        stage_end(kSortUp);
+        stage_begin(kSortScan);
+        launch_sort_binscan(hist_.p, bintot_.p, tiles, stream_);
+        stage_end(kSortScan);
+        stage_begin(kSortScatter);
+        launch_sort_scatter(n, kin, vin, kout, vout, shift, hist_.p, bintot_.p, tiles, stream_);
+        stage_end(kSortScatter);
+    }
+}
+
+void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zero_next) {
+    const int n = int(n_);
+    const int tiles = int((n_ + kSortTile - 1) / kSortTile);
+    int* cc_gen = collide ? cc_.p + slot : nullptr;
+    int* cc_solve = collide ? cc_.p + slot : cc_.p + contact_slot_;
+    const float inv_cell = 1.0f / cfg_.search_radius;
+    if (collide && !meshes_.empty()) {
+        MeshArgs m;
+        m.n = n; m.P = P_[s_].p; m.flags = flags_.p;
+        m.verts = verts_d_.p; m.tris = tris_d_.p; m.nl = nl_d_.p; m.nu = nu_d_.p; m.prim = prim_d_.p;
+        m.vvel = vvel_d_.p; m.info = info_d_.p; m.nmesh = int(meshes_.size());
+        m.margin = cfg_.soft_contact_margin; m.eps = avg_edge_ * 1e-3f;
+        m.cmax = int(cmax_); m.cc = cc_gen;
+        m.cpart = cpart_.p; m.cshape = cshape_.p; m.cbpos = cbpos_.p; m.cbvel = cbvel_.p; m.cnorm = cnorm_.p;
+        m.err = reinterpret_cast<uint32_t*>(cc_.p + 2);
+        stage_begin(kMesh);
+        launch_mesh_contacts(m, stream_);
+        stage_end(kMesh);
+    }
+    if (n > 0) {
+        IntegrateArgs a;
+        a.n = n; a.tiles = tiles;
+        a.Ps = P_[s_].p; a.Vs = V_[s_].p; a.flags = flags_.p; a.Pp = P_[p_].p; a.Vp = V_[p_].p;
+        a.keys = keysB_.p; a.hist = hist_.p;
+        a.dt = sdt; a.vmax = cfg_.particle_v_max;
+        a.gx = cfg_.gravity[0]; a.gy = cfg_.gravity[1]; a.gz = cfg_.gravity[2];
+        a.inv_cell = inv_cell;
+        a.has_plane = (collide && cfg_.has_ground_plane) ? 1 : 0;
+        a.margin = cfg_.soft_contact_margin; a.altitude = cfg_.ground_plane_altitude;
+        a.cmax = int(cmax_); a.cc = cc_gen; a.cc_next = zero_next ? cc_.p + (slot ^ 1) : nullptr;
+        a.cpart = cpart_.p; a.cshape = cshape_.p; a.cbpos = cbpos_.p; a.cbvel = cbvel_.p; a.cnorm = cnorm_.p;
+        stage_begin(kIntegrate);
+        launch_integrate(a, stream_);
+        stage_end(kIntegrate);
+        run_sort(tiles);
+        stage_begin(kRanges);
+        launch_cell_ranges(n, keysA_.p, range_.p, stream_);
+        stage_end(kRanges);
+        grid_valid_ = true;
+    }
+    int cur = p_, nxt = o_;
+    const int shape_grid = int(std::min<size_t>((cmax_ + 255) / 256, 2048));
+    for (int it = 0; it < iterations; it++) {
+        ShapeArgs sa;
+        sa.cmax = int(cmax_); sa.cc = cc_solve;
+        sa.cpart = cpart_.p; sa.cshape = cshape_.p; sa.cbpos = cbpos_.p; sa.cbvel = cbvel_.p; sa.cnorm = cnorm_.p;
+        sa.P = P_[cur].p; sa.V = V_[cur].p; sa.flags = flags_.p;
+        sa.body = body_args();
+        sa.pmu = cfg_.soft_contact_mu; sa.ka = cfg_.particle_adhesion; sa.dt = sdt; sa.relax = cfg_.soft_contact_relaxation;
+        sa.dshape = dshape_.p;
+        stage_begin(kShape);
+        launch_shape_solve(sa, shape_grid, stream_);
+        stage_end(kShape);
+        PPArgs pa;
+        pa.n = n; pa.keys = keysA_.p; pa.vals = valsA_.p; pa.range = range_.p;
+        pa.Ps = P_[s_].p; pa.Pp = P_[cur].p; pa.Vp = V_[cur].p; pa.flags = flags_.p;
+        pa.dshape = dshape_.p; pa.delta = delta_.p; pa.Po = P_[nxt].p; pa.Vo = V_[nxt].p;
+        pa.mu = cfg_.particle_mu; pa.coh = cfg_.particle_cohesion; pa.max_r = cfg_.particle_max_radius;
+        pa.dt = sdt; pa.relax = cfg_.soft_contact_relaxation; pa.inv_cell = inv_cell; pa.vmax = cfg_.particle_v_max;
+        pa.iter_index = uint32_t(it); pa.max_diag_pairs = max_pairs_;
+        pa.diag_header = diag_header_.p; pa.diag_pairs = diag_pairs_.p;
+        stage_begin(kPP);
+        launch_pp_apply(pa, cfg_.enable_diagnostics != 0, stream_);
+        stage_end(kPP);
+        std::swap(cur, nxt);
+    }
+    if (cfg_.enable_restitution) {
+        RestArgs ra;
+        ra.cmax = int(cmax_); ra.cc = cc_solve;
+        ra.cpart = cpart_.p; ra.cshape = cshape_.p; ra.cbpos = cbpos_.p; ra.cbvel = cbvel_.p; ra.cnorm = cnorm_.p;
+        ra.P0 = P_[s_].p; ra.V0 = V_[s_].p; ra.flags = flags_.p; ra.V = V_[cur].p;
+        ra.body = body_args();
+        ra.ka = cfg_.particle_adhesion; ra.restitution = cfg_.soft_contact_restitution;
+        stage_begin(kRest);
+        launch_restitution(ra, shape_grid, stream_);
+        stage_end(kRest);
+    }
+    const int old_s = s_;
+    s_ = cur;
+    p_ = old_s;
+    o_ = nxt;
+    HIPCHECK(hipGetLastError());
+}
+
+void Solver::read_count(int slot) {
+    stage_begin(kCopyOut);
+    HIPCHECK(hipMemcpyAsync(count_host_, cc_.p, 4 * sizeof(int32_t), hipMemcpyDeviceToHost, stream_));
+    stage_end(kCopyOut);
+    HIPCHECK(hipStreamSynchronize(stream_));
+    const int32_t c = count_host_[slot];
+    last_count_ = c > int32_t(cmax_) ? int32_t(cmax_) : c;
+    if (count_host_[2] != 0) {
+        const int32_t e = count_host_[2];
+        HIPCHECK(hipMemsetAsync(cc_.p + 2, 0, sizeof(int32_t), stream_));
+        HIPCHECK(hipStreamSynchronize(stream_));
+        throw std::runtime_error("mesh BVH traversal exceeded its 32-entry stack (flags " + std::to_string(e) + ")");
+    }
+}
+
+void Solver::step(float dt, int substeps, int iterations) {
+    require_init();
+    if (!finalized_) finalize_meshes();
+    DeviceGuard g(device_);
+    const auto t0 = std::chrono::high_resolution_clock::now();
+    const float sdt = dt / float(substeps);
+    launch_zero_int(cc_.p, 1, stream_);
+    for (int sub = 0; sub < substeps; sub++) substep(sdt, iterations, true, sub & 1, true);
+    if (substeps > 0) contact_slot_ = (substeps - 1) & 1;
+    read_count(substeps > 0 ? contact_slot_ : 0);
+    if (profiling_) {
+        sync_profile();
+        prof_substeps_ += uint64_t(std::max(substeps, 0));
+        prof_wall_ms_ += std::chrono::duration<double, std::milli>(std::chrono::high_resolution_clock::now() - t0).count();
+    }
+    const char* chk = std::getenv("VKXPBD_CHECK_SORT_ORDER");
+    if (chk && (std::strcmp(chk, "1") == 0 || std::strcmp(chk, "true") == 0) && n_ > 0) {
+        std::vector<uint32_t> k(n_);
+        get_sorted(false, k.data(), n_);
+        for (size_t i = 1; i < n_; i++)
+            if (k[i] < k[i - 1]) {
+                std::fprintf(stderr, "[VKXPBD SORT CHECK ERROR] keys[%zu] = %u < keys[%zu] = %u!\n", i, k[i], i - 1, k[i - 1]);
+                return;
+            }
+        std::printf("[VKXPBD DEBUG] Sort check passed: %zu keys strictly non-decreasing.\n", n_);
+    }
+}
+
+void Solver::step_solve_only(float dt, int substeps, int iterations) {
+    require_init();
+    if (!finalized_) finalize_meshes();
+    DeviceGuard g(device_);
+    const float sdt = dt / float(substeps);
+    for (int sub = 0; sub < substeps; sub++) substep(sdt, iterations, false, contact_slot_, false);
+    HIPCHECK(hipStreamSynchronize(stream_));
+    sync_profile();
+}
+
+void Solver::collide_only() {
+    require_init();
+    if (!finalized_) finalize_meshes();
+    DeviceGuard g(device_);
+    launch_zero_int(cc_.p, 1, stream_);
+    contact_slot_ = 0;
+    PlaneArgs pl;
+    pl.n = int(n_); pl.P = P_[s_].p; pl.flags = flags_.p;
+    pl.margin = cfg_.soft_contact_margin; pl.altitude = cfg_.ground_plane_altitude;
+    pl.cmax = int(cmax_); pl.cc = cc_.p;
+    pl.cpart = cpart_.p; pl.cshape = cshape_.p; pl.cbpos = cbpos_.p; pl.cbvel = cbvel_.p; pl.cnorm = cnorm_.p;
+    if (cfg_.has_ground_plane) launch_plane_contacts(pl, stream_);
+    if (!meshes_.empty()) {
+        MeshArgs m;
+        m.n = int(n_); m.P = P_[s_].p; m.flags = flags_.p;
+        m.verts = verts_d_.p; m.tris = tris_d_.p; m.nl = nl_d_.p; m.nu = nu_d_.p; m.prim = prim_d_.p;
+        m.vvel = vvel_d_.p; m.info = info_d_.p; m.nmesh = int(meshes_.size());
+        m.margin = cfg_.soft_contact_margin; m.eps = avg_edge_ * 1e-3f;
+        m.cmax = int(cmax_); m.cc = cc_.p;
+        m.cpart = cpart_.p; m.cshape = cshape_.p; m.cbpos = cbpos_.p; m.cbvel = cbvel_.p; m.cnorm = cnorm_.p;
+        m.err = reinterpret_cast<uint32_t*>(cc_.p + 2);
+        launch_mesh_contacts(m, stream_);
+    }
+    HIPCHECK(hipGetLastError());
+    read_count(0);
+    sync_profile();
+}
+
+// -------------------------------------------------------------- getters ----
+void Solver::get_positions(float* out, size_t n) {
+    require_init();
+    DeviceGuard g(device_);
+    n = std::min(n, n_);
+    std::vector<float4> t(n);
+    if (n) HIPCHECK(hipMemcpyAsync(t.data(), P_[s_].p, n * sizeof(float4), hipMemcpyDeviceToHost, stream_));
+    HIPCHECK(hipStreamSynchronize(stream_));
+    for (size_t i = 0; i < n; i++) { out[3 * i] = t[i].x; out[3 * i + 1] = t[i].y; out[3 * i + 2] = t[i].z; }
+}
+void Solver::get_velocities(float* out, size_t n) {
+    require_init();
+    DeviceGuard g(device_);
+    n = std::min(n, n_);
+    std::vector<float4> t(n);
+    if (n) HIPCHECK(hipMemcpyAsync(t.data(), V_[s_].p, n * sizeof(float4), hipMemcpyDeviceToHost, stream_));
+    HIPCHECK(hipStreamSynchronize(stream_));
+    for (size_t i = 0; i < n; i++) { out[3 * i] = t[i].x; out[3 * i + 1] = t[i].y; out[3 * i + 2] = t[i].z; }
+}
+void Solver::get_contact_ints(bool shapes, int32_t* out, size_t cap) {
+    require_init();
+    DeviceGuard g(device_);
+    const size_t c = std::min<size_t>(size_t(std::max(last_count_, 0)), cap);
+    if (c) HIPCHECK(hipMemcpyAsync(out, shapes ? cshape_.p : cpart_.p, c * sizeof(int32_t), hipMemcpyDeviceToHost, stream_));
+    HIPCHECK(hipStreamSynchronize(stream_));
+}
+void Solver::get_contact_vec(int which, float* out, size_t cap) {
+    require_init();
+    DeviceGuard g(device_);
+    const size_t c = std::min<size_t>(size_t(std::max(last_count_, 0)), cap);
+    std::vector<float4> t(c);
+    const float4* src = which == 0 ? cnorm_.p : (which == 1 ? cbpos_.p : cbvel_.p);
+    if (c) HIPCHECK(hipMemcpyAsync(t.data(), src, c * sizeof(float4), hipMemcpyDeviceToHost, stream_));
+    HIPCHECK(hipStreamSynchronize(stream_));
+    for (size_t i = 0; i < c; i++) { out[3 * i] = t[i].x; out[3 * i + 1] = t[i].y; out[3 * i + 2] = t[i].z; }
+}
+void Solver::set_contact_data(const int32_t* p, const int32_t* s, const float* bp, const float* nrm, int count) {
+    require_init();
+    DeviceGuard g(device_);
+    HIPCHECK(hipStreamSynchronize(stream_));
+    last_count_ = count;
+    const int32_t dev_count = std::max(count, 0);
+    const size_t c = std::min<size_t>(size_t(dev_count), cmax_);
+    if (c) {
+        std::vector<float4> b(c), nn(c);
+        for (size_t i = 0; i < c; i++) {
+            b[i] = make_float4(bp[3 * i], bp[3 * i + 1], bp[3 * i + 2], 0.f);
+            nn[i] = make_float4(nrm[3 * i], nrm[3 * i + 1], nrm[3 * i + 2], 0.f);
+        }
+        HIPCHECK(hipMemcpy(cpart_.p, p, c * sizeof(int32_t), hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(cshape_.p, s, c * sizeof(int32_t), hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(cbpos_.p, b.data(), c * sizeof(float4), hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(cnorm_.p, nn.data(), c * sizeof(float4), hipMemcpyHostToDevice));
+    }
+    HIPCHECK(hipMemcpy(cc_.p + contact_slot_, &dev_count, sizeof(int32_t), hipMemcpyHostToDevice));
+}
+void Solver::get_sorted(bool ids, uint32_t* out, size_t n) {
+    require_init();
+    DeviceGuard g(device_);
+    n = std::min(n, n_);
+    if (n) HIPCHECK(hipMemcpyAsync(out, ids ? valsA_.p : keysA_.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost, stream_));
+    HIPCHECK(hipStreamSynchronize(stream_));
+}
+void Solver::get_cell_table(bool ends, int32_t* out, size_t n) {
+    require_init();
+    DeviceGuard g(device_);
+    n = std::min<size_t>(n, kNumCells);
+    if (!grid_valid_ || n_ == 0) { std::memset(out, 0, n * sizeof(int32_t)); return; }
+    if (!tbl_.p) dalloc(tbl_, size_t(2) * kNumCells);
+    HIPCHECK(hipMemsetAsync(tbl_.p, 0, size_t(2) * kNumCells * sizeof(int32_t), stream_));
+    launch_cell_tables(int(n_), keysA_.p, tbl_.p, tbl_.p + kNumCells, stream_);
+    HIPCHECK(hipGetLastError());
+    if (n) HIPCHECK(hipMemcpyAsync(out, tbl_.p + (ends ? kNumCells : 0), n * sizeof(int32_t), hipMemcpyDeviceToHost, stream_));
+    HIPCHECK(hipStreamSynchronize(stream_));
+}
+void Solver::get_deltas(float* out, size_t n) {
+    require_init();
+    DeviceGuard g(device_);
+    n = std::min(n, n_);
+    if (n) HIPCHECK(hipMemcpyAsync(out, delta_.p, 3 * n * sizeof(float), hipMemcpyDeviceToHost, stream_));
+    HIPCHECK(hipStreamSynchronize(stream_));
+}
+uint32_t Solver::diag_word(int k) {
+    if (!initialized_) return 0;
+    DeviceGuard g(device_);
+    uint32_t v = 0;
+    HIPCHECK(hipMemcpyAsync(&v, diag_header_.p + k, sizeof(uint32_t), hipMemcpyDeviceToHost, stream_));
+    HIPCHECK(hipStreamSynchronize(stream_));
+    return v;
+}
+size_t Solver::get_diag_pairs(xpbd_pair_record* out, size_t cap) {
+    const uint32_t count = diag_word(0);
+    size_t rd = std::min<size_t>(std::min<size_t>(count, max_pairs_), cap);
+    if (!cfg_.enable_diagnostics) rd = std::min<size_t>(rd, 1);
+    if (rd == 0) return 0;
+    DeviceGuard g(device_);
+    HIPCHECK(hipMemcpyAsync(out, diag_pairs_.p, rd * sizeof(xpbd_pair_record), hipMemcpyDeviceToHost, stream_));
+    HIPCHECK(hipStreamSynchronize(stream_));
+    return rd;
+}
+void Solver::reset_diag() {
+    if (!initialized_) return;
+    DeviceGuard g(device_);
+    HIPCHECK(hipMemsetAsync(diag_header_.p, 0, 2 * sizeof(uint32_t), stream_));
+    HIPCHECK(hipStreamSynchronize(stream_));
+}
+size_t Solver::shape_contact_diag_len() const {
+    if (!initialized_) return 0;
+    return cfg_.enable_diagnostics ? size_t(4) * cmax_ * 32 : 32;
+}
+void Solver::bvh_half(size_t mi, bool upper, float* out, size_t cap) const {
+    if (mi >= bvh_.size()) return;
+    const std::vector<NodeHalf>& v = upper ? bvh_[mi].hi : bvh_[mi].lo;
+    std::memcpy(out, v.data(), std::min(cap, v.size()) * sizeof(NodeHalf));
+}
+
+void Solver::profile_reset() {
+    for (int k = 0; k < kNumStages; k++) { stage_ms_[k] = 0.0; stage_n_[k] = 0; }
+    prof_substeps_ = 0;
+    prof_wall_ms_ = 0.0;
+}
+
+size_t Solver::profile_stages(char* names, size_t names_len, double* ms, uint64_t* launches, size_t cap) {
+    size_t off = 0;
+    for (int k = 0; k < kNumStages; k++) {
+        if (size_t(k) < cap) {
+            if (ms) ms[k] = stage_ms_[k];
+            if (launches) launches[k] = stage_n_[k];
+        }
+        if (names) {
+            const size_t L = std::strlen(kStageNames[k]) + 1;
+            if (off + L <= names_len) { std::memcpy(names + off, kStageNames[k], L); off += L; }
+        }
+    }
+    return kNumStages;
+}
+
+void Solver::print_profiler_summary() {
+    if (!profiling_ || prof_substeps_ == 0) return;
+    double total = 0.0;
+    for (int k = 0; k < kNumStages; k++) total += stage_ms_[k];
+    const double ns = double(prof_substeps_);
+    const double idle = prof_wall_ms_ > total ? prof_wall_ms_ - total : 0.0;
+    std::printf("\n====================================================================================\n");
+    std::printf("                      VKXPBD (HIP) GPU EVENT PROFILER SUMMARY\n");
+    std::printf("====================================================================================\n");
+    std::printf("Total Substeps Recorded: %llu\n", (unsigned long long)prof_substeps_);
+    std::printf("Wall-Clock Step Time:    %10.2f ms (%8.4f ms/substep)\n", prof_wall_ms_, prof_wall_ms_ / ns);
+    std::printf("Summed GPU Active Time:  %10.2f ms (%8.4f ms/substep)\n", total, total / ns);
+    std::printf("Host / Sync Idle Gap:    %10.2f ms (%8.4f ms/substep, %5.1f%% of wall)\n", idle, idle / ns,
+                100.0 * idle / (prof_wall_ms_ > 0 ? prof_wall_ms_ : 1.0));
+    std::printf("------------------------------------------------------------------------------------\n");
+    std::printf("%-28s | %10s | %14s | %9s | %8s\n", "Kernel / Stage", "Total (ms)", "Per-Substep (ms)", "% GPU", "Launches");
+    for (int k = 0; k < kNumStages; k++)
+        std::printf("%-28s | %10.3f | %14.4f | %8.2f%% | %8llu\n", kStageNames[k], stage_ms_[k], stage_ms_[k] / ns,
+                    100.0 * stage_ms_[k] / (total > 0 ? total : 1.0), (unsigned long long)stage_n_[k]);
+    std::printf("====================================================================================\n\n");
+    std::fflush(stdout);
+}
+
+}  // namespace xpbd

@@ -1,0 +1,138 @@
+// solver.h -- host orchestrator of the MI355X XPBD step: owns device memory on
+// one HIP device and one stream, and records the substep pipeline
+// (src/scene.h:74-227 is the interface it reproduces).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/xpbd_capi.h"
+#include "lbvh.h"
+#include "xpbd_kernels.h"
+
+namespace xpbd {
+
+// Device allocation owned by the solver (freed in the destructor).
+template <class T> struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+};
+
+class Solver {
+public:
+    Solver();
+    ~Solver();
+
+    void init(const xpbd_config& cfg, size_t n, size_t max_contacts);
+    void add_mesh(const float* v, size_t nv, const uint32_t* t, size_t nt, float margin, const float* rot, int wt);
+    void finalize_meshes();
+    void set_particles(const float* q, const float* qd, const float* w, const float* r, const int32_t* f, size_t n);
+    void update_mesh(size_t mi, const float* v, size_t nv, const float* vel);
+    void step(float dt, int substeps, int iterations);
+    void collide_only();
+    void step_solve_only(float dt, int substeps, int iterations);
+
+    void get_positions(float* out, size_t n);
+    void get_velocities(float* out, size_t n);
+    size_t particle_count() const { return n_; }
+    size_t mesh_count() const { return meshes_.size(); }
+    int last_contact_count() const { return last_count_; }
+
+    void get_contact_ints(bool shapes, int32_t* out, size_t cap);
+    void get_contact_vec(int which, float* out, size_t cap);   // 0 normals, 1 body pos, 2 body vel
+    void set_contact_data(const int32_t* p, const int32_t* s, const float* bp, const float* nrm, int count);
+    void get_sorted(bool ids, uint32_t* out, size_t n);
+    void get_cell_table(bool ends, int32_t* out, size_t n);
+    void get_deltas(float* out, size_t n);
+
+    uint32_t diag_word(int k);
+    size_t get_diag_pairs(xpbd_pair_record* out, size_t cap);
+    void reset_diag();
+    size_t shape_contact_diag_len() const;
+
+    size_t bvh_nodes(size_t mi) const { return mi < bvh_.size() ? bvh_[mi].lo.size() : 0; }
+    void bvh_half(size_t mi, bool upper, float* out, size_t cap) const;
+
+    void print_profiler_summary();
+    size_t profile_stages(char* names, size_t names_len, double* ms, uint64_t* launches, size_t cap);
+    void profile_reset();
+
+private:
+    struct MeshHost {
+        std::vector<float> verts;
+        std::vector<uint32_t> tris;
+        float margin = 0.f;
+        float rot[4] = {0.f, 0.f, 0.f, 1.f};
+        bool watertight = false;
+    };
+    enum Stage { kMesh, kIntegrate, kSortUp, kSortScan, kSortScatter, kRanges, kShape, kPP, kRest, kCopyOut, kNumStages };
+
+    void require_init() const;
+    void alloc_all();
+    void free_all();
+    void upload_nodes();
+    BodyArgs body_args() const;
+    void substep(float sdt, int iterations, bool collide, int slot, bool zero_next);
+    void run_sort(int tiles);
+    void read_count(int slot);
+    void stage_begin(Stage s);
+    void stage_end(Stage s);
+    void sync_profile();
+
+    xpbd_config cfg_{};
+    bool initialized_ = false, finalized_ = false, grid_valid_ = false;
+    int device_ = 0;
+    hipStream_t stream_ = nullptr;
+    size_t n_ = 0, cmax_ = 0;
+
+    // particle state: 3 rotating float4 buffers each (state / predicted / out)
+    DevBuf<float4> P_[3], V_[3];
+    int s_ = 0, p_ = 1, o_ = 2;
+    DevBuf<int32_t> flags_;
+    std::vector<float> radius_h_, invm_h_;
+    // grid and sort
+    DevBuf<uint32_t> keysA_, valsA_, keysB_, valsB_, hist_, bintot_;
+    DevBuf<int32_t> range_;   // int2 per cell (start, end), never cleared
+    DevBuf<int32_t> tbl_;     // 2 x cells scratch for the dense getters
+    // contacts: SoA of float4 payloads, counter slots
+    DevBuf<int32_t> cpart_, cshape_, cc_;
+    DevBuf<float4> cbpos_, cbvel_, cnorm_;
+    int contact_slot_ = 0, last_count_ = 0;
+    int32_t* count_host_ = nullptr;   // pinned
+    DevBuf<float> dshape_, delta_;
+    // bodies (one static body per mesh, scene.cpp:220-238) and shapes
+    DevBuf<float4> body_q_, body_qd_;
+    DevBuf<float> body_com_, body_invI_, body_invm_, body_delta_, mat_mu_;
+    DevBuf<int32_t> body_flags_, shape_body_;
+    // meshes
+    std::vector<MeshHost> meshes_;
+    std::vector<MeshBvhHost> bvh_;
+    std::vector<float> all_verts_, all_vels_;
+    std::vector<uint32_t> all_tris_;
+    std::vector<size_t> vert_off_, tri_off_, node_off_, prim_off_;
+    DevBuf<float> verts_d_, vvel_d_;
+    DevBuf<uint32_t> tris_d_, prim_d_, err_d_;
+    DevBuf<float4> nl_d_, nu_d_;
+    DevBuf<MeshInfo> info_d_;
+    std::vector<MeshInfo> info_h_;
+    float avg_edge_ = 0.f;
+    // diagnostics
+    DevBuf<uint32_t> diag_header_;
+    DevBuf<float> diag_pairs_;
+    uint32_t max_pairs_ = 100000;
+    // profiler
+    bool profiling_ = false;
+    std::vector<hipEvent_t> ev_pool_;
+    struct Pending { int stage; hipEvent_t a, b; };
+    std::vector<Pending> pending_;
+    size_t ev_used_ = 0;
+    double stage_ms_[kNumStages] = {};
+    uint64_t stage_n_[kNumStages] = {};
+    uint64_t prof_substeps_ = 0;
+    double prof_wall_ms_ = 0.0;
+    hipEvent_t cur_a_ = nullptr;
+};
+
+}  // namespace xpbd

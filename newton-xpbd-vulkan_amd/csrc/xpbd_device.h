@@ -1,0 +1,130 @@
+// xpbd_device.h -- device-side arithmetic shared by the gfx950 kernels.
+//
+// All kernels are compiled with -ffp-contract=off and correctly rounded fp32
+// division/sqrt, so a plain a*b+c is two roundings and only the explicit
+// __fmaf_rn calls fuse.  This reproduces the reference shaders' evaluation:
+// GLSL fma()/`precise` sites become __fmaf_rn, everything else is evaluated
+// left to right (solve_particle_particle_contacts.comp:81-105,
+// generate_mesh_contacts.comp:29-75, apply_particle_deltas.comp:30-36).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace xpbd {
+
+constexpr int kGridDim = 128;
+constexpr int kNumCells = kGridDim * kGridDim * kGridDim;
+constexpr int kGridOrigin = 1 << 20;
+constexpr int kWave = 64;
+
+struct f3 { float x, y, z; };
+__device__ __forceinline__ f3 mk3(float x, float y, float z) { return f3{x, y, z}; }
+__device__ __forceinline__ f3 xyz(float4 a) { return f3{a.x, a.y, a.z}; }
+__device__ __forceinline__ f3 operator+(f3 a, f3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ f3 operator-(f3 a, f3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ f3 operator-(f3 a) { return {-a.x, -a.y, -a.z}; }
+__device__ __forceinline__ f3 operator*(f3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ f3 operator*(float s, f3 a) { return {s * a.x, s * a.y, s * a.z}; }
+__device__ __forceinline__ float fma_(float a, float b, float c) { return __fmaf_rn(a, b, c); }
+__device__ __forceinline__ float gmax(float a, float b) { return a < b ? b : a; }
+__device__ __forceinline__ float gmin(float a, float b) { return b < a ? b : a; }
+__device__ __forceinline__ float dot3(f3 a, f3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+__device__ __forceinline__ float len3(f3 a) { return sqrtf(dot3(a, a)); }
+__device__ __forceinline__ f3 cross3(f3 a, f3 b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ f3 normalize3(f3 a) {
+    float inv = 1.0f / sqrtf(dot3(a, a));
+    return a * inv;
+}
+__device__ __forceinline__ float comp3(f3 a, int k) { return k == 0 ? a.x : (k == 1 ? a.y : a.z); }
+
+// Newton-refined sqrt / length / dot / division used by the contact shaders.
+__device__ __forceinline__ float exact_sqrt(float a) {
+    float d = sqrtf(a);
+    float r = fma_(-d, d, a);
+    return fma_(r, 0.5f / d, d);
+}
+__device__ __forceinline__ float exact_length(f3 a) {
+    float yy = a.y * a.y;
+    float xx_yy = fma_(a.x, a.x, yy);
+    float sq = fma_(a.z, a.z, xx_yy);
+    return exact_sqrt(sq);
+}
+__device__ __forceinline__ float exact_dot(f3 a, f3 b) {
+    float y = a.y * b.y;
+    float xy = fma_(a.x, b.x, y);
+    return fma_(a.z, b.z, xy);
+}
+__device__ __forceinline__ f3 exact_div(f3 v, float d) {
+    float inv = 1.0f / d;
+    f3 q = v * inv;
+    f3 rem = {fma_(-d, q.x, v.x), fma_(-d, q.y, v.y), fma_(-d, q.z, v.z)};
+    return {fma_(rem.x, inv, q.x), fma_(rem.y, inv, q.y), fma_(rem.z, inv, q.z)};
+}
+__device__ __forceinline__ float exact_div1(float v, float d) {
+    float inv = 1.0f / d;
+    float q = v * inv;
+    float rem = fma_(-d, q, v);
+    return fma_(rem, inv, q);
+}
+// Markstein-refined reciprocal division of apply_particle_deltas.comp:30-36.
+__device__ __forceinline__ f3 markstein_div(f3 a, float b) {
+    float y = 1.0f / b;
+    float t = fma_(-b, y, 1.0f);
+    y = t * y + y;
+    f3 q = a * y;
+    f3 r = {fma_(-q.x, b, a.x), fma_(-q.y, b, a.y), fma_(-q.z, b, a.z)};
+    return {fma_(r.x, y, q.x), fma_(r.y, y, q.y), fma_(r.z, y, q.z)};
+}
+
+// Quaternion (x,y,z,w) rotation as written in solve_particle_shape_contacts.comp:50-68.
+__device__ __forceinline__ f3 quat_rotate_sc(float4 q, f3 x) {
+    float c = 2.0f * q.w * q.w - 1.0f;
+    float d = 2.0f * (q.x * x.x + q.y * x.y + q.z * x.z);
+    return {x.x * c + q.x * d + (q.y * x.z - q.z * x.y) * q.w * 2.0f,
+            x.y * c + q.y * d + (q.z * x.x - q.x * x.z) * q.w * 2.0f,
+            x.z * c + q.z * d + (q.x * x.y - q.y * x.x) * q.w * 2.0f};
+}
+__device__ __forceinline__ f3 quat_rotate_inv_sc(float4 q, f3 x) {
+    float c = 2.0f * q.w * q.w - 1.0f;
+    float d = 2.0f * (q.x * x.x + q.y * x.y + q.z * x.z);
+    return {x.x * c + q.x * d - (q.y * x.z - q.z * x.y) * q.w * 2.0f,
+            x.y * c + q.y * d - (q.z * x.x - q.x * x.z) * q.w * 2.0f,
+            x.z * c + q.z * d - (q.x * x.y - q.y * x.x) * q.w * 2.0f};
+}
+
+// Hash-grid cell of a position (grid_cell_indices.comp:17-25): floor first,
+// then the 2^20 origin offset, clamp at 0, wrap to 128 per axis, z-major.
+__device__ __forceinline__ uint32_t cell_of(float px, float py, float pz, float inv) {
+    int gx = (int)floorf(px * inv) + kGridOrigin;
+    int gy = (int)floorf(py * inv) + kGridOrigin;
+    int gz = (int)floorf(pz * inv) + kGridOrigin;
+    gx = gx < 0 ? 0 : gx;
+    gy = gy < 0 ? 0 : gy;
+    gz = gz < 0 ? 0 : gz;
+    return (uint32_t)((gz % kGridDim) * (kGridDim * kGridDim) + (gy % kGridDim) * kGridDim + (gx % kGridDim));
+}
+// Virtual cell coordinates -> physical cell (solve_particle_particle_contacts.comp:66-79).
+__device__ __forceinline__ int cell_index(int x, int y, int z) {
+    x += kGridOrigin; y += kGridOrigin; z += kGridOrigin;
+    x = x < 0 ? 0 : x; y = y < 0 ? 0 : y; z = z < 0 ? 0 : z;
+    return (z % kGridDim) * (kGridDim * kGridDim) + (y % kGridDim) * kGridDim + (x % kGridDim);
+}
+
+// Wave-aggregated slot reservation on a global counter: one atomic per wave
+// for all active lanes with want == true.  Returns the lane's slot (or -1).
+__device__ __forceinline__ int wave_reserve(int* counter, bool want) {
+    uint64_t mask = __ballot(want);
+    if (!want) return -1;
+    int lane = __lane_id();
+    int leader = __ffsll((unsigned long long)mask) - 1;
+    int total = __popcll(mask);
+    int base = 0;
+    if (lane == leader) base = atomicAdd(counter, total);
+    base = __shfl(base, leader);
+    uint64_t below = mask & ((1ull << lane) - 1ull);
+    return base + __popcll(below);
+}
+
+}  // namespace xpbd

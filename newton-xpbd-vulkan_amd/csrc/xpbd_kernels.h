@@ -1,0 +1,110 @@
+// xpbd_kernels.h -- launch interface between the host solver and the gfx950
+// kernels (argument blocks passed by value, one per kernel).
+#pragma once
+#include "xpbd_device.h"
+
+namespace xpbd {
+
+// Radix sort geometry: 21-bit cell keys sorted in three 7-bit LSD passes over
+// tiles of 256 threads x 4 keys.
+constexpr int kSortBits = 7;
+constexpr int kSortBins = 1 << kSortBits;
+constexpr int kSortPasses = 3;
+constexpr int kSortItems = 4;
+constexpr int kSortTile = 256 * kSortItems;
+
+struct BodyArgs {
+    const float4* q;      // 2 per body: (pos, 0), (quat xyzw)
+    const float4* qd;     // 2 per body: (lin, 0), (ang, 0)
+    const float* com;     // 3 per body
+    const float* invI;    // 9 per body (row-major as the particle-shape solve reads it)
+    const float* invm;
+    const int* flags;
+    const int* shape_body;
+    const float* mat_mu;
+    float* delta;         // 8 per body: lin at +0..2, ang at +4..6
+};
+
+struct IntegrateArgs {
+    int n, tiles;
+    const float4* Ps; const float4* Vs; const int* flags;
+    float4* Pp; float4* Vp;
+    uint32_t* keys; uint32_t* hist;
+    float dt, vmax, gx, gy, gz, inv_cell;
+    int has_plane; float margin, altitude;
+    int cmax; int* cc; int* cc_next;
+    int* cpart; int* cshape; float4* cbpos; float4* cbvel; float4* cnorm;
+};
+
+struct PlaneArgs {
+    int n; const float4* P; const int* flags;
+    float margin, altitude;
+    int cmax; int* cc;
+    int* cpart; int* cshape; float4* cbpos; float4* cbvel; float4* cnorm;
+};
+
+struct ShapeArgs {
+    int cmax; const int* cc;
+    const int* cpart; const int* cshape; const float4* cbpos; const float4* cbvel; const float4* cnorm;
+    const float4* P; const float4* V; const int* flags;
+    BodyArgs body;
+    float pmu, ka, dt, relax;
+    float* dshape;
+};
+
+struct PPArgs {
+    int n;
+    const uint32_t* keys; const uint32_t* vals; const int* range;
+    const float4* Ps; const float4* Pp; const float4* Vp; const int* flags;
+    float* dshape; float* delta;
+    float4* Po; float4* Vo;
+    float mu, coh, max_r, dt, relax, inv_cell, vmax;
+    uint32_t iter_index, max_diag_pairs;
+    uint32_t* diag_header; float* diag_pairs;
+};
+
+struct RestArgs {
+    int cmax; const int* cc;
+    const int* cpart; const int* cshape; const float4* cbpos; const float4* cbvel; const float4* cnorm;
+    const float4* P0; const float4* V0; const int* flags;
+    float4* V;
+    BodyArgs body;
+    float ka, restitution;
+};
+
+struct MeshInfo {
+    uint32_t root;
+    float margin;
+    uint32_t watertight;
+    uint32_t prim_off;
+    uint32_t prim_cnt;
+    uint32_t pad[3];
+};
+
+struct MeshArgs {
+    int n; const float4* P; const int* flags;
+    const float* verts; const uint32_t* tris; const float4* nl; const float4* nu; const uint32_t* prim;
+    const float* vvel; const MeshInfo* info; int nmesh;
+    float margin, eps;
+    int cmax; int* cc;
+    int* cpart; int* cshape; float4* cbpos; float4* cbvel; float4* cnorm;
+    uint32_t* err;
+};
+
+void launch_integrate(const IntegrateArgs& a, hipStream_t s);
+void launch_plane_contacts(const PlaneArgs& a, hipStream_t s);
+void launch_cell_ranges(int n, const uint32_t* keys, int* range, hipStream_t s);
+void launch_cell_tables(int n, const uint32_t* keys, int* starts, int* ends, hipStream_t s);
+void launch_shape_solve(const ShapeArgs& a, int grid, hipStream_t s);
+void launch_pp_apply(const PPArgs& a, bool diag, hipStream_t s);
+void launch_restitution(const RestArgs& a, int grid, hipStream_t s);
+void launch_zero_int(int* p, int count, hipStream_t s);
+void launch_mesh_contacts(const MeshArgs& a, hipStream_t s);
+
+// Radix sort: one pass = upsweep (tile histograms) -> bin-row scan -> scatter.
+void launch_sort_upsweep(int n, const uint32_t* keys, int shift, uint32_t* hist, int tiles, hipStream_t s);
+void launch_sort_binscan(uint32_t* hist, uint32_t* bintot, int tiles, hipStream_t s);
+void launch_sort_scatter(int n, const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out, uint32_t* vals_out,
+                         int shift, const uint32_t* hist, const uint32_t* bintot, int tiles, hipStream_t s);
+
+}  // namespace xpbd

@@ -1,0 +1,11 @@
+"""vkxpbd -- XPBD particle solver for AMD Instinct MI355X (gfx950).
+
+Same Python surface as the reference's `vkxpbd` pybind11 module
+(src/bindings.cpp:8-301): `SolverConfig`, `XpbdSolver`, `enumerate_devices`.
+The native module wraps the C-ABI of libvkxpbd.so (include/xpbd_capi.h),
+whose kernels are hand-written HIP for gfx950.  Importing fails loudly when
+the native library has not been built; there is no CPU fallback.
+"""
+from ._vkxpbd import SolverConfig, XpbdSolver, enumerate_devices  # noqa: F401
+
+__all__ = ["SolverConfig", "XpbdSolver", "enumerate_devices"]

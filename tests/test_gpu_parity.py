@@ -1,0 +1,251 @@
+"""HIP solver vs the oracle on identical inputs (through the vkxpbd module,
+which calls the C-ABI of libvkxpbd.so).
+
+Bar: integer stages (cell keys, sorted ids, cell tables, contact sets) bit
+exact; floating-point state bit exact against the fp32 restatement wherever
+the reference's float-atomic order cannot matter (at most two shape contacts
+per particle), and always within 1e-5 m of it."""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from gpu_util import assert_contacts_equal, gpu_contacts, log, make_pair, scene_from
+from vkxpbd import scenes
+
+pytestmark = pytest.mark.gpu
+
+TOL_M = 1e-5
+
+
+def compare_state(g, o, exact=True, what=""):
+    gp, op = g.get_positions().astype(np.float64), o.get_positions()
+    gv, ov = g.get_velocities().astype(np.float64), o.get_velocities()
+    dx = float(np.max(np.abs(gp - op))) if gp.size else 0.0
+    dv = float(np.max(np.abs(gv - ov))) if gv.size else 0.0
+    nbad = int(np.count_nonzero(gp != op))
+    log(f"  {what}: max|dx|={dx:.3e} max|dv|={dv:.3e} differing coords={nbad}")
+    assert dx <= TOL_M
+    if exact:
+        np.testing.assert_array_equal(gp, op)
+        np.testing.assert_array_equal(gv, ov)
+    return dx
+
+
+def compare_grid(g, o):
+    gk, gi = g.get_sorted_cells(), g.get_sorted_point_ids()
+    ok, oi = o.get_sorted()
+    np.testing.assert_array_equal(gk, ok)
+    np.testing.assert_array_equal(gi, oi)
+    os_, oe = o.get_cell_tables()
+    np.testing.assert_array_equal(g.get_cell_starts(), os_)
+    np.testing.assert_array_equal(g.get_cell_ends(), oe)
+
+
+def test_one_substep_bitexact_pile_4096(vk):
+    sc = scenes.config0()
+    g, o = make_pair(vk, sc)
+    log("pile_4096: step(1/60, 1, 1)")
+    g.step(1.0 / 60.0, 1, 1)
+    o.step(1.0 / 60.0, 1, 1)
+    compare_grid(g, o)
+    assert g.last_contact_count == o.last_contact_count
+    assert_contacts_equal(gpu_contacts(g), o.get_contacts())
+    np.testing.assert_array_equal(g.get_deltas().astype(np.float64), o.get_deltas())
+    compare_state(g, o, exact=True, what="1 substep")
+
+
+def test_frames_pile_4096(vk):
+    """config 0 for 30 frames x 8 substeps, compared every frame."""
+    sc = scenes.config0()
+    g, o = make_pair(vk, sc)
+    for f in range(30):
+        g.step(1.0 / 60.0, 8, 1)
+        o.step(1.0 / 60.0, 8, 1)
+        assert g.last_contact_count == o.last_contact_count
+        if f % 10 == 9:
+            compare_state(g, o, exact=True, what=f"frame {f + 1}")
+            compare_grid(g, o)
+    assert_contacts_equal(gpu_contacts(g), o.get_contacts())
+
+
+@pytest.mark.parametrize("n", [1, 2, 63, 1023, 1025, 5000])
+def test_ragged_sizes(vk, n):
+    rng = np.random.default_rng(n)
+    r = 0.01
+    q = rng.uniform([-0.1, -0.1, 0.0], [0.1, 0.1, 0.15], size=(n, 3)).astype(np.float32)
+    qd = rng.normal(0, 0.3, size=(n, 3)).astype(np.float32)
+    sc = scene_from(q, r, qd)
+    g, o = make_pair(vk, sc)
+    log(f"ragged n={n}")
+    for _ in range(3):
+        g.step(1.0 / 60.0, 4, 1)
+        o.step(1.0 / 60.0, 4, 1)
+    compare_grid(g, o)
+    compare_state(g, o, exact=True, what=f"n={n}")
+
+
+def test_two_iterations_cohesion_restitution(vk):
+    rng = np.random.default_rng(11)
+    r = 0.01
+    q = scenes.lattice(12, 12, 6, 2 * r * 0.95, (-0.11, -0.11, 0.004), 0.1 * r, rng)
+    qd = rng.normal(0, 0.2, size=q.shape).astype(np.float32)
+    qd[:, 2] -= 1.0
+    sc = scene_from(q, r, qd)
+    g, o = make_pair(vk, sc, particle_cohesion=0.002, particle_adhesion=0.001, enable_restitution=True,
+                     soft_contact_restitution=0.5)
+    for k in range(5):
+        g.step(1.0 / 60.0, 4, 2)
+        o.step(1.0 / 60.0, 4, 2)
+    # restitution updates velocities with float atomics on the velocity buffer
+    # it also reads (apply_particle_shape_restitution.comp:112-114): with one
+    # shape contact per particle the result is order independent
+    compare_state(g, o, exact=True, what="K=2 + cohesion + restitution")
+
+
+def test_inactive_and_proxy_particles(vk):
+    rng = np.random.default_rng(5)
+    r = 0.01
+    q = scenes.lattice(10, 10, 4, 2 * r * 0.9, (-0.09, -0.09, 0.005), 0.1 * r, rng)
+    flags = np.ones(q.shape[0], np.int32)
+    flags[::7] = 0          # inactive
+    flags[3::11] = 1 | 2    # active proxy
+    sc = scene_from(q, r, flags=flags)
+    g, o = make_pair(vk, sc)
+    for _ in range(4):
+        g.step(1.0 / 60.0, 8, 1)
+        o.step(1.0 / 60.0, 8, 1)
+    compare_state(g, o, exact=True, what="inactive/proxy")
+    # inactive particles never move (the build's documented passthrough)
+    np.testing.assert_array_equal(g.get_positions()[::7], q[::7])
+
+
+def test_collide_only_and_injected_contacts(vk):
+    sc = scenes.config0()
+    g, o = make_pair(vk, sc)
+    for _ in range(20):
+        g.step(1.0 / 60.0, 8, 1)
+        o.step(1.0 / 60.0, 8, 1)
+    g.collide_only()
+    o.collide_only()
+    assert g.last_contact_count == o.last_contact_count > 0
+    c = o.get_contacts()
+    assert_contacts_equal(gpu_contacts(g), c)
+    # inject a permuted half of the contact set and solve without collision
+    rng = np.random.default_rng(0)
+    sel = rng.permutation(c["particles"].size)[: c["particles"].size // 2]
+    args = (c["particles"][sel].astype(np.int32), c["shapes"][sel].astype(np.int32),
+            c["body_positions"][sel].astype(np.float32), c["normals"][sel].astype(np.float32), int(sel.size))
+    g.set_contact_data(*args)
+    o.set_contact_data(*args)
+    assert g.last_contact_count == sel.size
+    g.step_solve_only(1.0 / 60.0, 4, 1)
+    o.step_solve_only(1.0 / 60.0, 4, 1)
+    compare_state(g, o, exact=True, what="step_solve_only")
+
+
+def ico_shell(level, n, seed, band=0.02, r=0.005):
+    """Particles in a band around the icosphere surface, inside and outside."""
+    rng = np.random.default_rng(seed)
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rad = 0.5 + rng.uniform(-band, band, size=(n, 1))
+    return (d * rad + [0.0, 0.0, 0.5]).astype(np.float32), r
+
+
+@pytest.mark.parametrize("watertight", [True, False])
+def test_mesh_contacts_collide_only(vk, watertight):
+    verts, tris = scenes.icosphere(3)
+    q, r = ico_shell(3, 4000, seed=1)
+    sc = scene_from(q, r, meshes=[(verts, tris)])
+    cfg = scenes.solver_config(vk, r)
+    g = vk.XpbdSolver()
+    g.init(cfg, q.shape[0])
+    g.add_mesh(verts.reshape(-1), tris.reshape(-1), is_watertight=watertight)
+    g.finalize_meshes()
+    p = sc["particles"]
+    g.set_particles(p["q"], p["qd"], p["inv_mass"], p["radii"], p["flags"])
+    o = O.OracleSolver()
+    o.init(cfg, q.shape[0])
+    o.add_mesh(verts, tris, is_watertight=watertight)
+    o.finalize_meshes()
+    o.set_particles(p["q"], p["qd"], p["inv_mass"], p["radii"], p["flags"])
+    g.collide_only()
+    o.collide_only()
+    log(f"mesh collide_only watertight={watertight}: contacts gpu={g.last_contact_count} oracle={o.last_contact_count}")
+    assert g.last_contact_count == o.last_contact_count > 0
+    gc, oc = gpu_contacts(g), o.get_contacts()
+    if watertight:
+        assert_contacts_equal(gc, oc)
+    else:
+        # the pseudo-normal weights use acos (device vs host libm): sets and
+        # positions must agree, normals to a tolerance
+        a, b = (lambda c: {k: np.asarray(v)[np.lexsort((c["shapes"], c["particles"]))] for k, v in c.items()})(gc), \
+               (lambda c: {k: np.asarray(v)[np.lexsort((c["shapes"], c["particles"]))] for k, v in c.items()})(oc)
+        np.testing.assert_array_equal(a["particles"], b["particles"])
+        np.testing.assert_allclose(a["normals"], b["normals"], atol=1e-6)
+    # outward normals: the mesh contact's normal points away from the centre
+    m = gc["shapes"] == 1
+    rel = gc["body_positions"][m] - [0.0, 0.0, 0.5]
+    assert np.all(np.sum(rel * gc["normals"][m], axis=1) > 0.0)
+
+
+def test_mesh_pile_frames(vk):
+    """A pile falling onto a watertight icosphere: the config-1 path at small size."""
+    rng = np.random.default_rng(2)
+    r = 0.01
+    q = scenes.lattice(24, 24, 8, 2 * r, (-0.23, -0.23, 1.03), 0.1 * r, rng)
+    verts, tris = scenes.icosphere(3)
+    sc = scene_from(q, r, meshes=[(verts, tris)])
+    g, o = make_pair(vk, sc)
+    for f in range(24):
+        g.step(1.0 / 60.0, 8, 1)
+        o.step(1.0 / 60.0, 8, 1)
+        assert g.last_contact_count == o.last_contact_count
+        if f % 6 == 5:
+            log(f"  frame {f + 1}: contacts={g.last_contact_count}")
+            assert_contacts_equal(gpu_contacts(g), o.get_contacts())
+            compare_state(g, o, exact=True, what=f"mesh pile frame {f + 1}")
+
+
+def test_update_mesh_moving_collider(vk):
+    rng = np.random.default_rng(9)
+    r = 0.01
+    q = scenes.lattice(16, 16, 4, 2 * r, (-0.15, -0.15, 1.02), 0.1 * r, rng)
+    verts, tris = scenes.icosphere(2)
+    sc = scene_from(q, r, meshes=[(verts, tris)])
+    g, o = make_pair(vk, sc)
+    for f in range(12):
+        nv = (verts + np.float32([0.0, 0.0, 0.01 * f])).astype(np.float32)
+        vel = np.tile(np.float32([0.0, 0.0, 0.6]), (verts.shape[0], 1))
+        g.update_mesh(0, nv.reshape(-1), vel.reshape(-1))
+        o.update_mesh(0, nv, vel)
+        g.step(1.0 / 60.0, 8, 1)
+        o.step(1.0 / 60.0, 8, 1)
+    lo_g, lo_o = g.get_mesh_bvh_lowers(0), o.get_mesh_bvh(0)[0]
+    np.testing.assert_array_equal(lo_g.view(np.uint32), lo_o.view(np.uint32))
+    assert_contacts_equal(gpu_contacts(g), o.get_contacts())
+    compare_state(g, o, exact=True, what="moving collider")
+
+
+def test_large_grid_properties(vk):
+    """config 1 size (1,048,576 particles): size-independent properties of the
+    sort and cell tables after one substep with no iterations."""
+    sc = scenes.config1()
+    g = scenes.build_solver(vk, sc)
+    g.step(1.0 / 60.0, 1, 0)   # integrate + grid + sort only
+    x = g.get_positions()
+    keys, ids = g.get_sorted_cells(), g.get_sorted_point_ids()
+    n = x.shape[0]
+    assert np.all(keys[1:] >= keys[:-1])
+    assert np.array_equal(np.sort(ids), np.arange(n, dtype=np.uint32))
+    same = keys[1:] == keys[:-1]
+    assert np.all(ids[1:][same] > ids[:-1][same])          # stable: ids ascend inside a cell
+    inv = np.float32(1.0) / np.float32(2 * sc["r"])
+    c = np.floor(x * inv).astype(np.int64) + (1 << 20)
+    c = np.maximum(c, 0) % 128
+    cell = (c[:, 2] * 16384 + c[:, 1] * 128 + c[:, 0]).astype(np.uint32)
+    np.testing.assert_array_equal(keys, cell[ids])
+    st, en = g.get_cell_starts(), g.get_cell_ends()
+    assert int(np.sum(en - st)) == n
+    u = np.unique(keys)
+    np.testing.assert_array_equal(keys[st[u]], u)
