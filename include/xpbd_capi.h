@@ -180,6 +180,26 @@ int xpbd_profile_stages(xpbd_solver* h, char* names, size_t names_len, double* m
                         size_t* count);
 int xpbd_profile_reset(xpbd_solver* h);
 
+/* Dynamic rigid boxes.  No counterpart in the reference's API: its body
+ * shaders (integrate_bodies.comp, solve_body_contact_positions.comp,
+ * apply_body_deltas.comp) are never dispatched and nothing creates dynamic
+ * bodies.  Each box becomes a body and a shape (after the meshes); particles
+ * collide with it (particle-shape contacts) and its corners with the ground
+ * plane (rigid contacts).  pos xyz, quat xyzw, half extents xyz, density
+ * kg/m^3, flags (2 = kinematic).  Call before finalize_meshes/step. */
+int xpbd_add_box_body(xpbd_solver* h, const float* pos, const float* quat, const float* half_extents, float density,
+                      int flags);
+/* Body solver parameters (defaults 0.05, 0.8, 0, 0, 1): angular damping of
+ * integrate_bodies, relaxation of the rigid contacts, torsional and rolling
+ * friction, and whether apply_body_deltas divides by the rigid-contact count. */
+int xpbd_set_body_params(xpbd_solver* h, float angular_damping, float rigid_relaxation, float mu_torsional,
+                         float mu_rolling, int use_constraint_weights);
+int xpbd_body_count(const xpbd_solver* h, size_t* out);
+/* Body state: q7 = (px, py, pz, qx, qy, qz, qw), qd6 = (vx, vy, vz, wx, wy, wz)
+ * per box in add order; either pointer may be NULL; cap in bodies. */
+int xpbd_get_bodies(xpbd_solver* h, float* q7, float* qd6, size_t cap);
+int xpbd_set_body_velocities(xpbd_solver* h, const float* qd6, size_t n);
+
 /* Host-only LBVH build of one mesh, the builder finalize_meshes uses
  * (MeshBvh::build, src/mesh_bvh.h:40-69 -> src/mesh_lbvh.h:49-223).  Needs no
  * GPU.  lowers/uppers receive 2*num_tris packed nodes (4 floats each, w =

@@ -210,6 +210,24 @@ int xpbd_profile_stages(xpbd_solver* h, char* names, size_t names_len, double* m
 }
 int xpbd_profile_reset(xpbd_solver* h) { return guarded(h, [&] { h->s.profile_reset(); }); }
 
+int xpbd_add_box_body(xpbd_solver* h, const float* pos, const float* quat, const float* half, float density, int flags) {
+    return guarded(h, [&] { h->s.add_box_body(pos, quat, half, density, flags); });
+}
+int xpbd_set_body_params(xpbd_solver* h, float ad, float relax, float mt, float mr, int uw) {
+    return guarded(h, [&] { h->s.set_body_params(ad, relax, mt, mr, uw); });
+}
+int xpbd_body_count(const xpbd_solver* h, size_t* out) {
+    if (!h || !out) return 1;
+    *out = h->s.body_count();
+    return 0;
+}
+int xpbd_get_bodies(xpbd_solver* h, float* q7, float* qd6, size_t cap) {
+    return guarded(h, [&] { h->s.get_bodies(q7, qd6, cap); });
+}
+int xpbd_set_body_velocities(xpbd_solver* h, const float* qd6, size_t n) {
+    return guarded(h, [&] { h->s.set_body_velocities(qd6, n); });
+}
+
 int xpbd_lbvh_build(const float* verts, size_t nv, const uint32_t* tris, size_t nt, float* lowers, float* uppers,
                     uint32_t* prim, uint32_t* root) {
     (void)nv;

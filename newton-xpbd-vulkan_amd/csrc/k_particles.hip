@@ -16,6 +16,8 @@
 //   apply_particle_shape_restitution        -> k_restitution
 #include "xpbd_kernels.h"
 
+#include <algorithm>
+
 namespace xpbd {
 
 // Warp-striped tile mapping shared with the radix-sort kernels: wave w of a
@@ -37,44 +39,56 @@ __global__ __launch_bounds__(256) void k_integrate(IntegrateArgs a) {
     __shared__ uint32_t h[kSortBins];
     const int tile = blockIdx.x;
     if (threadIdx.x < kSortBins) h[threadIdx.x] = 0;
-    if (a.cc_next != nullptr && blockIdx.x == 0 && threadIdx.x == 0) *a.cc_next = 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (a.cc_next != nullptr) *a.cc_next = 0;
+        if (a.rcc_next != nullptr) *a.rcc_next = 0;
+    }
     __syncthreads();
+    float4 P[kSortItems], V[kSortItems];
+    int f[kSortItems];
+#pragma unroll
+    for (int it = 0; it < kSortItems; it++) {   // issue every load of the tile first
+        const int idx = tile_item(tile, it);
+        P[it] = make_float4(0.f, 0.f, 0.f, 0.f);
+        V[it] = P[it];
+        f[it] = 0;
+        if (idx < a.n) { P[it] = a.Ps[idx]; V[it] = a.Vs[idx]; f[it] = a.flags[idx]; }
+    }
+#pragma unroll
     for (int it = 0; it < kSortItems; it++) {
-        int idx = tile_item(tile, it);
-        bool valid = idx < a.n;
-        float4 P = make_float4(0.f, 0.f, 0.f, 0.f), V = P;
-        int f = 0;
-        if (valid) { P = a.Ps[idx]; V = a.Vs[idx]; f = a.flags[idx]; }
+        const int idx = tile_item(tile, it);
+        const bool valid = idx < a.n;
         bool want = false;
-        if (a.has_plane && valid && (f & 1)) {
-            float dist = P.z - a.altitude;
-            want = dist < a.margin + P.w;
+        if (a.has_plane && valid && (f[it] & 1)) {
+            const float dist = P[it].z - a.altitude;
+            want = dist < a.margin + P[it].w;
         }
         if (a.cc != nullptr) {
-            int slot = wave_reserve(a.cc, want);
+            const int slot = wave_reserve(a.cc, want);
             if (want && slot < a.cmax) {
                 a.cpart[slot] = idx;
                 a.cshape[slot] = 0;
-                a.cbpos[slot] = make_float4(P.x, P.y, a.altitude, 0.f);
+                a.cbpos[slot] = make_float4(P[it].x, P[it].y, a.altitude, 0.f);
                 a.cbvel[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
                 a.cnorm[slot] = make_float4(0.f, 0.f, 1.f, 0.f);
             }
         }
         if (!valid) continue;
-        float4 Pn = P, Vn = V;
-        if (f & 1) {
-            float w = V.w;
-            float gs = w > 0.0f ? 1.0f : 0.0f;
-            float fx = 0.0f * w, fy = 0.0f * w, fz = 0.0f * w;
-            f3 v1 = {V.x + (fx + a.gx * gs) * a.dt, V.y + (fy + a.gy * gs) * a.dt, V.z + (fz + a.gz * gs) * a.dt};
-            float mag = len3(v1);
+        float4 Pn = P[it], Vn = V[it];
+        if (f[it] & 1) {
+            const float w = V[it].w;
+            const float gs = w > 0.0f ? 1.0f : 0.0f;
+            const float fx = 0.0f * w, fy = 0.0f * w, fz = 0.0f * w;
+            f3 v1 = {V[it].x + (fx + a.gx * gs) * a.dt, V[it].y + (fy + a.gy * gs) * a.dt,
+                     V[it].z + (fz + a.gz * gs) * a.dt};
+            const float mag = len3(v1);
             if (mag > a.vmax) v1 = v1 * (a.vmax / mag);
-            Pn = make_float4(P.x + v1.x * a.dt, P.y + v1.y * a.dt, P.z + v1.z * a.dt, P.w);
-            Vn = make_float4(v1.x, v1.y, v1.z, V.w);
+            Pn = make_float4(P[it].x + v1.x * a.dt, P[it].y + v1.y * a.dt, P[it].z + v1.z * a.dt, P[it].w);
+            Vn = make_float4(v1.x, v1.y, v1.z, V[it].w);
         }
         a.Pp[idx] = Pn;
         a.Vp[idx] = Vn;
-        uint32_t key = cell_of(Pn.x, Pn.y, Pn.z, a.inv_cell);
+        const uint32_t key = cell_of(Pn.x, Pn.y, Pn.z, a.inv_cell);
         a.keys[idx] = key;
         atomicAdd(&h[key & (kSortBins - 1)], 1u);
     }
@@ -101,10 +115,9 @@ __global__ __launch_bounds__(256) void k_plane_contacts(PlaneArgs a) {
     }
 }
 
-// grid_offsets.comp:16-35 without the per-substep 16 MiB table clear of
-// scene.cpp:918-923: every occupied cell's (start, end) is rewritten here, and
-// readers accept an entry only if keys[start] == cell, so stale entries of
-// cells emptied since an earlier substep are never used.
+// grid_offsets.comp:16-35 into one interleaved (start, end) table; the table
+// is zeroed by k_fill_zero first (the two 8 MiB fills of scene.cpp:918-923
+// become one 16 MiB vectorised fill), so empty cells read (0, 0).
 __global__ __launch_bounds__(256) void k_cell_ranges(int n, const uint32_t* __restrict__ keys, int* __restrict__ range) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
@@ -211,6 +224,57 @@ __global__ __launch_bounds__(256) void k_shape_solve(ShapeArgs a) {
     }
 }
 
+// One neighbour candidate of solve_particle_particle_contacts.comp:150-210.
+template <bool DIAG>
+__device__ __forceinline__ void pp_pair(const PPArgs& a, int t, int k, int i, int proxy, f3 x, f3 v, float rad, float w1,
+                                        float R, f3& acc) {
+    if (k == t) return;   // j == i
+    const int nf = a.Fs[k];
+    if ((nf & 1) == 0) return;
+    const float4 Pj = a.Xs[k];
+    const float4 Vj = a.Vs[k];
+    if (proxy != 0 && (((nf & 2) != 0) || (Vj.w <= 0.0f))) return;
+    const f3 rd = x - xyz(Pj);
+    if (fabsf(rd.x) > R || fabsf(rd.y) > R || fabsf(rd.z) > R) return;
+    const float d = exact_length(rd);
+    const float err = d - rad - Pj.w;
+    if (err > a.coh || d <= 0.0f) return;
+    const float w2 = Vj.w;
+    const float denom = w1 + w2;
+    if (denom <= 0.0f) return;
+    const f3 nn = exact_div(rd, d);
+    const f3 vrel = v - xyz(Vj);
+    const float lambda_n = err;
+    const float vn = exact_dot(nn, vrel);
+    const f3 vt = {fma_(vn, -nn.x, vrel.x), fma_(vn, -nn.y, vrel.y), fma_(vn, -nn.z, vrel.z)};
+    const float vt_len = exact_length(vt);
+    const float lambda_f = gmax(a.mu * lambda_n, -vt_len * a.dt);
+    const f3 vt_dir = vt_len > 0.0f ? exact_div(vt, vt_len) : f3{0.f, 0.f, 0.f};
+    const f3 p2 = nn * err;
+    const f3 df = {fma_(vt_dir.x, lambda_f, -p2.x), fma_(vt_dir.y, lambda_f, -p2.y), fma_(vt_dir.z, lambda_f, -p2.z)};
+    const f3 pair_d = exact_div(df, denom);
+    acc = acc + pair_d;
+    if (DIAG) {
+        const uint32_t slot = atomicAdd(a.diag_header, 1u);
+        if (slot < a.max_diag_pairs) {
+            const f3 dw = {fma_(pair_d.x, w1, 0.f), fma_(pair_d.y, w1, 0.f), fma_(pair_d.z, w1, 0.f)};
+            const f3 di = {fma_(dw.x, a.relax, 0.f), fma_(dw.y, a.relax, 0.f), fma_(dw.z, a.relax, 0.f)};
+            float* rec = a.diag_pairs + 20 * size_t(slot);
+            reinterpret_cast<uint32_t*>(rec)[0] = uint32_t(i);
+            reinterpret_cast<uint32_t*>(rec)[1] = a.vals[k];
+            rec[2] = d; rec[3] = err;
+            rec[4] = nn.x; rec[5] = nn.y; rec[6] = nn.z;
+            rec[7] = vrel.x; rec[8] = vrel.y; rec[9] = vrel.z;
+            rec[10] = vn; rec[11] = vt_len; rec[12] = denom; rec[13] = lambda_n; rec[14] = lambda_f;
+            rec[15] = di.x; rec[16] = di.y; rec[17] = di.z;
+            reinterpret_cast<uint32_t*>(rec)[18] = a.iter_index;
+            reinterpret_cast<uint32_t*>(rec)[19] = 0u;
+        } else {
+            atomicAdd(a.diag_header + 1, 1u);
+        }
+    }
+}
+
 // solve_particle_particle_contacts.comp:107-218 fused with
 // apply_particle_deltas.comp:38-58.  One thread per sorted slot t (the
 // reference's thread order), particle i = vals[t].  Particle i's delta is
@@ -218,19 +282,26 @@ __global__ __launch_bounds__(256) void k_shape_solve(ShapeArgs a) {
 // accumulates exactly these with float atomics into a zeroed buffer, and with
 // at most two shape contacts per particle the sum is order independent, so a
 // single thread can finish the particle and write the applied state directly.
+//
+// Neighbour state is read from the sorted copies: the query box visits cells z
+// -> y -> x -> sorted slot, and the x-adjacent cells of one row are consecutive
+// cell ids, hence one contiguous run of sorted slots in the same order.  A row
+// is split where the x index wraps at 128; a virtual x below the clamp (the
+// reference maps every such cell to x = 0) is visited cell by cell.
 template <bool DIAG>
 __global__ __launch_bounds__(256) void k_pp_apply(PPArgs a) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= a.n) return;
     const int i = int(a.vals[t]);
-    const int pf = a.flags[i];
-    f3 ds = {a.dshape[3 * i], a.dshape[3 * i + 1], a.dshape[3 * i + 2]};
+    const int pf = a.Fs[t];
+    const f3 ds = {a.dshape[3 * i], a.dshape[3 * i + 1], a.dshape[3 * i + 2]};
     a.dshape[3 * i] = 0.0f; a.dshape[3 * i + 1] = 0.0f; a.dshape[3 * i + 2] = 0.0f;
-    const float4 Pi = a.Pp[i], Vi = a.Vp[i];
+    const float4 Pi = a.Xs[t], Vi = a.Vs[t];
     if ((pf & 1) == 0) {   // inactive: passthrough (the reference leaves x_out unwritten)
         a.delta[3 * i] = ds.x; a.delta[3 * i + 1] = ds.y; a.delta[3 * i + 2] = ds.z;
         a.Po[i] = Pi;
         a.Vo[i] = Vi;
+        if (a.Xs_next) { a.Xs_next[t] = Pi; a.Vs_next[t] = Vi; }
         return;
     }
     const int proxy = pf & 2;
@@ -242,63 +313,36 @@ __global__ __launch_bounds__(256) void k_pp_apply(PPArgs a) {
     const int xe = min((int)floorf((x.x + R) * inv), xs + kGridDim - 1);
     const int ye = min((int)floorf((x.y + R) * inv), ys + kGridDim - 1);
     const int ze = min((int)floorf((x.z + R) * inv), zs + kGridDim - 1);
+    const int2* range = reinterpret_cast<const int2*>(a.range);
     f3 acc = {0.f, 0.f, 0.f};
     for (int cz = zs; cz <= ze; cz++)
-        for (int cy = ys; cy <= ye; cy++)
-            for (int cx = xs; cx <= xe; cx++) {
-                const int cell = cell_index(cx, cy, cz);
-                const int2 rg = reinterpret_cast<const int2*>(a.range)[cell];
-                if (rg.x >= a.n || a.keys[rg.x] != uint32_t(cell)) continue;   // stale entry: cell empty now
-                for (int k = rg.x; k < rg.y; k++) {
-                    const int j = int(a.vals[k]);
-                    if (j == i) continue;
-                    const int nf = a.flags[j];
-                    if ((nf & 1) == 0) continue;
-                    const float4 Pj = a.Pp[j];
-                    const float4 Vj = a.Vp[j];
-                    if (proxy != 0 && (((nf & 2) != 0) || (Vj.w <= 0.0f))) continue;
-                    const f3 rd = x - xyz(Pj);
-                    if (fabsf(rd.x) > R || fabsf(rd.y) > R || fabsf(rd.z) > R) continue;
-                    const float d = exact_length(rd);
-                    const float err = d - rad - Pj.w;
-                    if (err > a.coh || d <= 0.0f) continue;
-                    const float w2 = Vj.w;
-                    const float denom = w1 + w2;
-                    if (denom <= 0.0f) continue;
-                    const f3 nn = exact_div(rd, d);
-                    const f3 vrel = v - xyz(Vj);
-                    const float lambda_n = err;
-                    const float vn = exact_dot(nn, vrel);
-                    const f3 vt = {fma_(vn, -nn.x, vrel.x), fma_(vn, -nn.y, vrel.y), fma_(vn, -nn.z, vrel.z)};
-                    const float vt_len = exact_length(vt);
-                    const float lambda_f = gmax(a.mu * lambda_n, -vt_len * a.dt);
-                    const f3 vt_dir = vt_len > 0.0f ? exact_div(vt, vt_len) : f3{0.f, 0.f, 0.f};
-                    const f3 p2 = nn * err;
-                    const f3 df = {fma_(vt_dir.x, lambda_f, -p2.x), fma_(vt_dir.y, lambda_f, -p2.y),
-                                   fma_(vt_dir.z, lambda_f, -p2.z)};
-                    const f3 pair_d = exact_div(df, denom);
-                    acc = acc + pair_d;
-                    if (DIAG) {
-                        uint32_t slot = atomicAdd(a.diag_header, 1u);
-                        if (slot < a.max_diag_pairs) {
-                            f3 dw = {fma_(pair_d.x, w1, 0.f), fma_(pair_d.y, w1, 0.f), fma_(pair_d.z, w1, 0.f)};
-                            f3 di = {fma_(dw.x, a.relax, 0.f), fma_(dw.y, a.relax, 0.f), fma_(dw.z, a.relax, 0.f)};
-                            float* rec = a.diag_pairs + 20 * size_t(slot);
-                            reinterpret_cast<uint32_t*>(rec)[0] = uint32_t(i);
-                            reinterpret_cast<uint32_t*>(rec)[1] = uint32_t(j);
-                            rec[2] = d; rec[3] = err;
-                            rec[4] = nn.x; rec[5] = nn.y; rec[6] = nn.z;
-                            rec[7] = vrel.x; rec[8] = vrel.y; rec[9] = vrel.z;
-                            rec[10] = vn; rec[11] = vt_len; rec[12] = denom; rec[13] = lambda_n; rec[14] = lambda_f;
-                            rec[15] = di.x; rec[16] = di.y; rec[17] = di.z;
-                            reinterpret_cast<uint32_t*>(rec)[18] = a.iter_index;
-                            reinterpret_cast<uint32_t*>(rec)[19] = 0u;
-                        } else {
-                            atomicAdd(a.diag_header + 1, 1u);
-                        }
+        for (int cy = ys; cy <= ye; cy++) {
+            int cx = xs;
+            while (cx <= xe) {
+                const int vx = cx + kGridOrigin;
+                if (vx < 0) {   // clamped virtual cell: one physical cell per step
+                    const int2 rg = range[cell_index(cx, cy, cz)];
+                    for (int k = rg.x; k < rg.y; k++) pp_pair<DIAG>(a, t, k, i, proxy, x, v, rad, w1, R, acc);
+                    cx++;
+                    continue;
+                }
+                const int px = vx % kGridDim;
+                const int len = min(xe - cx + 1, kGridDim - px);
+                const int c0 = cell_index(cx, cy, cz);
+                int s = 0, e = 0;
+                bool any = false;
+                for (int m = 0; m < len; m++) {
+                    const int2 rg = range[c0 + m];
+                    if (rg.y > rg.x) {
+                        if (!any) s = rg.x;
+                        e = rg.y;
+                        any = true;
                     }
                 }
+                for (int k = s; k < e; k++) pp_pair<DIAG>(a, t, k, i, proxy, x, v, rad, w1, R, acc);
+                cx += len;
             }
+        }
     const f3 dw = {fma_(acc.x, w1, 0.f), fma_(acc.y, w1, 0.f), fma_(acc.z, w1, 0.f)};
     const f3 tot = {fma_(dw.x, a.relax, 0.f), fma_(dw.y, a.relax, 0.f), fma_(dw.z, a.relax, 0.f)};
     const f3 dl = ds + tot;
@@ -312,8 +356,10 @@ __global__ __launch_bounds__(256) void k_pp_apply(PPArgs a) {
         vnew = vnew * (a.vmax / mag);
         xn = x0 + vnew * a.dt;
     }
-    a.Po[i] = make_float4(xn.x, xn.y, xn.z, rad);
-    a.Vo[i] = make_float4(vnew.x, vnew.y, vnew.z, w1);
+    const float4 Po = make_float4(xn.x, xn.y, xn.z, rad), Vo = make_float4(vnew.x, vnew.y, vnew.z, w1);
+    a.Po[i] = Po;
+    a.Vo[i] = Vo;
+    if (a.Xs_next) { a.Xs_next[t] = Po; a.Vs_next[t] = Vo; }
 }
 
 // apply_particle_shape_restitution.comp:48-116.  Body previous pose/velocity
@@ -360,9 +406,20 @@ __global__ __launch_bounds__(256) void k_restitution(RestArgs a) {
     }
 }
 
+__global__ __launch_bounds__(256) void k_fill_zero(int4* __restrict__ p, size_t n4) {
+    for (size_t t = size_t(blockIdx.x) * blockDim.x + threadIdx.x; t < n4; t += size_t(gridDim.x) * blockDim.x)
+        p[t] = make_int4(0, 0, 0, 0);
+}
+
 __global__ void k_zero_int(int* p, int count) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < count) p[t] = 0;
+}
+
+// Zeroes the two counters a frame's first substep uses (particle contacts at
+// cc[0], rigid contacts at cc[4]).
+__global__ void k_frame_begin(int* cc) {
+    if (threadIdx.x == 0) { cc[0] = 0; cc[4] = 0; }
 }
 
 // ------------------------------------------------------------- launchers --
@@ -391,6 +448,14 @@ void launch_pp_apply(const PPArgs& a, bool diag, hipStream_t s) {
 void launch_restitution(const RestArgs& a, int grid, hipStream_t s) {
     hipLaunchKernelGGL(k_restitution, dim3(grid), dim3(256), 0, s, a);
 }
+// count must be a multiple of 4 ints (16-byte stores)
+void launch_fill_zero(int* p, size_t count, hipStream_t s) {
+    const size_t n4 = count / 4;
+    if (n4 == 0) return;
+    const unsigned grid = unsigned(std::min<size_t>((n4 + 255) / 256, 4096));
+    hipLaunchKernelGGL(k_fill_zero, dim3(grid), dim3(256), 0, s, reinterpret_cast<int4*>(p), n4);
+}
+void launch_frame_begin(int* cc, hipStream_t s) { hipLaunchKernelGGL(k_frame_begin, dim3(1), dim3(64), 0, s, cc); }
 void launch_zero_int(int* p, int count, hipStream_t s) {
     if (count > 0) hipLaunchKernelGGL(k_zero_int, dim3(blocks_for(count, 256)), dim3(256), 0, s, p, count);
 }

@@ -70,12 +70,15 @@ __global__ __launch_bounds__(256) void k_sort_binscan(uint32_t* __restrict__ his
     if (threadIdx.x == 0) bintot[b] = carry;
 }
 
-template <bool IDENT>
+// GATHER (last pass only): also copy each particle's predicted position,
+// velocity and flags into sorted order, so the particle-particle solve reads
+// neighbours as contiguous runs.
+template <bool IDENT, bool GATHER>
 __global__ __launch_bounds__(256) void k_sort_scatter(int n, const uint32_t* __restrict__ keys_in,
                                                       const uint32_t* __restrict__ vals_in,
                                                       uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out,
                                                       int shift, const uint32_t* __restrict__ hist,
-                                                      const uint32_t* __restrict__ bintot, int tiles) {
+                                                      const uint32_t* __restrict__ bintot, int tiles, SortGather g) {
     __shared__ uint32_t s_cnt[4][kSortBins];    // per-wave bin counters, then per-wave bin offsets
     __shared__ uint32_t s_excl[kSortBins];      // tile-local exclusive bin starts (staging order)
     __shared__ uint32_t s_gbase[kSortBins];     // global destination of the tile's first key per bin
@@ -136,8 +139,10 @@ __global__ __launch_bounds__(256) void k_sort_scatter(int n, const uint32_t* __r
         if (idx < n) {
             const uint32_t d = (key[it] >> shift) & (kSortBins - 1);
             const uint32_t pos = s_excl[d] + s_cnt[w][d] + rank[it];
-            s_keys[pos] = key[it];
-            s_vals[pos] = val[it];
+            if (pos < uint32_t(kSortTile)) {
+                s_keys[pos] = key[it];
+                s_vals[pos] = val[it];
+            }
         }
     }
     __syncthreads();
@@ -146,8 +151,16 @@ __global__ __launch_bounds__(256) void k_sort_scatter(int n, const uint32_t* __r
         const uint32_t k = s_keys[e];
         const uint32_t d = (k >> shift) & (kSortBins - 1);
         const uint32_t dst = s_gbase[d] + (uint32_t(e) - s_excl[d]);
-        keys_out[dst] = k;
-        vals_out[dst] = s_vals[e];
+        if (dst < uint32_t(n)) {
+            const uint32_t v = s_vals[e];
+            keys_out[dst] = k;
+            vals_out[dst] = v;
+            if (GATHER) {
+                g.Xs[dst] = g.P[v];
+                g.Vs[dst] = g.V[v];
+                g.Fs[dst] = g.flags[v];
+            }
+        }
     }
 }
 
@@ -158,13 +171,24 @@ void launch_sort_binscan(uint32_t* hist, uint32_t* bintot, int tiles, hipStream_
     hipLaunchKernelGGL(k_sort_binscan, dim3(kSortBins), dim3(256), 0, s, hist, bintot, tiles);
 }
 void launch_sort_scatter(int n, const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out, uint32_t* vals_out,
-                         int shift, const uint32_t* hist, const uint32_t* bintot, int tiles, hipStream_t s) {
-    if (vals_in == nullptr)
-        hipLaunchKernelGGL(k_sort_scatter<true>, dim3(tiles), dim3(256), 0, s, n, keys_in, vals_in, keys_out, vals_out,
-                           shift, hist, bintot, tiles);
-    else
-        hipLaunchKernelGGL(k_sort_scatter<false>, dim3(tiles), dim3(256), 0, s, n, keys_in, vals_in, keys_out, vals_out,
-                           shift, hist, bintot, tiles);
+                         int shift, const uint32_t* hist, const uint32_t* bintot, int tiles, const SortGather* gather,
+                         hipStream_t s) {
+    const SortGather g = gather ? *gather : SortGather{};
+    if (vals_in == nullptr) {
+        if (gather)
+            hipLaunchKernelGGL((k_sort_scatter<true, true>), dim3(tiles), dim3(256), 0, s, n, keys_in, vals_in, keys_out,
+                               vals_out, shift, hist, bintot, tiles, g);
+        else
+            hipLaunchKernelGGL((k_sort_scatter<true, false>), dim3(tiles), dim3(256), 0, s, n, keys_in, vals_in, keys_out,
+                               vals_out, shift, hist, bintot, tiles, g);
+    } else {
+        if (gather)
+            hipLaunchKernelGGL((k_sort_scatter<false, true>), dim3(tiles), dim3(256), 0, s, n, keys_in, vals_in, keys_out,
+                               vals_out, shift, hist, bintot, tiles, g);
+        else
+            hipLaunchKernelGGL((k_sort_scatter<false, false>), dim3(tiles), dim3(256), 0, s, n, keys_in, vals_in, keys_out,
+                               vals_out, shift, hist, bintot, tiles, g);
+    }
 }
 
 }  // namespace xpbd

@@ -54,8 +54,10 @@ struct DeviceGuard {
     explicit DeviceGuard(int dev) { (void)hipSetDevice(dev); }
 };
 
-constexpr const char* kStageNames[] = {"mesh_contacts", "integrate",  "sort_upsweep", "sort_binscan", "sort_scatter",
-                                       "cell_ranges",   "shape_solve", "pp_solve_apply", "restitution", "readback"};
+constexpr const char* kStageNames[] = {"mesh_contacts", "integrate",      "sort_upsweep",   "sort_binscan",
+                                       "sort_scatter",  "cell_ranges",    "shape_solve",    "pp_solve_apply",
+                                       "restitution",   "readback",       "body_contacts",  "integrate_bodies",
+                                       "body_contact_solve", "apply_body_deltas"};
 
 }  // namespace
 
@@ -76,6 +78,9 @@ void Solver::require_init() const {
 void Solver::free_all() {
     for (int k = 0; k < 3; k++) { dfree(P_[k]); dfree(V_[k]); }
     dfree(flags_); dfree(keysA_); dfree(valsA_); dfree(keysB_); dfree(valsB_); dfree(hist_); dfree(bintot_);
+    for (int k = 0; k < 2; k++) { dfree(Xs_[k]); dfree(Vs_[k]); }
+    dfree(Fs_); dfree(body_half_); dfree(body_I_); dfree(body_invw_); dfree(mat_mu_t_); dfree(mat_mu_r_);
+    dfree(rc_s0_); dfree(rc_s1_); dfree(rc_p0_); dfree(rc_p1_); dfree(rc_n_); dfree(rc_impulse_);
     dfree(range_); dfree(tbl_); dfree(cpart_); dfree(cshape_); dfree(cc_); dfree(cbpos_); dfree(cbvel_); dfree(cnorm_);
     dfree(dshape_); dfree(delta_); dfree(body_q_); dfree(body_qd_); dfree(body_com_); dfree(body_invI_);
     dfree(body_invm_); dfree(body_delta_); dfree(mat_mu_); dfree(body_flags_); dfree(shape_body_);
@@ -94,13 +99,15 @@ void Solver::alloc_all() {
     radius_h_.assign(n_, 0.f);
     invm_h_.assign(n_, 0.f);
     dalloc(keysA_, n_); dalloc(valsA_, n_); dalloc(keysB_, n_); dalloc(valsB_, n_);
+    for (int k = 0; k < 2; k++) { dalloc(Xs_[k], n_); dalloc(Vs_[k], n_); }
+    dalloc(Fs_, n_);
     const size_t tiles = (n_ + kSortTile - 1) / kSortTile;
     dalloc(hist_, size_t(kSortBins) * std::max<size_t>(tiles, 1));
     dalloc(bintot_, kSortBins);
     dalloc(range_, size_t(2) * kNumCells);
     dalloc(cpart_, cmax_); dalloc(cshape_, cmax_);
     dalloc(cbpos_, cmax_); dalloc(cbvel_, cmax_); dalloc(cnorm_, cmax_);
-    dalloc(cc_, 4);   // [slot 0, slot 1, error flags, spare]
+    dalloc(cc_, 8);   // [particle slot 0, slot 1, error flags, spare, rigid slot 0, slot 1, spare, spare]
     dalloc(dshape_, 3 * n_); dalloc(delta_, 3 * n_);
     dalloc(diag_header_, 2);
     max_pairs_ = cfg_.max_diag_pairs ? cfg_.max_diag_pairs : 100000u;
@@ -193,23 +200,55 @@ void Solver::finalize_meshes() {
     if (!initialized_) throw std::runtime_error("XpbdSolver::init must be called before finalize_meshes");
     DeviceGuard g(device_);
     HIPCHECK(hipStreamSynchronize(stream_));
-    const size_t M = meshes_.size();
-    std::vector<int32_t> shape_body(M + 1, -1);
-    for (size_t c = 0; c < M; c++) shape_body[c + 1] = int32_t(c);
-    std::vector<float> mat_mu(M + 1, cfg_.shape_material_mu);
+    const size_t M = meshes_.size(), NB = boxes_.size();
+    // shapes: 0 = ground plane (body -1), 1..M = meshes (static bodies 0..M-1),
+    // M+1..M+NB = boxes (dynamic bodies M..M+NB-1)
+    std::vector<int32_t> shape_body(M + 1 + NB, -1);
+    for (size_t c = 0; c < M + NB; c++) shape_body[c + 1] = int32_t(c);
+    std::vector<float> mat_mu(M + 1 + NB, cfg_.shape_material_mu), mat_mu_t(M + 1 + NB, mu_t_),
+        mat_mu_r(M + 1 + NB, mu_r_);
     upload(shape_body_, shape_body.data(), shape_body.size());
     upload(mat_mu_, mat_mu.data(), mat_mu.size());
-    // one static body per mesh: identity pose, zero velocity / com / inverse mass and inertia
-    const size_t B = std::max<size_t>(M, 1);
-    std::vector<float4> bq(2 * B, make_float4(0.f, 0.f, 0.f, 0.f));
+    upload(mat_mu_t_, mat_mu_t.data(), mat_mu_t.size());
+    upload(mat_mu_r_, mat_mu_r.data(), mat_mu_r.size());
+    // mesh bodies: identity pose, zero velocity / com / inverse mass and inertia;
+    // boxes: solid boxes, m = rho * 8abc, I = m/3 (b^2 + c^2, a^2 + c^2, a^2 + b^2)
+    const size_t B = std::max<size_t>(M + NB, 1);
+    std::vector<float4> bq(2 * B, make_float4(0.f, 0.f, 0.f, 0.f)), bh(B, make_float4(0.f, 0.f, 0.f, 0.f));
     for (size_t b = 0; b < B; b++) bq[2 * b + 1] = make_float4(0.f, 0.f, 0.f, 1.f);
+    std::vector<float> bI(9 * B, 0.f), binvI(9 * B, 0.f), binvm(B, 0.f);
+    std::vector<int32_t> bflags(B, 0);
+    for (size_t k = 0; k < NB; k++) {
+        const BoxHost& x = boxes_[k];
+        const size_t b = M + k;
+        bq[2 * b] = make_float4(x.pos[0], x.pos[1], x.pos[2], 0.f);
+        bq[2 * b + 1] = make_float4(x.rot[0], x.rot[1], x.rot[2], x.rot[3]);
+        bh[b] = make_float4(x.half[0], x.half[1], x.half[2], 0.f);
+        bflags[b] = x.flags;
+        const float hx = x.half[0], hy = x.half[1], hz = x.half[2];
+        const float m = x.density * 8.0f * hx * hy * hz;
+        const float ix = m / 3.0f * (hy * hy + hz * hz), iy = m / 3.0f * (hx * hx + hz * hz),
+                    iz = m / 3.0f * (hx * hx + hy * hy);
+        binvm[b] = m > 0.0f ? 1.0f / m : 0.0f;
+        bI[9 * b] = ix; bI[9 * b + 4] = iy; bI[9 * b + 8] = iz;
+        binvI[9 * b] = ix > 0.f ? 1.0f / ix : 0.f;
+        binvI[9 * b + 4] = iy > 0.f ? 1.0f / iy : 0.f;
+        binvI[9 * b + 8] = iz > 0.f ? 1.0f / iz : 0.f;
+    }
     upload(body_q_, bq.data(), bq.size());
     dalloc(body_qd_, 2 * B);
+    upload(body_half_, bh.data(), bh.size());
     dalloc(body_com_, 3 * B);
-    dalloc(body_invI_, 9 * B);
-    dalloc(body_invm_, B);
-    dalloc(body_flags_, B);
+    upload(body_I_, bI.data(), bI.size());
+    upload(body_invI_, binvI.data(), binvI.size());
+    upload(body_invm_, binvm.data(), binvm.size());
+    upload(body_flags_, bflags.data(), bflags.size());
     dalloc(body_delta_, 8 * B);
+    dalloc(body_invw_, B);
+    const size_t rcmax = std::max<size_t>(8 * NB, 1);
+    dalloc(rc_s0_, rcmax); dalloc(rc_s1_, rcmax);
+    dalloc(rc_p0_, rcmax); dalloc(rc_p1_, rcmax); dalloc(rc_n_, rcmax);
+    dalloc(rc_impulse_, 6 * rcmax);
 
     all_verts_.clear(); all_tris_.clear(); vert_off_.clear(); tri_off_.clear(); node_off_.clear(); prim_off_.clear();
     bvh_.assign(M, MeshBvhHost{});
@@ -348,8 +387,9 @@ void Solver::run_sort(int tiles) {
     launch_sort_binscan(hist_.p, bintot_.p, tiles, stream_);
     stage_end(kSortScan);
     stage_begin(kSortScatter);
-    launch_sort_scatter(n, keysB_.p, nullptr, keysA_.p, valsA_.p, 0, hist_.p, bintot_.p, tiles, stream_);
+    launch_sort_scatter(n, keysB_.p, nullptr, keysA_.p, valsA_.p, 0, hist_.p, bintot_.p, tiles, nullptr, stream_);
     stage_end(kSortScatter);
+    const SortGather g{P_[p_].p, V_[p_].p, flags_.p, Xs_[0].p, Vs_[0].p, Fs_.p};
     for (int pass = 1; pass < kSortPasses; pass++) {
         const int shift = pass * kSortBits;
         uint32_t* kin = (pass == 1) ? keysA_.p : keysB_.p;
@@ -363,22 +403,38 @@ void Solver::run_sort(int tiles) {
         launch_sort_binscan(hist_.p, bintot_.p, tiles, stream_);
         stage_end(kSortScan);
         stage_begin(kSortScatter);
-        launch_sort_scatter(n, kin, vin, kout, vout, shift, hist_.p, bintot_.p, tiles, stream_);
+        launch_sort_scatter(n, kin, vin, kout, vout, shift, hist_.p, bintot_.p, tiles,
+                            pass == kSortPasses - 1 ? &g : nullptr, stream_);
         stage_end(kSortScatter);
     }
+}
+
+BodyStepArgs Solver::body_step_args(float sdt) {
+    BodyStepArgs b;
+    b.b0 = int(meshes_.size());
+    b.b1 = int(meshes_.size() + boxes_.size());
+    b.q = body_q_.p; b.qd = body_qd_.p; b.com = body_com_.p; b.I = body_I_.p; b.invI = body_invI_.p;
+    b.invm = body_invm_.p; b.flags = body_flags_.p; b.delta = body_delta_.p; b.invw = body_invw_.p;
+    b.dt = sdt; b.gx = cfg_.gravity[0]; b.gy = cfg_.gravity[1]; b.gz = cfg_.gravity[2];
+    b.angular_damping = angular_damping_;
+    b.use_weights = use_weights_;
+    return b;
 }
 
 void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zero_next) {
     const int n = int(n_);
     const int tiles = int((n_ + kSortTile - 1) / kSortTile);
+    const int nbox = int(boxes_.size());
+    const int M = int(meshes_.size());
     int* cc_gen = collide ? cc_.p + slot : nullptr;
     int* cc_solve = collide ? cc_.p + slot : cc_.p + contact_slot_;
+    int* rcc_solve = collide ? cc_.p + 4 + slot : cc_.p + 4 + rigid_slot_;
     const float inv_cell = 1.0f / cfg_.search_radius;
     if (collide && !meshes_.empty()) {
         MeshArgs m;
         m.n = n; m.P = P_[s_].p; m.flags = flags_.p;
         m.verts = verts_d_.p; m.tris = tris_d_.p; m.nl = nl_d_.p; m.nu = nu_d_.p; m.prim = prim_d_.p;
-        m.vvel = vvel_d_.p; m.info = info_d_.p; m.nmesh = int(meshes_.size());
+        m.vvel = vvel_d_.p; m.info = info_d_.p; m.nmesh = M;
         m.margin = cfg_.soft_contact_margin; m.eps = avg_edge_ * 1e-3f;
         m.cmax = int(cmax_); m.cc = cc_gen;
         m.cpart = cpart_.p; m.cshape = cshape_.p; m.cbpos = cbpos_.p; m.cbvel = cbvel_.p; m.cnorm = cnorm_.p;
@@ -386,6 +442,27 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
         stage_begin(kMesh);
         launch_mesh_contacts(m, stream_);
         stage_end(kMesh);
+    }
+    if (collide && nbox > 0) {
+        stage_begin(kBodyContacts);
+        ParticleBoxArgs pb;
+        pb.n = n; pb.P = P_[s_].p; pb.flags = flags_.p;
+        pb.nbox = nbox; pb.b0 = M; pb.shape0 = M + 1;
+        pb.q = body_q_.p; pb.half = body_half_.p; pb.margin = cfg_.soft_contact_margin;
+        pb.cmax = int(cmax_); pb.cc = cc_gen;
+        pb.cpart = cpart_.p; pb.cshape = cshape_.p; pb.cbpos = cbpos_.p; pb.cbvel = cbvel_.p; pb.cnorm = cnorm_.p;
+        launch_particle_box_contacts(pb, stream_);
+        if (cfg_.has_ground_plane) {
+            BodyContactGenArgs bg;
+            bg.nbox = nbox; bg.b0 = M; bg.shape0 = M + 1;
+            bg.q = body_q_.p; bg.half = body_half_.p;
+            bg.margin = cfg_.soft_contact_margin; bg.altitude = cfg_.ground_plane_altitude;
+            bg.rcmax = 8 * nbox; bg.rcc = rcc_solve;
+            bg.s0 = rc_s0_.p; bg.s1 = rc_s1_.p; bg.p0 = rc_p0_.p; bg.p1 = rc_p1_.p; bg.n = rc_n_.p;
+            launch_body_plane_contacts(bg, stream_);
+        }
+        launch_fill_zero(reinterpret_cast<int*>(rc_impulse_.p), rc_impulse_.n & ~size_t(3), stream_);
+        stage_end(kBodyContacts);
     }
     if (n > 0) {
         IntegrateArgs a;
@@ -397,19 +474,31 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
         a.inv_cell = inv_cell;
         a.has_plane = (collide && cfg_.has_ground_plane) ? 1 : 0;
         a.margin = cfg_.soft_contact_margin; a.altitude = cfg_.ground_plane_altitude;
-        a.cmax = int(cmax_); a.cc = cc_gen; a.cc_next = zero_next ? cc_.p + (slot ^ 1) : nullptr;
+        a.cmax = int(cmax_); a.cc = cc_gen;
+        a.cc_next = zero_next ? cc_.p + (slot ^ 1) : nullptr;
+        a.rcc_next = zero_next ? cc_.p + 4 + (slot ^ 1) : nullptr;
         a.cpart = cpart_.p; a.cshape = cshape_.p; a.cbpos = cbpos_.p; a.cbvel = cbvel_.p; a.cnorm = cnorm_.p;
         stage_begin(kIntegrate);
         launch_integrate(a, stream_);
         stage_end(kIntegrate);
+    }
+    const BodyStepArgs bs = body_step_args(sdt);
+    if (nbox > 0) {
+        stage_begin(kBodyIntegrate);
+        launch_integrate_bodies(bs, stream_);
+        stage_end(kBodyIntegrate);
+    }
+    if (n > 0) {
         run_sort(tiles);
         stage_begin(kRanges);
+        launch_fill_zero(range_.p, size_t(2) * kNumCells, stream_);
         launch_cell_ranges(n, keysA_.p, range_.p, stream_);
         stage_end(kRanges);
         grid_valid_ = true;
     }
     int cur = p_, nxt = o_;
-    const int shape_grid = int(std::min<size_t>((cmax_ + 255) / 256, 2048));
+    int xs = 0;   // sorted copy holding the current iteration's input
+    const int shape_grid = int(std::max<size_t>(1, std::min<size_t>((cmax_ + 255) / 256, 2048)));
     for (int it = 0; it < iterations; it++) {
         ShapeArgs sa;
         sa.cmax = int(cmax_); sa.cc = cc_solve;
@@ -422,9 +511,12 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
         launch_shape_solve(sa, shape_grid, stream_);
         stage_end(kShape);
         PPArgs pa;
-        pa.n = n; pa.keys = keysA_.p; pa.vals = valsA_.p; pa.range = range_.p;
-        pa.Ps = P_[s_].p; pa.Pp = P_[cur].p; pa.Vp = V_[cur].p; pa.flags = flags_.p;
+        pa.n = n; pa.vals = valsA_.p; pa.range = range_.p;
+        pa.Ps = P_[s_].p; pa.Xs = Xs_[xs].p; pa.Vs = Vs_[xs].p; pa.Fs = Fs_.p;
         pa.dshape = dshape_.p; pa.delta = delta_.p; pa.Po = P_[nxt].p; pa.Vo = V_[nxt].p;
+        const bool more = it + 1 < iterations;
+        pa.Xs_next = more ? Xs_[xs ^ 1].p : nullptr;
+        pa.Vs_next = more ? Vs_[xs ^ 1].p : nullptr;
         pa.mu = cfg_.particle_mu; pa.coh = cfg_.particle_cohesion; pa.max_r = cfg_.particle_max_radius;
         pa.dt = sdt; pa.relax = cfg_.soft_contact_relaxation; pa.inv_cell = inv_cell; pa.vmax = cfg_.particle_v_max;
         pa.iter_index = uint32_t(it); pa.max_diag_pairs = max_pairs_;
@@ -433,6 +525,23 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
         launch_pp_apply(pa, cfg_.enable_diagnostics != 0, stream_);
         stage_end(kPP);
         std::swap(cur, nxt);
+        xs ^= 1;
+        if (nbox > 0) {
+            BodySolveArgs bv;
+            bv.rcmax = 8 * nbox; bv.rcc = rcc_solve;
+            bv.s0 = rc_s0_.p; bv.s1 = rc_s1_.p; bv.p0 = rc_p0_.p; bv.p1 = rc_p1_.p; bv.n = rc_n_.p;
+            bv.shape_body = shape_body_.p; bv.mu = mat_mu_.p; bv.mu_t = mat_mu_t_.p; bv.mu_r = mat_mu_r_.p;
+            bv.q = body_q_.p; bv.qd = body_qd_.p; bv.com = body_com_.p; bv.invI = body_invI_.p;
+            bv.invm = body_invm_.p; bv.flags = body_flags_.p;
+            bv.delta = body_delta_.p; bv.invw = body_invw_.p; bv.impulse = rc_impulse_.p;
+            bv.relax = rigid_relax_; bv.dt = sdt;
+            stage_begin(kBodySolve);
+            launch_body_contact_solve(bv, stream_);
+            stage_end(kBodySolve);
+            stage_begin(kBodyApply);
+            launch_apply_body_deltas(bs, stream_);
+            stage_end(kBodyApply);
+        }
     }
     if (cfg_.enable_restitution) {
         RestArgs ra;
@@ -473,9 +582,9 @@ void Solver::step(float dt, int substeps, int iterations) {
     DeviceGuard g(device_);
     const auto t0 = std::chrono::high_resolution_clock::now();
     const float sdt = dt / float(substeps);
-    launch_zero_int(cc_.p, 1, stream_);
+    launch_frame_begin(cc_.p, stream_);
     for (int sub = 0; sub < substeps; sub++) substep(sdt, iterations, true, sub & 1, true);
-    if (substeps > 0) contact_slot_ = (substeps - 1) & 1;
+    if (substeps > 0) contact_slot_ = rigid_slot_ = (substeps - 1) & 1;
     read_count(substeps > 0 ? contact_slot_ : 0);
     if (profiling_) {
         sync_profile();
@@ -517,6 +626,15 @@ void Solver::collide_only() {
     pl.cmax = int(cmax_); pl.cc = cc_.p;
     pl.cpart = cpart_.p; pl.cshape = cshape_.p; pl.cbpos = cbpos_.p; pl.cbvel = cbvel_.p; pl.cnorm = cnorm_.p;
     if (cfg_.has_ground_plane) launch_plane_contacts(pl, stream_);
+    if (!boxes_.empty()) {
+        ParticleBoxArgs pb;
+        pb.n = int(n_); pb.P = P_[s_].p; pb.flags = flags_.p;
+        pb.nbox = int(boxes_.size()); pb.b0 = int(meshes_.size()); pb.shape0 = int(meshes_.size()) + 1;
+        pb.q = body_q_.p; pb.half = body_half_.p; pb.margin = cfg_.soft_contact_margin;
+        pb.cmax = int(cmax_); pb.cc = cc_.p;
+        pb.cpart = cpart_.p; pb.cshape = cshape_.p; pb.cbpos = cbpos_.p; pb.cbvel = cbvel_.p; pb.cnorm = cnorm_.p;
+        launch_particle_box_contacts(pb, stream_);
+    }
     if (!meshes_.empty()) {
         MeshArgs m;
         m.n = int(n_); m.P = P_[s_].p; m.flags = flags_.p;
@@ -573,9 +691,17 @@ void Solver::set_contact_data(const int32_t* p, const int32_t* s, const float* b
     require_init();
     DeviceGuard g(device_);
     HIPCHECK(hipStreamSynchronize(stream_));
-    last_count_ = count;
     const int32_t dev_count = std::max(count, 0);
     const size_t c = std::min<size_t>(size_t(dev_count), cmax_);
+    // the solve kernels index particles and shapes with these: reject ids a
+    // kernel would read out of bounds
+    const int32_t nshapes = int32_t(meshes_.size()) + 1;
+    for (size_t i = 0; i < c; i++) {
+        if (p[i] < 0 || size_t(p[i]) >= n_) throw std::runtime_error("set_contact_data: particle index out of range");
+        if (s[i] < 0 || s[i] >= nshapes) throw std::runtime_error("set_contact_data: shape index out of range");
+    }
+    if (!finalized_) finalize_meshes();
+    last_count_ = count;
     if (c) {
         std::vector<float4> b(c), nn(c);
         for (size_t i = 0; i < c; i++) {
@@ -647,6 +773,69 @@ void Solver::bvh_half(size_t mi, bool upper, float* out, size_t cap) const {
     if (mi >= bvh_.size()) return;
     const std::vector<NodeHalf>& v = upper ? bvh_[mi].hi : bvh_[mi].lo;
     std::memcpy(out, v.data(), std::min(cap, v.size()) * sizeof(NodeHalf));
+}
+
+void Solver::add_box_body(const float* pos, const float* quat, const float* half, float density, int flags) {
+    BoxHost b;
+    std::memcpy(b.pos, pos, sizeof(b.pos));
+    std::memcpy(b.rot, quat, sizeof(b.rot));
+    std::memcpy(b.half, half, sizeof(b.half));
+    b.density = density;
+    b.flags = flags;
+    boxes_.push_back(b);
+    finalized_ = false;
+}
+
+void Solver::set_body_params(float angular_damping, float rigid_relax, float mu_t, float mu_r, int use_weights) {
+    angular_damping_ = angular_damping;
+    rigid_relax_ = rigid_relax;
+    mu_t_ = mu_t;
+    mu_r_ = mu_r;
+    use_weights_ = use_weights;
+    if (finalized_) {   // update the per-shape coefficients in place; body state is kept
+        DeviceGuard g(device_);
+        HIPCHECK(hipStreamSynchronize(stream_));
+        std::vector<float> t(mat_mu_t_.n, mu_t), r(mat_mu_r_.n, mu_r);
+        if (!t.empty()) HIPCHECK(hipMemcpy(mat_mu_t_.p, t.data(), t.size() * sizeof(float), hipMemcpyHostToDevice));
+        if (!r.empty()) HIPCHECK(hipMemcpy(mat_mu_r_.p, r.data(), r.size() * sizeof(float), hipMemcpyHostToDevice));
+    }
+}
+
+void Solver::get_bodies(float* q7, float* qd6, size_t cap) {
+    require_init();
+    if (!finalized_) finalize_meshes();
+    DeviceGuard g(device_);
+    const size_t M = meshes_.size(), nb = std::min(cap, boxes_.size());
+    std::vector<float4> q(2 * nb), qd(2 * nb);
+    if (nb) {
+        HIPCHECK(hipMemcpyAsync(q.data(), body_q_.p + 2 * M, 2 * nb * sizeof(float4), hipMemcpyDeviceToHost, stream_));
+        HIPCHECK(hipMemcpyAsync(qd.data(), body_qd_.p + 2 * M, 2 * nb * sizeof(float4), hipMemcpyDeviceToHost, stream_));
+    }
+    HIPCHECK(hipStreamSynchronize(stream_));
+    for (size_t k = 0; k < nb; k++) {
+        if (q7) {
+            const float v[7] = {q[2 * k].x, q[2 * k].y, q[2 * k].z, q[2 * k + 1].x, q[2 * k + 1].y, q[2 * k + 1].z, q[2 * k + 1].w};
+            std::memcpy(q7 + 7 * k, v, sizeof(v));
+        }
+        if (qd6) {
+            const float v[6] = {qd[2 * k].x, qd[2 * k].y, qd[2 * k].z, qd[2 * k + 1].x, qd[2 * k + 1].y, qd[2 * k + 1].z};
+            std::memcpy(qd6 + 6 * k, v, sizeof(v));
+        }
+    }
+}
+
+void Solver::set_body_velocities(const float* qd6, size_t n) {
+    require_init();
+    if (!finalized_) finalize_meshes();
+    if (n != boxes_.size()) throw std::runtime_error("set_body_velocities: body count mismatch");
+    DeviceGuard g(device_);
+    HIPCHECK(hipStreamSynchronize(stream_));
+    std::vector<float4> qd(2 * n);
+    for (size_t k = 0; k < n; k++) {
+        qd[2 * k] = make_float4(qd6[6 * k], qd6[6 * k + 1], qd6[6 * k + 2], 0.f);
+        qd[2 * k + 1] = make_float4(qd6[6 * k + 3], qd6[6 * k + 4], qd6[6 * k + 5], 0.f);
+    }
+    if (n) HIPCHECK(hipMemcpy(body_qd_.p + 2 * meshes_.size(), qd.data(), 2 * n * sizeof(float4), hipMemcpyHostToDevice));
 }
 
 void Solver::profile_reset() {

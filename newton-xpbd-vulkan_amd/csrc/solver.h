@@ -55,6 +55,13 @@ public:
     size_t bvh_nodes(size_t mi) const { return mi < bvh_.size() ? bvh_[mi].lo.size() : 0; }
     void bvh_half(size_t mi, bool upper, float* out, size_t cap) const;
 
+    // Dynamic box bodies (builder extension; the reference wires no body path).
+    void add_box_body(const float* pos, const float* quat, const float* half, float density, int flags);
+    void set_body_params(float angular_damping, float rigid_relax, float mu_t, float mu_r, int use_weights);
+    size_t body_count() const { return boxes_.size(); }
+    void get_bodies(float* q7, float* qd6, size_t cap);
+    void set_body_velocities(const float* qd6, size_t n);
+
     void print_profiler_summary();
     size_t profile_stages(char* names, size_t names_len, double* ms, uint64_t* launches, size_t cap);
     void profile_reset();
@@ -67,7 +74,14 @@ private:
         float rot[4] = {0.f, 0.f, 0.f, 1.f};
         bool watertight = false;
     };
-    enum Stage { kMesh, kIntegrate, kSortUp, kSortScan, kSortScatter, kRanges, kShape, kPP, kRest, kCopyOut, kNumStages };
+    struct BoxHost {
+        float pos[3], rot[4], half[3], density;
+        int flags;
+    };
+    enum Stage {
+        kMesh, kIntegrate, kSortUp, kSortScan, kSortScatter, kRanges, kShape, kPP, kRest, kCopyOut,
+        kBodyContacts, kBodyIntegrate, kBodySolve, kBodyApply, kNumStages
+    };
 
     void require_init() const;
     void alloc_all();
@@ -75,6 +89,7 @@ private:
     void upload_nodes();
     BodyArgs body_args() const;
     void substep(float sdt, int iterations, bool collide, int slot, bool zero_next);
+    BodyStepArgs body_step_args(float sdt);
     void run_sort(int tiles);
     void read_count(int slot);
     void stage_begin(Stage s);
@@ -94,7 +109,9 @@ private:
     std::vector<float> radius_h_, invm_h_;
     // grid and sort
     DevBuf<uint32_t> keysA_, valsA_, keysB_, valsB_, hist_, bintot_;
-    DevBuf<int32_t> range_;   // int2 per cell (start, end), never cleared
+    DevBuf<float4> Xs_[2], Vs_[2];   // predicted state in sorted order (ping-pong across iterations)
+    DevBuf<int32_t> Fs_;
+    DevBuf<int32_t> range_;   // int2 per cell (start, end), zero-filled every substep
     DevBuf<int32_t> tbl_;     // 2 x cells scratch for the dense getters
     // contacts: SoA of float4 payloads, counter slots
     DevBuf<int32_t> cpart_, cshape_, cc_;
@@ -103,9 +120,17 @@ private:
     int32_t* count_host_ = nullptr;   // pinned
     DevBuf<float> dshape_, delta_;
     // bodies (one static body per mesh, scene.cpp:220-238) and shapes
-    DevBuf<float4> body_q_, body_qd_;
-    DevBuf<float> body_com_, body_invI_, body_invm_, body_delta_, mat_mu_;
+    DevBuf<float4> body_q_, body_qd_, body_half_;
+    DevBuf<float> body_com_, body_I_, body_invI_, body_invm_, body_delta_, body_invw_, mat_mu_, mat_mu_t_, mat_mu_r_;
     DevBuf<int32_t> body_flags_, shape_body_;
+    std::vector<BoxHost> boxes_;
+    float angular_damping_ = 0.05f, rigid_relax_ = 0.8f, mu_t_ = 0.f, mu_r_ = 0.f;
+    int use_weights_ = 1;
+    // rigid contacts (box corners vs the ground plane)
+    DevBuf<int32_t> rc_s0_, rc_s1_;
+    DevBuf<float4> rc_p0_, rc_p1_, rc_n_;
+    DevBuf<float> rc_impulse_;
+    int rigid_slot_ = 0;
     // meshes
     std::vector<MeshHost> meshes_;
     std::vector<MeshBvhHost> bvh_;

@@ -396,5 +396,46 @@ PYBIND11_MODULE(_vkxpbd, m) {
                  return d;
              },
              "Per-stage accumulated GPU milliseconds and launch counts (enable_profile).")
-        .def("profile_reset", [](PySolver& s) { s.check(xpbd_profile_reset(s.h)); });
+        .def("profile_reset", [](PySolver& s) { s.check(xpbd_profile_reset(s.h)); })
+        .def(
+            "add_box_body",
+            [](PySolver& s, farr pos, farr quat, farr half_extents, float density, int flags) {
+                if (pos.size() < 3 || quat.size() < 4 || half_extents.size() < 3)
+                    throw std::runtime_error("add_box_body: pos[3], quat[4] (xyzw), half_extents[3] expected");
+                s.check(xpbd_add_box_body(s.h, pos.data(), quat.data(), half_extents.data(), density, flags));
+            },
+            py::arg("pos"), py::arg("quat"), py::arg("half_extents"), py::arg("density") = 1000.0f, py::arg("flags") = 0,
+            "Add a dynamic rigid box (extension: the reference wires no body path).")
+        .def(
+            "set_body_params",
+            [](PySolver& s, float ad, float relax, float mt, float mr, bool uw) {
+                s.check(xpbd_set_body_params(s.h, ad, relax, mt, mr, uw ? 1 : 0));
+            },
+            py::arg("angular_damping") = 0.05f, py::arg("rigid_contact_relaxation") = 0.8f,
+            py::arg("mu_torsional") = 0.0f, py::arg("mu_rolling") = 0.0f, py::arg("use_constraint_weights") = true)
+        .def_property_readonly("body_count",
+                               [](const PySolver& s) {
+                                   size_t v = 0;
+                                   xpbd_body_count(s.h, &v);
+                                   return v;
+                               })
+        .def("get_body_q",
+             [](PySolver& s) {
+                 size_t nb = 0;
+                 xpbd_body_count(s.h, &nb);
+                 py::array_t<float> r({nb, size_t(7)});
+                 s.check(xpbd_get_bodies(s.h, r.mutable_data(), nullptr, nb));
+                 return r;
+             })
+        .def("get_body_qd",
+             [](PySolver& s) {
+                 size_t nb = 0;
+                 xpbd_body_count(s.h, &nb);
+                 py::array_t<float> r({nb, size_t(6)});
+                 s.check(xpbd_get_bodies(s.h, nullptr, r.mutable_data(), nb));
+                 return r;
+             })
+        .def("set_body_velocities", [](PySolver& s, farr qd) {
+            s.check(xpbd_set_body_velocities(s.h, qd.data(), size_t(qd.size()) / 6));
+        });
 }

@@ -13,6 +13,13 @@ constexpr int kSortPasses = 3;
 constexpr int kSortItems = 4;
 constexpr int kSortTile = 256 * kSortItems;
 
+// Sorted-order copies of the predicted particle state, written by the last
+// radix pass (positions with radius, velocities with inverse mass, flags).
+struct SortGather {
+    const float4* P; const float4* V; const int* flags;
+    float4* Xs; float4* Vs; int* Fs;
+};
+
 struct BodyArgs {
     const float4* q;      // 2 per body: (pos, 0), (quat xyzw)
     const float4* qd;     // 2 per body: (lin, 0), (ang, 0)
@@ -32,7 +39,7 @@ struct IntegrateArgs {
     uint32_t* keys; uint32_t* hist;
     float dt, vmax, gx, gy, gz, inv_cell;
     int has_plane; float margin, altitude;
-    int cmax; int* cc; int* cc_next;
+    int cmax; int* cc; int* cc_next; int* rcc_next;
     int* cpart; int* cshape; float4* cbpos; float4* cbvel; float4* cnorm;
 };
 
@@ -54,10 +61,12 @@ struct ShapeArgs {
 
 struct PPArgs {
     int n;
-    const uint32_t* keys; const uint32_t* vals; const int* range;
-    const float4* Ps; const float4* Pp; const float4* Vp; const int* flags;
+    const uint32_t* vals; const int* range;
+    const float4* Ps;                               // substep-start positions, original order
+    const float4* Xs; const float4* Vs; const int* Fs;   // predicted state, sorted order
     float* dshape; float* delta;
-    float4* Po; float4* Vo;
+    float4* Po; float4* Vo;                         // output state, original order
+    float4* Xs_next; float4* Vs_next;               // output state, sorted order (when another iteration follows)
     float mu, coh, max_r, dt, relax, inv_cell, vmax;
     uint32_t iter_index, max_diag_pairs;
     uint32_t* diag_header; float* diag_pairs;
@@ -91,6 +100,48 @@ struct MeshArgs {
     uint32_t* err;
 };
 
+// Rigid bodies (dynamic bodies occupy indices [b0, b1) of the body arrays).
+struct BodyStepArgs {
+    int b0, b1;
+    float4* q; float4* qd;
+    const float* com; const float* I; const float* invI; const float* invm; const int* flags;
+    float* delta; float* invw;
+    float dt, gx, gy, gz, angular_damping;
+    int use_weights;
+};
+
+struct BodyContactGenArgs {
+    int nbox, b0, shape0;
+    const float4* q; const float4* half;
+    float margin, altitude;
+    int rcmax; int* rcc;
+    int* s0; int* s1; float4* p0; float4* p1; float4* n;
+};
+
+struct ParticleBoxArgs {
+    int n; const float4* P; const int* flags;
+    int nbox, b0, shape0;
+    const float4* q; const float4* half;
+    float margin;
+    int cmax; int* cc;
+    int* cpart; int* cshape; float4* cbpos; float4* cbvel; float4* cnorm;
+};
+
+struct BodySolveArgs {
+    int rcmax; const int* rcc;
+    const int* s0; const int* s1; const float4* p0; const float4* p1; const float4* n;
+    const int* shape_body; const float* mu; const float* mu_t; const float* mu_r;
+    const float4* q; const float4* qd; const float* com; const float* invI; const float* invm; const int* flags;
+    float* delta; float* invw; float* impulse;
+    float relax, dt;
+};
+
+void launch_integrate_bodies(const BodyStepArgs& a, hipStream_t s);
+void launch_apply_body_deltas(const BodyStepArgs& a, hipStream_t s);
+void launch_body_plane_contacts(const BodyContactGenArgs& a, hipStream_t s);
+void launch_particle_box_contacts(const ParticleBoxArgs& a, hipStream_t s);
+void launch_body_contact_solve(const BodySolveArgs& a, hipStream_t s);
+
 void launch_integrate(const IntegrateArgs& a, hipStream_t s);
 void launch_plane_contacts(const PlaneArgs& a, hipStream_t s);
 void launch_cell_ranges(int n, const uint32_t* keys, int* range, hipStream_t s);
@@ -99,12 +150,15 @@ void launch_shape_solve(const ShapeArgs& a, int grid, hipStream_t s);
 void launch_pp_apply(const PPArgs& a, bool diag, hipStream_t s);
 void launch_restitution(const RestArgs& a, int grid, hipStream_t s);
 void launch_zero_int(int* p, int count, hipStream_t s);
+void launch_frame_begin(int* cc, hipStream_t s);
 void launch_mesh_contacts(const MeshArgs& a, hipStream_t s);
 
 // Radix sort: one pass = upsweep (tile histograms) -> bin-row scan -> scatter.
 void launch_sort_upsweep(int n, const uint32_t* keys, int shift, uint32_t* hist, int tiles, hipStream_t s);
 void launch_sort_binscan(uint32_t* hist, uint32_t* bintot, int tiles, hipStream_t s);
 void launch_sort_scatter(int n, const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out, uint32_t* vals_out,
-                         int shift, const uint32_t* hist, const uint32_t* bintot, int tiles, hipStream_t s);
+                         int shift, const uint32_t* hist, const uint32_t* bintot, int tiles, const SortGather* gather,
+                         hipStream_t s);
+void launch_fill_zero(int* p, size_t count, hipStream_t s);
 
 }  // namespace xpbd

@@ -42,6 +42,8 @@
 //   * contact lists are emitted in (particle, shape) order; the reference's
 //     order is whatever its global atomic counter produced.
 #include <algorithm>
+#include <array>
+#include <type_traits>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -385,9 +387,30 @@ template <class R> struct Solver {
     std::vector<NodeHalf> nl, nu;   // packed, re-based
     std::vector<uint32_t> gprim, roots, prim_off, prim_cnt;
     float avg_edge = 0;
-    // bodies: one static body per mesh with identity pose (scene.cpp:220-238)
+    // bodies: one static body per mesh with identity pose (scene.cpp:220-238),
+    // then the dynamic boxes added with add_box_body (builder extension).
+    // Shapes: 0 = ground plane (body -1), 1..M = meshes, M+1.. = boxes.
     std::vector<int32_t> shape_body;
-    std::vector<R> mat_mu;
+    std::vector<R> mat_mu, mat_mu_t, mat_mu_r;
+    struct Box { float pos[3], rot[4], half[3], density; int32_t flags; };
+    std::vector<Box> boxes;
+    std::vector<V3<R>> b_pos, b_v, b_w, b_com, b_half;
+    std::vector<std::array<R, 4>> b_rot;
+    std::vector<std::array<R, 9>> b_I, b_invI;
+    std::vector<R> b_invm;
+    std::vector<int32_t> b_flags;
+    std::vector<V3<R>> b_pos0, b_v0, b_w0;   // state at substep start
+    std::vector<std::array<R, 4>> b_rot0;
+    std::vector<R> b_delta;                  // 8 per body: lin at +0..2, ang at +4..6
+    std::vector<R> b_invw;                   // rigid contacts touching each body
+    // rigid contacts (box corners vs ground plane)
+    int32_t rcc = 0;
+    std::vector<int32_t> rc_s0, rc_s1;
+    std::vector<V3<R>> rc_p0, rc_p1, rc_n;
+    std::vector<R> rc_impulse;
+    // body parameters (builder choices, settable)
+    R angular_damping = R(0.05f), rigid_relax = R(0.8f), mu_torsional = 0, mu_rolling = 0;
+    int32_t use_weights = 1;
 
     void init(const OrcConfig& c, int64_t np, int64_t mc) {
         cfg = c; n = np; cmax = mc ? mc : np * 2;
@@ -453,10 +476,37 @@ template <class R> struct Solver {
 
     void finalize_meshes() {   // scene.cpp:204-390
         if (!initialized) throw std::runtime_error("XpbdSolver::init must be called before finalize_meshes");
-        size_t M = meshes.size();
-        shape_body.assign(M + 1, -1);
-        for (size_t c = 0; c < M; c++) shape_body[c + 1] = int32_t(c);
-        mat_mu.assign(M + 1, R(cfg.shape_material_mu));
+        size_t M = meshes.size(), NB = boxes.size();
+        shape_body.assign(M + 1 + NB, -1);
+        for (size_t c = 0; c < M + NB; c++) shape_body[c + 1] = int32_t(c);
+        mat_mu.assign(M + 1 + NB, R(cfg.shape_material_mu));
+        mat_mu_t.assign(M + 1 + NB, mu_torsional);
+        mat_mu_r.assign(M + 1 + NB, mu_rolling);
+        const size_t B = M + NB;
+        b_pos.assign(B, {0, 0, 0}); b_v = b_w = b_com = b_half = b_pos;
+        b_rot.assign(B, {R(0), R(0), R(0), R(1)});
+        b_I.assign(B, std::array<R, 9>{}); b_invI = b_I;
+        b_invm.assign(B, R(0)); b_flags.assign(B, 0);
+        for (size_t k = 0; k < NB; k++) {   // solid box: m = rho*8abc, I = m/3*(b^2+c^2, ...)
+            const Box& bx = boxes[k];
+            size_t b = M + k;
+            b_pos[b] = {R(bx.pos[0]), R(bx.pos[1]), R(bx.pos[2])};
+            b_rot[b] = {R(bx.rot[0]), R(bx.rot[1]), R(bx.rot[2]), R(bx.rot[3])};
+            b_half[b] = {R(bx.half[0]), R(bx.half[1]), R(bx.half[2])};
+            b_flags[b] = bx.flags;
+            float hx = bx.half[0], hy = bx.half[1], hz = bx.half[2];
+            float m = bx.density * 8.0f * hx * hy * hz;
+            float ix = m / 3.0f * (hy * hy + hz * hz), iy = m / 3.0f * (hx * hx + hz * hz), iz = m / 3.0f * (hx * hx + hy * hy);
+            b_invm[b] = m > 0.0f ? R(1.0f / m) : R(0);
+            b_I[b] = {R(ix), R(0), R(0), R(0), R(iy), R(0), R(0), R(0), R(iz)};
+            b_invI[b] = {ix > 0 ? R(1.0f / ix) : R(0), R(0), R(0), R(0), iy > 0 ? R(1.0f / iy) : R(0), R(0), R(0), R(0),
+                         iz > 0 ? R(1.0f / iz) : R(0)};
+        }
+        b_delta.assign(8 * B, R(0));
+        b_invw.assign(B, R(0));
+        rc_s0.assign(8 * NB, 0); rc_s1 = rc_s0;
+        rc_p0.assign(8 * NB, {0, 0, 0}); rc_p1 = rc_n = rc_p0;
+        rc_impulse.assign(6 * 8 * NB, R(0));
         all_verts.clear(); all_tris.clear(); vert_off.clear(); tri_off.clear(); bvh.assign(M, Lbvh{});
         gprim.clear(); roots.clear(); prim_off.clear(); prim_cnt.clear();
         size_t nb = 0;
@@ -725,8 +775,20 @@ template <class R> struct Solver {
                 x.z * c + qr[2] * d - (qr[0] * x.y - qr[1] * x.x) * qr[3] * R(2)};
     }
 
-    // solve_particle_shape_contacts.comp:104-200.  Mesh bodies are static with
-    // identity pose, zero com, zero velocity, zero inverse mass/inertia.
+    // body frame of a shape's body; identity / zeros for body < 0 (the plane)
+    // (solve_particle_shape_contacts.comp:123-131, :137-141)
+    struct Frame { V3<R> pos; R rot[4]; V3<R> com, v, w; };
+    Frame frame(int body) const {
+        Frame F{{0, 0, 0}, {0, 0, 0, 1}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+        if (body >= 0) {
+            F.pos = b_pos[body];
+            for (int k = 0; k < 4; k++) F.rot[k] = b_rot[body][k];
+            F.com = b_com[body]; F.v = b_v[body]; F.w = b_w[body];
+        }
+        return F;
+    }
+
+    // solve_particle_shape_contacts.comp:104-200
     void shape_solve(const std::vector<V3<R>>& X, const std::vector<V3<R>>& Vv, R dt) {
         int32_t count = std::min(int32_t(cmax), cc);
         R relax = R(cfg.soft_contact_relaxation), ka = R(cfg.particle_adhesion), pmu = R(cfg.soft_contact_mu);
@@ -736,22 +798,21 @@ template <class R> struct Solver {
             int pi = cpart[t];
             int pf = flags[pi];
             if ((pf & 1) == 0) continue;
-            if (pf & 2) {   // PROXY: static mesh bodies have inverse mass 0
+            if (pf & 2) {
                 if (body < 0) continue;
-                continue;
+                if (b_flags[body] & 4) continue;
+                if (b_invm[body] == 0) continue;
             }
             V3<R> px = X[pi], pv = Vv[pi];
-            V3<R> Xp = {0, 0, 0}, Xc = {0, 0, 0};
-            R Xr[4] = {0, 0, 0, 1};
-            V3<R> bx = Xp + quat_rotate(Xr, cbpos[t]);
-            V3<R> r = bx - (Xp + quat_rotate(Xr, Xc));
+            Frame F = frame(body);
+            V3<R> bx = F.pos + quat_rotate(F.rot, cbpos[t]);
+            V3<R> r = bx - (F.pos + quat_rotate(F.rot, F.com));
             V3<R> nn = cnorm[t];
             V3<R> dp = px - bx;
             R c = fmaR(nn.z, dp.z, fmaR(nn.x, dp.x, nn.y * dp.y)) - radius[pi];
             if (c > ka) continue;
             R mu = R(0.5) * (pmu + mat_mu[s]);
-            V3<R> bw = {0, 0, 0}, bvl = {0, 0, 0};
-            V3<R> bv = bvl + cross(bw, r) + quat_rotate(Xr, cbvel[t]);
+            V3<R> bv = F.v + cross(F.w, r) + quat_rotate(F.rot, cbvel[t]);
             V3<R> v = pv - bv;
             R lambda_n = c;
             V3<R> delta_n = nn * lambda_n;
@@ -763,10 +824,11 @@ template <class R> struct Solver {
             R w2 = 0;
             if (body >= 0) {
                 V3<R> ang = cross(r, nn);
-                V3<R> ra = quat_rotate_inv(Xr, ang);
-                V3<R> Ira = {R(0) * ra.x + R(0) * ra.y + R(0) * ra.z, R(0) * ra.x + R(0) * ra.y + R(0) * ra.z,
-                             R(0) * ra.x + R(0) * ra.y + R(0) * ra.z};
-                w2 = R(0) + dot(ra, Ira);
+                V3<R> ra = quat_rotate_inv(F.rot, ang);
+                const std::array<R, 9>& I = b_invI[body];   // row-major read, as this shader does
+                V3<R> Ira = {I[0] * ra.x + I[1] * ra.y + I[2] * ra.z, I[3] * ra.x + I[4] * ra.y + I[5] * ra.z,
+                             I[6] * ra.x + I[7] * ra.y + I[8] * ra.z};
+                w2 = b_invm[body] + dot(ra, Ira);
             }
             R denom = w1 + w2;
             if (denom == 0) continue;
@@ -783,6 +845,275 @@ template <class R> struct Solver {
             delta[pi * 3 + 0] += fd.x;
             delta[pi * 3 + 1] += fd.y;
             delta[pi * 3 + 2] += fd.z;
+            if (body >= 0 && b_invm[body] > 0) {
+                V3<R> dv = {dtot.x / dt, dtot.y / dt, dtot.z / dt};
+                V3<R> dw = cross(r, dv);
+                R* bd = &b_delta[8 * body];
+                bd[0] += -dv.x; bd[1] += -dv.y; bd[2] += -dv.z;
+                bd[4] += -dw.x; bd[5] += -dw.y; bd[6] += -dw.z;
+            }
+        }
+    }
+
+    // ---- rigid bodies (integrate_bodies.comp, solve_body_contact_positions.comp,
+    //      apply_body_deltas.comp); the reference ships these unwired ----
+    static V3<R> qrot(const std::array<R, 4>& q, V3<R> v) {   // integrate_bodies.comp:62-65
+        V3<R> u = {q[0], q[1], q[2]};
+        return v + R(2) * cross(u, cross(u, v) + v * q[3]);
+    }
+    static V3<R> qrot_inv(const std::array<R, 4>& q, V3<R> v) { return qrot({-q[0], -q[1], -q[2], q[3]}, v); }
+    static std::array<R, 4> qmul(const std::array<R, 4>& p, const std::array<R, 4>& q) {   // :55-60
+        V3<R> pu = {p[0], p[1], p[2]}, qu = {q[0], q[1], q[2]};
+        R w = p[3] * q[3] - dot(pu, qu);
+        V3<R> xyz = p[3] * qu + q[3] * pu + cross(pu, qu);
+        return {xyz.x, xyz.y, xyz.z, w};
+    }
+    static std::array<R, 4> qnormalize(const std::array<R, 4>& q) {
+        R d = ((q[0] * q[0] + q[1] * q[1]) + q[2] * q[2]) + q[3] * q[3];
+        R inv = R(1) / std::sqrt(d);
+        return {q[0] * inv, q[1] * inv, q[2] * inv, q[3] * inv};
+    }
+    static V3<R> mcol(const std::array<R, 9>& m, V3<R> v) {   // column-major 3x3 times v
+        return V3<R>{m[0], m[1], m[2]} * v.x + V3<R>{m[3], m[4], m[5]} * v.y + V3<R>{m[6], m[7], m[8]} * v.z;
+    }
+    static V3<R> tf_point(V3<R> pos, const std::array<R, 4>& q, V3<R> p) {   // solve_body_contact_positions.comp:58-62
+        V3<R> u = {q[0], q[1], q[2]};
+        return p + R(2) * cross(u, cross(u, p) + p * q[3]) + pos;
+    }
+    static V3<R> qrot_inv_b(const std::array<R, 4>& q, V3<R> v) {   // :64-67
+        V3<R> nq = {-q[0], -q[1], -q[2]};
+        return v + R(2) * cross(nq, cross(nq, v) + v * q[3]);
+    }
+
+    void integrate_bodies(R dt) {   // integrate_bodies.comp:73-130 (force, torque, world index zero)
+        V3<R> g = {R(cfg.gravity[0]), R(cfg.gravity[1]), R(cfg.gravity[2])};
+        for (size_t b = meshes.size(); b < b_pos.size(); b++) {
+            if (b_flags[b] & 2) continue;   // KINEMATIC: pass through
+            V3<R> x0 = b_pos[b], v0 = b_v[b], w0 = b_w[b], com = b_com[b];
+            std::array<R, 4> r0 = b_rot[b];
+            R im = b_invm[b];
+            V3<R> f0 = {0, 0, 0}, t0 = {0, 0, 0};
+            V3<R> xc = x0 + qrot(r0, com);
+            R ms = im != 0 ? R(1) : R(0);
+            V3<R> v1 = v0 + (f0 * im + g * ms) * dt;
+            V3<R> x1 = xc + v1 * dt;
+            V3<R> wb = qrot_inv(r0, w0);
+            V3<R> Iwb = mcol(b_I[b], wb);
+            V3<R> tb = qrot_inv(r0, t0) - cross(wb, Iwb);
+            V3<R> dwb = mcol(b_invI[b], tb) * dt;
+            V3<R> w1 = qrot(r0, wb + dwb);
+            std::array<R, 4> wq = qmul({w1.x, w1.y, w1.z, R(0)}, r0);
+            R hdt = R(0.5) * dt;
+            std::array<R, 4> r1 = qnormalize({r0[0] + wq[0] * hdt, r0[1] + wq[1] * hdt, r0[2] + wq[2] * hdt, r0[3] + wq[3] * hdt});
+            w1 = w1 * (R(1) - angular_damping * dt);
+            b_pos[b] = x1 - qrot(r1, com);
+            b_rot[b] = r1;
+            b_v[b] = v1;
+            b_w[b] = w1;
+        }
+    }
+
+    // Builder-written rigid contact generator: the 8 corners of every box
+    // against the ground plane; shape0 = box, shape1 = plane (shape 0, body
+    // -1), normal from shape0 to shape1 = (0,0,-1), point0 in the box frame,
+    // point1 in world.
+    void body_plane_contacts() {
+        rcc = 0;
+        if (!cfg.has_ground_plane) return;
+        R alt = R(cfg.ground_plane_altitude), margin = R(cfg.soft_contact_margin);
+        size_t M = meshes.size();
+        for (size_t k = 0; k < boxes.size(); k++) {
+            size_t b = M + k;
+            for (int c = 0; c < 8; c++) {
+                V3<R> h = b_half[b];
+                V3<R> lc = {(c & 1) ? h.x : -h.x, (c & 2) ? h.y : -h.y, (c & 4) ? h.z : -h.z};
+                V3<R> wc = tf_point(b_pos[b], b_rot[b], lc);
+                if (!(wc.z - alt < margin)) continue;
+                int32_t slot = rcc++;
+                rc_s0[slot] = int32_t(M + 1 + k);
+                rc_s1[slot] = 0;
+                rc_p0[slot] = lc;
+                rc_p1[slot] = {wc.x, wc.y, alt};
+                rc_n[slot] = {0, 0, -1};
+            }
+        }
+    }
+
+    // Builder-written particle-box contact generator: closest point of the
+    // oriented box to the particle centre; contact when the signed distance is
+    // below soft_contact_margin + radius.  body_pos in the box frame, normal in
+    // world from the box surface towards the particle, body_vel 0.
+    void particle_box_contacts() {
+        if (boxes.empty()) return;
+        R margin = R(cfg.soft_contact_margin);
+        size_t M = meshes.size();
+        for (int64_t i = 0; i < n; i++) {
+            if ((flags[i] & 1) == 0) continue;
+            V3<R> p = q[i];
+            for (size_t k = 0; k < boxes.size(); k++) {
+                size_t b = M + k;
+                V3<R> h = b_half[b];
+                V3<R> rel = p - b_pos[b];
+                R reach = (h.x + h.y + h.z) + margin + radius[i];
+                if (dot(rel, rel) > reach * reach) continue;
+                V3<R> l = qrot_inv(b_rot[b], rel);
+                V3<R> cp = {gmin(gmax(l.x, -h.x), h.x), gmin(gmax(l.y, -h.y), h.y), gmin(gmax(l.z, -h.z), h.z)};
+                V3<R> nl;
+                R sd;
+                bool inside = std::fabs(l.x) < h.x && std::fabs(l.y) < h.y && std::fabs(l.z) < h.z;
+                if (!inside) {
+                    V3<R> d = l - cp;
+                    sd = length(d);
+                    if (!(sd > 0)) continue;
+                    nl = d * (R(1) / sd);
+                } else {
+                    R gx = h.x - std::fabs(l.x), gy = h.y - std::fabs(l.y), gz = h.z - std::fabs(l.z);
+                    int ax = (gx <= gy && gx <= gz) ? 0 : (gy <= gz ? 1 : 2);
+                    R sg = comp(l, ax) < 0 ? R(-1) : R(1);
+                    nl = {ax == 0 ? sg : R(0), ax == 1 ? sg : R(0), ax == 2 ? sg : R(0)};
+                    if (ax == 0) { cp.x = sg * h.x; sd = -gx; }
+                    else if (ax == 1) { cp.y = sg * h.y; sd = -gy; }
+                    else { cp.z = sg * h.z; sd = -gz; }
+                }
+                if (!(sd < margin + radius[i])) continue;
+                emit(int32_t(i), int32_t(M + 1 + k), cp, {0, 0, 0}, qrot(b_rot[b], nl));
+            }
+        }
+    }
+
+    // solve_body_contact_positions.comp:70-85
+    R ccd(R err, const std::array<R, 4>& qa, const std::array<R, 4>& qb, R mia, R mib, const std::array<R, 9>& Ia,
+          const std::array<R, 9>& Ib, V3<R> la, V3<R> lb, V3<R> aa, V3<R> ab, R relax, R dt) const {
+        R den = 0;
+        den += dot(la, la) * mia;
+        den += dot(lb, lb) * mib;
+        V3<R> ra = qrot_inv_b(qa, aa), rb = qrot_inv_b(qb, ab);
+        den += dot(ra, mcol(Ia, ra));
+        den += dot(rb, mcol(Ib, rb));
+        R dl = -err;
+        if (den > 0) dl /= dt * den;
+        return dl * relax;
+    }
+
+    // solve_body_contact_positions.comp:97-279 (offsets and thicknesses zero)
+    void body_contact_solve(R dt) {
+        const std::array<R, 4> ident = {0, 0, 0, 1};
+        const std::array<R, 9> zero9{};
+        for (int32_t ci = 0; ci < rcc; ci++) {
+            int sa = rc_s0[ci], sb = rc_s1[ci];
+            if (sa == sb) continue;
+            int ba = sa >= 0 ? shape_body[sa] : -1, bb = sb >= 0 ? shape_body[sb] : -1;
+            if (ba == bb) continue;
+            V3<R> pa = {0, 0, 0}, pb = {0, 0, 0};
+            std::array<R, 4> qa = ident, qb = ident;
+            if (ba >= 0) { pa = b_pos[ba]; qa = b_rot[ba]; }
+            if (bb >= 0) { pb = b_pos[bb]; qb = b_rot[bb]; }
+            V3<R> p0 = rc_p0[ci], p1 = rc_p1[ci];
+            V3<R> bxa = tf_point(pa, qa, p0), bxb = tf_point(pb, qb, p1);
+            V3<R> nn = rc_n[ci];
+            R d = dot(nn, bxb - bxa) - (R(0) + R(0));
+            if (d >= 0) continue;
+            R mia = 0, mib = 0;
+            std::array<R, 9> Ia = zero9, Ib = zero9;
+            V3<R> ca = {0, 0, 0}, cb = {0, 0, 0}, wa = {0, 0, 0}, wb = {0, 0, 0};
+            V3<R> oa = {0, 0, 0}, ob = {0, 0, 0};
+            if (ba >= 0) { mia = b_invm[ba]; Ia = b_invI[ba]; ca = b_com[ba]; wa = b_w[ba]; }
+            if (bb >= 0) { mib = b_invm[bb]; Ib = b_invI[bb]; cb = b_com[bb]; wb = b_w[bb]; }
+            int nz = 0;
+            R mu = 0, mut = 0, mur = 0;
+            if (sa >= 0) { nz += 1; mu += mat_mu[sa]; mut += mat_mu_t[sa]; mur += mat_mu_r[sa]; }
+            if (sb >= 0) { nz += 1; mu += mat_mu[sb]; mut += mat_mu_t[sb]; mur += mat_mu_r[sb]; }
+            if (nz > 0) { mu /= R(nz); mut /= R(nz); mur /= R(nz); }
+            V3<R> ra = bxa - tf_point(pa, qa, ca), rb = bxb - tf_point(pb, qb, cb);
+            V3<R> anga = -cross(ra, nn), angb = cross(rb, nn);
+            if (ba >= 0) b_invw[ba] += R(1);
+            if (bb >= 0) b_invw[bb] += R(1);
+            R ln = ccd(d, qa, qb, mia, mib, Ia, Ib, -nn, nn, anga, angb, rigid_relax, dt);
+            V3<R> lda = -nn * ln, ldb = nn * ln, ada = anga * ln, adb = angb * ln;
+            if (mu > 0) {
+                V3<R> sxa = tf_point(pa, qa, p0 + oa), sxb = tf_point(pb, qb, p1 + ob);
+                V3<R> dl = sxb - sxa;
+                V3<R> fd = dl - dot(nn, dl) * nn;
+                V3<R> fra = sxa - tf_point(pa, qa, ca), frb = sxb - tf_point(pb, qb, cb);
+                V3<R> rv = {0, 0, 0};
+                if (ba >= 0 && (b_flags[ba] & 2)) {
+                    V3<R> va = b_v[ba] + cross(wa, fra);
+                    rv = rv - (va - dot(nn, va) * nn);
+                }
+                if (bb >= 0 && (b_flags[bb] & 2)) {
+                    V3<R> vb = b_v[bb] + cross(wb, frb);
+                    rv = rv + (vb - dot(nn, vb) * nn);
+                }
+                fd = fd + rv * dt;
+                V3<R> perp = normalize(fd);
+                V3<R> afa = -cross(fra, perp), afb = cross(frb, perp);
+                R err = length(fd);
+                if (err > 0) {
+                    R lf = ccd(err, qa, qb, mia, mib, Ia, Ib, -perp, perp, afa, afb, rigid_relax, dt);
+                    lf = gmax(lf, -ln * mu);
+                    lda = lda - perp * lf; ldb = ldb + perp * lf;
+                    ada = ada + afa * lf; adb = adb + afb * lf;
+                }
+            }
+            V3<R> dom = wb - wa;
+            if (mut > 0) {
+                R err = dot(dom, nn) * dt;
+                if (std::fabs(err) > 0) {
+                    R lt = ccd(err, qa, qb, mia, mib, Ia, Ib, {0, 0, 0}, {0, 0, 0}, -nn, nn, rigid_relax, dt);
+                    lt = gmin(gmax(lt, -ln * mut), ln * mut);
+                    ada = ada - nn * lt; adb = adb + nn * lt;
+                }
+            }
+            if (mur > 0) {
+                dom = dom - dot(nn, dom) * nn;
+                R err = length(dom) * dt;
+                if (err > 0) {
+                    V3<R> rn = normalize(dom);
+                    R lr = ccd(err, qa, qb, mia, mib, Ia, Ib, {0, 0, 0}, {0, 0, 0}, -rn, rn, rigid_relax, dt);
+                    lr = gmax(lr, -ln * mur);
+                    ada = ada - rn * lr; adb = adb + rn * lr;
+                }
+            }
+            if (ba >= 0) {
+                R* d8 = &b_delta[8 * ba];
+                d8[0] += lda.x; d8[1] += lda.y; d8[2] += lda.z; d8[4] += ada.x; d8[5] += ada.y; d8[6] += ada.z;
+            }
+            if (bb >= 0) {
+                R* d8 = &b_delta[8 * bb];
+                d8[0] += ldb.x; d8[1] += ldb.y; d8[2] += ldb.z; d8[4] += adb.x; d8[5] += adb.y; d8[6] += adb.z;
+            }
+            R* imp = &rc_impulse[6 * ci];
+            imp[0] += lda.x; imp[1] += lda.y; imp[2] += lda.z; imp[3] += ada.x; imp[4] += ada.y; imp[5] += ada.z;
+        }
+    }
+
+    void apply_body_deltas(R dt) {   // apply_body_deltas.comp:58-119
+        for (size_t b = meshes.size(); b < b_pos.size(); b++) {
+            R im = b_invm[b];
+            if (im == 0) continue;
+            V3<R> p0 = b_pos[b], v0 = b_v[b], w0 = b_w[b], com = b_com[b];
+            std::array<R, 4> q0 = b_rot[b];
+            V3<R> dp = {b_delta[8 * b], b_delta[8 * b + 1], b_delta[8 * b + 2]};
+            V3<R> dq = {b_delta[8 * b + 4], b_delta[8 * b + 5], b_delta[8 * b + 6]};
+            R wgt = 1;
+            if (use_weights && b_invw[b] > 0) wgt = R(1) / b_invw[b];
+            dp = dp * (im * wgt);
+            V3<R> dqw = dq * wgt;
+            V3<R> wb = qrot_inv(q0, w0);
+            V3<R> dwb = mcol(b_invI[b], qrot_inv(q0, dqw));
+            V3<R> tb = cross(dwb, mcol(b_I[b], wb + dwb)) + cross(wb, mcol(b_I[b], dwb));
+            V3<R> dw1 = qrot(q0, dwb - dt * mcol(b_invI[b], tb));
+            V3<R> dwdt = dw1 * dt;
+            std::array<R, 4> wq = qmul({dwdt.x, dwdt.y, dwdt.z, R(0)}, q0);
+            std::array<R, 4> q1 = qnormalize({q0[0] + wq[0] * R(0.5), q0[1] + wq[1] * R(0.5), q0[2] + wq[2] * R(0.5),
+                                              q0[3] + wq[3] * R(0.5)});
+            V3<R> xc = p0 + qrot(q0, com);
+            V3<R> p1 = xc + dp * dt;
+            p1 = p1 - qrot(q1, com);
+            V3<R> v1 = v0 + dp, w1 = w0 + dw1;
+            if (length(v1) < R(1e-4f)) v1 = {0, 0, 0};
+            if (length(w1) < R(1e-4f)) w1 = {0, 0, 0};
+            b_pos[b] = p1; b_rot[b] = q1; b_v[b] = v1; b_w[b] = w1;
         }
     }
 
@@ -868,31 +1199,30 @@ template <class R> struct Solver {
         }
     }
 
-    // apply_particle_shape_restitution.comp:48-116; the reference reads and
-    // atomically updates the same qd buffer, here sequential in contact order.
+    // apply_particle_shape_restitution.comp:48-116.  The reference reads and
+    // atomically updates the same qd buffer; here sequential in contact order.
+    // Previous body pose / velocity are the current ones (scene.cpp:570-574
+    // binds the same buffers twice).
     void restitution() {
         int32_t count = std::min(int32_t(cmax), cc);
         R ka = R(cfg.particle_adhesion), rest = R(cfg.soft_contact_restitution);
-        R Xr[4] = {0, 0, 0, 1};
         for (int32_t t = 0; t < count; t++) {
             int s = cshape[t], body = shape_body[s], pi = cpart[t];
-            (void)body;
             if ((flags[pi] & 1) == 0) continue;
             V3<R> vnew = qd[pi], px = q_init[pi], vold = qd_init[pi];
-            V3<R> Xp = {0, 0, 0}, Xc = {0, 0, 0};
-            V3<R> bx = Xp + quat_rotate(Xr, cbpos[t]);
+            Frame F = frame(body);
+            V3<R> bx = F.pos + quat_rotate(F.rot, cbpos[t]);
             V3<R> nn = cnorm[t];
             V3<R> dp = px - bx;
             R c = fmaR(nn.z, dp.z, fmaR(nn.x, dp.x, nn.y * dp.y)) - radius[pi];
             if (c > ka) continue;
-            V3<R> bxp = Xp + quat_rotate(Xr, cbpos[t]);
-            V3<R> r = bxp - (Xp + quat_rotate(Xr, Xc));
-            V3<R> bvc = quat_rotate(Xr, cbvel[t]);
+            V3<R> bxp = F.pos + quat_rotate(F.rot, cbpos[t]);
+            V3<R> r = bxp - (F.pos + quat_rotate(F.rot, F.com));
+            V3<R> bvc = quat_rotate(F.rot, cbvel[t]);
             V3<R> bvo = bvc, bvn = bvc;
             if (body >= 0) {
-                V3<R> z = {0, 0, 0};
-                bvo = (z + cross(z, r)) + bvc;
-                bvn = (z + cross(z, r)) + bvc;
+                bvo = (F.v + cross(F.w, r)) + bvc;
+                bvn = (F.v + cross(F.w, r)) + bvc;
             }
             V3<R> dvo = vold - bvo;
             R rvo = fmaR(nn.z, dvo.z, fmaR(nn.y, dvo.y, nn.x * dvo.x));
@@ -906,6 +1236,8 @@ template <class R> struct Solver {
         }
     }
 
+    bool has_dynamic_bodies() const { return !boxes.empty(); }
+
     void iterations_and_finish(R sdt, int iters, bool pingpong) {
         for (int it = 0; it < iters; it++) {
             bool flip = pingpong && (it % 2 != 0);
@@ -914,23 +1246,43 @@ template <class R> struct Solver {
             std::vector<V3<R>>& Xo = flip ? q : q_out;
             std::vector<V3<R>>& Vo = flip ? qd : qd_out;
             std::fill(delta.begin(), delta.end(), R(0));
+            std::fill(b_delta.begin(), b_delta.end(), R(0));
+            std::fill(b_invw.begin(), b_invw.end(), R(0));
             shape_solve(X, Vv, sdt);
             pp_solve(X, Vv, sdt);
             apply(X, Vv, Xo, Vo, sdt);
             if (!pingpong) { q = q_out; qd = qd_out; }
+            if (has_dynamic_bodies()) {
+                body_contact_solve(sdt);
+                apply_body_deltas(sdt);
+            }
         }
         if (pingpong && iters % 2 != 0) { q = q_out; qd = qd_out; }
         if (cfg.enable_restitution) restitution();
     }
 
-    void step(float dt, int substeps, int iters) {   // scene.cpp:698-1092
+    void generate_contacts(bool rigid) {
+        cc = 0;
+        if (cfg.has_ground_plane) plane_contacts();
+        mesh_contacts();
+        particle_box_contacts();
+        if (rigid) {
+            body_plane_contacts();
+            std::fill(rc_impulse.begin(), rc_impulse.end(), R(0));
+        }
+    }
+
+    // scene.cpp:698-1092, extended with the rigid-body stages (builder order:
+    // rigid contacts are generated with the particle contacts; bodies are
+    // integrated after the particles; each iteration solves rigid contacts and
+    // applies body deltas after the particle apply).
+    void step(float dt, int substeps, int iters) {
         if (!finalized) finalize_meshes();
         R sdt = R(dt / float(substeps));
         for (int sub = 0; sub < substeps; sub++) {
-            cc = 0;
-            if (cfg.has_ground_plane) plane_contacts();
-            mesh_contacts();
+            generate_contacts(true);
             integrate(sdt);
+            integrate_bodies(sdt);
             grid_and_sort();
             iterations_and_finish(sdt, iters, true);
         }
@@ -942,6 +1294,7 @@ template <class R> struct Solver {
         R sdt = R(dt / float(substeps));
         for (int sub = 0; sub < substeps; sub++) {
             integrate(sdt);
+            integrate_bodies(sdt);
             grid_and_sort();
             iterations_and_finish(sdt, iters, false);
         }
@@ -949,9 +1302,7 @@ template <class R> struct Solver {
 
     void collide_only() {   // scene.cpp:1229-1245
         if (!finalized) finalize_meshes();
-        cc = 0;
-        if (cfg.has_ground_plane) plane_contacts();
-        mesh_contacts();
+        generate_contacts(false);
         last_count = std::min<int32_t>(cc, int32_t(cmax));
     }
 };
@@ -1074,6 +1425,64 @@ void orc_get_mesh_bvh(void* p, int64_t mi, float* lowers, float* uppers, uint32_
     std::memcpy(uppers, b.uppers.data(), 16 * b.uppers.size());
     std::memcpy(prim, b.prim.data(), 4 * b.prim.size());
     *root = b.root;
+}
+int orc_add_box_body(void* p, const float* pos, const float* rot, const float* half, float density, int32_t flags) {
+    Handle* h = static_cast<Handle*>(p);
+    auto go = [&](auto& s) {
+        typename std::remove_reference<decltype(s)>::type::Box b;
+        for (int k = 0; k < 3; k++) { b.pos[k] = pos[k]; b.half[k] = half[k]; }
+        for (int k = 0; k < 4; k++) b.rot[k] = rot[k];
+        b.density = density; b.flags = flags;
+        s.boxes.push_back(b);
+        s.finalized = false;
+    };
+    if (h->fp64) go(h->d); else go(h->f);
+    return 0;
+}
+void orc_set_body_params(void* p, float angular_damping, float rigid_relax, float mu_t, float mu_r, int use_weights) {
+    Handle* h = static_cast<Handle*>(p);
+    auto go = [&](auto& s) {
+        using RR = decltype(s.rigid_relax);
+        s.angular_damping = RR(angular_damping); s.rigid_relax = RR(rigid_relax);
+        s.mu_torsional = RR(mu_t); s.mu_rolling = RR(mu_r); s.use_weights = use_weights;
+        for (auto& x : s.mat_mu_t) x = RR(mu_t);
+        for (auto& x : s.mat_mu_r) x = RR(mu_r);
+    };
+    if (h->fp64) go(h->d); else go(h->f);
+}
+// dynamic bodies only (boxes), in add order: q = 7 (pos, quat xyzw), qd = 6 (v, w)
+int64_t orc_get_bodies(void* p, double* q7, double* qd6) {
+    Handle* h = static_cast<Handle*>(p);
+    auto go = [&](auto& s) -> int64_t {
+        size_t M = s.meshes.size();
+        int64_t nb = int64_t(s.boxes.size());
+        if (s.b_pos.size() < M + s.boxes.size()) return 0;
+        for (int64_t k = 0; k < nb; k++) {
+            size_t b = M + size_t(k);
+            if (q7) {
+                q7[7 * k] = s.b_pos[b].x; q7[7 * k + 1] = s.b_pos[b].y; q7[7 * k + 2] = s.b_pos[b].z;
+                for (int c = 0; c < 4; c++) q7[7 * k + 3 + c] = s.b_rot[b][c];
+            }
+            if (qd6) {
+                qd6[6 * k] = s.b_v[b].x; qd6[6 * k + 1] = s.b_v[b].y; qd6[6 * k + 2] = s.b_v[b].z;
+                qd6[6 * k + 3] = s.b_w[b].x; qd6[6 * k + 4] = s.b_w[b].y; qd6[6 * k + 5] = s.b_w[b].z;
+            }
+        }
+        return nb;
+    };
+    return h->fp64 ? go(h->d) : go(h->f);
+}
+void orc_set_body_velocities(void* p, const float* qd6) {
+    Handle* h = static_cast<Handle*>(p);
+    auto go = [&](auto& s) {
+        using RR = decltype(s.rigid_relax);
+        size_t M = s.meshes.size();
+        for (size_t k = 0; k < s.boxes.size(); k++) {
+            s.b_v[M + k] = {RR(qd6[6 * k]), RR(qd6[6 * k + 1]), RR(qd6[6 * k + 2])};
+            s.b_w[M + k] = {RR(qd6[6 * k + 3]), RR(qd6[6 * k + 4]), RR(qd6[6 * k + 5])};
+        }
+    };
+    if (h->fp64) go(h->d); else go(h->f);
 }
 int orc_mesh_watertight(void* p, int64_t mi) { Handle* h = static_cast<Handle*>(p); return h->f.meshes[mi].watertight ? 1 : 0; }
 

@@ -67,6 +67,11 @@ def lib():
         L.orc_mesh_node_count.argtypes = [C.c_void_p, C.c_int64]
         L.orc_get_mesh_bvh.argtypes = [C.c_void_p, C.c_int64, _fp, _fp, _up, _up]
         L.orc_mesh_watertight.argtypes = [C.c_void_p, C.c_int64]
+        L.orc_add_box_body.argtypes = [C.c_void_p, _fp, _fp, _fp, C.c_float, C.c_int32]
+        L.orc_set_body_params.argtypes = [C.c_void_p, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int]
+        L.orc_get_bodies.restype = C.c_int64
+        L.orc_get_bodies.argtypes = [C.c_void_p, _dp, _dp]
+        L.orc_set_body_velocities.argtypes = [C.c_void_p, _fp]
         L.orc_lbvh_build.argtypes = [_fp, _up, C.c_uint32, _fp, _fp, _up, _up]
         L.orc_ray_tri_woop_f.argtypes = [_fp] * 5
         L.orc_closest_point_triangle_f.argtypes = [_fp] * 6
@@ -121,6 +126,32 @@ class OracleSolver:
         v = np.ascontiguousarray(verts, np.float32).reshape(-1)
         w = None if vels is None else np.ascontiguousarray(vels, np.float32).reshape(-1)
         self._ck(self.L.orc_update_mesh(self.h, idx, _ptr(v, _fp), v.size // 3, _ptr(w, _fp)))
+
+    def add_box_body(self, pos, quat, half_extents, density=1000.0, flags=0):
+        a = [np.ascontiguousarray(x, np.float32) for x in (pos, quat, half_extents)]
+        self.nbodies = getattr(self, "nbodies", 0) + 1
+        self.L.orc_add_box_body(self.h, *[_ptr(x, _fp) for x in a], density, flags)
+
+    def set_body_params(self, angular_damping=0.05, rigid_contact_relaxation=0.8, mu_torsional=0.0, mu_rolling=0.0,
+                        use_constraint_weights=True):
+        self.L.orc_set_body_params(self.h, angular_damping, rigid_contact_relaxation, mu_torsional, mu_rolling,
+                                   int(use_constraint_weights))
+
+    def get_body_q(self):
+        nb = getattr(self, "nbodies", 0)
+        q = np.zeros((nb, 7), np.float64)
+        self.L.orc_get_bodies(self.h, _ptr(q, _dp), None)
+        return q
+
+    def get_body_qd(self):
+        nb = getattr(self, "nbodies", 0)
+        qd = np.zeros((nb, 6), np.float64)
+        self.L.orc_get_bodies(self.h, None, _ptr(qd, _dp))
+        return qd
+
+    def set_body_velocities(self, qd):
+        a = np.ascontiguousarray(qd, np.float32).reshape(-1)
+        self.L.orc_set_body_velocities(self.h, _ptr(a, _fp))
 
     def set_particles(self, q, qd=None, inv_mass=None, radii=None, flags=None):
         q = np.ascontiguousarray(q, np.float32).reshape(-1)
