@@ -200,6 +200,33 @@ int xpbd_body_count(const xpbd_solver* h, size_t* out);
 int xpbd_get_bodies(xpbd_solver* h, float* q7, float* qd6, size_t cap);
 int xpbd_set_body_velocities(xpbd_solver* h, const float* qd6, size_t n);
 
+/* Mesh narrowphase counters accumulated while profiling (enable_profile or
+ * VKXPBD_PROFILE=1) since the last xpbd_profile_reset: out[0] active
+ * particle x mesh pairs, [1] root-box passes, [2] BVH node pops, [3]
+ * closest-point triangle tests, [4] parity ray/triangle tests, [5] contacts.
+ * Extension (the reference's mesh diagnostics buffer is never written). */
+int xpbd_mesh_query_stats(xpbd_solver* h, uint64_t* out6);
+
+/* Batched scenes: num_scenes independent particle scenes (counts[k] particles
+ * each, concatenated in scene order in every particle array) advanced together
+ * with one launch per stage.  Each scene has its own hash grid (the sort key
+ * carries the scene index above the 21 cell bits) and sees the ground plane
+ * and all meshes/boxes.  At most 2048 scenes.  Extension without a reference
+ * counterpart: the reference steps one scene per XpbdSolver
+ * (src/scene.cpp:65-152 is the single-scene init it generalises). */
+int xpbd_init_batch(xpbd_solver* h, const xpbd_config* cfg, const uint64_t* counts, size_t num_scenes,
+                    size_t max_contacts);
+int xpbd_scene_count(const xpbd_solver* h, size_t* out);
+/* Frame capture: when on (default; environment VKXPBD_GRAPHS=0 turns it off),
+ * xpbd_step records each distinct frame once as a hipGraph and replays it.
+ * Results are identical either way.  No reference counterpart (the reference
+ * re-records its command buffer every frame, src/scene.cpp:805-992). */
+int xpbd_set_graph_mode(xpbd_solver* h, int on);
+/* Times the candidate particle-particle block sizes over `frames` frames of
+ * the current state, restores the state, keeps the fastest and returns it in
+ * *block (may be NULL).  Results do not depend on the choice. */
+int xpbd_autotune(xpbd_solver* h, int frames, float dt, int substeps, int iterations, int* block);
+
 /* Host-only LBVH build of one mesh, the builder finalize_meshes uses
  * (MeshBvh::build, src/mesh_bvh.h:40-69 -> src/mesh_lbvh.h:49-223).  Needs no
  * GPU.  lowers/uppers receive 2*num_tris packed nodes (4 floats each, w =
@@ -210,6 +237,15 @@ int xpbd_lbvh_build(const float* verts, size_t num_verts, const uint32_t* tris, 
  * src/mesh_bvh.h:142-181).  Host only. */
 int xpbd_lbvh_refit(const float* verts, const float* new_verts, size_t num_verts, const uint32_t* tris,
                     size_t num_tris, float* lowers, float* uppers);
+/* Host only: builds the projected ray-parity grid the watertight sign test
+ * uses and returns, per query point, the triangles registered in the point's
+ * cell: counts[k] for point k, ids concatenated in point order up to
+ * ids_cap entries (*total receives the full length).  Exists so the grid's
+ * conservativeness (every triangle the ray test hits is a candidate) can be
+ * checked without a GPU. */
+int xpbd_parity_grid_candidates(const float* verts, size_t num_verts, const uint32_t* tris, size_t num_tris,
+                                const float* points, size_t num_points, uint32_t* counts, uint32_t* ids,
+                                size_t ids_cap, size_t* total);
 
 #ifdef __cplusplus
 }

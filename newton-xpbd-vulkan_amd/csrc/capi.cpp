@@ -216,6 +216,27 @@ int xpbd_add_box_body(xpbd_solver* h, const float* pos, const float* quat, const
 int xpbd_set_body_params(xpbd_solver* h, float ad, float relax, float mt, float mr, int uw) {
     return guarded(h, [&] { h->s.set_body_params(ad, relax, mt, mr, uw); });
 }
+int xpbd_mesh_query_stats(xpbd_solver* h, uint64_t* out6) { return guarded(h, [&] { h->s.mesh_query_stats(out6); }); }
+int xpbd_init_batch(xpbd_solver* h, const xpbd_config* c, const uint64_t* counts, size_t ns, size_t mc) {
+    return guarded(h, [&] {
+        if (!c) throw std::runtime_error("null config");
+        if (!counts && ns) throw std::runtime_error("null scene counts");
+        std::vector<size_t> cnt(counts, counts + ns);
+        h->s.init_batch(*c, cnt.data(), ns, mc);
+    });
+}
+int xpbd_scene_count(const xpbd_solver* h, size_t* out) {
+    if (!h || !out) return 1;
+    *out = h->s.scene_count();
+    return 0;
+}
+int xpbd_set_graph_mode(xpbd_solver* h, int on) { return guarded(h, [&] { h->s.set_graph_mode(on != 0); }); }
+int xpbd_autotune(xpbd_solver* h, int frames, float dt, int substeps, int iterations, int* block) {
+    return guarded(h, [&] {
+        const int b = h->s.autotune(frames, dt, substeps, iterations);
+        if (block) *block = b;
+    });
+}
 int xpbd_body_count(const xpbd_solver* h, size_t* out) {
     if (!h || !out) return 1;
     *out = h->s.body_count();
@@ -254,6 +275,30 @@ int xpbd_lbvh_refit(const float* verts, const float* new_verts, size_t nv, const
         xpbd::lbvh_refit(b, pts.data(), tris);
         std::memcpy(lowers, b.lo.data(), b.lo.size() * sizeof(xpbd::NodeHalf));
         std::memcpy(uppers, b.hi.data(), b.hi.size() * sizeof(xpbd::NodeHalf));
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return 1;
+    }
+}
+
+int xpbd_parity_grid_candidates(const float* verts, size_t nv, const uint32_t* tris, size_t nt, const float* points,
+                                size_t np, uint32_t* counts, uint32_t* ids, size_t ids_cap, size_t* total) {
+    try {
+        for (size_t k = 0; k < 3 * nt; k++)
+            if (tris[k] >= nv) throw std::runtime_error("parity grid: vertex index out of range");
+        xpbd::ParityGrid g;
+        xpbd::parity_grid_build(verts, tris, uint32_t(nt), 0u, g);
+        size_t w = 0;
+        for (size_t k = 0; k < np; k++) {
+            const long c = g.dim ? xpbd::parity_grid_cell(g, points + 3 * k) : -1;
+            uint32_t cnt = 0;
+            if (c >= 0)
+                for (uint32_t e = g.start[size_t(c)]; e < g.start[size_t(c) + 1]; e++, cnt++, w++)
+                    if (w < ids_cap) ids[w] = g.tri[e];
+            counts[k] = cnt;
+        }
+        if (total) *total = w;
         return 0;
     } catch (const std::exception& e) {
         g_err = e.what();

@@ -42,6 +42,7 @@ __global__ __launch_bounds__(256) void k_integrate(IntegrateArgs a) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         if (a.cc_next != nullptr) *a.cc_next = 0;
         if (a.rcc_next != nullptr) *a.rcc_next = 0;
+        if (a.mesh_next != nullptr) { a.mesh_next[0] = 0; a.mesh_next[1] = 0; }
     }
     __syncthreads();
     float4 P[kSortItems], V[kSortItems];
@@ -54,25 +55,40 @@ __global__ __launch_bounds__(256) void k_integrate(IntegrateArgs a) {
         f[it] = 0;
         if (idx < a.n) { P[it] = a.Ps[idx]; V[it] = a.Vs[idx]; f[it] = a.flags[idx]; }
     }
+    // plane contacts of this thread's items (on the pre-integration position),
+    // slots reserved once per block
+    bool want[kSortItems];
+    int mine = 0;
 #pragma unroll
     for (int it = 0; it < kSortItems; it++) {
         const int idx = tile_item(tile, it);
-        const bool valid = idx < a.n;
-        bool want = false;
-        if (a.has_plane && valid && (f[it] & 1)) {
+        want[it] = false;
+        if (a.has_plane && idx < a.n && (f[it] & 1)) {
             const float dist = P[it].z - a.altitude;
-            want = dist < a.margin + P[it].w;
+            want[it] = dist < a.margin + P[it].w;
         }
-        if (a.cc != nullptr) {
-            const int slot = wave_reserve(a.cc, want);
-            if (want && slot < a.cmax) {
-                a.cpart[slot] = idx;
+        mine += want[it] ? 1 : 0;
+    }
+    if (a.cc != nullptr) {
+        __shared__ int s_tmp[256 / 64 + 1];
+        int slot = block_reserve(a.cc, mine, s_tmp);
+#pragma unroll
+        for (int it = 0; it < kSortItems; it++) {
+            if (!want[it]) continue;
+            if (slot < a.cmax) {
+                a.cpart[slot] = tile_item(tile, it);
                 a.cshape[slot] = 0;
                 a.cbpos[slot] = make_float4(P[it].x, P[it].y, a.altitude, 0.f);
                 a.cbvel[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
                 a.cnorm[slot] = make_float4(0.f, 0.f, 1.f, 0.f);
             }
+            slot++;
         }
+    }
+#pragma unroll
+    for (int it = 0; it < kSortItems; it++) {
+        const int idx = tile_item(tile, it);
+        const bool valid = idx < a.n;
         if (!valid) continue;
         float4 Pn = P[it], Vn = V[it];
         if (f[it] & 1) {
@@ -88,7 +104,8 @@ __global__ __launch_bounds__(256) void k_integrate(IntegrateArgs a) {
         }
         a.Pp[idx] = Pn;
         a.Vp[idx] = Vn;
-        const uint32_t key = cell_of(Pn.x, Pn.y, Pn.z, a.inv_cell);
+        uint32_t key = cell_of(Pn.x, Pn.y, Pn.z, a.inv_cell);
+        if (a.scene != nullptr) key |= uint32_t(a.scene[idx]) << kCellBits;
         a.keys[idx] = key;
         atomicAdd(&h[key & (kSortBins - 1)], 1u);
     }
@@ -121,9 +138,10 @@ __global__ __launch_bounds__(256) void k_plane_contacts(PlaneArgs a) {
 __global__ __launch_bounds__(256) void k_cell_ranges(int n, const uint32_t* __restrict__ keys, int* __restrict__ range) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
-    uint32_t c = keys[t];
-    if (t == 0 || keys[t - 1] != c) range[2 * c] = t;
-    if (t == n - 1 || keys[t + 1] != c) range[2 * c + 1] = t + 1;
+    const uint32_t c = keys[t];
+    const size_t row = size_t(2) * c;   // batched keys reach 2^32 - 1
+    if (t == 0 || keys[t - 1] != c) range[row] = t;
+    if (t == n - 1 || keys[t + 1] != c) range[row + 1] = t + 1;
 }
 
 // Dense tables exactly as the reference leaves them after grid_offsets
@@ -226,13 +244,10 @@ __global__ __launch_bounds__(256) void k_shape_solve(ShapeArgs a) {
 
 // One neighbour candidate of solve_particle_particle_contacts.comp:150-210.
 template <bool DIAG>
-__device__ __forceinline__ void pp_pair(const PPArgs& a, int t, int k, int i, int proxy, f3 x, f3 v, float rad, float w1,
-                                        float R, f3& acc) {
+__device__ __forceinline__ void pp_pair(const PPArgs& a, int t, int k, int nf, float4 Pj, float4 Vj, int i, int proxy, f3 x,
+                                        f3 v, float rad, float w1, float R, f3& acc) {
     if (k == t) return;   // j == i
-    const int nf = a.Fs[k];
     if ((nf & 1) == 0) return;
-    const float4 Pj = a.Xs[k];
-    const float4 Vj = a.Vs[k];
     if (proxy != 0 && (((nf & 2) != 0) || (Vj.w <= 0.0f))) return;
     const f3 rd = x - xyz(Pj);
     if (fabsf(rd.x) > R || fabsf(rd.y) > R || fabsf(rd.z) > R) return;
@@ -288,7 +303,7 @@ __device__ __forceinline__ void pp_pair(const PPArgs& a, int t, int k, int i, in
 // cell ids, hence one contiguous run of sorted slots in the same order.  A row
 // is split where the x index wraps at 128; a virtual x below the clamp (the
 // reference maps every such cell to x = 0) is visited cell by cell.
-template <bool DIAG>
+template <bool DIAG, bool VALIDATE>
 __global__ __launch_bounds__(256) void k_pp_apply(PPArgs a) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= a.n) return;
@@ -314,6 +329,15 @@ __global__ __launch_bounds__(256) void k_pp_apply(PPArgs a) {
     const int ye = min((int)floorf((x.y + R) * inv), ys + kGridDim - 1);
     const int ze = min((int)floorf((x.z + R) * inv), zs + kGridDim - 1);
     const int2* range = reinterpret_cast<const int2*>(a.range);
+    // batched mode: this particle's scene selects its own 2^21-cell table
+    // (keys reach 2^32 - 1 with 2048 scenes: row arithmetic stays unsigned)
+    const uint32_t base = VALIDATE ? (a.keys[t] & ~((1u << kCellBits) - 1u)) : 0u;
+    auto lookup = [&](int cell) -> int2 {
+        int2 rg = range[size_t(base) + size_t(cell)];
+        if (VALIDATE && !(rg.x >= 0 && rg.x < a.n && rg.y > rg.x && a.keys[rg.x] == base + uint32_t(cell)))
+            rg = make_int2(0, 0);
+        return rg;
+    };
     f3 acc = {0.f, 0.f, 0.f};
     for (int cz = zs; cz <= ze; cz++)
         for (int cy = ys; cy <= ye; cy++) {
@@ -321,8 +345,9 @@ __global__ __launch_bounds__(256) void k_pp_apply(PPArgs a) {
             while (cx <= xe) {
                 const int vx = cx + kGridOrigin;
                 if (vx < 0) {   // clamped virtual cell: one physical cell per step
-                    const int2 rg = range[cell_index(cx, cy, cz)];
-                    for (int k = rg.x; k < rg.y; k++) pp_pair<DIAG>(a, t, k, i, proxy, x, v, rad, w1, R, acc);
+                    const int2 rg = lookup(cell_index(cx, cy, cz));
+                    for (int k = rg.x; k < rg.y; k++)
+                        pp_pair<DIAG>(a, t, k, a.Fs[k], a.Xs[k], a.Vs[k], i, proxy, x, v, rad, w1, R, acc);
                     cx++;
                     continue;
                 }
@@ -332,14 +357,23 @@ __global__ __launch_bounds__(256) void k_pp_apply(PPArgs a) {
                 int s = 0, e = 0;
                 bool any = false;
                 for (int m = 0; m < len; m++) {
-                    const int2 rg = range[c0 + m];
+                    const int2 rg = lookup(c0 + m);
                     if (rg.y > rg.x) {
                         if (!any) s = rg.x;
                         e = rg.y;
                         any = true;
                     }
                 }
-                for (int k = s; k < e; k++) pp_pair<DIAG>(a, t, k, i, proxy, x, v, rad, w1, R, acc);
+                // two candidates per step: both loads issue before either's math
+                int k = s;
+                for (; k + 1 < e; k += 2) {
+                    const int f0 = a.Fs[k], f1 = a.Fs[k + 1];
+                    const float4 P0 = a.Xs[k], P1 = a.Xs[k + 1];
+                    const float4 V0 = a.Vs[k], V1 = a.Vs[k + 1];
+                    pp_pair<DIAG>(a, t, k, f0, P0, V0, i, proxy, x, v, rad, w1, R, acc);
+                    pp_pair<DIAG>(a, t, k + 1, f1, P1, V1, i, proxy, x, v, rad, w1, R, acc);
+                }
+                if (k < e) pp_pair<DIAG>(a, t, k, a.Fs[k], a.Xs[k], a.Vs[k], i, proxy, x, v, rad, w1, R, acc);
                 cx += len;
             }
         }
@@ -416,10 +450,10 @@ __global__ void k_zero_int(int* p, int count) {
     if (t < count) p[t] = 0;
 }
 
-// Zeroes the two counters a frame's first substep uses (particle contacts at
-// cc[0], rigid contacts at cc[4]).
+// Zeroes the counters a frame's first substep uses: particle contacts cc[0],
+// rigid contacts cc[4], mesh narrowphase queues cc[6], cc[7].
 __global__ void k_frame_begin(int* cc) {
-    if (threadIdx.x == 0) { cc[0] = 0; cc[4] = 0; }
+    if (threadIdx.x == 0) { cc[0] = 0; cc[4] = 0; cc[6] = 0; cc[7] = 0; }
 }
 
 // ------------------------------------------------------------- launchers --
@@ -440,10 +474,17 @@ void launch_cell_tables(int n, const uint32_t* keys, int* starts, int* ends, hip
 void launch_shape_solve(const ShapeArgs& a, int grid, hipStream_t s) {
     hipLaunchKernelGGL(k_shape_solve, dim3(grid), dim3(256), 0, s, a);
 }
-void launch_pp_apply(const PPArgs& a, bool diag, hipStream_t s) {
+void launch_pp_apply(const PPArgs& a, bool diag, bool validate, int block, hipStream_t s) {
     if (a.n <= 0) return;
-    if (diag) hipLaunchKernelGGL(k_pp_apply<true>, dim3(blocks_for(a.n, 256)), dim3(256), 0, s, a);
-    else hipLaunchKernelGGL(k_pp_apply<false>, dim3(blocks_for(a.n, 256)), dim3(256), 0, s, a);
+    const int bs = (block == 64 || block == 128) ? block : 256;
+    const dim3 g(blocks_for(a.n, bs)), b(bs);
+    if (diag) {
+        if (validate) hipLaunchKernelGGL((k_pp_apply<true, true>), g, b, 0, s, a);
+        else hipLaunchKernelGGL((k_pp_apply<true, false>), g, b, 0, s, a);
+    } else {
+        if (validate) hipLaunchKernelGGL((k_pp_apply<false, true>), g, b, 0, s, a);
+        else hipLaunchKernelGGL((k_pp_apply<false, false>), g, b, 0, s, a);
+    }
 }
 void launch_restitution(const RestArgs& a, int grid, hipStream_t s) {
     hipLaunchKernelGGL(k_restitution, dim3(grid), dim3(256), 0, s, a);

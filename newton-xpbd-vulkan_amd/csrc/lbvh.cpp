@@ -8,6 +8,7 @@
 #include "lbvh.h"
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 
 namespace xpbd {
@@ -179,6 +180,91 @@ void refit_node(MeshBvhHost& b, const float* pts, const uint32_t* tris, uint32_t
 
 void lbvh_refit(MeshBvhHost& b, const float* pts, const uint32_t* tris) {
     if (!b.lo.empty()) refit_node(b, pts, tris, b.root);
+}
+
+namespace {
+struct RayAxes { int kx, ky, kz; float Sx, Sy; };
+RayAxes parity_axes() {
+    const float dir[3] = {1.029700994f, 1.086472273f, 1.034694910f};
+    const float ad[3] = {std::fabs(dir[0]), std::fabs(dir[1]), std::fabs(dir[2])};
+    int kz = 0;
+    if (ad[1] > ad[0]) kz = 1;
+    if (ad[2] > (kz == 1 ? ad[1] : ad[0])) kz = 2;
+    int kx = kz + 1; if (kx == 3) kx = 0;
+    int ky = kx + 1; if (ky == 3) ky = 0;
+    if (dir[kz] < 0.0f) std::swap(kx, ky);
+    return RayAxes{kx, ky, kz, dir[kx] / dir[kz], dir[ky] / dir[kz]};
+}
+}  // namespace
+
+long parity_grid_cell(const ParityGrid& g, const float p[3]) {
+    const RayAxes r = parity_axes();
+    const float u = p[r.kx] - r.Sx * p[r.kz];
+    const float w = p[r.ky] - r.Sy * p[r.kz];
+    const float fu = std::floor((u - g.u0) * g.inv), fw = std::floor((w - g.w0) * g.inv);
+    if (!(fu >= 0.0f && fw >= 0.0f && fu < float(g.dim) && fw < float(g.dim))) return -1;
+    return long(fw) * long(g.dim) + long(fu);
+}
+
+void parity_grid_build(const float* pts, const uint32_t* tris, uint32_t nt, uint32_t tri_base, ParityGrid& g) {
+    g = ParityGrid{};
+    if (nt == 0) return;
+    const RayAxes r = parity_axes();
+    const int kx = r.kx, ky = r.ky, kz = r.kz;
+    const double Sx = double(r.Sx), Sy = double(r.Sy);
+    double scale = 0.0;
+    std::vector<double> bu0(nt), bu1(nt), bw0(nt), bw1(nt);
+    double umin = 1e300, umax = -1e300, wmin = 1e300, wmax = -1e300;
+    for (uint32_t t = 0; t < nt; t++) {
+        double lo_u = 1e300, hi_u = -1e300, lo_w = 1e300, hi_w = -1e300;
+        for (int c = 0; c < 3; c++) {
+            const float* v = pts + size_t(tris[3 * t + c]) * 3;
+            const double u = double(v[kx]) - Sx * double(v[kz]), w = double(v[ky]) - Sy * double(v[kz]);
+            lo_u = std::min(lo_u, u); hi_u = std::max(hi_u, u);
+            lo_w = std::min(lo_w, w); hi_w = std::max(hi_w, w);
+            for (int k = 0; k < 3; k++) scale = std::max(scale, double(std::fabs(v[k])));
+        }
+        bu0[t] = lo_u; bu1[t] = hi_u; bw0[t] = lo_w; bw1[t] = hi_w;
+    }
+    // rounding of the device's projection and of the ray test is a few ulps of
+    // the coordinate scale; the pad is orders of magnitude larger
+    const double pad = 1e-4 * (1.0 + scale);
+    for (uint32_t t = 0; t < nt; t++) {
+        bu0[t] -= pad; bu1[t] += pad; bw0[t] -= pad; bw1[t] += pad;
+        umin = std::min(umin, bu0[t]); umax = std::max(umax, bu1[t]);
+        wmin = std::min(wmin, bw0[t]); wmax = std::max(wmax, bw1[t]);
+    }
+    uint32_t dim = uint32_t(std::ceil(std::sqrt(double(nt)) / 2.0));
+    dim = std::max<uint32_t>(1, std::min<uint32_t>(dim, 2048));
+    const double ext = std::max(umax - umin, wmax - wmin);
+    const float cell = float(ext / double(dim)) * 1.0001f;
+    g.u0 = float(umin);
+    g.w0 = float(wmin);
+    g.inv = 1.0f / cell;
+    g.dim = dim;
+    auto cell_of = [&](double x, float o) {
+        const long c = long(std::floor((x - double(o)) * double(g.inv)));
+        return uint32_t(std::max<long>(0, std::min<long>(long(dim) - 1, c)));
+    };
+    std::vector<uint32_t> cnt(size_t(dim) * dim + 1, 0);
+    for (int pass = 0; pass < 2; pass++) {
+        if (pass == 1) {
+            uint32_t run = 0;
+            for (auto& c : cnt) { const uint32_t v = c; c = run; run += v; }
+            g.start = cnt;
+            g.tri.assign(run, 0);
+        }
+        for (uint32_t t = 0; t < nt; t++) {
+            const uint32_t a0 = cell_of(bu0[t], g.u0), a1 = cell_of(bu1[t], g.u0);
+            const uint32_t b0 = cell_of(bw0[t], g.w0), b1 = cell_of(bw1[t], g.w0);
+            for (uint32_t b = b0; b <= b1; b++)
+                for (uint32_t a = a0; a <= a1; a++) {
+                    const size_t c = size_t(b) * dim + a;
+                    if (pass == 0) cnt[c]++;
+                    else g.tri[cnt[c]++] = tri_base + t;
+                }
+        }
+    }
 }
 
 }  // namespace xpbd

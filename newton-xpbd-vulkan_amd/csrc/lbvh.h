@@ -32,4 +32,22 @@ void lbvh_build(const float* pts, const uint32_t* tris, uint32_t num_tris, MeshB
 // reachable from the root (src/mesh_bvh.h:142-181); topology is unchanged.
 void lbvh_refit(MeshBvhHost& b, const float* pts, const uint32_t* tris);
 
+// Grid of triangle projections along the fixed parity-ray direction of the
+// watertight inside/outside test (generate_mesh_contacts.comp:375).  A ray
+// with that direction can only hit a triangle whose projection contains the
+// ray origin's projection; each triangle is registered in every cell its
+// projected box, grown by `pad`, touches.  start has dim*dim + 1 entries.
+struct ParityGrid {
+    float u0 = 0.f, w0 = 0.f, inv = 0.f;
+    uint32_t dim = 0;
+    std::vector<uint32_t> start, tri;
+};
+
+// pts/tris: vertex xyz and global vertex indices; tri_base: global id of the
+// mesh's first triangle (stored in the grid).
+void parity_grid_build(const float* pts, const uint32_t* tris, uint32_t num_tris, uint32_t tri_base, ParityGrid& g);
+
+// Cell of a ray origin, in the device's float arithmetic; -1 outside the grid.
+long parity_grid_cell(const ParityGrid& g, const float p[3]);
+
 }  // namespace xpbd

@@ -84,10 +84,12 @@ void Solver::free_all() {
     dfree(range_); dfree(tbl_); dfree(cpart_); dfree(cshape_); dfree(cc_); dfree(cbpos_); dfree(cbvel_); dfree(cnorm_);
     dfree(dshape_); dfree(delta_); dfree(body_q_); dfree(body_qd_); dfree(body_com_); dfree(body_invI_);
     dfree(body_invm_); dfree(body_delta_); dfree(mat_mu_); dfree(body_flags_); dfree(shape_body_);
-    dfree(verts_d_); dfree(vvel_d_); dfree(tris_d_); dfree(prim_d_); dfree(err_d_); dfree(nl_d_); dfree(nu_d_);
-    dfree(info_d_); dfree(diag_header_); dfree(diag_pairs_);
+    dfree(verts_d_); dfree(vvel_d_); dfree(tris_d_); dfree(prim_d_); dfree(err_d_); dfree(nodes_d_); dfree(pg_start_); dfree(pg_tri_);
+    dfree(info_d_); dfree(diag_header_); dfree(diag_pairs_); dfree(mesh_stats_);
     for (hipEvent_t e : ev_pool_) (void)hipEventDestroy(e);
     ev_pool_.clear();
+    invalidate_graphs();
+    dfree(scene_d_);
     if (count_host_) { (void)hipHostFree(count_host_); count_host_ = nullptr; }
     if (stream_) { (void)hipStreamDestroy(stream_); stream_ = nullptr; }
 }
@@ -104,18 +106,20 @@ void Solver::alloc_all() {
     const size_t tiles = (n_ + kSortTile - 1) / kSortTile;
     dalloc(hist_, size_t(kSortBins) * std::max<size_t>(tiles, 1));
     dalloc(bintot_, kSortBins);
-    dalloc(range_, size_t(2) * kNumCells);
+    // one 2^21-cell table per scene; the batched key scene << 21 | cell is the row
+    dalloc(range_, size_t(2) * kNumCells * num_scenes_);
     dalloc(cpart_, cmax_); dalloc(cshape_, cmax_);
     dalloc(cbpos_, cmax_); dalloc(cbvel_, cmax_); dalloc(cnorm_, cmax_);
     dalloc(cc_, 8);   // [particle slot 0, slot 1, error flags, spare, rigid slot 0, slot 1, spare, spare]
     dalloc(dshape_, 3 * n_); dalloc(delta_, 3 * n_);
     dalloc(diag_header_, 2);
+    dalloc(mesh_stats_, 8);
     max_pairs_ = cfg_.max_diag_pairs ? cfg_.max_diag_pairs : 100000u;
     dalloc(diag_pairs_, cfg_.enable_diagnostics ? size_t(20) * max_pairs_ : 20);
     HIPCHECK(hipHostMalloc(reinterpret_cast<void**>(&count_host_), 4 * sizeof(int32_t), hipHostMallocDefault));
 }
 
-void Solver::init(const xpbd_config& cfg, size_t n, size_t max_contacts) {
+void Solver::init_common(const xpbd_config& cfg, size_t n, size_t max_contacts) {
     int count = 0;
     if (hipGetDeviceCount(&count) != hipSuccess || count == 0) throw std::runtime_error("No HIP devices found");
     if (n > (size_t(1) << 28)) throw std::runtime_error("num_particles exceeds the supported maximum of 2^28");
@@ -132,11 +136,39 @@ void Solver::init(const xpbd_config& cfg, size_t n, size_t max_contacts) {
     bool env_on = env && (std::strcmp(env, "1") == 0 || std::strcmp(env, "true") == 0);
     profiling_ = cfg.enable_profile || env_on;
     if (profiling_) std::printf("[vkxpbd] GPU profiler enabled (HIP events)\n");
+    const char* genv = std::getenv("VKXPBD_GRAPHS");
+    use_graphs_ = !(genv && std::strcmp(genv, "0") == 0);
     grid_valid_ = false;
     contact_slot_ = 0;
+    rigid_slot_ = 0;
     last_count_ = 0;
     initialized_ = true;
     finalized_ = false;
+}
+
+void Solver::init(const xpbd_config& cfg, size_t n, size_t max_contacts) {
+    num_scenes_ = 1;
+    scene_off_.assign({0, n});
+    sort_passes_ = kSortPasses;
+    init_common(cfg, n, max_contacts);
+}
+
+void Solver::init_batch(const xpbd_config& cfg, const size_t* counts, size_t num_scenes, size_t max_contacts) {
+    if (num_scenes == 0) throw std::runtime_error("init_batch: num_scenes must be positive");
+    if (num_scenes > 2048) throw std::runtime_error("init_batch: at most 2048 scenes (11 key bits above the cell)");
+    size_t total = 0;
+    scene_off_.assign(1, 0);
+    for (size_t k = 0; k < num_scenes; k++) { total += counts[k]; scene_off_.push_back(total); }
+    num_scenes_ = num_scenes;
+    int bits = kCellBits;
+    while ((size_t(1) << (bits - kCellBits)) < num_scenes) bits++;
+    sort_passes_ = (bits + kSortBits - 1) / kSortBits;
+    init_common(cfg, total, max_contacts);
+    std::vector<int32_t> sc(total);
+    for (size_t k = 0; k < num_scenes; k++)
+        for (size_t i = scene_off_[k]; i < scene_off_[k + 1]; i++) sc[i] = int32_t(k);
+    DeviceGuard g(device_);
+    upload(scene_d_, sc.data(), sc.size());
 }
 
 void Solver::add_mesh(const float* v, size_t nv, const uint32_t* t, size_t nt, float margin, const float* rot, int wt) {
@@ -192,14 +224,38 @@ void Solver::upload_nodes() {
             nu[nb + i] = make_float4(U.x, U.y, U.z, uf);
         }
     }
-    upload(nl_d_, nl.data(), nl.size());
-    upload(nu_d_, nu.data(), nu.size());
+    std::vector<float4> inter(2 * total);
+    for (size_t i = 0; i < total; i++) { inter[2 * i] = nl[i]; inter[2 * i + 1] = nu[i]; }
+    upload(nodes_d_, inter.data(), inter.size());
+}
+
+// Concatenates the per-mesh parity grids into one cell-start table (each
+// mesh's cells at pg_cell_off, entries offset into one triangle list) and
+// refreshes the mesh descriptors that carry their geometry.
+void Solver::upload_parity_grids() {
+    std::vector<uint32_t> start, tri;
+    for (size_t c = 0; c < pgrid_.size(); c++) {
+        const ParityGrid& g = pgrid_[c];
+        info_h_[c].pg_u0 = g.u0;
+        info_h_[c].pg_w0 = g.w0;
+        info_h_[c].pg_inv = g.inv;
+        info_h_[c].pg_dim = g.dim;
+        info_h_[c].pg_cell_off = uint32_t(start.size());
+        const uint32_t base = uint32_t(tri.size());
+        for (size_t k = 0; k + 1 < g.start.size(); k++) start.push_back(base + g.start[k]);
+        tri.insert(tri.end(), g.tri.begin(), g.tri.end());
+    }
+    start.push_back(uint32_t(tri.size()));
+    upload(pg_start_, start.data(), start.size());
+    upload(pg_tri_, tri.data(), tri.size());
+    upload(info_d_, info_h_.data(), info_h_.size());
 }
 
 void Solver::finalize_meshes() {
     if (!initialized_) throw std::runtime_error("XpbdSolver::init must be called before finalize_meshes");
     DeviceGuard g(device_);
     HIPCHECK(hipStreamSynchronize(stream_));
+    invalidate_graphs();
     const size_t M = meshes_.size(), NB = boxes_.size();
     // shapes: 0 = ground plane (body -1), 1..M = meshes (static bodies 0..M-1),
     // M+1..M+NB = boxes (dynamic bodies M..M+NB-1)
@@ -299,8 +355,13 @@ void Solver::finalize_meshes() {
     upload(vvel_d_, all_vels_.data(), all_vels_.size());
     upload(tris_d_, all_tris_.data(), all_tris_.size());
     upload(prim_d_, prim_global.data(), prim_global.size());
-    upload(info_d_, info_h_.data(), info_h_.size());
     upload_nodes();
+    pgrid_.assign(M, ParityGrid{});
+    for (size_t c = 0; c < M; c++)
+        if (meshes_[c].watertight)
+            parity_grid_build(all_verts_.data(), all_tris_.data() + 3 * tri_off_[c], uint32_t(meshes_[c].tris.size() / 3),
+                              uint32_t(tri_off_[c]), pgrid_[c]);
+    upload_parity_grids();
     finalized_ = true;
 }
 
@@ -329,12 +390,18 @@ void Solver::update_mesh(size_t mi, const float* v, size_t nv, const float* vel)
     if (nv != meshes_[mi].verts.size() / 3) throw std::runtime_error("update_mesh: vertex count mismatch");
     DeviceGuard g(device_);
     HIPCHECK(hipStreamSynchronize(stream_));
+    invalidate_graphs();
     const size_t vs = vert_off_[mi];
     std::memcpy(all_verts_.data() + 3 * vs, v, nv * 3 * sizeof(float));
     if (vel) std::memcpy(all_vels_.data() + 3 * vs, vel, nv * 3 * sizeof(float));
     else std::fill(all_vels_.begin() + 3 * vs, all_vels_.begin() + 3 * (vs + nv), 0.f);
     std::memcpy(meshes_[mi].verts.data(), v, nv * 3 * sizeof(float));
     lbvh_refit(bvh_[mi], all_verts_.data(), all_tris_.data() + 3 * tri_off_[mi]);
+    if (meshes_[mi].watertight) {
+        parity_grid_build(all_verts_.data(), all_tris_.data() + 3 * tri_off_[mi], uint32_t(meshes_[mi].tris.size() / 3),
+                          uint32_t(tri_off_[mi]), pgrid_[mi]);
+        upload_parity_grids();
+    }
     HIPCHECK(hipMemcpy(verts_d_.p + 3 * vs, v, nv * 3 * sizeof(float), hipMemcpyHostToDevice));
     HIPCHECK(hipMemcpy(vvel_d_.p + 3 * vs, all_vels_.data() + 3 * vs, nv * 3 * sizeof(float), hipMemcpyHostToDevice));
     upload_nodes();
@@ -353,7 +420,9 @@ void Solver::stage_begin(Stage) {
     if (ev_used_ + 2 > ev_pool_.size()) {
         for (int k = 0; k < 64; k++) {
             hipEvent_t e;
-            HIPCHECK(hipEventCreate(&e));
+            // no system-scope fence: a default event record writes back the L2,
+            // which would inflate the very stage times being measured
+            HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
             ev_pool_.push_back(e);
         }
     }
@@ -380,32 +449,33 @@ void Solver::sync_profile() {
 }
 
 // ----------------------------------------------------------------- step ----
+// Key buffers alternate between passes and the result must land in A (the
+// buffers the getters read), so k_integrate writes its keys into A when the
+// pass count is even and into B when it is odd.
+uint32_t* Solver::first_keys() { return (sort_passes_ & 1) ? keysB_.p : keysA_.p; }
+
 void Solver::run_sort(int tiles) {
     const int n = int(n_);
-    // pass 0: tile histograms came from k_integrate; ids are implicit
-    stage_begin(kSortScan);
-    launch_sort_binscan(hist_.p, bintot_.p, tiles, stream_);
-    stage_end(kSortScan);
-    stage_begin(kSortScatter);
-    launch_sort_scatter(n, keysB_.p, nullptr, keysA_.p, valsA_.p, 0, hist_.p, bintot_.p, tiles, nullptr, stream_);
-    stage_end(kSortScatter);
+    uint32_t* kb[2] = {keysA_.p, keysB_.p};
+    uint32_t* vb[2] = {valsA_.p, valsB_.p};
+    int src = (sort_passes_ & 1) ? 1 : 0;
     const SortGather g{P_[p_].p, V_[p_].p, flags_.p, Xs_[0].p, Vs_[0].p, Fs_.p};
-    for (int pass = 1; pass < kSortPasses; pass++) {
+    for (int pass = 0; pass < sort_passes_; pass++) {
         const int shift = pass * kSortBits;
-        uint32_t* kin = (pass == 1) ? keysA_.p : keysB_.p;
-        uint32_t* vin = (pass == 1) ? valsA_.p : valsB_.p;
-        uint32_t* kout = (pass == 1) ? keysB_.p : keysA_.p;
-        uint32_t* vout = (pass == 1) ? valsB_.p : valsA_.p;
-        stage_begin(kSortUp);
-        launch_sort_upsweep(n, kin, shift, hist_.p, tiles, stream_);
-        stage_end(kSortUp);
+        const int dst = src ^ 1;
+        if (pass > 0) {   // pass 0's tile histograms came from k_integrate
+            stage_begin(kSortUp);
+            launch_sort_upsweep(n, kb[src], shift, hist_.p, tiles, stream_);
+            stage_end(kSortUp);
+        }
         stage_begin(kSortScan);
         launch_sort_binscan(hist_.p, bintot_.p, tiles, stream_);
         stage_end(kSortScan);
         stage_begin(kSortScatter);
-        launch_sort_scatter(n, kin, vin, kout, vout, shift, hist_.p, bintot_.p, tiles,
-                            pass == kSortPasses - 1 ? &g : nullptr, stream_);
+        launch_sort_scatter(n, kb[src], pass == 0 ? nullptr : vb[src], kb[dst], vb[dst], shift, hist_.p, bintot_.p,
+                            tiles, pass == sort_passes_ - 1 ? &g : nullptr, stream_);
         stage_end(kSortScatter);
+        src = dst;
     }
 }
 
@@ -433,12 +503,14 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
     if (collide && !meshes_.empty()) {
         MeshArgs m;
         m.n = n; m.P = P_[s_].p; m.flags = flags_.p;
-        m.verts = verts_d_.p; m.tris = tris_d_.p; m.nl = nl_d_.p; m.nu = nu_d_.p; m.prim = prim_d_.p;
+        m.verts = verts_d_.p; m.tris = tris_d_.p; m.nodes = nodes_d_.p; m.prim = prim_d_.p;
+        m.pg_start = pg_start_.p; m.pg_tri = pg_tri_.p;
         m.vvel = vvel_d_.p; m.info = info_d_.p; m.nmesh = M;
         m.margin = cfg_.soft_contact_margin; m.eps = avg_edge_ * 1e-3f;
         m.cmax = int(cmax_); m.cc = cc_gen;
         m.cpart = cpart_.p; m.cshape = cshape_.p; m.cbpos = cbpos_.p; m.cbvel = cbvel_.p; m.cnorm = cnorm_.p;
         m.err = reinterpret_cast<uint32_t*>(cc_.p + 2);
+        m.stats = profiling_ ? mesh_stats_.p : nullptr;
         stage_begin(kMesh);
         launch_mesh_contacts(m, stream_);
         stage_end(kMesh);
@@ -468,7 +540,7 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
         IntegrateArgs a;
         a.n = n; a.tiles = tiles;
         a.Ps = P_[s_].p; a.Vs = V_[s_].p; a.flags = flags_.p; a.Pp = P_[p_].p; a.Vp = V_[p_].p;
-        a.keys = keysB_.p; a.hist = hist_.p;
+        a.keys = first_keys(); a.hist = hist_.p;
         a.dt = sdt; a.vmax = cfg_.particle_v_max;
         a.gx = cfg_.gravity[0]; a.gy = cfg_.gravity[1]; a.gz = cfg_.gravity[2];
         a.inv_cell = inv_cell;
@@ -477,6 +549,8 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
         a.cmax = int(cmax_); a.cc = cc_gen;
         a.cc_next = zero_next ? cc_.p + (slot ^ 1) : nullptr;
         a.rcc_next = zero_next ? cc_.p + 4 + (slot ^ 1) : nullptr;
+        a.mesh_next = zero_next ? cc_.p + 6 : nullptr;
+        a.scene = num_scenes_ > 1 ? scene_d_.p : nullptr;
         a.cpart = cpart_.p; a.cshape = cshape_.p; a.cbpos = cbpos_.p; a.cbvel = cbvel_.p; a.cnorm = cnorm_.p;
         stage_begin(kIntegrate);
         launch_integrate(a, stream_);
@@ -491,7 +565,7 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
     if (n > 0) {
         run_sort(tiles);
         stage_begin(kRanges);
-        launch_fill_zero(range_.p, size_t(2) * kNumCells, stream_);
+        if (num_scenes_ == 1) launch_fill_zero(range_.p, size_t(2) * kNumCells, stream_);
         launch_cell_ranges(n, keysA_.p, range_.p, stream_);
         stage_end(kRanges);
         grid_valid_ = true;
@@ -511,7 +585,7 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
         launch_shape_solve(sa, shape_grid, stream_);
         stage_end(kShape);
         PPArgs pa;
-        pa.n = n; pa.vals = valsA_.p; pa.range = range_.p;
+        pa.n = n; pa.keys = keysA_.p; pa.vals = valsA_.p; pa.range = range_.p;
         pa.Ps = P_[s_].p; pa.Xs = Xs_[xs].p; pa.Vs = Vs_[xs].p; pa.Fs = Fs_.p;
         pa.dshape = dshape_.p; pa.delta = delta_.p; pa.Po = P_[nxt].p; pa.Vo = V_[nxt].p;
         const bool more = it + 1 < iterations;
@@ -522,7 +596,7 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
         pa.iter_index = uint32_t(it); pa.max_diag_pairs = max_pairs_;
         pa.diag_header = diag_header_.p; pa.diag_pairs = diag_pairs_.p;
         stage_begin(kPP);
-        launch_pp_apply(pa, cfg_.enable_diagnostics != 0, stream_);
+        launch_pp_apply(pa, cfg_.enable_diagnostics != 0, num_scenes_ > 1, pp_block_, stream_);
         stage_end(kPP);
         std::swap(cur, nxt);
         xs ^= 1;
@@ -576,14 +650,113 @@ void Solver::read_count(int slot) {
     }
 }
 
+// ---------------------------------------------------------- frame graphs ----
+// A frame (counter reset + all substeps) is captured once per rotation state of
+// the three particle buffers and per (dt, substeps, iterations), then replayed
+// with one hipGraphLaunch.  Every device pointer and scalar a frame bakes in is
+// fixed between invalidations: the methods that reallocate buffers or change
+// kernel arguments (finalize_meshes, update_mesh, body parameters, init) drop
+// the cache.
+void Solver::invalidate_graphs() {
+    for (FrameGraph& f : graphs_)
+        if (f.exec) (void)hipGraphExecDestroy(f.exec);
+    graphs_.clear();
+}
+
+void Solver::record_frame(float dt, int substeps, int iterations) {
+    const float sdt = dt / float(substeps);
+    for (const FrameGraph& f : graphs_)
+        if (f.s == s_ && f.p == p_ && f.o == o_ && f.dt == dt && f.substeps == substeps && f.iterations == iterations) {
+            HIPCHECK(hipGraphLaunch(f.exec, stream_));
+            s_ = f.s2; p_ = f.p2; o_ = f.o2;
+            grid_valid_ = true;
+            return;
+        }
+    if (graphs_.size() >= 8) invalidate_graphs();
+    FrameGraph f{s_, p_, o_, dt, substeps, iterations, 0, 0, 0, nullptr};
+    HIPCHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+    hipGraph_t graph = nullptr;
+    try {
+        launch_frame_begin(cc_.p, stream_);
+        for (int sub = 0; sub < substeps; sub++) substep(sdt, iterations, true, sub & 1, true);
+    } catch (...) {
+        (void)hipStreamEndCapture(stream_, &graph);
+        if (graph) (void)hipGraphDestroy(graph);
+        s_ = f.s; p_ = f.p; o_ = f.o;
+        throw;
+    }
+    HIPCHECK(hipStreamEndCapture(stream_, &graph));
+    const hipError_t e = hipGraphInstantiate(&f.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    HIPCHECK(e);
+    f.s2 = s_; f.p2 = p_; f.o2 = o_;
+    graphs_.push_back(f);
+    HIPCHECK(hipGraphLaunch(f.exec, stream_));
+}
+
+int Solver::autotune(int frames, float dt, int substeps, int iterations) {
+    require_init();
+    if (!finalized_) finalize_meshes();
+    DeviceGuard g(device_);
+    if (n_ == 0 || substeps <= 0) return pp_block_;
+    frames = std::max(frames, 1);
+    // snapshot of everything a frame advances
+    DevBuf<float4> P0, V0, bq, bqd;
+    dalloc(P0, n_); dalloc(V0, n_);
+    HIPCHECK(hipMemcpyAsync(P0.p, P_[s_].p, n_ * sizeof(float4), hipMemcpyDeviceToDevice, stream_));
+    HIPCHECK(hipMemcpyAsync(V0.p, V_[s_].p, n_ * sizeof(float4), hipMemcpyDeviceToDevice, stream_));
+    const size_t nb = body_q_.n;
+    if (nb) {
+        dalloc(bq, nb); dalloc(bqd, nb);
+        HIPCHECK(hipMemcpyAsync(bq.p, body_q_.p, nb * sizeof(float4), hipMemcpyDeviceToDevice, stream_));
+        HIPCHECK(hipMemcpyAsync(bqd.p, body_qd_.p, nb * sizeof(float4), hipMemcpyDeviceToDevice, stream_));
+    }
+    auto restore = [&] {
+        HIPCHECK(hipMemcpyAsync(P_[s_].p, P0.p, n_ * sizeof(float4), hipMemcpyDeviceToDevice, stream_));
+        HIPCHECK(hipMemcpyAsync(V_[s_].p, V0.p, n_ * sizeof(float4), hipMemcpyDeviceToDevice, stream_));
+        if (nb) {
+            HIPCHECK(hipMemcpyAsync(body_q_.p, bq.p, nb * sizeof(float4), hipMemcpyDeviceToDevice, stream_));
+            HIPCHECK(hipMemcpyAsync(body_qd_.p, bqd.p, nb * sizeof(float4), hipMemcpyDeviceToDevice, stream_));
+        }
+        HIPCHECK(hipStreamSynchronize(stream_));
+    };
+    const int cands[] = {64, 128, 256};
+    int best = pp_block_;
+    double best_ms = 1e300;
+    try {
+        for (int c : cands) {
+            pp_block_ = c;
+            invalidate_graphs();
+            step(dt, substeps, iterations);   // warm-up (and capture)
+            restore();
+            const auto t0 = std::chrono::high_resolution_clock::now();
+            for (int f = 0; f < frames; f++) step(dt, substeps, iterations);
+            const double ms = std::chrono::duration<double, std::milli>(std::chrono::high_resolution_clock::now() - t0).count();
+            restore();
+            if (ms < best_ms) { best_ms = ms; best = c; }
+        }
+    } catch (...) {
+        dfree(P0); dfree(V0); dfree(bq); dfree(bqd);
+        throw;
+    }
+    dfree(P0); dfree(V0); dfree(bq); dfree(bqd);
+    pp_block_ = best;
+    invalidate_graphs();
+    return best;
+}
+
 void Solver::step(float dt, int substeps, int iterations) {
     require_init();
     if (!finalized_) finalize_meshes();
     DeviceGuard g(device_);
     const auto t0 = std::chrono::high_resolution_clock::now();
     const float sdt = dt / float(substeps);
-    launch_frame_begin(cc_.p, stream_);
-    for (int sub = 0; sub < substeps; sub++) substep(sdt, iterations, true, sub & 1, true);
+    if (use_graphs_ && !profiling_ && substeps > 0) {
+        record_frame(dt, substeps, iterations);
+    } else {
+        launch_frame_begin(cc_.p, stream_);
+        for (int sub = 0; sub < substeps; sub++) substep(sdt, iterations, true, sub & 1, true);
+    }
     if (substeps > 0) contact_slot_ = rigid_slot_ = (substeps - 1) & 1;
     read_count(substeps > 0 ? contact_slot_ : 0);
     if (profiling_) {
@@ -618,7 +791,7 @@ void Solver::collide_only() {
     require_init();
     if (!finalized_) finalize_meshes();
     DeviceGuard g(device_);
-    launch_zero_int(cc_.p, 1, stream_);
+    launch_frame_begin(cc_.p, stream_);
     contact_slot_ = 0;
     PlaneArgs pl;
     pl.n = int(n_); pl.P = P_[s_].p; pl.flags = flags_.p;
@@ -638,12 +811,14 @@ void Solver::collide_only() {
     if (!meshes_.empty()) {
         MeshArgs m;
         m.n = int(n_); m.P = P_[s_].p; m.flags = flags_.p;
-        m.verts = verts_d_.p; m.tris = tris_d_.p; m.nl = nl_d_.p; m.nu = nu_d_.p; m.prim = prim_d_.p;
+        m.verts = verts_d_.p; m.tris = tris_d_.p; m.nodes = nodes_d_.p; m.prim = prim_d_.p;
+        m.pg_start = pg_start_.p; m.pg_tri = pg_tri_.p;
         m.vvel = vvel_d_.p; m.info = info_d_.p; m.nmesh = int(meshes_.size());
         m.margin = cfg_.soft_contact_margin; m.eps = avg_edge_ * 1e-3f;
         m.cmax = int(cmax_); m.cc = cc_.p;
         m.cpart = cpart_.p; m.cshape = cshape_.p; m.cbpos = cbpos_.p; m.cbvel = cbvel_.p; m.cnorm = cnorm_.p;
         m.err = reinterpret_cast<uint32_t*>(cc_.p + 2);
+        m.stats = profiling_ ? mesh_stats_.p : nullptr;
         launch_mesh_contacts(m, stream_);
     }
     HIPCHECK(hipGetLastError());
@@ -724,6 +899,7 @@ void Solver::get_sorted(bool ids, uint32_t* out, size_t n) {
 }
 void Solver::get_cell_table(bool ends, int32_t* out, size_t n) {
     require_init();
+    if (num_scenes_ > 1) throw std::runtime_error("cell tables are per scene in a batched solver; not available");
     DeviceGuard g(device_);
     n = std::min<size_t>(n, kNumCells);
     if (!grid_valid_ || n_ == 0) { std::memset(out, 0, n * sizeof(int32_t)); return; }
@@ -792,6 +968,7 @@ void Solver::set_body_params(float angular_damping, float rigid_relax, float mu_
     mu_t_ = mu_t;
     mu_r_ = mu_r;
     use_weights_ = use_weights;
+    invalidate_graphs();
     if (finalized_) {   // update the per-shape coefficients in place; body state is kept
         DeviceGuard g(device_);
         HIPCHECK(hipStreamSynchronize(stream_));
@@ -842,6 +1019,20 @@ void Solver::profile_reset() {
     for (int k = 0; k < kNumStages; k++) { stage_ms_[k] = 0.0; stage_n_[k] = 0; }
     prof_substeps_ = 0;
     prof_wall_ms_ = 0.0;
+    if (initialized_) {
+        DeviceGuard g(device_);
+        HIPCHECK(hipStreamSynchronize(stream_));
+        HIPCHECK(hipMemset(mesh_stats_.p, 0, 8 * sizeof(uint64_t)));
+    }
+}
+
+void Solver::mesh_query_stats(uint64_t out[6]) {
+    require_init();
+    DeviceGuard g(device_);
+    uint64_t v[8];
+    HIPCHECK(hipStreamSynchronize(stream_));
+    HIPCHECK(hipMemcpy(v, mesh_stats_.p, sizeof(v), hipMemcpyDeviceToHost));
+    std::memcpy(out, v, 6 * sizeof(uint64_t));
 }
 
 size_t Solver::profile_stages(char* names, size_t names_len, double* ms, uint64_t* launches, size_t cap) {

@@ -26,6 +26,16 @@ public:
     ~Solver();
 
     void init(const xpbd_config& cfg, size_t n, size_t max_contacts);
+    // Batched mode: num_scenes independent particle scenes (counts[k]
+    // particles each, concatenated in order), each with its own hash grid and
+    // ground plane, advanced together (one launch per stage for all scenes).
+    void init_batch(const xpbd_config& cfg, const size_t* counts, size_t num_scenes, size_t max_contacts);
+    size_t scene_count() const { return num_scenes_; }
+    // hipGraph capture of whole frames in step() (default on; VKXPBD_GRAPHS=0 disables)
+    void set_graph_mode(bool on) { use_graphs_ = on; invalidate_graphs(); }
+    // Times candidate launch shapes on the current state (then restores it) and
+    // keeps the fastest; returns the chosen particle-particle block size.
+    int autotune(int frames, float dt, int substeps, int iterations);
     void add_mesh(const float* v, size_t nv, const uint32_t* t, size_t nt, float margin, const float* rot, int wt);
     void finalize_meshes();
     void set_particles(const float* q, const float* qd, const float* w, const float* r, const int32_t* f, size_t n);
@@ -65,6 +75,10 @@ public:
     void print_profiler_summary();
     size_t profile_stages(char* names, size_t names_len, double* ms, uint64_t* launches, size_t cap);
     void profile_reset();
+    // mesh query counters gathered while profiling: active particle x mesh
+    // pairs, root-box passes, BVH node pops, closest-point tests, parity
+    // triangle tests, contacts
+    void mesh_query_stats(uint64_t out[6]);
 
 private:
     struct MeshHost {
@@ -84,13 +98,18 @@ private:
     };
 
     void require_init() const;
+    void init_common(const xpbd_config& cfg, size_t n, size_t max_contacts);
+    void invalidate_graphs();
+    void record_frame(float dt, int substeps, int iterations);
     void alloc_all();
     void free_all();
     void upload_nodes();
+    void upload_parity_grids();
     BodyArgs body_args() const;
     void substep(float sdt, int iterations, bool collide, int slot, bool zero_next);
     BodyStepArgs body_step_args(float sdt);
     void run_sort(int tiles);
+    uint32_t* first_keys();
     void read_count(int slot);
     void stage_begin(Stage s);
     void stage_end(Stage s);
@@ -102,6 +121,22 @@ private:
     hipStream_t stream_ = nullptr;
     size_t n_ = 0, cmax_ = 0;
 
+    // batched scenes
+    size_t num_scenes_ = 1;
+    std::vector<size_t> scene_off_;
+    DevBuf<int32_t> scene_d_;
+    int sort_passes_ = kSortPasses;
+    int pp_block_ = 256;
+    // captured frames: rotation state and parameters -> executable graph
+    struct FrameGraph {
+        int s, p, o;
+        float dt;
+        int substeps, iterations;
+        int s2, p2, o2;
+        hipGraphExec_t exec;
+    };
+    std::vector<FrameGraph> graphs_;
+    bool use_graphs_ = true;
     // particle state: 3 rotating float4 buffers each (state / predicted / out)
     DevBuf<float4> P_[3], V_[3];
     int s_ = 0, p_ = 1, o_ = 2;
@@ -139,12 +174,15 @@ private:
     std::vector<size_t> vert_off_, tri_off_, node_off_, prim_off_;
     DevBuf<float> verts_d_, vvel_d_;
     DevBuf<uint32_t> tris_d_, prim_d_, err_d_;
-    DevBuf<float4> nl_d_, nu_d_;
+    DevBuf<float4> nodes_d_;   // interleaved (lower, upper) node boxes, re-based
+    std::vector<ParityGrid> pgrid_;        // per mesh (empty for non-watertight)
+    DevBuf<uint32_t> pg_start_, pg_tri_;   // all meshes' grids, concatenated
     DevBuf<MeshInfo> info_d_;
     std::vector<MeshInfo> info_h_;
     float avg_edge_ = 0.f;
     // diagnostics
     DevBuf<uint32_t> diag_header_;
+    DevBuf<uint64_t> mesh_stats_;
     DevBuf<float> diag_pairs_;
     uint32_t max_pairs_ = 100000;
     // profiler

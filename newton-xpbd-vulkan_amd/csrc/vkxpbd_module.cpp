@@ -398,6 +398,50 @@ PYBIND11_MODULE(_vkxpbd, m) {
              "Per-stage accumulated GPU milliseconds and launch counts (enable_profile).")
         .def("profile_reset", [](PySolver& s) { s.check(xpbd_profile_reset(s.h)); })
         .def(
+            "init_batch",
+            [](PySolver& s, const CfgWrap& c, py::array_t<int64_t, py::array::c_style | py::array::forcecast> counts,
+               size_t mc) {
+                std::vector<uint64_t> cnt(size_t(counts.size()));
+                for (size_t k = 0; k < cnt.size(); k++) {
+                    if (counts.data()[k] < 0) throw std::runtime_error("init_batch: negative scene size");
+                    cnt[k] = uint64_t(counts.data()[k]);
+                }
+                s.check(xpbd_init_batch(s.h, &c.c, cnt.data(), cnt.size(), mc));
+            },
+            py::arg("config"), py::arg("scene_counts"), py::arg("max_contacts") = 0,
+            "Batched scenes (extension): scene_counts[k] particles per scene, concatenated in scene order.")
+        .def("mesh_query_stats",
+             [](PySolver& s) {
+                 uint64_t v[6];
+                 s.check(xpbd_mesh_query_stats(s.h, v));
+                 py::dict d;
+                 const char* k[6] = {"pairs", "root_passes", "node_pops", "closest_tests", "parity_tests", "contacts"};
+                 for (int i = 0; i < 6; i++) d[k[i]] = v[i];
+                 return d;
+             })
+        .def_property_readonly("scene_count",
+                               [](const PySolver& s) {
+                                   size_t v = 0;
+                                   xpbd_scene_count(s.h, &v);
+                                   return v;
+                               })
+        .def(
+            "set_graph_mode", [](PySolver& s, bool on) { s.check(xpbd_set_graph_mode(s.h, on ? 1 : 0)); },
+            py::arg("enabled"), "Capture and replay whole frames as hipGraphs (default on).")
+        .def(
+            "autotune",
+            [](PySolver& s, int frames, float dt, int substeps, int iterations) {
+                int b = 0, rc;
+                {
+                    py::gil_scoped_release nogil;
+                    rc = xpbd_autotune(s.h, frames, dt, substeps, iterations, &b);
+                }
+                s.check(rc);
+                return b;
+            },
+            py::arg("frames") = 3, py::arg("dt") = 1.0f / 60.0f, py::arg("substeps") = 8, py::arg("iterations") = 1,
+            "Pick the fastest particle-particle block size on the current state (state is restored).")
+        .def(
             "add_box_body",
             [](PySolver& s, farr pos, farr quat, farr half_extents, float density, int flags) {
                 if (pos.size() < 3 || quat.size() < 4 || half_extents.size() < 3)

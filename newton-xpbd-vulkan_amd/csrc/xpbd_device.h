@@ -127,4 +127,37 @@ __device__ __forceinline__ int wave_reserve(int* counter, bool want) {
     return base + __popcll(below);
 }
 
+// Inclusive prefix sum over the 64 lanes of a wave (all lanes active).
+__device__ __forceinline__ int wave_incl_scan_i(int x) {
+    const int lane = __lane_id();
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    return x;
+}
+
+// Block-wide slot reservation: every thread of the block calls this once with
+// the number of slots it needs; it returns the thread's first slot.  One
+// global atomic per block (none if the block needs nothing): same-address
+// atomics serialise across the chip, so aggregating them is what keeps
+// contact emission from throttling the kernels that emit.
+// s_tmp must hold blockDim.x / 64 + 1 ints.
+__device__ __forceinline__ int block_reserve(int* counter, int count, int* s_tmp) {
+    const int lane = __lane_id(), w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const int incl = wave_incl_scan_i(count);
+    if (lane == 63) s_tmp[w] = incl;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int tot = 0;
+        for (int k = 0; k < nw; k++) { const int c = s_tmp[k]; s_tmp[k] = tot; tot += c; }
+        s_tmp[nw] = tot > 0 ? atomicAdd(counter, tot) : 0;
+    }
+    __syncthreads();
+    const int base = s_tmp[nw] + s_tmp[w] + (incl - count);
+    __syncthreads();
+    return base;
+}
+
 }  // namespace xpbd

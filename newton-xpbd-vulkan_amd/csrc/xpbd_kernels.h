@@ -9,7 +9,8 @@ namespace xpbd {
 // tiles of 256 threads x 4 keys.
 constexpr int kSortBits = 7;
 constexpr int kSortBins = 1 << kSortBits;
-constexpr int kSortPasses = 3;
+constexpr int kSortPasses = 3;      // plain solver: 21-bit cell keys
+constexpr int kCellBits = 21;       // batched solver: key = scene << 21 | cell
 constexpr int kSortItems = 4;
 constexpr int kSortTile = 256 * kSortItems;
 
@@ -39,7 +40,8 @@ struct IntegrateArgs {
     uint32_t* keys; uint32_t* hist;
     float dt, vmax, gx, gy, gz, inv_cell;
     int has_plane; float margin, altitude;
-    int cmax; int* cc; int* cc_next; int* rcc_next;
+    int cmax; int* cc; int* cc_next; int* rcc_next; int* mesh_next;
+    const int* scene;   // batched mode: scene of each particle (key = scene << 21 | cell); else null
     int* cpart; int* cshape; float4* cbpos; float4* cbvel; float4* cnorm;
 };
 
@@ -61,6 +63,7 @@ struct ShapeArgs {
 
 struct PPArgs {
     int n;
+    const uint32_t* keys;   // sorted keys (batched mode: scene << 21 | cell)
     const uint32_t* vals; const int* range;
     const float4* Ps;                               // substep-start positions, original order
     const float4* Xs; const float4* Vs; const int* Fs;   // predicted state, sorted order
@@ -87,17 +90,26 @@ struct MeshInfo {
     uint32_t watertight;
     uint32_t prim_off;
     uint32_t prim_cnt;
-    uint32_t pad[3];
+    // projected parity grid (watertight meshes): origin, 1/cell, cells per
+    // axis, first cell in the global cell-start table
+    float pg_u0, pg_w0, pg_inv;
+    uint32_t pg_dim, pg_cell_off;
+    uint32_t pad[2];
 };
 
 struct MeshArgs {
     int n; const float4* P; const int* flags;
-    const float* verts; const uint32_t* tris; const float4* nl; const float4* nu; const uint32_t* prim;
+    const float* verts; const uint32_t* tris;
+    const float4* nodes;   // interleaved (lower, upper) per node
+    const uint32_t* prim;
     const float* vvel; const MeshInfo* info; int nmesh;
     float margin, eps;
     int cmax; int* cc;
     int* cpart; int* cshape; float4* cbpos; float4* cbvel; float4* cnorm;
     uint32_t* err;
+    const uint32_t* pg_start;   // projected parity grids: CSR cell starts ...
+    const uint32_t* pg_tri;     // ... and global triangle ids
+    uint64_t* stats;            // optional query counters (6 words), null = off
 };
 
 // Rigid bodies (dynamic bodies occupy indices [b0, b1) of the body arrays).
@@ -147,7 +159,9 @@ void launch_plane_contacts(const PlaneArgs& a, hipStream_t s);
 void launch_cell_ranges(int n, const uint32_t* keys, int* range, hipStream_t s);
 void launch_cell_tables(int n, const uint32_t* keys, int* starts, int* ends, hipStream_t s);
 void launch_shape_solve(const ShapeArgs& a, int grid, hipStream_t s);
-void launch_pp_apply(const PPArgs& a, bool diag, hipStream_t s);
+// validate: the cell table is not cleared (batched mode); an entry is used
+// only if keys[start] equals the looked-up key.  block: 64, 128 or 256.
+void launch_pp_apply(const PPArgs& a, bool diag, bool validate, int block, hipStream_t s);
 void launch_restitution(const RestArgs& a, int grid, hipStream_t s);
 void launch_zero_int(int* p, int count, hipStream_t s);
 void launch_frame_begin(int* cc, hipStream_t s);

@@ -94,6 +94,48 @@ def config1(seed=1, nx=128, ny=128, nz=64, level=6):
                 meshes=[(verts, tris)], frames=60, dt=1.0 / 60.0, substeps=8, iterations=1)
 
 
+def pile(n, r, rng, z0=None):
+    """n particles of radius r stacked on a near-cubic lattice (spacing 2r,
+    jitter 0.1r) above the origin; the last layer may be partial."""
+    side = max(1, int(round(n ** (1.0 / 3.0))))
+    nz = -(-n // (side * side))
+    q = lattice(side, side, nz, 2 * r, (-(side - 1) * r, -(side - 1) * r, r * 1.2 if z0 is None else z0), 0.1 * r, rng)
+    return q[:n]
+
+
+def config3(seed=3, num_scenes=256, lo=1024, hi=65536):
+    """num_scenes independent piles, particle counts log-uniform on [lo, hi]
+    (seeded), r = 0.005, each over its own ground plane and hash grid."""
+    rng = np.random.default_rng(seed)
+    r = 0.005
+    counts = np.exp(rng.uniform(np.log(lo), np.log(hi), num_scenes)).astype(np.int64)
+    parts = [pile(int(c), r, rng) for c in counts]
+    q = np.concatenate(parts).astype(np.float32)
+    return dict(name=f"batch_{num_scenes}_scenes_{q.shape[0]}", r=r, particles=_particles(q, r), meshes=[],
+                counts=counts, frames=60, dt=1.0 / 60.0, substeps=8, iterations=1)
+
+
+def scene_slice(scene, k):
+    """Scene k of a batched scene dict, as a plain single-scene dict."""
+    off = np.concatenate([[0], np.cumsum(scene["counts"])])
+    a, b = int(off[k]), int(off[k + 1])
+    p = {key: v[a:b] for key, v in scene["particles"].items()}
+    return dict(name=f"{scene['name']}[{k}]", r=scene["r"], particles=p, meshes=list(scene["meshes"]))
+
+
+def build_batch_solver(vkxpbd_module, scene, max_contacts=0, **cfg_overrides):
+    """Initialised batched XpbdSolver for a config3-style scene dict."""
+    cfg = solver_config(vkxpbd_module, scene["r"], **cfg_overrides)
+    s = vkxpbd_module.XpbdSolver()
+    p = scene["particles"]
+    s.init_batch(cfg, np.asarray(scene["counts"], np.int64), max_contacts)
+    for verts, tris in scene["meshes"]:
+        s.add_mesh(verts.reshape(-1), tris.reshape(-1))
+    s.finalize_meshes()
+    s.set_particles(p["q"], p["qd"], p["inv_mass"], p["radii"], p["flags"])
+    return s
+
+
 def config4(seed=4):
     """32,768 particles (32^3, r = 0.01, spacing 2r, jitter 0.1r) with
     N(0, 0.5) initial velocities; 64 substeps of 1/480 s; seeded targets."""

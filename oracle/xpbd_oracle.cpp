@@ -1500,6 +1500,31 @@ int orc_ray_tri_woop_f(const float* p, const float* dir, const float* a, const f
     return orc::ray_tri_woop<float>(V{p[0], p[1], p[2]}, V{dir[0], dir[1], dir[2]}, V{a[0], a[1], a[2]}, V{b[0], b[1], b[2]},
                                     V{c[0], c[1], c[2]}) ? 1 : 0;
 }
+// Hits of the reference's fixed-direction parity ray (generate_mesh_contacts.comp:372-388)
+// from each point over all triangles: out_hits[k] = number of hits; hit ids
+// (point-major) written to ids up to ids_cap.  Returns the total hit count.
+int64_t orc_ray_hit_lists(const float* verts, const uint32_t* tris, int64_t nt, const float* pts, int64_t np,
+                          int32_t* out_hits, uint32_t* ids, int64_t ids_cap) {
+    using V = orc::V3<float>;
+    const V dir{1.029700994f, 1.086472273f, 1.034694910f};
+    int64_t w = 0;
+    for (int64_t k = 0; k < np; k++) {
+        const V p{pts[3 * k], pts[3 * k + 1], pts[3 * k + 2]};
+        int32_t h = 0;
+        for (int64_t t = 0; t < nt; t++) {
+            const float* a = verts + 3 * size_t(tris[3 * t]);
+            const float* b = verts + 3 * size_t(tris[3 * t + 1]);
+            const float* c = verts + 3 * size_t(tris[3 * t + 2]);
+            if (orc::ray_tri_woop<float>(p, dir, V{a[0], a[1], a[2]}, V{b[0], b[1], b[2]}, V{c[0], c[1], c[2]})) {
+                if (w < ids_cap) ids[w] = uint32_t(t);
+                w++;
+                h++;
+            }
+        }
+        out_hits[k] = h;
+    }
+    return w;
+}
 void orc_closest_point_triangle_f(const float* p, const float* a, const float* b, const float* c, float* cp, float* bary) {
     using V = orc::V3<float>;
     V o, tn, bc;

@@ -75,6 +75,8 @@ def lib():
         L.orc_lbvh_build.argtypes = [_fp, _up, C.c_uint32, _fp, _fp, _up, _up]
         L.orc_ray_tri_woop_f.argtypes = [_fp] * 5
         L.orc_closest_point_triangle_f.argtypes = [_fp] * 6
+        L.orc_ray_hit_lists.restype = C.c_int64
+        L.orc_ray_hit_lists.argtypes = [_fp, _up, C.c_int64, _fp, C.c_int64, _ip, _up, C.c_int64]
         _lib = L
     return _lib
 
@@ -243,6 +245,24 @@ def lbvh_build(verts, tris):
 def ray_tri(p, d, a, b, c):
     arrs = [np.ascontiguousarray(x, np.float32) for x in (p, d, a, b, c)]
     return bool(lib().orc_ray_tri_woop_f(*[_ptr(x, _fp) for x in arrs]))
+
+
+def ray_hit_lists(verts, tris, pts):
+    """Per point, the triangle ids the fixed-direction parity ray hits (all-triangle loop)."""
+    v = np.ascontiguousarray(verts, np.float32).reshape(-1)
+    t = np.ascontiguousarray(tris, np.uint32).reshape(-1)
+    p = np.ascontiguousarray(pts, np.float32).reshape(-1)
+    nt, npt = t.size // 3, p.size // 3
+    hits = np.zeros(npt, np.int32)
+    cap = 8 * npt + 16
+    ids = np.zeros(cap, np.uint32)
+    tot = lib().orc_ray_hit_lists(_ptr(v, _fp), _ptr(t, _up), nt, _ptr(p, _fp), npt, _ptr(hits, _ip), _ptr(ids, _up), cap)
+    assert tot <= cap
+    out, o = [], 0
+    for h in hits:
+        out.append(ids[o:o + h].copy())
+        o += h
+    return out
 
 
 def closest_point_triangle(p, a, b, c):

@@ -303,3 +303,71 @@ def test_boxes_and_particles(vk):
     gq = g.get_body_q()
     assert np.all(np.isfinite(gq)) and np.all(gq[:, 2] > 0.0)        # boxes rest above the plane
     np.testing.assert_allclose(np.linalg.norm(gq[:, 3:], axis=1), 1.0, atol=1e-5)
+
+
+def test_graph_replay_equals_eager(vk):
+    """Captured frames (hipGraph replay) give the same bits as eager launches,
+    across an update_mesh (which drops the captured frames) and a change of
+    the step parameters."""
+    sc = scenes.config0()
+    v, t = scenes.icosphere(2, 0.1, (0.0, 0.0, 0.1))
+    sc["meshes"] = [(v, t)]
+    a = scenes.build_solver(vk, sc)
+    b = scenes.build_solver(vk, sc)
+    b.set_graph_mode(False)
+    for f in range(12):
+        if f == 6:
+            v2 = v + np.float32([0.01, 0.0, 0.0])
+            a.update_mesh(0, v2.reshape(-1))
+            b.update_mesh(0, v2.reshape(-1))
+        it = 1 if f < 9 else 2
+        a.step(1.0 / 60.0, 8, it)
+        b.step(1.0 / 60.0, 8, it)
+        np.testing.assert_array_equal(a.get_positions(), b.get_positions())
+        np.testing.assert_array_equal(a.get_velocities(), b.get_velocities())
+        assert a.last_contact_count == b.last_contact_count
+    log(f"graph replay == eager over 12 frames, contacts {a.last_contact_count}")
+
+
+def test_batched_scenes_match_sequential(vk):
+    """A batched solver over uneven scenes equals one plain solver per scene,
+    bit for bit (each scene keeps its own grid, neighbour order and plane)."""
+    sc = scenes.config3(seed=9, num_scenes=7, lo=64, hi=3000)
+    sc["counts"][3] = 1   # a single-particle scene
+    off = np.concatenate([[0], np.cumsum(sc["counts"])])
+    sc["particles"] = {k: np.concatenate([scenes.scene_slice(sc, j)["particles"][k] for j in range(7)])
+                       for k in sc["particles"]}
+    log(f"batched scenes: counts {list(sc['counts'])}")
+    bs = scenes.build_batch_solver(vk, sc)
+    assert bs.scene_count == 7
+    singles = [scenes.build_solver(vk, scenes.scene_slice(sc, k)) for k in range(7)]
+    for f in range(20):
+        bs.step(1.0 / 60.0, 8, 1)
+        tot = 0
+        for s in singles:
+            s.step(1.0 / 60.0, 8, 1)
+            tot += s.last_contact_count
+        gp = bs.get_positions()
+        gv = bs.get_velocities()
+        for k, s in enumerate(singles):
+            np.testing.assert_array_equal(gp[off[k]:off[k + 1]], s.get_positions(), err_msg=f"scene {k} frame {f}")
+            np.testing.assert_array_equal(gv[off[k]:off[k + 1]], s.get_velocities(), err_msg=f"scene {k} frame {f}")
+        assert bs.last_contact_count == tot
+    # scenes overlap in space but never interact: keys carry the scene index
+    keys = bs.get_sorted_cells()
+    assert np.all(np.diff(keys.astype(np.int64)) >= 0)
+    assert int(keys.max()) >> 21 == 6
+
+
+def test_autotune_keeps_state(vk):
+    sc = scenes.config0()
+    a = scenes.build_solver(vk, sc)
+    b = scenes.build_solver(vk, sc)
+    blk = a.autotune(2, 1.0 / 60.0, 8, 1)
+    log(f"autotune picked pp block {blk}")
+    assert blk in (64, 128, 256)
+    for _ in range(3):
+        a.step(1.0 / 60.0, 8, 1)
+        b.step(1.0 / 60.0, 8, 1)
+    np.testing.assert_array_equal(a.get_positions(), b.get_positions())
+    np.testing.assert_array_equal(a.get_velocities(), b.get_velocities())
