@@ -57,6 +57,78 @@ def reset(s, sc):
     s.set_particles(p["q"], p["qd"], p["inv_mass"], p["radii"], p["flags"])
 
 
+def bench_batched(vkxpbd, scenes, frames):
+    """256 scenes (log-uniform 1,024..65,536 particles) advanced by one batched
+    solver (frame graphs, autotuned block size) against 256 plain solvers
+    stepped one after another, same frames; final states compared."""
+    sc = scenes.config3()
+    counts = sc["counts"]
+    total = int(counts.sum())
+    b = scenes.build_batch_solver(vkxpbd, sc)
+    block = b.autotune(2, DT, SUBSTEPS, 1)
+    b.step(DT, SUBSTEPS, 1)                       # capture
+    p = sc["particles"]
+    b.set_particles(p["q"], p["qd"], p["inv_mass"], p["radii"], p["flags"])
+    t0 = time.perf_counter()
+    for _ in range(frames):
+        b.step(DT, SUBSTEPS, 1)
+    tb = time.perf_counter() - t0
+    xb = b.get_positions()
+    del b
+    singles = [scenes.build_solver(vkxpbd, scenes.scene_slice(sc, k)) for k in range(len(counts))]
+    for s in singles:                             # capture each solver's frame graph
+        s.step(DT, SUBSTEPS, 1)
+    for k, s in enumerate(singles):
+        q = scenes.scene_slice(sc, k)["particles"]
+        s.set_particles(q["q"], q["qd"], q["inv_mass"], q["radii"], q["flags"])
+    t0 = time.perf_counter()
+    for _ in range(frames):
+        for s in singles:
+            s.step(DT, SUBSTEPS, 1)
+    ts = time.perf_counter() - t0
+    xs = np.concatenate([s.get_positions() for s in singles])
+    del singles
+    return {"workload": sc["name"], "scenes": int(len(counts)), "particles": total, "frames_timed": frames,
+            "value": total * SUBSTEPS * frames / tb, "unit": "particle-substeps/s",
+            "ms_per_frame": 1e3 * tb / frames, "sequential_ms_per_frame": 1e3 * ts / frames,
+            "speedup_vs_sequential": ts / tb, "pp_block": int(block),
+            "max_abs_diff_vs_sequential_m": float(np.max(np.abs(xb.astype(np.float64) - xs)))}
+
+
+def bench_adjoint(vkxpbd, scenes, reps=3):
+    """Differentiable rollout of 32,768 particles, 64 substeps of 1/480 s,
+    checkpoints every 8: forward and reverse pass times (GPU-resident
+    tensors), and their ratio."""
+    import torch
+    from vkxpbd.autograd import rollout
+    sc = scenes.config4()
+    p = sc["particles"]
+    s = scenes.build_solver(vkxpbd, sc)
+    dev = torch.device("cuda:0")
+    x0 = torch.tensor(p["q"], device=dev)
+    tgt = torch.tensor(sc["target"], device=dev)
+    fw, bw = [], []
+    for k in range(reps + 1):
+        v0 = torch.tensor(p["qd"], device=dev, requires_grad=True)
+        relax = torch.tensor(0.9, device=dev, requires_grad=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        xs, _ = rollout(s, x0, v0, relax, 1.0 / 480.0, 64, 8)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        loss = ((xs - tgt) ** 2).sum(dim=1).mean()
+        loss.backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        if k > 0:
+            fw.append(t1 - t0)
+            bw.append(t2 - t1)
+    f, b = float(np.median(fw)), float(np.median(bw))
+    return {"workload": "rollout_32768_64_substeps_checkpoint8", "forward_ms": 1e3 * f, "backward_ms": 1e3 * b,
+            "bwd_fwd_ratio": b / f, "loss": float(loss.detach().cpu()),
+            "grad_relax": float(relax.grad.cpu()), "grad_v0_max": float(v0.grad.abs().max().cpu())}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -64,6 +136,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--batched-frames", type=int, default=10)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -144,6 +219,11 @@ def main():
         out["gpu_ms_per_frame_profiled"] = sum(v[0] for v in prof.values()) / args.steps
         out["contacts_mean"] = C
         out["occupied_cells"] = U
+
+    if not args.no_batched:
+        out["batched"] = bench_batched(vkxpbd, scenes, args.batched_frames)
+    if not args.no_adjoint:
+        out["adjoint"] = bench_adjoint(vkxpbd, scenes)
 
     if not args.no_cpu_baseline:
         sys.path.insert(0, os.path.join(ROOT, "tests"))

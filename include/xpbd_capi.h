@@ -200,9 +200,25 @@ int xpbd_body_count(const xpbd_solver* h, size_t* out);
 int xpbd_get_bodies(xpbd_solver* h, float* q7, float* qd6, size_t cap);
 int xpbd_set_body_velocities(xpbd_solver* h, const float* qd6, size_t n);
 
-/* Mesh narrowphase counters accumulated while profiling (enable_profile or
- * VKXPBD_PROFILE=1) since the last xpbd_profile_reset: out[0] active
- * particle x mesh pairs, [1] root-box passes, [2] BVH node pops, [3]
+/* Differentiable rollout (extension; the reference has no gradient path).
+ * Runs `substeps` substeps of length sdt (one iteration each) from positions
+ * x0 and velocities v0 with contact relaxation `relax`, keeping state
+ * checkpoints every `checkpoint_every` substeps.  x0, v0, x_out, v_out are
+ * DEVICE pointers to 3 floats per particle on the solver's device (x0/v0 NULL:
+ * current state; outputs may be NULL).  The solver's state afterwards is the
+ * final state.  Dynamic boxes and restitution are not supported. */
+int xpbd_rollout_forward(xpbd_solver* h, const float* x0, const float* v0, float relax, float sdt, int substeps,
+                         int checkpoint_every, float* x_out, float* v_out);
+/* Reverse pass of the last rollout_forward: cotangents of the final positions
+ * gx (required) and velocities gv (NULL = zero) -> cotangents of x0 and v0
+ * (device pointers, NULL to skip) and of the relaxation (*grelax, host).  The
+ * contact set of every substep is the forward's (frozen).  Restores the
+ * forward's final state. */
+int xpbd_rollout_backward(xpbd_solver* h, const float* gx, const float* gv, float* gx0, float* gv0, double* grelax);
+
+/* Mesh narrowphase counters accumulated while the environment variable
+ * VKXPBD_MESH_STATS=1 was set at init, since the last xpbd_profile_reset: out[0] active
+ * particle x mesh pairs, [1] particles needing the tree search, [2] BVH node pops, [3]
  * closest-point triangle tests, [4] parity ray/triangle tests, [5] contacts.
  * Extension (the reference's mesh diagnostics buffer is never written). */
 int xpbd_mesh_query_stats(xpbd_solver* h, uint64_t* out6);
@@ -246,6 +262,12 @@ int xpbd_lbvh_refit(const float* verts, const float* new_verts, size_t num_verts
 int xpbd_parity_grid_candidates(const float* verts, size_t num_verts, const uint32_t* tris, size_t num_tris,
                                 const float* points, size_t num_points, uint32_t* counts, uint32_t* ids,
                                 size_t ids_cap, size_t* total);
+/* Host only: builds the distance lower-bound grid of a mesh for the given cap
+ * and returns, per point, the bound the mesh query uses for it (-1 for a
+ * point outside the grid, which is farther than cap from every triangle).
+ * Exists so the bounds can be checked against exact distances without a GPU. */
+int xpbd_dist_grid_bounds(const float* verts, size_t num_verts, const uint32_t* tris, size_t num_tris, float cap,
+                          const float* points, size_t num_points, float* bounds);
 
 #ifdef __cplusplus
 }

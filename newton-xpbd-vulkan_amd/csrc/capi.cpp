@@ -216,6 +216,13 @@ int xpbd_add_box_body(xpbd_solver* h, const float* pos, const float* quat, const
 int xpbd_set_body_params(xpbd_solver* h, float ad, float relax, float mt, float mr, int uw) {
     return guarded(h, [&] { h->s.set_body_params(ad, relax, mt, mr, uw); });
 }
+int xpbd_rollout_forward(xpbd_solver* h, const float* x0, const float* v0, float relax, float sdt, int substeps,
+                         int every, float* x_out, float* v_out) {
+    return guarded(h, [&] { h->s.rollout_forward(x0, v0, relax, sdt, substeps, every, x_out, v_out); });
+}
+int xpbd_rollout_backward(xpbd_solver* h, const float* gx, const float* gv, float* gx0, float* gv0, double* grelax) {
+    return guarded(h, [&] { h->s.rollout_backward(gx, gv, gx0, gv0, grelax); });
+}
 int xpbd_mesh_query_stats(xpbd_solver* h, uint64_t* out6) { return guarded(h, [&] { h->s.mesh_query_stats(out6); }); }
 int xpbd_init_batch(xpbd_solver* h, const xpbd_config* c, const uint64_t* counts, size_t ns, size_t mc) {
     return guarded(h, [&] {
@@ -299,6 +306,21 @@ int xpbd_parity_grid_candidates(const float* verts, size_t nv, const uint32_t* t
             counts[k] = cnt;
         }
         if (total) *total = w;
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return 1;
+    }
+}
+
+int xpbd_dist_grid_bounds(const float* verts, size_t nv, const uint32_t* tris, size_t nt, float cap, const float* points,
+                          size_t np, float* bounds) {
+    try {
+        for (size_t k = 0; k < 3 * nt; k++)
+            if (tris[k] >= nv) throw std::runtime_error("distance grid: vertex index out of range");
+        xpbd::DistGrid g;
+        xpbd::dist_grid_build(verts, tris, uint32_t(nt), cap, g);
+        for (size_t k = 0; k < np; k++) bounds[k] = g.lb.empty() ? -1.0f : xpbd::dist_grid_bound(g, points + 3 * k);
         return 0;
     } catch (const std::exception& e) {
         g_err = e.what();

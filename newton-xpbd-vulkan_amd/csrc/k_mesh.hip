@@ -1,9 +1,11 @@
 // k_mesh.hip -- particle-vs-triangle-mesh narrowphase over the packed LBVH,
 // the replacement of shaders/generate_mesh_contacts.comp:181-414.
 //
-// One kernel, 1,024 particles per workgroup (4 per thread), one block-level
-// slot reservation per mesh for the contacts it emits (same-address global
-// atomics serialise, so per-wave reservations throttle the kernel):
+// One kernel, one particle per thread, one block-level slot reservation per
+// mesh for the contacts it emits (same-address global atomics serialise, so
+// per-wave reservations throttle the kernel):
+//   * early outs: the root box and a grid of distance lower bounds; the
+//     particles left are compacted within the block before the tree walk;
 //   * closest triangle: depth-first search with the DFS stack in LDS (one
 //     32-deep column per thread, bank = thread), the reference's pruning, push
 //     order and strict-less tie rule, so the chosen face is the same;
@@ -22,7 +24,6 @@
 namespace xpbd {
 
 constexpr int kMeshBlock = 256;
-constexpr int kMeshItems = 4;
 constexpr int kStack = 32;
 
 __device__ __forceinline__ f3 ldv(const float* v, uint32_t k) { return {v[3 * k], v[3 * k + 1], v[3 * k + 2]}; }
@@ -166,14 +167,15 @@ __device__ __forceinline__ uint32_t parity_hits(const MeshArgs& a, const MeshInf
 // strict-less tie rule.  Returns whether a contact is emitted, with the face,
 // barycentrics (v, w) and, for non-watertight meshes, the pseudo-normal sign.
 template <bool STATS>
-__device__ bool closest_face(const MeshArgs& a, const MeshInfo& mi, f3 p, float rad, uint32_t (*stk)[kMeshBlock], int lt,
-                             uint32_t& face, float& sv_out, float& sw_out, float& sign, uint32_t& pops, uint32_t& tests) {
+__device__ bool closest_face(const MeshArgs& a, const MeshInfo& mi, uint32_t start, f3 p, float rad,
+                             uint32_t (*stk)[kMeshBlock], int lt, uint32_t& face, float& sv_out, float& sw_out,
+                             float& sign, uint32_t& pops, uint32_t& tests) {
     const float eps = a.eps, eps2 = eps * eps;
     const float PI = 3.14159265359f;
     const float max_dist = (a.margin + mi.margin) + rad;
     const bool wt = mi.watertight != 0u;
     int sp = 0;
-    stk[sp++][lt] = mi.root;
+    stk[sp++][lt] = start;
     float min_dist = max_dist;
     f3 acc_n = {0.f, 0.f, 0.f};
     uint32_t min_face = 0;
@@ -267,67 +269,87 @@ __device__ bool closest_face(const MeshArgs& a, const MeshInfo& mi, f3 p, float 
     return true;
 }
 
-// One block = kMeshItems x 128 particles; per mesh: the queries, then one
-// block-level reservation, then the contact payloads
-// (generate_mesh_contacts.comp:333-411).
-// STATS: query counters for tuning (active particle x mesh pairs, root-box
-// passes, node pops, closest-point tests, parity triangle tests, contacts),
-// summed per wave into a.stats.
+// Conservative early out from the mesh's distance lower-bound grid: returns
+// the node the closest-face search starts at, or kNoNode if no triangle lies
+// within thr of p.  The bounds are valid below the grid's cap; beyond the
+// grid every triangle is farther than the cap.  The pad (2e-5 at unit
+// coordinates) is far above the rounding of the device's distance and cell
+// computations.  The start node is the root unless the grid applies.
+constexpr uint32_t kNoNode = 0xFFFFFFFFu;
+__device__ __forceinline__ uint32_t search_start(const MeshArgs& a, const MeshInfo& mi, f3 p, float thr) {
+    const float sc = fmaxf(fmaxf(fabsf(p.x), fabsf(p.y)), fabsf(p.z));
+    const float lim = thr + 1e-5f * (1.0f + sc);
+    if (!(lim < mi.lb_cap)) return mi.root;
+    const float fx = floorf((p.x - mi.lb_ox) * mi.lb_inv), fy = floorf((p.y - mi.lb_oy) * mi.lb_inv),
+                fz = floorf((p.z - mi.lb_oz) * mi.lb_inv);
+    if (!(fx >= 0.0f && fy >= 0.0f && fz >= 0.0f && fx < float(mi.lb_nx) && fy < float(mi.lb_ny) && fz < float(mi.lb_nz)))
+        return kNoNode;
+    const uint32_t c = mi.lb_off + (uint32_t(fz) * mi.lb_ny + uint32_t(fy)) * mi.lb_nx + uint32_t(fx);
+    return a.lb[c] > lim ? kNoNode : a.lb_node[c];
+}
+
+// STATS: query counters for tuning (active particle x mesh pairs, particles
+// that need the tree walk, node pops, closest-point tests, parity triangle
+// tests, contacts), summed per wave into a.stats.
 __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
     return v;
 }
 
+// One thread per particle.  Per mesh: the reference's root-box test plus the
+// distance-bound early out, then the particles that remain are compacted to
+// the front of the block (block order kept) so the tree walks run on dense
+// waves, then one block-level reservation for the contacts and the payloads
+// (generate_mesh_contacts.comp:333-411).
 template <bool STATS>
 __global__ __launch_bounds__(kMeshBlock) void k_mesh_contacts(MeshArgs a) {
     __shared__ uint32_t stk[kStack][kMeshBlock];
-    uint32_t st[6] = {0, 0, 0, 0, 0, 0};
+    __shared__ int s_list[kMeshBlock];
+    __shared__ uint32_t s_start[kMeshBlock];
     __shared__ int s_tmp[kMeshBlock / 64 + 1];
+    uint32_t st[6] = {0, 0, 0, 0, 0, 0};
     const int lt = threadIdx.x;
-    float4 P[kMeshItems];
-    bool act[kMeshItems];
-#pragma unroll
-    for (int it = 0; it < kMeshItems; it++) {
-        const int i = (blockIdx.x * kMeshItems + it) * kMeshBlock + lt;
-        act[it] = false;
-        P[it] = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (i < a.n && (a.flags[i] & 1)) { act[it] = true; P[it] = a.P[i]; }
-    }
+    const int i = blockIdx.x * kMeshBlock + lt;
+    const bool act = i < a.n && (a.flags[i] & 1);
+    const float4 Pi = act ? a.P[i] : make_float4(0.f, 0.f, 0.f, 0.f);
     for (int ci = 0; ci < a.nmesh; ci++) {
         const MeshInfo mi = a.info[ci];
-        uint32_t face[kMeshItems];
-        float sv[kMeshItems], sw[kMeshItems], sg[kMeshItems];
-        bool hit[kMeshItems];
-        int mine = 0;
-#pragma unroll
-        for (int it = 0; it < kMeshItems; it++) {
-            hit[it] = false;
-            face[it] = 0; sv[it] = 0.f; sw[it] = 0.f; sg[it] = 1.0f;
-            if (!act[it]) continue;
+        bool cand = false;
+        uint32_t start = kNoNode;
+        if (act) {
             if (STATS) st[0]++;
-            const f3 p = xyz(P[it]);
+            const f3 p = xyz(Pi);
             // the reference's first pop: the root box against the search distance
-            const float max_dist = (a.margin + mi.margin) + P[it].w;
+            const float max_dist = (a.margin + mi.margin) + Pi.w;
+            const float thr = mi.watertight ? max_dist : max_dist + a.eps;
             const float nds = dist_sq_aabb(p, xyz(node_lo(a.nodes, mi.root)), xyz(node_hi(a.nodes, mi.root)));
-            const bool reach = mi.watertight ? !(nds > max_dist * max_dist)
-                                             : !(nds > (max_dist + a.eps) * (max_dist + a.eps));
-            if (!reach) continue;
-            if (STATS) st[1]++;
-            hit[it] = closest_face<STATS>(a, mi, p, P[it].w, stk, lt, face[it], sv[it], sw[it], sg[it], st[2], st[3]);
-            if (hit[it] && mi.watertight) sg[it] = (parity_hits(a, mi, make_ray(p), st[4]) & 1u) ? -1.0f : 1.0f;
-            mine += hit[it] ? 1 : 0;
+            if (!(nds > thr * thr)) start = search_start(a, mi, p, thr);
+            cand = start != kNoNode;
         }
-        if (STATS) st[5] += mine;
-        int slot = block_reserve(a.cc, mine, s_tmp);
-#pragma unroll
-        for (int it = 0; it < kMeshItems; it++) {
-            if (!hit[it]) continue;
-            const int i = (blockIdx.x * kMeshItems + it) * kMeshBlock + lt;
-            const f3 p = xyz(P[it]);
-            const float u_m = (1.0f - sv[it]) - sw[it], v_m = sv[it];
-            const uint32_t f = face[it];
-            const uint32_t i0 = a.tris[3 * f], i1 = a.tris[3 * f + 1], i2 = a.tris[3 * f + 2];
+        int m = 0;
+        const int pos = block_exclusive_scan(cand ? 1 : 0, s_tmp, &m);
+        if (cand) { s_list[pos] = lt; s_start[pos] = start; }
+        __syncthreads();
+        bool hit = false;
+        uint32_t face = 0;
+        float sv = 0.f, sw = 0.f, sg = 1.0f;
+        int pi = 0;
+        float4 P = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (lt < m) {
+            pi = blockIdx.x * kMeshBlock + s_list[lt];
+            P = a.P[pi];
+            const f3 p = xyz(P);
+            if (STATS) st[1]++;
+            hit = closest_face<STATS>(a, mi, s_start[lt], p, P.w, stk, lt, face, sv, sw, sg, st[2], st[3]);
+            if (hit && mi.watertight) sg = (parity_hits(a, mi, make_ray(p), st[4]) & 1u) ? -1.0f : 1.0f;
+        }
+        if (STATS) st[5] += hit ? 1u : 0u;
+        const int slot = block_reserve(a.cc, hit ? 1 : 0, s_tmp);
+        if (hit && slot < a.cmax) {
+            const f3 p = xyz(P);
+            const float u_m = (1.0f - sv) - sw, v_m = sv;
+            const uint32_t i0 = a.tris[3 * face], i1 = a.tris[3 * face + 1], i2 = a.tris[3 * face + 2];
             const f3 v0 = ldv(a.verts, i0), v1 = ldv(a.verts, i1), v2 = ldv(a.verts, i2);
             const float w_e = (1.0f - u_m) - v_m;
             const f3 tb = v1 * v_m;
@@ -339,17 +361,14 @@ __global__ __launch_bounds__(kMeshBlock) void k_mesh_contacts(MeshArgs a) {
             const f3 svel = {fma_(w2.x, w_e, vab.x), fma_(w2.y, w_e, vab.y), fma_(w2.z, w_e, vab.z)};
             const f3 dl = p - shp;
             const float dlen = exact_length(dl);
-            const f3 nn = dlen > 0.0f ? exact_div(dl, dlen) * sg[it] : f3{0.f, 0.f, 1.f};
-            const float d = dlen * sg[it];
+            const f3 nn = dlen > 0.0f ? exact_div(dl, dlen) * sg : f3{0.f, 0.f, 1.f};
+            const float d = dlen * sg;
             const f3 bp = {fma_(-nn.x, d, p.x), fma_(-nn.y, d, p.y), fma_(-nn.z, d, p.z)};
-            if (slot < a.cmax) {
-                a.cpart[slot] = i;
-                a.cshape[slot] = ci + 1;
-                a.cbpos[slot] = make_float4(bp.x, bp.y, bp.z, 0.f);
-                a.cbvel[slot] = make_float4(svel.x, svel.y, svel.z, 0.f);
-                a.cnorm[slot] = make_float4(nn.x, nn.y, nn.z, 0.f);
-            }
-            slot++;
+            a.cpart[slot] = pi;
+            a.cshape[slot] = ci + 1;
+            a.cbpos[slot] = make_float4(bp.x, bp.y, bp.z, 0.f);
+            a.cbvel[slot] = make_float4(svel.x, svel.y, svel.z, 0.f);
+            a.cnorm[slot] = make_float4(nn.x, nn.y, nn.z, 0.f);
         }
     }
     if (STATS) {
@@ -363,8 +382,7 @@ __global__ __launch_bounds__(kMeshBlock) void k_mesh_contacts(MeshArgs a) {
 
 void launch_mesh_contacts(const MeshArgs& a, hipStream_t s) {
     if (a.n <= 0 || a.nmesh <= 0) return;
-    const int per_block = kMeshItems * kMeshBlock;
-    const dim3 g((a.n + per_block - 1) / per_block), b(kMeshBlock);
+    const dim3 g((a.n + kMeshBlock - 1) / kMeshBlock), b(kMeshBlock);
     if (a.stats) hipLaunchKernelGGL(k_mesh_contacts<true>, g, b, 0, s, a);
     else hipLaunchKernelGGL(k_mesh_contacts<false>, g, b, 0, s, a);
 }

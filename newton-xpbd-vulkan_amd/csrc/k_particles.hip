@@ -159,21 +159,6 @@ __global__ __launch_bounds__(256) void k_cell_tables(int n, const uint32_t* __re
     if (t == n - 1) ends[c] = t + 1;
 }
 
-// Rigid transform of a shape's body; identity and zeros for body < 0 (the
-// ground plane), as solve_particle_shape_contacts.comp:123-131.
-struct BodyFrame { f3 pos; float4 rot; f3 com; f3 v; f3 w; };
-__device__ __forceinline__ BodyFrame load_body(const BodyArgs& b, int body) {
-    BodyFrame F{{0.f, 0.f, 0.f}, make_float4(0.f, 0.f, 0.f, 1.f), {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
-    if (body >= 0) {
-        F.pos = xyz(b.q[2 * body]);
-        F.rot = b.q[2 * body + 1];
-        F.com = {b.com[3 * body], b.com[3 * body + 1], b.com[3 * body + 2]};
-        F.v = xyz(b.qd[2 * body]);
-        F.w = xyz(b.qd[2 * body + 1]);
-    }
-    return F;
-}
-
 // solve_particle_shape_contacts.comp:104-200.  Accumulates into dshape, which
 // k_pp_apply consumes and re-zeroes.
 __global__ __launch_bounds__(256) void k_shape_solve(ShapeArgs a) {
@@ -251,6 +236,11 @@ __device__ __forceinline__ void pp_pair(const PPArgs& a, int t, int k, int nf, f
     if (proxy != 0 && (((nf & 2) != 0) || (Vj.w <= 0.0f))) return;
     const f3 rd = x - xyz(Pj);
     if (fabsf(rd.x) > R || fabsf(rd.y) > R || fabsf(rd.z) > R) return;
+    // cheap reject before the refined length: a pair the exact test accepts
+    // has |rd| <= (r_i + r_j + coh)(1 + 5 ulp), and the plain squared length
+    // is within 3 ulp of |rd|^2, far inside the 1e-4 relative slack
+    const float lim = (rad + Pj.w) + fabsf(a.coh);
+    if ((rd.x * rd.x + rd.y * rd.y) + rd.z * rd.z > lim * lim * 1.0001f + 1e-30f) return;
     const float d = exact_length(rd);
     const float err = d - rad - Pj.w;
     if (err > a.coh || d <= 0.0f) return;

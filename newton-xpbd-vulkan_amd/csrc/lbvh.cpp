@@ -197,6 +197,126 @@ RayAxes parity_axes() {
 }
 }  // namespace
 
+void dist_grid_build(const float* pts, const uint32_t* tris, uint32_t nt, float cap, DistGrid& g, const MeshBvhHost* bvh) {
+    g = DistGrid{};
+    if (nt == 0 || !(cap > 0.0f)) return;
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    std::vector<double> tb(size_t(6) * nt);
+    for (uint32_t t = 0; t < nt; t++) {
+        double a[3] = {1e300, 1e300, 1e300}, b[3] = {-1e300, -1e300, -1e300};
+        for (int c = 0; c < 3; c++)
+            for (int k = 0; k < 3; k++) {
+                const double x = pts[size_t(tris[3 * t + c]) * 3 + k];
+                a[k] = std::min(a[k], x);
+                b[k] = std::max(b[k], x);
+            }
+        for (int k = 0; k < 3; k++) {
+            tb[6 * t + k] = a[k]; tb[6 * t + 3 + k] = b[k];
+            lo[k] = std::min(lo[k], a[k]); hi[k] = std::max(hi[k], b[k]);
+        }
+    }
+    // cell edge: half the cap, coarsened so no axis exceeds 256 cells
+    double ext = 0.0;
+    for (int k = 0; k < 3; k++) { lo[k] -= cap; hi[k] += cap; ext = std::max(ext, hi[k] - lo[k]); }
+    const double h = std::max(double(cap) * 0.5, ext / 256.0);
+    const uint32_t n[3] = {uint32_t(std::ceil((hi[0] - lo[0]) / h)) + 1, uint32_t(std::ceil((hi[1] - lo[1]) / h)) + 1,
+                           uint32_t(std::ceil((hi[2] - lo[2]) / h)) + 1};
+    g.ox = float(lo[0]); g.oy = float(lo[1]); g.oz = float(lo[2]);
+    g.inv = float(1.0 / h);
+    g.cap = cap;
+    g.nx = n[0]; g.ny = n[1]; g.nz = n[2];
+    // cell (i, j, k) covers [o + c*h', o + (c+1)*h'] with h' = 1/inv as the
+    // device computes cells; bounds use that box grown by a relative 1e-6
+    // so float rounding of the device's cell index cannot leave the box
+    const double o[3] = {double(g.ox), double(g.oy), double(g.oz)};
+    const double hc = 1.0 / double(g.inv);
+    std::vector<double> best(size_t(n[0]) * n[1] * n[2], double(cap));
+    // leaf-order span of the triangles within cap of each cell
+    std::vector<uint32_t> pos, span_lo, span_hi;
+    if (bvh) {
+        pos.assign(nt, 0);
+        for (uint32_t k = 0; k < nt; k++) pos[bvh->prim[k]] = k;
+        span_lo.assign(best.size(), 0xFFFFFFFFu);
+        span_hi.assign(best.size(), 0u);
+    }
+    for (uint32_t t = 0; t < nt; t++) {
+        long c0[3], c1[3];
+        for (int k = 0; k < 3; k++) {
+            c0[k] = std::max<long>(0, long(std::floor((tb[6 * t + k] - cap - o[k]) / hc)) - 1);
+            c1[k] = std::min<long>(long(n[k]) - 1, long(std::floor((tb[6 * t + 3 + k] + cap - o[k]) / hc)) + 1);
+        }
+        for (long z = c0[2]; z <= c1[2]; z++)
+            for (long y = c0[1]; y <= c1[1]; y++)
+                for (long x = c0[0]; x <= c1[0]; x++) {
+                    const long cc[3] = {x, y, z};
+                    double d2 = 0.0;
+                    for (int k = 0; k < 3; k++) {
+                        const double slack = 1e-6 * (std::fabs(o[k]) + double(n[k]) * hc) + 1e-9;
+                        const double blo = o[k] + double(cc[k]) * hc - slack, bhi = o[k] + double(cc[k] + 1) * hc + slack;
+                        const double gap = std::max(0.0, std::max(tb[6 * t + k] - bhi, blo - tb[6 * t + 3 + k]));
+                        d2 += gap * gap;
+                    }
+                    const size_t ci = (size_t(z) * n[1] + size_t(y)) * n[0] + size_t(x);
+                    const double d = std::sqrt(d2);
+                    if (d < best[ci]) best[ci] = d;
+                    if (bvh && d <= double(cap)) {
+                        span_lo[ci] = std::min(span_lo[ci], pos[t]);
+                        span_hi[ci] = std::max(span_hi[ci], pos[t]);
+                    }
+                }
+    }
+    if (bvh) {
+        // leaf-order range [first, last] covered by every node reachable from the root
+        const size_t nn = bvh->lo.size();
+        std::vector<uint32_t> first(nn, 0), last(nn, 0);
+        std::vector<uint32_t> order;
+        order.reserve(nn);
+        order.push_back(bvh->root);
+        for (size_t k = 0; k < order.size(); k++) {
+            const uint32_t v = order[k];
+            const uint32_t lp = bvh->lo[v].packed;
+            if (!((lp >> 30) & 3u)) { order.push_back(lp & 0x3FFFFFFFu); order.push_back(bvh->hi[v].packed & 0x3FFFFFFFu); }
+        }
+        for (size_t k = order.size(); k-- > 0;) {
+            const uint32_t v = order[k];
+            const uint32_t lp = bvh->lo[v].packed;
+            if ((lp >> 30) & 3u) { first[v] = lp & 0x3FFFFFFFu; last[v] = (bvh->hi[v].packed & 0x3FFFFFFFu) - 1u; }
+            else {
+                const uint32_t l = lp & 0x3FFFFFFFu, r = bvh->hi[v].packed & 0x3FFFFFFFu;
+                first[v] = std::min(first[l], first[r]);
+                last[v] = std::max(last[l], last[r]);
+            }
+        }
+        g.start.assign(best.size(), bvh->root);
+        for (size_t ci = 0; ci < best.size(); ci++) {
+            if (span_lo[ci] > span_hi[ci]) continue;
+            uint32_t v = bvh->root;
+            for (;;) {
+                const uint32_t lp = bvh->lo[v].packed;
+                if ((lp >> 30) & 3u) break;
+                const uint32_t l = lp & 0x3FFFFFFFu, r = bvh->hi[v].packed & 0x3FFFFFFFu;
+                if (first[l] <= span_lo[ci] && span_hi[ci] <= last[l]) v = l;
+                else if (first[r] <= span_lo[ci] && span_hi[ci] <= last[r]) v = r;
+                else break;
+            }
+            g.start[ci] = v;
+        }
+    }
+    g.lb.resize(best.size());
+    for (size_t k = 0; k < best.size(); k++) {
+        float f = float(best[k]);
+        if (double(f) > best[k]) f = std::nextafter(f, 0.0f);   // round down
+        g.lb[k] = f;
+    }
+}
+
+float dist_grid_bound(const DistGrid& g, const float p[3]) {
+    const float fx = std::floor((p[0] - g.ox) * g.inv), fy = std::floor((p[1] - g.oy) * g.inv),
+                fz = std::floor((p[2] - g.oz) * g.inv);
+    if (!(fx >= 0.0f && fy >= 0.0f && fz >= 0.0f && fx < float(g.nx) && fy < float(g.ny) && fz < float(g.nz))) return -1.0f;
+    return g.lb[(size_t(fz) * g.ny + size_t(fy)) * g.nx + size_t(fx)];
+}
+
 long parity_grid_cell(const ParityGrid& g, const float p[3]) {
     const RayAxes r = parity_axes();
     const float u = p[r.kx] - r.Sx * p[r.kz];

@@ -47,6 +47,28 @@ struct ParityGrid {
 // mesh's first triangle (stored in the grid).
 void parity_grid_build(const float* pts, const uint32_t* tris, uint32_t num_tris, uint32_t tri_base, ParityGrid& g);
 
+// Uniform grid over a mesh's box grown by `cap`; each cell holds a lower bound
+// of the distance from any point of the cell to any triangle of the mesh
+// (box-to-triangle-box distance, rounded down), capped at `cap`.  A point
+// whose cell bound exceeds a query distance d < cap has no triangle within d,
+// and a point outside the grid is farther than cap from every triangle.
+//
+// With a BVH, each cell also records the deepest node whose subtree holds
+// every triangle within `cap` of the cell (the lowest common ancestor of
+// their leaf positions; the root when none).  The depth-first closest-face
+// search started there visits the same candidates in the same order as the
+// search from the root, whose other branches hold no triangle within reach.
+struct DistGrid {
+    float ox = 0.f, oy = 0.f, oz = 0.f, inv = 0.f, cap = 0.f;
+    uint32_t nx = 0, ny = 0, nz = 0;
+    std::vector<float> lb;
+    std::vector<uint32_t> start;   // local node index per cell (with a BVH)
+};
+void dist_grid_build(const float* pts, const uint32_t* tris, uint32_t num_tris, float cap, DistGrid& g,
+                     const MeshBvhHost* bvh = nullptr);
+// Bound for a point in the device's float arithmetic; -1 outside the grid.
+float dist_grid_bound(const DistGrid& g, const float p[3]);
+
 // Cell of a ray origin, in the device's float arithmetic; -1 outside the grid.
 long parity_grid_cell(const ParityGrid& g, const float p[3]);
 

@@ -21,38 +21,11 @@
 #include <cstring>
 #include <stdexcept>
 
-#define HIPCHECK(x)                                                                                          \
-    do {                                                                                                     \
-        hipError_t e_ = (x);                                                                                 \
-        if (e_ != hipSuccess)                                                                                \
-            throw std::runtime_error(std::string("HIP error ") + hipGetErrorString(e_) + " at " + __FILE__ + \
-                                     ":" + std::to_string(__LINE__));                                        \
-    } while (0)
+#include "devbuf.h"
 
 namespace xpbd {
 
 namespace {
-
-template <class T> void dalloc(DevBuf<T>& b, size_t n) {
-    if (b.p) { (void)hipFree(b.p); b.p = nullptr; b.n = 0; }
-    size_t m = n ? n : 1;
-    HIPCHECK(hipMalloc(reinterpret_cast<void**>(&b.p), m * sizeof(T)));
-    HIPCHECK(hipMemset(b.p, 0, m * sizeof(T)));
-    b.n = n;
-}
-template <class T> void dfree(DevBuf<T>& b) {
-    if (b.p) (void)hipFree(b.p);
-    b.p = nullptr;
-    b.n = 0;
-}
-template <class T> void upload(DevBuf<T>& b, const T* src, size_t n) {
-    dalloc(b, n);
-    if (n) HIPCHECK(hipMemcpy(b.p, src, n * sizeof(T), hipMemcpyHostToDevice));
-}
-
-struct DeviceGuard {
-    explicit DeviceGuard(int dev) { (void)hipSetDevice(dev); }
-};
 
 constexpr const char* kStageNames[] = {"mesh_contacts", "integrate",      "sort_upsweep",   "sort_binscan",
                                        "sort_scatter",  "cell_ranges",    "shape_solve",    "pp_solve_apply",
@@ -84,11 +57,12 @@ void Solver::free_all() {
     dfree(range_); dfree(tbl_); dfree(cpart_); dfree(cshape_); dfree(cc_); dfree(cbpos_); dfree(cbvel_); dfree(cnorm_);
     dfree(dshape_); dfree(delta_); dfree(body_q_); dfree(body_qd_); dfree(body_com_); dfree(body_invI_);
     dfree(body_invm_); dfree(body_delta_); dfree(mat_mu_); dfree(body_flags_); dfree(shape_body_);
-    dfree(verts_d_); dfree(vvel_d_); dfree(tris_d_); dfree(prim_d_); dfree(err_d_); dfree(nodes_d_); dfree(pg_start_); dfree(pg_tri_);
+    dfree(verts_d_); dfree(vvel_d_); dfree(tris_d_); dfree(prim_d_); dfree(err_d_); dfree(nodes_d_); dfree(pg_start_); dfree(pg_tri_); dfree(lb_); dfree(lb_node_);
     dfree(info_d_); dfree(diag_header_); dfree(diag_pairs_); dfree(mesh_stats_);
     for (hipEvent_t e : ev_pool_) (void)hipEventDestroy(e);
     ev_pool_.clear();
     invalidate_graphs();
+    free_tape();
     dfree(scene_d_);
     if (count_host_) { (void)hipHostFree(count_host_); count_host_ = nullptr; }
     if (stream_) { (void)hipStreamDestroy(stream_); stream_ = nullptr; }
@@ -125,6 +99,7 @@ void Solver::init_common(const xpbd_config& cfg, size_t n, size_t max_contacts) 
     if (n > (size_t(1) << 28)) throw std::runtime_error("num_particles exceeds the supported maximum of 2^28");
     if (initialized_) { DeviceGuard g(device_); (void)hipDeviceSynchronize(); free_all(); initialized_ = false; }
     cfg_ = cfg;
+    relax_ = cfg.soft_contact_relaxation;
     n_ = n;
     cmax_ = max_contacts ? max_contacts : n * 2;
     if (cmax_ > size_t(0x7FFFFFFF)) throw std::runtime_error("max_contacts exceeds the int32 range");
@@ -136,6 +111,8 @@ void Solver::init_common(const xpbd_config& cfg, size_t n, size_t max_contacts) 
     bool env_on = env && (std::strcmp(env, "1") == 0 || std::strcmp(env, "true") == 0);
     profiling_ = cfg.enable_profile || env_on;
     if (profiling_) std::printf("[vkxpbd] GPU profiler enabled (HIP events)\n");
+    const char* senv = std::getenv("VKXPBD_MESH_STATS");
+    mesh_stats_on_ = senv && (std::strcmp(senv, "1") == 0 || std::strcmp(senv, "true") == 0);
     const char* genv = std::getenv("VKXPBD_GRAPHS");
     use_graphs_ = !(genv && std::strcmp(genv, "0") == 0);
     grid_valid_ = false;
@@ -229,11 +206,26 @@ void Solver::upload_nodes() {
     upload(nodes_d_, inter.data(), inter.size());
 }
 
-// Concatenates the per-mesh parity grids into one cell-start table (each
-// mesh's cells at pg_cell_off, entries offset into one triangle list) and
-// refreshes the mesh descriptors that carry their geometry.
-void Solver::upload_parity_grids() {
-    std::vector<uint32_t> start, tri;
+// Host-side query accelerators of one mesh: the projected parity grid
+// (watertight meshes) and the distance lower-bound grid.  The bound grid's cap
+// covers the search distance of particles up to twice particle_max_radius;
+// larger particles take the full tree walk.
+void Solver::build_query_grids(size_t c) {
+    const uint32_t nt = uint32_t(meshes_[c].tris.size() / 3);
+    const uint32_t* tris = all_tris_.data() + 3 * tri_off_[c];
+    pgrid_[c] = ParityGrid{};
+    if (meshes_[c].watertight) parity_grid_build(all_verts_.data(), tris, nt, uint32_t(tri_off_[c]), pgrid_[c]);
+    const float cap = 1.25f * ((cfg_.soft_contact_margin + meshes_[c].margin) + 2.0f * cfg_.particle_max_radius) +
+                      avg_edge_ * 1e-3f;
+    dist_grid_build(all_verts_.data(), tris, nt, cap, dgrid_[c], &bvh_[c]);
+}
+
+// Concatenates the per-mesh grids (parity: one cell-start table with each
+// mesh's cells at pg_cell_off and one triangle list; bounds: one array with
+// each mesh's cells at lb_off) and refreshes the mesh descriptors.
+void Solver::upload_query_grids() {
+    std::vector<uint32_t> start, tri, lbn;
+    std::vector<float> lb;
     for (size_t c = 0; c < pgrid_.size(); c++) {
         const ParityGrid& g = pgrid_[c];
         info_h_[c].pg_u0 = g.u0;
@@ -244,10 +236,20 @@ void Solver::upload_parity_grids() {
         const uint32_t base = uint32_t(tri.size());
         for (size_t k = 0; k + 1 < g.start.size(); k++) start.push_back(base + g.start[k]);
         tri.insert(tri.end(), g.tri.begin(), g.tri.end());
+        const DistGrid& d = dgrid_[c];
+        info_h_[c].lb_ox = d.ox; info_h_[c].lb_oy = d.oy; info_h_[c].lb_oz = d.oz; info_h_[c].lb_inv = d.inv;
+        info_h_[c].lb_nx = d.nx; info_h_[c].lb_ny = d.ny; info_h_[c].lb_nz = d.nz;
+        info_h_[c].lb_off = uint32_t(lb.size());
+        info_h_[c].lb_cap = d.lb.empty() ? 0.0f : d.cap;   // cap 0: no early out
+        lb.insert(lb.end(), d.lb.begin(), d.lb.end());
+        for (size_t k = 0; k < d.lb.size(); k++)   // start nodes, re-based like the packed nodes
+            lbn.push_back(uint32_t(node_off_[c]) + (k < d.start.size() ? d.start[k] : bvh_[c].root));
     }
     start.push_back(uint32_t(tri.size()));
     upload(pg_start_, start.data(), start.size());
     upload(pg_tri_, tri.data(), tri.size());
+    upload(lb_, lb.data(), lb.size());
+    upload(lb_node_, lbn.data(), lbn.size());
     upload(info_d_, info_h_.data(), info_h_.size());
 }
 
@@ -357,11 +359,9 @@ void Solver::finalize_meshes() {
     upload(prim_d_, prim_global.data(), prim_global.size());
     upload_nodes();
     pgrid_.assign(M, ParityGrid{});
-    for (size_t c = 0; c < M; c++)
-        if (meshes_[c].watertight)
-            parity_grid_build(all_verts_.data(), all_tris_.data() + 3 * tri_off_[c], uint32_t(meshes_[c].tris.size() / 3),
-                              uint32_t(tri_off_[c]), pgrid_[c]);
-    upload_parity_grids();
+    dgrid_.assign(M, DistGrid{});
+    for (size_t c = 0; c < M; c++) build_query_grids(c);
+    upload_query_grids();
     finalized_ = true;
 }
 
@@ -397,11 +397,8 @@ void Solver::update_mesh(size_t mi, const float* v, size_t nv, const float* vel)
     else std::fill(all_vels_.begin() + 3 * vs, all_vels_.begin() + 3 * (vs + nv), 0.f);
     std::memcpy(meshes_[mi].verts.data(), v, nv * 3 * sizeof(float));
     lbvh_refit(bvh_[mi], all_verts_.data(), all_tris_.data() + 3 * tri_off_[mi]);
-    if (meshes_[mi].watertight) {
-        parity_grid_build(all_verts_.data(), all_tris_.data() + 3 * tri_off_[mi], uint32_t(meshes_[mi].tris.size() / 3),
-                          uint32_t(tri_off_[mi]), pgrid_[mi]);
-        upload_parity_grids();
-    }
+    build_query_grids(mi);
+    upload_query_grids();
     HIPCHECK(hipMemcpy(verts_d_.p + 3 * vs, v, nv * 3 * sizeof(float), hipMemcpyHostToDevice));
     HIPCHECK(hipMemcpy(vvel_d_.p + 3 * vs, all_vels_.data() + 3 * vs, nv * 3 * sizeof(float), hipMemcpyHostToDevice));
     upload_nodes();
@@ -504,13 +501,13 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
         MeshArgs m;
         m.n = n; m.P = P_[s_].p; m.flags = flags_.p;
         m.verts = verts_d_.p; m.tris = tris_d_.p; m.nodes = nodes_d_.p; m.prim = prim_d_.p;
-        m.pg_start = pg_start_.p; m.pg_tri = pg_tri_.p;
+        m.pg_start = pg_start_.p; m.pg_tri = pg_tri_.p; m.lb = lb_.p; m.lb_node = lb_node_.p;
         m.vvel = vvel_d_.p; m.info = info_d_.p; m.nmesh = M;
         m.margin = cfg_.soft_contact_margin; m.eps = avg_edge_ * 1e-3f;
         m.cmax = int(cmax_); m.cc = cc_gen;
         m.cpart = cpart_.p; m.cshape = cshape_.p; m.cbpos = cbpos_.p; m.cbvel = cbvel_.p; m.cnorm = cnorm_.p;
         m.err = reinterpret_cast<uint32_t*>(cc_.p + 2);
-        m.stats = profiling_ ? mesh_stats_.p : nullptr;
+        m.stats = mesh_stats_on_ ? mesh_stats_.p : nullptr;
         stage_begin(kMesh);
         launch_mesh_contacts(m, stream_);
         stage_end(kMesh);
@@ -579,7 +576,7 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
         sa.cpart = cpart_.p; sa.cshape = cshape_.p; sa.cbpos = cbpos_.p; sa.cbvel = cbvel_.p; sa.cnorm = cnorm_.p;
         sa.P = P_[cur].p; sa.V = V_[cur].p; sa.flags = flags_.p;
         sa.body = body_args();
-        sa.pmu = cfg_.soft_contact_mu; sa.ka = cfg_.particle_adhesion; sa.dt = sdt; sa.relax = cfg_.soft_contact_relaxation;
+        sa.pmu = cfg_.soft_contact_mu; sa.ka = cfg_.particle_adhesion; sa.dt = sdt; sa.relax = relax_;
         sa.dshape = dshape_.p;
         stage_begin(kShape);
         launch_shape_solve(sa, shape_grid, stream_);
@@ -592,7 +589,7 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
         pa.Xs_next = more ? Xs_[xs ^ 1].p : nullptr;
         pa.Vs_next = more ? Vs_[xs ^ 1].p : nullptr;
         pa.mu = cfg_.particle_mu; pa.coh = cfg_.particle_cohesion; pa.max_r = cfg_.particle_max_radius;
-        pa.dt = sdt; pa.relax = cfg_.soft_contact_relaxation; pa.inv_cell = inv_cell; pa.vmax = cfg_.particle_v_max;
+        pa.dt = sdt; pa.relax = relax_; pa.inv_cell = inv_cell; pa.vmax = cfg_.particle_v_max;
         pa.iter_index = uint32_t(it); pa.max_diag_pairs = max_pairs_;
         pa.diag_header = diag_header_.p; pa.diag_pairs = diag_pairs_.p;
         stage_begin(kPP);
@@ -812,13 +809,13 @@ void Solver::collide_only() {
         MeshArgs m;
         m.n = int(n_); m.P = P_[s_].p; m.flags = flags_.p;
         m.verts = verts_d_.p; m.tris = tris_d_.p; m.nodes = nodes_d_.p; m.prim = prim_d_.p;
-        m.pg_start = pg_start_.p; m.pg_tri = pg_tri_.p;
+        m.pg_start = pg_start_.p; m.pg_tri = pg_tri_.p; m.lb = lb_.p; m.lb_node = lb_node_.p;
         m.vvel = vvel_d_.p; m.info = info_d_.p; m.nmesh = int(meshes_.size());
         m.margin = cfg_.soft_contact_margin; m.eps = avg_edge_ * 1e-3f;
         m.cmax = int(cmax_); m.cc = cc_.p;
         m.cpart = cpart_.p; m.cshape = cshape_.p; m.cbpos = cbpos_.p; m.cbvel = cbvel_.p; m.cnorm = cnorm_.p;
         m.err = reinterpret_cast<uint32_t*>(cc_.p + 2);
-        m.stats = profiling_ ? mesh_stats_.p : nullptr;
+        m.stats = mesh_stats_on_ ? mesh_stats_.p : nullptr;
         launch_mesh_contacts(m, stream_);
     }
     HIPCHECK(hipGetLastError());

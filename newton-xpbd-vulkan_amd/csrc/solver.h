@@ -72,10 +72,20 @@ public:
     void get_bodies(float* q7, float* qd6, size_t cap);
     void set_body_velocities(const float* qd6, size_t n);
 
+    // Differentiable rollout (one iteration per substep, contact set frozen
+    // from the forward pass in the reverse pass).  Device pointers hold 3
+    // floats per particle; x0/v0 may be null (current state).  The forward
+    // keeps state checkpoints every `checkpoint_every` substeps; the backward
+    // recomputes each segment from its checkpoint and runs the reverse
+    // kernels, then restores the forward's final state.
+    void rollout_forward(const float* x0, const float* v0, float relax, float sdt, int substeps, int checkpoint_every,
+                         float* x_out, float* v_out);
+    void rollout_backward(const float* gx, const float* gv, float* gx0, float* gv0, double* grelax);
+
     void print_profiler_summary();
     size_t profile_stages(char* names, size_t names_len, double* ms, uint64_t* launches, size_t cap);
     void profile_reset();
-    // mesh query counters gathered while profiling: active particle x mesh
+    // mesh query counters (environment VKXPBD_MESH_STATS=1): active particle x mesh
     // pairs, root-box passes, BVH node pops, closest-point tests, parity
     // triangle tests, contacts
     void mesh_query_stats(uint64_t out[6]);
@@ -104,7 +114,11 @@ private:
     void alloc_all();
     void free_all();
     void upload_nodes();
-    void upload_parity_grids();
+    void free_tape();
+    void swap_tape(int j);
+    void ensure_tape();
+    void build_query_grids(size_t mesh);
+    void upload_query_grids();
     BodyArgs body_args() const;
     void substep(float sdt, int iterations, bool collide, int slot, bool zero_next);
     BodyStepArgs body_step_args(float sdt);
@@ -136,6 +150,24 @@ private:
         hipGraphExec_t exec;
     };
     std::vector<FrameGraph> graphs_;
+    float relax_ = 0.f;   // contact relaxation of the solves (a rollout may override the config's)
+    // rollout tape: checkpoints, per-substep records of the segment being
+    // reversed, and the reverse-pass buffers
+    struct Tape {
+        bool valid = false;
+        float sdt = 0.f, relax = 0.f;
+        int substeps = 0, every = 0;
+        std::vector<DevBuf<float4>> ckP, ckV;
+        std::vector<DevBuf<float4>> P0, V0, Xs, Vs, cbpos, cbvel, cnorm;
+        std::vector<DevBuf<uint32_t>> keys, vals;
+        std::vector<DevBuf<int32_t>> Fs, cpart, cshape;
+        std::vector<DevBuf<float>> delta;
+        DevBuf<int32_t> counts;
+        DevBuf<float4> lx[2], lv[2], A, As, finP, finV;
+        DevBuf<uint32_t> rank;
+        DevBuf<float> shx, shv;
+        DevBuf<double> relax_part;
+    } tape_;
     bool use_graphs_ = true;
     // particle state: 3 rotating float4 buffers each (state / predicted / out)
     DevBuf<float4> P_[3], V_[3];
@@ -177,12 +209,16 @@ private:
     DevBuf<float4> nodes_d_;   // interleaved (lower, upper) node boxes, re-based
     std::vector<ParityGrid> pgrid_;        // per mesh (empty for non-watertight)
     DevBuf<uint32_t> pg_start_, pg_tri_;   // all meshes' grids, concatenated
+    std::vector<DistGrid> dgrid_;          // per mesh distance lower bounds
+    DevBuf<float> lb_;                     // all meshes' bounds, concatenated
+    DevBuf<uint32_t> lb_node_;             // per bound cell: search start node
     DevBuf<MeshInfo> info_d_;
     std::vector<MeshInfo> info_h_;
     float avg_edge_ = 0.f;
     // diagnostics
     DevBuf<uint32_t> diag_header_;
     DevBuf<uint64_t> mesh_stats_;
+    bool mesh_stats_on_ = false;   // VKXPBD_MESH_STATS=1: count mesh query work (adds atomics)
     DevBuf<float> diag_pairs_;
     uint32_t max_pairs_ = 100000;
     // profiler

@@ -410,6 +410,37 @@ PYBIND11_MODULE(_vkxpbd, m) {
             },
             py::arg("config"), py::arg("scene_counts"), py::arg("max_contacts") = 0,
             "Batched scenes (extension): scene_counts[k] particles per scene, concatenated in scene order.")
+        .def(
+            "rollout_forward",
+            [](PySolver& s, uintptr_t x0, uintptr_t v0, float relax, float sdt, int substeps, int every, uintptr_t xo,
+               uintptr_t vo) {
+                int rc;
+                {
+                    py::gil_scoped_release nogil;
+                    rc = xpbd_rollout_forward(s.h, reinterpret_cast<const float*>(x0), reinterpret_cast<const float*>(v0),
+                                              relax, sdt, substeps, every, reinterpret_cast<float*>(xo),
+                                              reinterpret_cast<float*>(vo));
+                }
+                s.check(rc);
+            },
+            py::arg("x0_ptr"), py::arg("v0_ptr"), py::arg("relax"), py::arg("sdt"), py::arg("substeps"),
+            py::arg("checkpoint_every"), py::arg("x_out_ptr"), py::arg("v_out_ptr"),
+            "Differentiable rollout forward on device pointers (0 = null); see vkxpbd.autograd.")
+        .def(
+            "rollout_backward",
+            [](PySolver& s, uintptr_t gx, uintptr_t gv, uintptr_t gx0, uintptr_t gv0) {
+                double gr = 0.0;
+                int rc;
+                {
+                    py::gil_scoped_release nogil;
+                    rc = xpbd_rollout_backward(s.h, reinterpret_cast<const float*>(gx), reinterpret_cast<const float*>(gv),
+                                               reinterpret_cast<float*>(gx0), reinterpret_cast<float*>(gv0), &gr);
+                }
+                s.check(rc);
+                return gr;
+            },
+            py::arg("gx_ptr"), py::arg("gv_ptr"), py::arg("gx0_ptr"), py::arg("gv0_ptr"),
+            "Reverse pass of the last rollout_forward; returns d/d relax.")
         .def("mesh_query_stats",
              [](PySolver& s) {
                  uint64_t v[6];

@@ -160,4 +160,24 @@ __device__ __forceinline__ int block_reserve(int* counter, int count, int* s_tmp
     return base;
 }
 
+// Block-wide exclusive prefix sum of one int per thread, in thread order.
+// Returns the thread's offset; *total receives the block sum.  s_tmp must
+// hold blockDim.x / 64 + 1 ints.
+__device__ __forceinline__ int block_exclusive_scan(int count, int* s_tmp, int* total) {
+    const int lane = __lane_id(), w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const int incl = wave_incl_scan_i(count);
+    if (lane == 63) s_tmp[w] = incl;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int tot = 0;
+        for (int k = 0; k < nw; k++) { const int c = s_tmp[k]; s_tmp[k] = tot; tot += c; }
+        s_tmp[nw] = tot;
+    }
+    __syncthreads();
+    const int base = s_tmp[w] + (incl - count);
+    *total = s_tmp[nw];
+    __syncthreads();
+    return base;
+}
+
 }  // namespace xpbd

@@ -52,6 +52,22 @@ struct PlaneArgs {
     int* cpart; int* cshape; float4* cbpos; float4* cbvel; float4* cnorm;
 };
 
+// Rigid transform of a shape's body; identity and zeros for body < 0 (the
+// ground plane), as solve_particle_shape_contacts.comp:123-131.
+struct BodyFrame { f3 pos; float4 rot; f3 com; f3 v; f3 w; };
+__device__ __forceinline__ BodyFrame load_body(const BodyArgs& b, int body) {
+    BodyFrame F{{0.f, 0.f, 0.f}, make_float4(0.f, 0.f, 0.f, 1.f), {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
+    if (body >= 0) {
+        F.pos = xyz(b.q[2 * body]);
+        F.rot = b.q[2 * body + 1];
+        F.com = {b.com[3 * body], b.com[3 * body + 1], b.com[3 * body + 2]};
+        F.v = xyz(b.qd[2 * body]);
+        F.w = xyz(b.qd[2 * body + 1]);
+    }
+    return F;
+}
+
+
 struct ShapeArgs {
     int cmax; const int* cc;
     const int* cpart; const int* cshape; const float4* cbpos; const float4* cbvel; const float4* cnorm;
@@ -94,8 +110,14 @@ struct MeshInfo {
     // axis, first cell in the global cell-start table
     float pg_u0, pg_w0, pg_inv;
     uint32_t pg_dim, pg_cell_off;
-    uint32_t pad[2];
+    // distance lower-bound grid: origin, 1/cell, cells per axis, first cell in
+    // the global table, and the distance cap the bounds are valid below
+    float lb_ox, lb_oy, lb_oz, lb_inv;
+    uint32_t lb_nx, lb_ny, lb_nz, lb_off;
+    float lb_cap;
+    uint32_t pad[5];
 };
+static_assert(sizeof(MeshInfo) == 96, "MeshInfo layout");
 
 struct MeshArgs {
     int n; const float4* P; const int* flags;
@@ -109,6 +131,8 @@ struct MeshArgs {
     uint32_t* err;
     const uint32_t* pg_start;   // projected parity grids: CSR cell starts ...
     const uint32_t* pg_tri;     // ... and global triangle ids
+    const float* lb;            // distance lower bounds of all meshes' grids
+    const uint32_t* lb_node;    // per bound cell: node the closest-face search starts at
     uint64_t* stats;            // optional query counters (6 words), null = off
 };
 
@@ -147,6 +171,36 @@ struct BodySolveArgs {
     float* delta; float* invw; float* impulse;
     float relax, dt;
 };
+
+// Reverse pass of one substep (differentiable rollout).
+struct AdjArgs {
+    int n;
+    const uint32_t* keys; const uint32_t* vals; const int* range; const int* Fs;
+    const float4* Xs; const float4* Vs;     // sorted predicted state (w: radius, inverse mass)
+    const float4* P0; const float4* V0;     // substep-start state, particle order
+    const float* delta;                     // final deltas, particle order
+    const float4* lx_in; const float4* lv_in;
+    float4* lx_out; float4* lv_out;
+    float4* A; float4* As; uint32_t* rank;
+    float* shx; float* shv;
+    double* relax_part;
+    float mu, coh, max_r, dt, relax, inv_cell, vmax, gx, gy, gz;
+};
+struct AdjShapeArgs {
+    int cmax; const int* cc;
+    const int* cpart; const int* cshape; const float4* cbpos; const float4* cbvel; const float4* cnorm;
+    const int* flags; const uint32_t* rank; const float4* Xs; const float4* Vs; const float4* A;
+    BodyArgs body;
+    float pmu, ka, dt, relax;
+    float* shx; float* shv; double* relax_part;
+};
+
+void launch_adj_prep(const AdjArgs& a, hipStream_t s);
+void launch_adj_shape(const AdjShapeArgs& a, int grid, hipStream_t s);
+void launch_adj_pp(const AdjArgs& a, bool validate, hipStream_t s);
+void launch_xyz_in(float4* dst, const float* src, int n, hipStream_t s);
+void launch_xyz_out(float* dst, const float4* src, int n, hipStream_t s);
+void launch_fill_f4(float4* dst, int n, hipStream_t s);
 
 void launch_integrate_bodies(const BodyStepArgs& a, hipStream_t s);
 void launch_apply_body_deltas(const BodyStepArgs& a, hipStream_t s);

@@ -412,8 +412,14 @@ template <class R> struct Solver {
     R angular_damping = R(0.05f), rigid_relax = R(0.8f), mu_torsional = 0, mu_rolling = 0;
     int32_t use_weights = 1;
 
+    // contact relaxation as used by the solves (the config's value unless a
+    // differentiable rollout overrides it in the solver's precision)
+    R relax_r = R(0);
+    std::vector<std::pair<int32_t, int32_t>>* pair_log = nullptr;   // contributing (i, j) pairs
+
     void init(const OrcConfig& c, int64_t np, int64_t mc) {
         cfg = c; n = np; cmax = mc ? mc : np * 2;
+        relax_r = R(cfg.soft_contact_relaxation);
         q.assign(n, {0, 0, 0}); qd = q_init = qd_init = q_out = qd_out = q;
         invm.assign(n, 0); radius.assign(n, 0); flags.assign(n, 0);
         delta.assign(3 * n, 0);
@@ -791,7 +797,7 @@ template <class R> struct Solver {
     // solve_particle_shape_contacts.comp:104-200
     void shape_solve(const std::vector<V3<R>>& X, const std::vector<V3<R>>& Vv, R dt) {
         int32_t count = std::min(int32_t(cmax), cc);
-        R relax = R(cfg.soft_contact_relaxation), ka = R(cfg.particle_adhesion), pmu = R(cfg.soft_contact_mu);
+        R relax = relax_r, ka = R(cfg.particle_adhesion), pmu = R(cfg.soft_contact_mu);
         for (int32_t t = 0; t < count; t++) {
             int s = cshape[t];
             int body = shape_body[s];
@@ -1126,7 +1132,7 @@ template <class R> struct Solver {
     // solve_particle_particle_contacts.comp:107-218
     void pp_solve(const std::vector<V3<R>>& X, const std::vector<V3<R>>& Vv, R dt) {
         R kmu = R(cfg.particle_mu), coh = R(cfg.particle_cohesion), maxr = R(cfg.particle_max_radius);
-        R relax = R(cfg.soft_contact_relaxation);
+        R relax = relax_r;
         R inv = R(1.0f / cfg.search_radius);
         for (int64_t tid = 0; tid < n; tid++) {
             int i = int(svals[tid]);
@@ -1171,6 +1177,7 @@ template <class R> struct Solver {
                             V3<R> df = {fmaR(vt_dir.x, lambda_f, -p2.x), fmaR(vt_dir.y, lambda_f, -p2.y),
                                         fmaR(vt_dir.z, lambda_f, -p2.z)};
                             acc = acc + exact_div(df, denom);
+                            if (pair_log) pair_log->push_back({int32_t(i), int32_t(j)});
                         }
                     }
             V3<R> dw = {fmaR(acc.x, w1, R(0)), fmaR(acc.y, w1, R(0)), fmaR(acc.z, w1, R(0))};
@@ -1300,6 +1307,226 @@ template <class R> struct Solver {
         }
     }
 
+
+    // ---- differentiable rollout (builder extension; no reference counterpart) ----
+    // A rollout is `substeps` substeps of length sdt with one iteration each,
+    // recorded on a tape: per substep the state at its start (x0, v0), the
+    // predicted state (x1, v1), the contacts generated on x0, the contributing
+    // particle pairs and the final deltas.  The adjoint treats the contact
+    // set and each contact's body point/normal/velocity as constants (the
+    // contact set is frozen from the forward pass) and differentiates every
+    // other operation of integrate, the shape and particle-particle
+    // projections and the delta apply, in double precision.
+    struct SubTape {
+        std::vector<V3<R>> x0, v0, x1, v1;
+        std::vector<R> delta;
+        std::vector<int32_t> cpart, cshape;
+        std::vector<V3<R>> cbpos, cbvel, cnorm;
+        std::vector<std::pair<int32_t, int32_t>> pairs;
+    };
+    std::vector<SubTape> tape;
+    R tape_dt = R(0);
+
+    void rollout(const double* v0, double relax, R sdt, int substeps, bool keep_tape) {
+        if (!finalized) finalize_meshes();
+        if (!boxes.empty()) throw std::runtime_error("rollout: dynamic bodies are not differentiated");
+        if (cfg.enable_restitution) throw std::runtime_error("rollout: restitution is not differentiated");
+        relax_r = R(relax);
+        if (v0) for (int64_t i = 0; i < n; i++) qd[i] = {R(v0[3 * i]), R(v0[3 * i + 1]), R(v0[3 * i + 2])};
+        tape.clear();
+        tape_dt = sdt;
+        for (int sub = 0; sub < substeps; sub++) {
+            generate_contacts(false);
+            SubTape T;
+            if (keep_tape) {
+                T.x0 = q; T.v0 = qd;
+                const int32_t c = std::min<int32_t>(cc, int32_t(cmax));
+                T.cpart.assign(cpart.begin(), cpart.begin() + c);
+                T.cshape.assign(cshape.begin(), cshape.begin() + c);
+                T.cbpos.assign(cbpos.begin(), cbpos.begin() + c);
+                T.cbvel.assign(cbvel.begin(), cbvel.begin() + c);
+                T.cnorm.assign(cnorm.begin(), cnorm.begin() + c);
+            }
+            integrate(sdt);
+            if (keep_tape) { T.x1 = q; T.v1 = qd; }
+            grid_and_sort();
+            pair_log = keep_tape ? &T.pairs : nullptr;
+            iterations_and_finish(sdt, 1, true);
+            pair_log = nullptr;
+            if (keep_tape) { T.delta = delta; tape.push_back(std::move(T)); }
+        }
+        last_count = std::min<int32_t>(cc, int32_t(cmax));
+    }
+
+    using D3 = V3<double>;
+    static D3 dv(V3<R> a) { return {double(a.x), double(a.y), double(a.z)}; }
+    static D3 perp(D3 u, D3 nn) { return u - nn * dot(nn, u); }   // (I - n n^T) u
+
+    // Friction-and-normal term g(rd, vrel) of a particle pair
+    // (solve_particle_particle_contacts.comp:170-205) and its vector-Jacobian
+    // product with cotangent C.  Returns g.
+    static D3 pair_vjp(D3 rd, D3 vrel, double ri, double rj, double kmu, double h, D3 C, D3& g_rd, D3& g_vrel) {
+        const double d = length(rd);
+        const D3 nn = rd * (1.0 / d);
+        const double err = d - ri - rj;
+        const double vn = dot(nn, vrel);
+        const D3 vt = vrel - nn * vn;
+        const double L = length(vt);
+        const double la = kmu * err, lb = -L * h;
+        const D3 pc = perp(C, nn);
+        const double nc = dot(nn, C);
+        if (!(la < lb)) {   // friction bounded by the cone: lambda_f = kmu * err
+            const D3 t = L > 0 ? vt * (1.0 / L) : D3{0, 0, 0};
+            g_rd = nn * (kmu * dot(t, C) - nc) - pc * (err / d);
+            g_vrel = {0, 0, 0};
+            if (L > 0) {
+                const D3 u = (C - t * dot(t, C)) * (kmu * err / L);
+                const D3 pu = perp(u, nn);
+                g_vrel = pu;
+                g_rd = g_rd - (vt * dot(nn, u) + pu * vn) * (1.0 / d);
+            }
+            return t * la - nn * err;
+        }
+        g_vrel = pc * (-h);
+        g_rd = (vt * nc + pc * vn) * (h / d) - nn * nc - pc * (err / d);
+        return vt * (-h) - nn * err;
+    }
+
+    // Shape-contact term (solve_particle_shape_contacts.comp:143-180) with a
+    // constant normal: c = n.(x1 - bx) - r, v = v1 - bv.  VJP with cotangent C.
+    static D3 shape_vjp(double c, D3 v, D3 nn, double mu, double h, D3 C, D3& g_x, D3& g_v) {
+        const double vn = dot(nn, v);
+        const D3 vt = v - nn * vn;
+        const double L = length(vt);
+        const double la = mu * c, lb = -L * h;
+        if (!(la < lb)) {
+            const D3 t = L > 0 ? vt * (1.0 / L) : D3{0, 0, 0};
+            g_x = nn * (mu * dot(t, C) - dot(nn, C));
+            g_v = L > 0 ? perp((C - t * dot(t, C)) * (mu * c / L), nn) : D3{0, 0, 0};
+            return t * la - nn * c;
+        }
+        g_x = nn * (-dot(nn, C));
+        g_v = perp(C, nn) * (-h);
+        return vt * (-h) - nn * c;
+    }
+
+    // Reverse pass over the tape: cotangents of the final positions gx and
+    // velocities gv (3N doubles each, gv may be null) -> cotangents of the
+    // initial positions and velocities and of the contact relaxation.
+    void rollout_backward(const double* gx, const double* gv, double* gx0, double* gv0, double* grelax) {
+        if (tape.empty()) throw std::runtime_error("rollout_backward: no taped rollout");
+        const double h = double(tape_dt), vmax = double(cfg.particle_v_max);
+        const double relax = double(relax_r), kmu = double(cfg.particle_mu), ka = double(cfg.particle_adhesion);
+        const double pmu = double(cfg.soft_contact_mu);
+        const D3 grav = {double(cfg.gravity[0]), double(cfg.gravity[1]), double(cfg.gravity[2])};
+        std::vector<D3> lx(n), lv(n), a(n), lx1(n), lv1(n);
+        for (int64_t i = 0; i < n; i++) {
+            lx[i] = {gx[3 * i], gx[3 * i + 1], gx[3 * i + 2]};
+            lv[i] = gv ? D3{gv[3 * i], gv[3 * i + 1], gv[3 * i + 2]} : D3{0, 0, 0};
+        }
+        double gr = 0.0;
+        const R vmax_r = R(cfg.particle_v_max);
+        for (size_t s = tape.size(); s-- > 0;) {
+            const SubTape& T = tape[s];
+            std::vector<D3> lx0(n);
+            // apply (apply_particle_deltas.comp:44-57): x2 = xn or clamped
+            for (int64_t i = 0; i < n; i++) {
+                if ((flags[i] & 1) == 0) { a[i] = {0, 0, 0}; continue; }
+                const V3<R> dl = {T.delta[3 * i], T.delta[3 * i + 1], T.delta[3 * i + 2]};
+                const V3<R> xn = T.x1[i] + dl;
+                const V3<R> vr = apply_exact_div(xn - T.x0[i], R(tape_dt));
+                if (length(vr) > vmax_r) {
+                    const D3 u = (dv(xn) - dv(T.x0[i])) * (1.0 / h);
+                    const double m = length(u);
+                    const D3 uh = u * (1.0 / m);
+                    const D3 lt = lv[i] + lx[i] * h;
+                    const D3 lu = (lt - uh * dot(uh, lt)) * (vmax / m);
+                    a[i] = lu * (1.0 / h);
+                    lx0[i] = lx[i] - lu * (1.0 / h);
+                } else {
+                    a[i] = lx[i] + lv[i] * (1.0 / h);
+                    lx0[i] = lv[i] * (-1.0 / h);
+                }
+            }
+            for (int64_t i = 0; i < n; i++) { lx1[i] = a[i]; lv1[i] = {0, 0, 0}; }
+            // shape contacts
+            for (size_t t = 0; t < T.cpart.size(); t++) {
+                const int si = T.cshape[t], body = shape_body[si], pi = T.cpart[t];
+                const int pf = flags[pi];
+                if ((pf & 1) == 0) continue;
+                if (pf & 2) {
+                    if (body < 0) continue;
+                    if (b_flags[body] & 4) continue;
+                    if (b_invm[body] == 0) continue;
+                }
+                Frame F = frame(body);
+                const V3<R> bx = F.pos + quat_rotate(F.rot, T.cbpos[t]);
+                const V3<R> rr = bx - (F.pos + quat_rotate(F.rot, F.com));
+                const V3<R> nnr = T.cnorm[t];
+                const V3<R> dp = T.x1[pi] - bx;
+                const R cr = fmaR(nnr.z, dp.z, fmaR(nnr.x, dp.x, nnr.y * dp.y)) - radius[pi];
+                if (cr > R(ka)) continue;
+                double w2 = 0;
+                if (body >= 0) {
+                    const V3<R> ang = cross(rr, nnr);
+                    const V3<R> ra = quat_rotate_inv(F.rot, ang);
+                    const std::array<R, 9>& I = b_invI[body];
+                    const V3<R> Ira = {I[0] * ra.x + I[1] * ra.y + I[2] * ra.z, I[3] * ra.x + I[4] * ra.y + I[5] * ra.z,
+                                       I[6] * ra.x + I[7] * ra.y + I[8] * ra.z};
+                    w2 = double(b_invm[body] + dot(ra, Ira));
+                }
+                const double w1 = double(invm[pi]);
+                if (w1 + w2 == 0) continue;
+                const V3<R> bvr = F.v + cross(F.w, rr) + quat_rotate(F.rot, T.cbvel[t]);
+                const D3 nn = dv(nnr);
+                const double c = dot(nn, dv(T.x1[pi]) - dv(bx)) - double(radius[pi]);
+                const D3 v = dv(T.v1[pi]) - dv(bvr);
+                const double mu = 0.5 * (pmu + double(mat_mu[si]));
+                const double sc = w1 / (w1 + w2);
+                D3 g_x, g_v;
+                const D3 g = shape_vjp(c, v, nn, mu, h, a[pi] * (sc * relax), g_x, g_v);
+                lx1[pi] = lx1[pi] + g_x;
+                lv1[pi] = lv1[pi] + g_v;
+                gr += dot(a[pi], g) * sc;
+            }
+            // particle pairs (scatter form of the gather in the forward)
+            for (const auto& pr : T.pairs) {
+                const int i = pr.first, j = pr.second;
+                const double wi = double(invm[i]), wj = double(invm[j]);
+                const double sc = wi / (wi + wj);
+                D3 g_rd, g_vr;
+                const D3 g = pair_vjp(dv(T.x1[i]) - dv(T.x1[j]), dv(T.v1[i]) - dv(T.v1[j]), double(radius[i]),
+                                      double(radius[j]), kmu, h, a[i] * (sc * relax), g_rd, g_vr);
+                lx1[i] = lx1[i] + g_rd; lx1[j] = lx1[j] - g_rd;
+                lv1[i] = lv1[i] + g_vr; lv1[j] = lv1[j] - g_vr;
+                gr += dot(a[i], g) * sc;
+            }
+            // integrate (integrate_particles.comp:44-57)
+            for (int64_t i = 0; i < n; i++) {
+                if ((flags[i] & 1) == 0) continue;   // passes through unchanged
+                lx0[i] = lx0[i] + lx1[i];
+                const D3 lvv = lv1[i] + lx1[i] * h;
+                const R gs = invm[i] > 0 ? R(1) : R(0);
+                const V3<R> gr_ = {R(cfg.gravity[0]), R(cfg.gravity[1]), R(cfg.gravity[2])};
+                const V3<R> w = T.v0[i] + (V3<R>{0, 0, 0} * invm[i] + gr_ * gs) * R(tape_dt);
+                if (length(w) > vmax_r) {
+                    const D3 wd = dv(T.v0[i]) + grav * (double(gs) * h);
+                    const double m = length(wd);
+                    const D3 wh = wd * (1.0 / m);
+                    lv[i] = (lvv - wh * dot(wh, lvv)) * (vmax / m);
+                } else {
+                    lv[i] = lvv;
+                }
+                lx[i] = lx0[i];
+            }
+        }
+        for (int64_t i = 0; i < n; i++) {
+            if (gx0) { gx0[3 * i] = lx[i].x; gx0[3 * i + 1] = lx[i].y; gx0[3 * i + 2] = lx[i].z; }
+            if (gv0) { gv0[3 * i] = lv[i].x; gv0[3 * i + 1] = lv[i].y; gv0[3 * i + 2] = lv[i].z; }
+        }
+        if (grelax) *grelax = gr;
+    }
+
     void collide_only() {   // scene.cpp:1229-1245
         if (!finalized) finalize_meshes();
         generate_contacts(false);
@@ -1368,6 +1595,21 @@ int orc_collide_only(void* p) {
     return guarded(h, [&] { DISPATCH(h, collide_only()); });
 }
 
+
+// Differentiable rollout (builder extension): v0 3N doubles (NULL keeps the
+// current velocities), relax the contact relaxation, `substeps` substeps of
+// sdt with one iteration each; keep_tape records what rollout_backward needs.
+int orc_rollout(void* p, const double* v0, double relax, double sdt, int substeps, int keep_tape) {
+    Handle* h = static_cast<Handle*>(p);
+    return guarded(h, [&] {
+        if (h->fp64) h->d.rollout(v0, relax, double(float(sdt)), substeps, keep_tape != 0);
+        else h->f.rollout(v0, relax, float(sdt), substeps, keep_tape != 0);
+    });
+}
+int orc_rollout_backward(void* p, const double* gx, const double* gv, double* gx0, double* gv0, double* grelax) {
+    Handle* h = static_cast<Handle*>(p);
+    return guarded(h, [&] { DISPATCH(h, rollout_backward(gx, gv, gx0, gv0, grelax)); });
+}
 
 // Getters return doubles (exact for the fp32 state) so one ctypes signature
 // serves both precisions.

@@ -75,6 +75,8 @@ def lib():
         L.orc_lbvh_build.argtypes = [_fp, _up, C.c_uint32, _fp, _fp, _up, _up]
         L.orc_ray_tri_woop_f.argtypes = [_fp] * 5
         L.orc_closest_point_triangle_f.argtypes = [_fp] * 6
+        L.orc_rollout.argtypes = [C.c_void_p, _dp, C.c_double, C.c_double, C.c_int, C.c_int]
+        L.orc_rollout_backward.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp]
         L.orc_ray_hit_lists.restype = C.c_int64
         L.orc_ray_hit_lists.argtypes = [_fp, _up, C.c_int64, _fp, C.c_int64, _ip, _up, C.c_int64]
         _lib = L
@@ -123,6 +125,24 @@ class OracleSolver:
 
     def finalize_meshes(self):
         self._ck(self.L.orc_finalize_meshes(self.h))
+
+    def rollout(self, v0, relax, sdt, substeps, keep_tape=True):
+        """Taped rollout of `substeps` one-iteration substeps from the current
+        positions with initial velocities v0 ((N,3), None keeps the current)."""
+        v = None if v0 is None else np.ascontiguousarray(v0, np.float64).reshape(-1)
+        self._ck(self.L.orc_rollout(self.h, _ptr(v, _dp) if v is not None else None, float(relax), float(sdt),
+                                    int(substeps), 1 if keep_tape else 0))
+
+    def rollout_backward(self, gx, gv=None):
+        """FP64 adjoint of the last taped rollout: (gx0, gv0, grelax)."""
+        gx = np.ascontiguousarray(gx, np.float64).reshape(-1)
+        gvv = None if gv is None else np.ascontiguousarray(gv, np.float64).reshape(-1)
+        gx0 = np.zeros_like(gx)
+        gv0 = np.zeros_like(gx)
+        gr = np.zeros(1, np.float64)
+        self._ck(self.L.orc_rollout_backward(self.h, _ptr(gx, _dp), _ptr(gvv, _dp) if gvv is not None else None,
+                                             _ptr(gx0, _dp), _ptr(gv0, _dp), _ptr(gr, _dp)))
+        return gx0.reshape(-1, 3), gv0.reshape(-1, 3), float(gr[0])
 
     def update_mesh(self, idx, verts, vels=None):
         v = np.ascontiguousarray(verts, np.float32).reshape(-1)

@@ -1,0 +1,120 @@
+"""Differentiable rollout on the GPU (vkxpbd.autograd over the C-ABI reverse
+pass) against the FP64 host adjoint of the oracle evaluated on the same fp32
+trajectory (the oracle's fp32 forward is bit-identical to the GPU forward).
+
+Tolerance: the GPU reverse pass runs in fp32; its gradient must match the FP64
+adjoint to 1e-3 relative in norm and on 16 seeded coordinates (relative to the
+gradient's largest component).  The FP64 adjoint itself is checked against
+finite differences in test_adjoint_oracle.py."""
+import numpy as np
+import pytest
+import torch
+
+import oracle_lib as O
+import vkxpbd
+from gpu_util import log
+from vkxpbd import scenes
+from vkxpbd.autograd import rollout
+
+pytestmark = pytest.mark.gpu
+
+SDT = 1.0 / 480.0
+
+
+def setup(q, v0, r, substeps, **cfg):
+    N = q.shape[0]
+    c = scenes.solver_config(vkxpbd, r, **cfg)
+    s = vkxpbd.XpbdSolver()
+    s.init(c, N)
+    s.finalize_meshes()
+    inv_m = np.full(N, scenes.PARTICLE_INV_MASS, np.float32)
+    rad = np.full(N, r, np.float32)
+    s.set_particles(q, v0.astype(np.float32), inv_m, rad, np.ones(N, np.int32))
+    o = O.OracleSolver(fp64=False)
+    o.init(c, N)
+    o.finalize_meshes()
+    o.set_particles(q, v0.astype(np.float32), inv_m, rad, np.ones(N, np.int32))
+    return s, o
+
+
+def gpu_grad(s, q, v0, tgt, relax, substeps, every):
+    dev = torch.device("cuda:0")
+    x0 = torch.tensor(q, dtype=torch.float32, device=dev, requires_grad=True)
+    v = torch.tensor(v0, dtype=torch.float32, device=dev, requires_grad=True)
+    rl = torch.tensor(relax, dtype=torch.float32, device=dev, requires_grad=True)
+    xs, _ = rollout(s, x0, v, rl, SDT, substeps, every)
+    t = torch.tensor(tgt, dtype=torch.float32, device=dev)
+    loss = ((xs - t) ** 2).sum(dim=1).mean()
+    loss.backward()
+    return xs.detach().cpu().numpy(), x0.grad.cpu().numpy(), v.grad.cpu().numpy(), float(rl.grad.cpu()), float(loss)
+
+
+def host_grad(o, xs_gpu, q, v0, tgt, relax, substeps):
+    o.rollout(v0.astype(np.float32).astype(np.float64), float(np.float32(relax)), SDT, substeps, keep_tape=True)
+    x = o.get_positions()
+    np.testing.assert_array_equal(x, xs_gpu.astype(np.float64))   # same trajectory, bit for bit
+    gx = (2.0 * (xs_gpu.astype(np.float32) - tgt.astype(np.float32)) / x.shape[0]).astype(np.float32)
+    return o.rollout_backward(gx.astype(np.float64))
+
+
+def compare(name, g, h, rng, tol=1e-3):
+    g, h = np.asarray(g, np.float64).reshape(-1), np.asarray(h, np.float64).reshape(-1)
+    scale = np.abs(h).max()
+    rel = np.linalg.norm(g - h) / max(np.linalg.norm(h), 1e-30)
+    idx = rng.choice(h.size, min(16, h.size), replace=False)
+    worst = float(np.max(np.abs(g[idx] - h[idx])) / max(scale, 1e-30))
+    log(f"  {name}: |g-h|/|h| = {rel:.2e}, worst of 16 coords {worst:.2e} (scale {scale:.3e})")
+    assert rel <= tol
+    assert worst <= tol
+
+
+@pytest.mark.parametrize("every", [1, 8])
+def test_rollout_gradients_small(vk, every):
+    rng = np.random.default_rng(4)
+    r = 0.01
+    q = scenes.lattice(6, 6, 6, 2 * r, (-0.1, -0.1, 0.02), 0.1 * r, rng)
+    v0 = rng.normal(0, 0.5, size=q.shape).astype(np.float32)
+    tgt = (q.astype(np.float64) + rng.uniform(-0.05, 0.05, size=q.shape)).astype(np.float32)
+    s, o = setup(q, v0, r, 32)
+    log(f"small rollout: 216 particles, 32 substeps, checkpoint every {every}")
+    xs, gx0, gv0, gr, loss = gpu_grad(s, q, v0, tgt, 0.9, 32, every)
+    hx0, hv0, hr = host_grad(o, xs, q, v0, tgt, 0.9, 32)
+    compare("dL/dv0", gv0, hv0, rng)
+    compare("dL/dx0", gx0, hx0, rng)
+    log(f"  dL/drelax gpu {gr:.6e} host {hr:.6e}")
+    assert abs(gr - hr) <= 1e-3 * abs(hr)
+    # the solver's state after the reverse pass is the forward's final state
+    np.testing.assert_array_equal(s.get_positions(), xs)
+
+
+def test_checkpoint_interval_does_not_change_gradients(vk):
+    rng = np.random.default_rng(6)
+    r = 0.01
+    q = scenes.lattice(8, 8, 4, 2 * r, (-0.1, -0.1, 0.02), 0.1 * r, rng)
+    v0 = rng.normal(0, 0.5, size=q.shape).astype(np.float32)
+    tgt = q + np.float32(0.01)
+    out = []
+    for every in (1, 5, 24):
+        s, _ = setup(q, v0, r, 24)
+        out.append(gpu_grad(s, q, v0, tgt, 0.9, 24, every))
+    for o_ in out[1:]:
+        np.testing.assert_array_equal(o_[0], out[0][0])
+        np.testing.assert_array_equal(o_[2], out[0][2])
+        assert o_[3] == out[0][3]
+
+
+def test_rollout_config4(vk):
+    """32,768 particles, 64 substeps of 1/480 s, checkpoints every 8."""
+    sc = scenes.config4()
+    p = sc["particles"]
+    q, v0, tgt = p["q"], p["qd"], sc["target"]
+    s, o = setup(q, v0, sc["r"], 64)
+    log("config 4 rollout: 32768 particles, 64 substeps, checkpoint every 8")
+    xs, gx0, gv0, gr, loss = gpu_grad(s, q, v0, tgt, 0.9, 64, 8)
+    hx0, hv0, hr = host_grad(o, xs, q, v0, tgt, 0.9, 64)
+    rng = np.random.default_rng(44)
+    log(f"  loss {loss:.6e}")
+    compare("dL/dv0", gv0, hv0, rng)
+    compare("dL/dx0", gx0, hx0, rng)
+    log(f"  dL/drelax gpu {gr:.6e} host {hr:.6e}")
+    assert abs(gr - hr) <= 1e-3 * abs(hr)

@@ -109,3 +109,81 @@ def test_parity_grid_inside_outside_sphere():
     cands = candidates(v, t, pts)
     for h, c in zip(hits, cands):
         assert np.setdiff1d(h, c).size == 0
+
+
+def dist_bounds(verts, tris, cap, pts):
+    L = C.CDLL(CAPI)
+    fp, up = C.POINTER(C.c_float), C.POINTER(C.c_uint32)
+    L.xpbd_dist_grid_bounds.argtypes = [fp, C.c_size_t, up, C.c_size_t, C.c_float, fp, C.c_size_t, fp]
+    v = np.ascontiguousarray(verts, np.float32).reshape(-1)
+    t = np.ascontiguousarray(tris, np.uint32).reshape(-1)
+    p = np.ascontiguousarray(pts, np.float32).reshape(-1)
+    out = np.zeros(p.size // 3, np.float32)
+    assert L.xpbd_dist_grid_bounds(v.ctypes.data_as(fp), v.size // 3, t.ctypes.data_as(up), t.size // 3, cap,
+                                   p.ctypes.data_as(fp), p.size // 3, out.ctypes.data_as(fp)) == 0
+    return out
+
+
+def exact_min_dist(verts, tris, pts):
+    """Brute-force point-to-triangle distance (double, textbook Voronoi-region
+    closest point), minimum over all triangles."""
+    v = np.asarray(verts, np.float64)
+    t = np.asarray(tris, np.int64).reshape(-1, 3)
+    a, b, c = v[t[:, 0]], v[t[:, 1]], v[t[:, 2]]
+    out = []
+    for p in np.asarray(pts, np.float64):
+        best = np.inf
+        for k in range(len(t)):
+            cp = _closest(p, a[k], b[k], c[k])
+            best = min(best, float(np.linalg.norm(p - cp)))
+        out.append(best)
+    return np.array(out)
+
+
+def _closest(p, a, b, c):
+    ab, ac, ap = b - a, c - a, p - a
+    d1, d2 = ab @ ap, ac @ ap
+    if d1 <= 0 and d2 <= 0:
+        return a
+    bp = p - b
+    d3, d4 = ab @ bp, ac @ bp
+    if d3 >= 0 and d4 <= d3:
+        return b
+    vc = d1 * d4 - d3 * d2
+    if vc <= 0 and d1 >= 0 and d3 <= 0:
+        return a + ab * (d1 / (d1 - d3))
+    cp_ = p - c
+    d5, d6 = ab @ cp_, ac @ cp_
+    if d6 >= 0 and d5 <= d6:
+        return c
+    vb = d5 * d2 - d1 * d6
+    if vb <= 0 and d2 >= 0 and d6 <= 0:
+        return a + ac * (d2 / (d2 - d6))
+    va = d3 * d6 - d5 * d4
+    if va <= 0 and (d4 - d3) >= 0 and (d5 - d6) >= 0:
+        return b + (c - b) * ((d4 - d3) / ((d4 - d3) + (d5 - d6)))
+    den = 1.0 / (va + vb + vc)
+    return a + ab * (vb * den) + ac * (vc * den)
+
+
+def test_distance_bounds_are_lower_bounds():
+    """The mesh query skips a particle when its cell's bound exceeds the search
+    distance: every bound must be <= the exact distance to the mesh, and a
+    point outside the grid must be farther than the cap."""
+    v, t = icosphere(2, radius=0.5, center=(0.0, 0.0, 0.5))
+    cap = 0.05
+    rng = np.random.default_rng(8)
+    d = rng.normal(size=(600, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    r = rng.uniform(0.3, 0.7, 600)
+    pts = np.concatenate([d * r[:, None] + [0, 0, 0.5], rng.uniform(-0.7, 0.7, (200, 3)) + [0, 0, 0.5]])
+    pts = pts.astype(np.float32)
+    lb = dist_bounds(v, t, cap, pts)
+    ex = exact_min_dist(v, t, pts)
+    inside = lb >= 0
+    assert np.all(lb[inside] <= ex[inside] + 1e-7)
+    assert np.all(ex[~inside] > cap)
+    # the bound is useful: tight to about a cell diagonal near the surface
+    near = inside & (ex < cap)
+    assert np.all(ex[near] - lb[near] <= 0.05 * np.sqrt(3) * 0.5 + 0.06)
+    assert np.count_nonzero(inside & (lb > 0.01)) > 100
