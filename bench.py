@@ -144,7 +144,20 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--batched-frames", type=int, default=10)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 256 MiB device copy (3 times): the reference kernel for scaling "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE counts to bytes")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = 64 * 1024 * 1024
+        src = torch.arange(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "copies": 3}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU sharding is a design, not a run)", file=sys.stderr)

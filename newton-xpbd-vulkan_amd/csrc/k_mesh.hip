@@ -21,9 +21,11 @@
 // node.
 #include "xpbd_kernels.h"
 
+#include <algorithm>
+
 namespace xpbd {
 
-constexpr int kMeshBlock = 256;
+constexpr int kMeshBlock = 64;   // tree-walk workgroup: one wave
 constexpr int kStack = 32;
 
 __device__ __forceinline__ f3 ldv(const float* v, uint32_t k) { return {v[3 * k], v[3 * k + 1], v[3 * k + 2]}; }
@@ -66,10 +68,12 @@ __device__ __forceinline__ bool ray_tri(const Ray& r, f3 a, f3 b, f3 c) {
     return true;
 }
 
-// closest_point_triangle of generate_mesh_contacts.comp:115-179.
+// closest_point_triangle of generate_mesh_contacts.comp:115-179 (NORMAL:
+// also the unit face normal, which only the non-watertight sign uses).
+template <bool NORMAL>
 __device__ void closest_point_triangle(f3 p, f3 a, f3 b, f3 c, f3& cp, f3& tn, f3& bary) {
     f3 ab = b - a, ac = c - a, ap = p - a;
-    tn = normalize3(cross3(ab, ac));
+    if (NORMAL) tn = normalize3(cross3(ab, ac));
     float u, v;
     float d1 = exact_dot3(ab, ap), d2 = exact_dot3(ac, ap);
     if (d1 <= 0.0f && d2 <= 0.0f) { v = 0.0f; u = 1.0f; }
@@ -155,9 +159,8 @@ __device__ __forceinline__ uint32_t parity_hits(const MeshArgs& a, const MeshInf
     tested += e - b;
     uint32_t hits = 0;
     for (uint32_t k = b; k < e; k++) {
-        const uint32_t t = a.pg_tri[k];
-        if (ray_tri(ray, ldv(a.verts, a.tris[3 * t]), ldv(a.verts, a.tris[3 * t + 1]), ldv(a.verts, a.tris[3 * t + 2])))
-            hits++;
+        const float4 A = a.pg_soup[3 * k], B = a.pg_soup[3 * k + 1], C = a.pg_soup[3 * k + 2];
+        if (ray_tri(ray, xyz(A), xyz(B), xyz(C))) hits++;
     }
     return hits;
 }
@@ -183,31 +186,36 @@ __device__ bool closest_face(const MeshArgs& a, const MeshInfo& mi, uint32_t sta
     bool found = false;
     float min_dist_sq = max_dist * max_dist;
     while (sp > 0) {
-        const uint32_t ni = stk[--sp][lt];
+        const uint32_t word = stk[--sp][lt];
+        const uint32_t ni = word & 0x7FFFFFFFu;
         if (STATS) pops++;
-        const float4 L = node_lo(a.nodes, ni), U = node_hi(a.nodes, ni);
-        const float nds = dist_sq_aabb(p, xyz(L), xyz(U));
-        if (wt) { if (nds > min_dist_sq) continue; }
-        else if (nds > (min_dist + eps) * (min_dist + eps)) continue;
-        const uint32_t lp = __float_as_uint(L.w);
-        const uint32_t li = lp & 0x3FFFFFFFu, leaf = (lp >> 30) & 3u, ri = __float_as_uint(U.w) & 0x3FFFFFFFu;
-        if (leaf) {
+        if (word >> 31) {   // leaf: the reference's pop-time box test, then its triangles
+            const float4 L = node_lo(a.nodes, ni), U = node_hi(a.nodes, ni);
+            const float nds = dist_sq_aabb(p, xyz(L), xyz(U));
+            if (wt) { if (nds > min_dist_sq) continue; }
+            else if (nds > (min_dist + eps) * (min_dist + eps)) continue;
+            const uint32_t li = __float_as_uint(L.w) & 0x3FFFFFFFu, ri = __float_as_uint(U.w) & 0x3FFFFFFFu;
             for (uint32_t pc = li; pc < ri; pc++) {
-                const uint32_t t = a.prim[pc];
+                const float4 A = a.tsoup[3 * pc], B = a.tsoup[3 * pc + 1], C = a.tsoup[3 * pc + 2];
+                const uint32_t t = __float_as_uint(A.w);
                 if (STATS) tests++;
-                const f3 va = ldv(a.verts, a.tris[3 * t]), vb = ldv(a.verts, a.tris[3 * t + 1]),
-                         vc = ldv(a.verts, a.tris[3 * t + 2]);
+                const f3 va = xyz(A), vb = xyz(B), vc = xyz(C);
                 const f3 e0 = vb - va, e1 = vc - va, e2 = vc - vb;
-                const f3 nrm = cross3(e0, e1);
                 const float e0n = dot3(e0, e0), e1n = dot3(e1, e1), e2n = dot3(e2, e2);
-                if (len3(nrm) / ((e0n + e1n) + e2n) < 1.0e-6f) continue;
                 f3 cp, tn, bc;
-                closest_point_triangle(p, va, vb, vc, cp, tn, bc);
                 if (wt) {
+                    // the degenerate-triangle skip (:250-251) only matters for a
+                    // triangle that would become the closest: test it there
+                    closest_point_triangle<false>(p, va, vb, vc, cp, tn, bc);
                     const f3 df = cp - p;
                     const float dsq = dot3(df, df);
-                    if (dsq < min_dist_sq) { min_dist_sq = dsq; min_face = t; sv = bc.y; sw = bc.z; found = true; }
+                    if (dsq < min_dist_sq) {
+                        if (len3(cross3(e0, e1)) / ((e0n + e1n) + e2n) < 1.0e-6f) continue;
+                        min_dist_sq = dsq; min_face = t; sv = bc.y; sw = bc.z; found = true;
+                    }
                 } else {
+                    if (len3(cross3(e0, e1)) / ((e0n + e1n) + e2n) < 1.0e-6f) continue;
+                    closest_point_triangle<true>(p, va, vb, vc, cp, tn, bc);
                     const float dist = exact_length(cp - p);
                     if (dist < min_dist + eps) {
                         float weight = 0.0f;
@@ -235,18 +243,24 @@ __device__ bool closest_face(const MeshArgs& a, const MeshInfo& mi, uint32_t sta
                 }
             }
         } else {
-            const float4 LL = node_lo(a.nodes, li), LU = node_hi(a.nodes, li);
-            const float4 RL = node_lo(a.nodes, ri), RU = node_hi(a.nodes, ri);
+            // internal node: one read of both children's boxes.  Its own
+            // pop-time box test is not repeated: a node it would prune has
+            // children at least as far (their boxes lie inside its box), so
+            // neither is pushed and nothing changes.
+            const float4* rec = a.cnodes + 4 * size_t(ni);
+            const float4 LL = rec[0], LU = rec[1], RL = rec[2], RU = rec[3];
             const float lds = dist_sq_aabb(p, xyz(LL), xyz(LU));
             const float rds = dist_sq_aabb(p, xyz(RL), xyz(RU));
             const float th = wt ? min_dist_sq : (min_dist + eps) * (min_dist + eps);
+            const uint32_t lw = __float_as_uint(LL.w), rw = __float_as_uint(RL.w);
+            if (!(lds < th) && !(rds < th)) continue;
             if (sp + 2 > kStack) { atomicOr(a.err, 1u); continue; }
             if (lds < rds) {
-                if (rds < th) stk[sp++][lt] = ri;
-                if (lds < th) stk[sp++][lt] = li;
+                if (rds < th) stk[sp++][lt] = rw;
+                if (lds < th) stk[sp++][lt] = lw;
             } else {
-                if (lds < th) stk[sp++][lt] = li;
-                if (rds < th) stk[sp++][lt] = ri;
+                if (lds < th) stk[sp++][lt] = lw;
+                if (rds < th) stk[sp++][lt] = rw;
             }
         }
     }
@@ -275,11 +289,11 @@ __device__ bool closest_face(const MeshArgs& a, const MeshInfo& mi, uint32_t sta
 // grid every triangle is farther than the cap.  The pad (2e-5 at unit
 // coordinates) is far above the rounding of the device's distance and cell
 // computations.  The start node is the root unless the grid applies.
-constexpr uint32_t kNoNode = 0xFFFFFFFFu;
+constexpr uint32_t kNoNode = 0xFFFFFFFFu;   // (node words carry a leaf flag in bit 31)
 __device__ __forceinline__ uint32_t search_start(const MeshArgs& a, const MeshInfo& mi, f3 p, float thr) {
     const float sc = fmaxf(fmaxf(fabsf(p.x), fabsf(p.y)), fabsf(p.z));
     const float lim = thr + 1e-5f * (1.0f + sc);
-    if (!(lim < mi.lb_cap)) return mi.root;
+    if (!(lim < mi.lb_cap)) return mi.root_word;
     const float fx = floorf((p.x - mi.lb_ox) * mi.lb_inv), fy = floorf((p.y - mi.lb_oy) * mi.lb_inv),
                 fz = floorf((p.z - mi.lb_oz) * mi.lb_inv);
     if (!(fx >= 0.0f && fy >= 0.0f && fz >= 0.0f && fx < float(mi.lb_nx) && fy < float(mi.lb_ny) && fz < float(mi.lb_nz)))
@@ -297,51 +311,93 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
     return v;
 }
 
-// One thread per particle.  Per mesh: the reference's root-box test plus the
-// distance-bound early out, then the particles that remain are compacted to
-// the front of the block (block order kept) so the tree walks run on dense
-// waves, then one block-level reservation for the contacts and the payloads
-// (generate_mesh_contacts.comp:333-411).
+// Candidate test of particle i against mesh mi: the reference's first pop
+// (the root box against the search distance) and the distance-bound early
+// out.  Returns the search start node word or kNoNode.
+__device__ __forceinline__ uint32_t mesh_candidate(const MeshArgs& a, const MeshInfo& mi, int i) {
+    if (i >= a.n || (a.flags[i] & 1) == 0) return kNoNode;
+    const float4 P = a.P[i];
+    const f3 p = xyz(P);
+    const float max_dist = (a.margin + mi.margin) + P.w;
+    const float thr = mi.watertight ? max_dist : max_dist + a.eps;
+    const float nds = dist_sq_aabb(p, xyz(node_lo(a.nodes, mi.root)), xyz(node_hi(a.nodes, mi.root)));
+    if (nds > thr * thr) return kNoNode;
+    return search_start(a, mi, p, thr);
+}
+
+// Pass 1: candidates per 256-particle block.
 template <bool STATS>
-__global__ __launch_bounds__(kMeshBlock) void k_mesh_contacts(MeshArgs a) {
+__global__ __launch_bounds__(256) void k_mesh_count(MeshArgs a, int ci) {
+    const MeshInfo mi = a.info[ci];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const bool cand = mesh_candidate(a, mi, i) != kNoNode;
+    const int c = __syncthreads_count(cand);
+    if (threadIdx.x == 0) a.mq_count[blockIdx.x] = uint32_t(c);
+    if (STATS) {
+        const uint64_t act = wave_sum_u64((i < a.n && (a.flags[i] & 1)) ? 1u : 0u);
+        if (__lane_id() == 0 && act) atomicAdd(reinterpret_cast<unsigned long long*>(a.stats), (unsigned long long)act);
+    }
+}
+
+// Pass 2 (one block): exclusive scan of the block counts in place; the total
+// goes to mq_count[nb].
+__global__ __launch_bounds__(1024) void k_mesh_scan(uint32_t* cnt, int nb) {
+    __shared__ uint32_t part[1024];
+    const int t = threadIdx.x;
+    const int per = (nb + 1023) / 1024;
+    const int b0 = min(nb, t * per), b1 = min(nb, b0 + per);
+    uint32_t sum = 0;
+    for (int k = b0; k < b1; k++) sum += cnt[k];
+    part[t] = sum;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {   // Hillis-Steele over 1024 partial sums
+        const uint32_t v = t >= o ? part[t - o] : 0u;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    uint32_t run = part[t] - sum;
+    for (int k = b0; k < b1; k++) { const uint32_t c = cnt[k]; cnt[k] = run; run += c; }
+    if (t == 1023) cnt[nb] = part[1023];
+}
+
+// Pass 3: candidates into the queue in particle order.
+__global__ __launch_bounds__(256) void k_mesh_enqueue(MeshArgs a, int ci) {
+    __shared__ int s_tmp[256 / 64 + 1];
+    const MeshInfo mi = a.info[ci];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const uint32_t start = mesh_candidate(a, mi, i);
+    int tot = 0;
+    const int pos = block_exclusive_scan(start != kNoNode ? 1 : 0, s_tmp, &tot);
+    if (start != kNoNode) a.mq[a.mq_count[blockIdx.x] + uint32_t(pos)] = make_int2(i, int(start));
+}
+
+// Pass 4: one queued particle per thread, one wave per block so the walks
+// spread over every SIMD; contact slots reserved once per block, then the
+// payloads (generate_mesh_contacts.comp:333-411).  Grid-stride over the
+// queue with a block-uniform trip count.
+template <bool STATS>
+__global__ __launch_bounds__(kMeshBlock) void k_mesh_walk(MeshArgs a, int ci, int nb) {
     __shared__ uint32_t stk[kStack][kMeshBlock];
-    __shared__ int s_list[kMeshBlock];
-    __shared__ uint32_t s_start[kMeshBlock];
     __shared__ int s_tmp[kMeshBlock / 64 + 1];
     uint32_t st[6] = {0, 0, 0, 0, 0, 0};
+    const MeshInfo mi = a.info[ci];
     const int lt = threadIdx.x;
-    const int i = blockIdx.x * kMeshBlock + lt;
-    const bool act = i < a.n && (a.flags[i] & 1);
-    const float4 Pi = act ? a.P[i] : make_float4(0.f, 0.f, 0.f, 0.f);
-    for (int ci = 0; ci < a.nmesh; ci++) {
-        const MeshInfo mi = a.info[ci];
-        bool cand = false;
-        uint32_t start = kNoNode;
-        if (act) {
-            if (STATS) st[0]++;
-            const f3 p = xyz(Pi);
-            // the reference's first pop: the root box against the search distance
-            const float max_dist = (a.margin + mi.margin) + Pi.w;
-            const float thr = mi.watertight ? max_dist : max_dist + a.eps;
-            const float nds = dist_sq_aabb(p, xyz(node_lo(a.nodes, mi.root)), xyz(node_hi(a.nodes, mi.root)));
-            if (!(nds > thr * thr)) start = search_start(a, mi, p, thr);
-            cand = start != kNoNode;
-        }
-        int m = 0;
-        const int pos = block_exclusive_scan(cand ? 1 : 0, s_tmp, &m);
-        if (cand) { s_list[pos] = lt; s_start[pos] = start; }
-        __syncthreads();
+    const uint32_t total = a.mq_count[nb];
+    for (uint32_t base = blockIdx.x * kMeshBlock; base < total; base += gridDim.x * kMeshBlock) {
+        const uint32_t q = base + uint32_t(lt);
         bool hit = false;
         uint32_t face = 0;
         float sv = 0.f, sw = 0.f, sg = 1.0f;
         int pi = 0;
         float4 P = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (lt < m) {
-            pi = blockIdx.x * kMeshBlock + s_list[lt];
+        if (q < total) {
+            const int2 e = a.mq[q];
+            pi = e.x;
             P = a.P[pi];
             const f3 p = xyz(P);
             if (STATS) st[1]++;
-            hit = closest_face<STATS>(a, mi, s_start[lt], p, P.w, stk, lt, face, sv, sw, sg, st[2], st[3]);
+            hit = closest_face<STATS>(a, mi, uint32_t(e.y), p, P.w, stk, lt, face, sv, sw, sg, st[2], st[3]);
             if (hit && mi.watertight) sg = (parity_hits(a, mi, make_ray(p), st[4]) & 1u) ? -1.0f : 1.0f;
         }
         if (STATS) st[5] += hit ? 1u : 0u;
@@ -373,7 +429,7 @@ __global__ __launch_bounds__(kMeshBlock) void k_mesh_contacts(MeshArgs a) {
     }
     if (STATS) {
 #pragma unroll
-        for (int k = 0; k < 6; k++) {
+        for (int k = 1; k < 6; k++) {
             const uint64_t v = wave_sum_u64(st[k]);
             if (__lane_id() == 0 && v) atomicAdd(reinterpret_cast<unsigned long long*>(a.stats + k), (unsigned long long)v);
         }
@@ -382,9 +438,16 @@ __global__ __launch_bounds__(kMeshBlock) void k_mesh_contacts(MeshArgs a) {
 
 void launch_mesh_contacts(const MeshArgs& a, hipStream_t s) {
     if (a.n <= 0 || a.nmesh <= 0) return;
-    const dim3 g((a.n + kMeshBlock - 1) / kMeshBlock), b(kMeshBlock);
-    if (a.stats) hipLaunchKernelGGL(k_mesh_contacts<true>, g, b, 0, s, a);
-    else hipLaunchKernelGGL(k_mesh_contacts<false>, g, b, 0, s, a);
+    const int nb = (a.n + 255) / 256;
+    const unsigned walk_grid = unsigned(std::min(4096, (a.n + kMeshBlock - 1) / kMeshBlock));
+    for (int ci = 0; ci < a.nmesh; ci++) {
+        if (a.stats) hipLaunchKernelGGL(k_mesh_count<true>, dim3(nb), dim3(256), 0, s, a, ci);
+        else hipLaunchKernelGGL(k_mesh_count<false>, dim3(nb), dim3(256), 0, s, a, ci);
+        hipLaunchKernelGGL(k_mesh_scan, dim3(1), dim3(1024), 0, s, a.mq_count, nb);
+        hipLaunchKernelGGL(k_mesh_enqueue, dim3(nb), dim3(256), 0, s, a, ci);
+        if (a.stats) hipLaunchKernelGGL(k_mesh_walk<true>, dim3(walk_grid), dim3(kMeshBlock), 0, s, a, ci, nb);
+        else hipLaunchKernelGGL(k_mesh_walk<false>, dim3(walk_grid), dim3(kMeshBlock), 0, s, a, ci, nb);
+    }
 }
 
 }  // namespace xpbd

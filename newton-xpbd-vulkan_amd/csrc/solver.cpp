@@ -57,7 +57,7 @@ void Solver::free_all() {
     dfree(range_); dfree(tbl_); dfree(cpart_); dfree(cshape_); dfree(cc_); dfree(cbpos_); dfree(cbvel_); dfree(cnorm_);
     dfree(dshape_); dfree(delta_); dfree(body_q_); dfree(body_qd_); dfree(body_com_); dfree(body_invI_);
     dfree(body_invm_); dfree(body_delta_); dfree(mat_mu_); dfree(body_flags_); dfree(shape_body_);
-    dfree(verts_d_); dfree(vvel_d_); dfree(tris_d_); dfree(prim_d_); dfree(err_d_); dfree(nodes_d_); dfree(pg_start_); dfree(pg_tri_); dfree(lb_); dfree(lb_node_);
+    dfree(verts_d_); dfree(vvel_d_); dfree(tris_d_); dfree(prim_d_); dfree(err_d_); dfree(nodes_d_); dfree(cnodes_d_); dfree(pg_start_); dfree(lb_); dfree(lb_node_); dfree(tsoup_); dfree(pg_soup_); dfree(mq_count_); dfree(mq_);
     dfree(info_d_); dfree(diag_header_); dfree(diag_pairs_); dfree(mesh_stats_);
     for (hipEvent_t e : ev_pool_) (void)hipEventDestroy(e);
     ev_pool_.clear();
@@ -88,6 +88,8 @@ void Solver::alloc_all() {
     dalloc(dshape_, 3 * n_); dalloc(delta_, 3 * n_);
     dalloc(diag_header_, 2);
     dalloc(mesh_stats_, 8);
+    dalloc(mq_count_, (n_ + 255) / 256 + 1);   // mesh candidate queue
+    dalloc(mq_, n_);
     max_pairs_ = cfg_.max_diag_pairs ? cfg_.max_diag_pairs : 100000u;
     dalloc(diag_pairs_, cfg_.enable_diagnostics ? size_t(20) * max_pairs_ : 20);
     HIPCHECK(hipHostMalloc(reinterpret_cast<void**>(&count_host_), 4 * sizeof(int32_t), hipHostMallocDefault));
@@ -204,6 +206,31 @@ void Solver::upload_nodes() {
     std::vector<float4> inter(2 * total);
     for (size_t i = 0; i < total; i++) { inter[2 * i] = nl[i]; inter[2 * i + 1] = nu[i]; }
     upload(nodes_d_, inter.data(), inter.size());
+    // child-box records: per internal node both children's boxes, each lower
+    // corner's w = child index | leaf flag << 31 (one 64-byte read per expansion)
+    std::vector<float4> rec(4 * total, make_float4(0.f, 0.f, 0.f, 0.f));
+    for (size_t c = 0; c < bvh_.size(); c++) {
+        const uint32_t nb = uint32_t(node_off_[c]);
+        const MeshBvhHost& b = bvh_[c];
+        for (size_t i = 0; i < b.lo.size(); i++) {
+            if ((b.lo[i].packed >> 30) & 3u) continue;
+            const uint32_t ch[2] = {b.lo[i].packed & 0x3FFFFFFFu, b.hi[i].packed & 0x3FFFFFFFu};
+            for (int k = 0; k < 2; k++) {
+                const uint32_t g = (nb + ch[k]) | ((((b.lo[ch[k]].packed >> 30) & 3u) != 0u) ? 0x80000000u : 0u);
+                float gw;
+                std::memcpy(&gw, &g, 4);
+                const NodeHalf L = b.lo[ch[k]], U = b.hi[ch[k]];
+                rec[4 * (nb + i) + 2 * k] = make_float4(L.x, L.y, L.z, gw);
+                rec[4 * (nb + i) + 2 * k + 1] = make_float4(U.x, U.y, U.z, 0.f);
+            }
+        }
+    }
+    upload(cnodes_d_, rec.data(), rec.size());
+    for (size_t c = 0; c < bvh_.size() && c < info_h_.size(); c++) {
+        const MeshBvhHost& b = bvh_[c];
+        const bool leaf = !b.lo.empty() && ((b.lo[b.root].packed >> 30) & 3u) != 0u;
+        info_h_[c].root_word = uint32_t(node_off_[c] + b.root) | (leaf ? 0x80000000u : 0u);
+    }
 }
 
 // Host-side query accelerators of one mesh: the projected parity grid
@@ -226,6 +253,7 @@ void Solver::build_query_grids(size_t c) {
 void Solver::upload_query_grids() {
     std::vector<uint32_t> start, tri, lbn;
     std::vector<float> lb;
+    std::vector<float4> psoup;
     for (size_t c = 0; c < pgrid_.size(); c++) {
         const ParityGrid& g = pgrid_[c];
         info_h_[c].pg_u0 = g.u0;
@@ -236,21 +264,46 @@ void Solver::upload_query_grids() {
         const uint32_t base = uint32_t(tri.size());
         for (size_t k = 0; k + 1 < g.start.size(); k++) start.push_back(base + g.start[k]);
         tri.insert(tri.end(), g.tri.begin(), g.tri.end());
+        for (uint32_t t : g.tri)   // the registered triangles' corners, in cell order
+            for (int q = 0; q < 3; q++) {
+                const uint32_t v = all_tris_[3 * size_t(t) + q];
+                psoup.push_back(make_float4(all_verts_[3 * v], all_verts_[3 * v + 1], all_verts_[3 * v + 2], 0.f));
+            }
         const DistGrid& d = dgrid_[c];
         info_h_[c].lb_ox = d.ox; info_h_[c].lb_oy = d.oy; info_h_[c].lb_oz = d.oz; info_h_[c].lb_inv = d.inv;
         info_h_[c].lb_nx = d.nx; info_h_[c].lb_ny = d.ny; info_h_[c].lb_nz = d.nz;
         info_h_[c].lb_off = uint32_t(lb.size());
         info_h_[c].lb_cap = d.lb.empty() ? 0.0f : d.cap;   // cap 0: no early out
         lb.insert(lb.end(), d.lb.begin(), d.lb.end());
-        for (size_t k = 0; k < d.lb.size(); k++)   // start nodes, re-based like the packed nodes
-            lbn.push_back(uint32_t(node_off_[c]) + (k < d.start.size() ? d.start[k] : bvh_[c].root));
+        for (size_t k = 0; k < d.lb.size(); k++) {   // start nodes, re-based, with the leaf flag
+            const uint32_t v = k < d.start.size() ? d.start[k] : bvh_[c].root;
+            const bool leaf = ((bvh_[c].lo[v].packed >> 30) & 3u) != 0u;
+            lbn.push_back((uint32_t(node_off_[c]) + v) | (leaf ? 0x80000000u : 0u));
+        }
     }
     start.push_back(uint32_t(tri.size()));
     upload(pg_start_, start.data(), start.size());
-    upload(pg_tri_, tri.data(), tri.size());
+    upload(pg_soup_, psoup.data(), psoup.size());
     upload(lb_, lb.data(), lb.size());
     upload(lb_node_, lbn.data(), lbn.size());
     upload(info_d_, info_h_.data(), info_h_.size());
+}
+
+// Corners of every triangle in leaf order (the order the tree search reads
+// them), the first corner's w carrying the global triangle id: one contiguous
+// 48-byte read per triangle instead of an index-then-vertex gather.
+void Solver::upload_tri_soup() {
+    std::vector<float4> soup(3 * prim_h_.size());
+    for (size_t k = 0; k < prim_h_.size(); k++) {
+        const uint32_t t = prim_h_[k];
+        for (int q = 0; q < 3; q++) {
+            const uint32_t v = all_tris_[3 * size_t(t) + q];
+            float w = 0.f;
+            if (q == 0) std::memcpy(&w, &t, sizeof(w));
+            soup[3 * k + q] = make_float4(all_verts_[3 * v], all_verts_[3 * v + 1], all_verts_[3 * v + 2], w);
+        }
+    }
+    upload(tsoup_, soup.data(), soup.size());
 }
 
 void Solver::finalize_meshes() {
@@ -357,7 +410,9 @@ void Solver::finalize_meshes() {
     upload(vvel_d_, all_vels_.data(), all_vels_.size());
     upload(tris_d_, all_tris_.data(), all_tris_.size());
     upload(prim_d_, prim_global.data(), prim_global.size());
+    prim_h_ = prim_global;
     upload_nodes();
+    upload_tri_soup();
     pgrid_.assign(M, ParityGrid{});
     dgrid_.assign(M, DistGrid{});
     for (size_t c = 0; c < M; c++) build_query_grids(c);
@@ -402,6 +457,7 @@ void Solver::update_mesh(size_t mi, const float* v, size_t nv, const float* vel)
     HIPCHECK(hipMemcpy(verts_d_.p + 3 * vs, v, nv * 3 * sizeof(float), hipMemcpyHostToDevice));
     HIPCHECK(hipMemcpy(vvel_d_.p + 3 * vs, all_vels_.data() + 3 * vs, nv * 3 * sizeof(float), hipMemcpyHostToDevice));
     upload_nodes();
+    upload_tri_soup();
 }
 
 BodyArgs Solver::body_args() const {
@@ -500,8 +556,9 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
     if (collide && !meshes_.empty()) {
         MeshArgs m;
         m.n = n; m.P = P_[s_].p; m.flags = flags_.p;
-        m.verts = verts_d_.p; m.tris = tris_d_.p; m.nodes = nodes_d_.p; m.prim = prim_d_.p;
-        m.pg_start = pg_start_.p; m.pg_tri = pg_tri_.p; m.lb = lb_.p; m.lb_node = lb_node_.p;
+        m.verts = verts_d_.p; m.tris = tris_d_.p; m.nodes = nodes_d_.p; m.cnodes = cnodes_d_.p; m.prim = prim_d_.p;
+        m.pg_start = pg_start_.p; m.pg_soup = pg_soup_.p; m.tsoup = tsoup_.p; m.lb = lb_.p; m.lb_node = lb_node_.p;
+        m.mq_count = mq_count_.p; m.mq = mq_.p;
         m.vvel = vvel_d_.p; m.info = info_d_.p; m.nmesh = M;
         m.margin = cfg_.soft_contact_margin; m.eps = avg_edge_ * 1e-3f;
         m.cmax = int(cmax_); m.cc = cc_gen;
@@ -808,8 +865,9 @@ void Solver::collide_only() {
     if (!meshes_.empty()) {
         MeshArgs m;
         m.n = int(n_); m.P = P_[s_].p; m.flags = flags_.p;
-        m.verts = verts_d_.p; m.tris = tris_d_.p; m.nodes = nodes_d_.p; m.prim = prim_d_.p;
-        m.pg_start = pg_start_.p; m.pg_tri = pg_tri_.p; m.lb = lb_.p; m.lb_node = lb_node_.p;
+        m.verts = verts_d_.p; m.tris = tris_d_.p; m.nodes = nodes_d_.p; m.cnodes = cnodes_d_.p; m.prim = prim_d_.p;
+        m.pg_start = pg_start_.p; m.pg_soup = pg_soup_.p; m.tsoup = tsoup_.p; m.lb = lb_.p; m.lb_node = lb_node_.p;
+        m.mq_count = mq_count_.p; m.mq = mq_.p;
         m.vvel = vvel_d_.p; m.info = info_d_.p; m.nmesh = int(meshes_.size());
         m.margin = cfg_.soft_contact_margin; m.eps = avg_edge_ * 1e-3f;
         m.cmax = int(cmax_); m.cc = cc_.p;

@@ -118,6 +118,7 @@ private:
     void swap_tape(int j);
     void ensure_tape();
     void build_query_grids(size_t mesh);
+    void upload_tri_soup();
     void upload_query_grids();
     BodyArgs body_args() const;
     void substep(float sdt, int iterations, bool collide, int slot, bool zero_next);
@@ -207,11 +208,17 @@ private:
     DevBuf<float> verts_d_, vvel_d_;
     DevBuf<uint32_t> tris_d_, prim_d_, err_d_;
     DevBuf<float4> nodes_d_;   // interleaved (lower, upper) node boxes, re-based
+    DevBuf<float4> cnodes_d_;  // per internal node: both children's boxes and tagged indices
     std::vector<ParityGrid> pgrid_;        // per mesh (empty for non-watertight)
-    DevBuf<uint32_t> pg_start_, pg_tri_;   // all meshes' grids, concatenated
+    DevBuf<uint32_t> pg_start_;            // all meshes' parity-grid cell starts, concatenated
     std::vector<DistGrid> dgrid_;          // per mesh distance lower bounds
     DevBuf<float> lb_;                     // all meshes' bounds, concatenated
     DevBuf<uint32_t> lb_node_;             // per bound cell: search start node
+    std::vector<uint32_t> prim_h_;         // global triangle id per leaf position
+    DevBuf<float4> tsoup_;                 // triangle corners in leaf order
+    DevBuf<float4> pg_soup_;               // triangle corners of every parity-grid entry
+    DevBuf<uint32_t> mq_count_;            // mesh candidates per 256-particle block, scanned
+    DevBuf<int2> mq_;                      // mesh candidate queue (particle, start node)
     DevBuf<MeshInfo> info_d_;
     std::vector<MeshInfo> info_h_;
     float avg_edge_ = 0.f;

@@ -115,7 +115,8 @@ struct MeshInfo {
     float lb_ox, lb_oy, lb_oz, lb_inv;
     uint32_t lb_nx, lb_ny, lb_nz, lb_off;
     float lb_cap;
-    uint32_t pad[5];
+    uint32_t root_word;   // root index | leaf flag << 31
+    uint32_t pad[4];
 };
 static_assert(sizeof(MeshInfo) == 96, "MeshInfo layout");
 
@@ -123,6 +124,7 @@ struct MeshArgs {
     int n; const float4* P; const int* flags;
     const float* verts; const uint32_t* tris;
     const float4* nodes;   // interleaved (lower, upper) per node
+    const float4* cnodes;  // per internal node: (lower, upper) of both children; lower.w = child | leaf << 31
     const uint32_t* prim;
     const float* vvel; const MeshInfo* info; int nmesh;
     float margin, eps;
@@ -130,10 +132,13 @@ struct MeshArgs {
     int* cpart; int* cshape; float4* cbpos; float4* cbvel; float4* cnorm;
     uint32_t* err;
     const uint32_t* pg_start;   // projected parity grids: CSR cell starts ...
-    const uint32_t* pg_tri;     // ... and global triangle ids
+    const float4* pg_soup;      // ... and each entry's triangle corners (3 float4)
+    const float4* tsoup;        // triangle corners in leaf order (3 float4; first w = triangle id)
     const float* lb;            // distance lower bounds of all meshes' grids
     const uint32_t* lb_node;    // per bound cell: node the closest-face search starts at
     uint64_t* stats;            // optional query counters (6 words), null = off
+    uint32_t* mq_count;         // per 256-particle block candidate counts (+ total), scanned in place
+    int2* mq;                   // candidate queue: (particle, search start node word)
 };
 
 // Rigid bodies (dynamic bodies occupy indices [b0, b1) of the body arrays).
