@@ -228,19 +228,20 @@ __global__ __launch_bounds__(256) void k_shape_solve(ShapeArgs a) {
 }
 
 // One neighbour candidate of solve_particle_particle_contacts.comp:150-210.
+// The candidate at sorted slot k has passed the cheap distance reject; the
+// rest of the reference's tests and the projection follow in its order
+// (self, inactive, proxy, box, refined distance).  Every test is a pure
+// reject, so testing the distance first changes nothing.
 template <bool DIAG>
-__device__ __forceinline__ void pp_pair(const PPArgs& a, int t, int k, int nf, float4 Pj, float4 Vj, int i, int proxy, f3 x,
-                                        f3 v, float rad, float w1, float R, f3& acc) {
+__device__ __forceinline__ void pp_pair(const PPArgs& a, int t, int k, float4 Pj, int i, int proxy, f3 x, f3 v, float rad,
+                                        float w1, float R, f3& acc) {
     if (k == t) return;   // j == i
+    const int nf = a.Fs[k];
     if ((nf & 1) == 0) return;
+    const float4 Vj = a.Vs[k];
     if (proxy != 0 && (((nf & 2) != 0) || (Vj.w <= 0.0f))) return;
     const f3 rd = x - xyz(Pj);
     if (fabsf(rd.x) > R || fabsf(rd.y) > R || fabsf(rd.z) > R) return;
-    // cheap reject before the refined length: a pair the exact test accepts
-    // has |rd| <= (r_i + r_j + coh)(1 + 5 ulp), and the plain squared length
-    // is within 3 ulp of |rd|^2, far inside the 1e-4 relative slack
-    const float lim = (rad + Pj.w) + fabsf(a.coh);
-    if ((rd.x * rd.x + rd.y * rd.y) + rd.z * rd.z > lim * lim * 1.0001f + 1e-30f) return;
     const float d = exact_length(rd);
     const float err = d - rad - Pj.w;
     if (err > a.coh || d <= 0.0f) return;
@@ -328,6 +329,15 @@ __global__ __launch_bounds__(256) void k_pp_apply(PPArgs a) {
             rg = make_int2(0, 0);
         return rg;
     };
+    // cheap reject before the refined length: a pair the exact test accepts
+    // has |rd| <= (r_i + r_j + coh)(1 + 5 ulp), and the plain squared length is
+    // within 3 ulp of |rd|^2, far inside the 1e-4 relative slack
+    const float lim0 = rad + fabsf(a.coh);
+    auto near = [&](float4 Pj) -> bool {
+        const float dx = x.x - Pj.x, dy = x.y - Pj.y, dz = x.z - Pj.z;
+        const float lim = lim0 + Pj.w;
+        return !((dx * dx + dy * dy) + dz * dz > lim * lim * 1.0001f + 1e-30f);
+    };
     f3 acc = {0.f, 0.f, 0.f};
     for (int cz = zs; cz <= ze; cz++)
         for (int cy = ys; cy <= ye; cy++) {
@@ -336,8 +346,10 @@ __global__ __launch_bounds__(256) void k_pp_apply(PPArgs a) {
                 const int vx = cx + kGridOrigin;
                 if (vx < 0) {   // clamped virtual cell: one physical cell per step
                     const int2 rg = lookup(cell_index(cx, cy, cz));
-                    for (int k = rg.x; k < rg.y; k++)
-                        pp_pair<DIAG>(a, t, k, a.Fs[k], a.Xs[k], a.Vs[k], i, proxy, x, v, rad, w1, R, acc);
+                    for (int k = rg.x; k < rg.y; k++) {
+                        const float4 Pj = a.Xs[k];
+                        if (near(Pj)) pp_pair<DIAG>(a, t, k, Pj, i, proxy, x, v, rad, w1, R, acc);
+                    }
                     cx++;
                     continue;
                 }
@@ -354,16 +366,19 @@ __global__ __launch_bounds__(256) void k_pp_apply(PPArgs a) {
                         any = true;
                     }
                 }
-                // two candidates per step: both loads issue before either's math
+                // two positions per step: both loads issue before the cheap
+                // reject; survivors are projected in slot order
                 int k = s;
                 for (; k + 1 < e; k += 2) {
-                    const int f0 = a.Fs[k], f1 = a.Fs[k + 1];
                     const float4 P0 = a.Xs[k], P1 = a.Xs[k + 1];
-                    const float4 V0 = a.Vs[k], V1 = a.Vs[k + 1];
-                    pp_pair<DIAG>(a, t, k, f0, P0, V0, i, proxy, x, v, rad, w1, R, acc);
-                    pp_pair<DIAG>(a, t, k + 1, f1, P1, V1, i, proxy, x, v, rad, w1, R, acc);
+                    const bool n0 = near(P0), n1 = near(P1);
+                    if (n0) pp_pair<DIAG>(a, t, k, P0, i, proxy, x, v, rad, w1, R, acc);
+                    if (n1) pp_pair<DIAG>(a, t, k + 1, P1, i, proxy, x, v, rad, w1, R, acc);
                 }
-                if (k < e) pp_pair<DIAG>(a, t, k, a.Fs[k], a.Xs[k], a.Vs[k], i, proxy, x, v, rad, w1, R, acc);
+                for (; k < e; k++) {
+                    const float4 Pj = a.Xs[k];
+                    if (near(Pj)) pp_pair<DIAG>(a, t, k, Pj, i, proxy, x, v, rad, w1, R, acc);
+                }
                 cx += len;
             }
         }
