@@ -192,9 +192,23 @@ __global__ __launch_bounds__(256) void k_adj_pp(AdjArgs a) {
             const uint32_t base = VALIDATE ? (a.keys[t] & ~((1u << kCellBits) - 1u)) : 0u;
             f3 gx = {0.f, 0.f, 0.f}, gv = {0.f, 0.f, 0.f};
             float rsum = 0.0f;
+            // the forward's cell skipping (every active pair, from either side,
+            // lies within r_k + r_max + |coh|); the rollout runs one iteration,
+            // so the positions are those the cells were computed from
+            const float ux = x.x * inv, uy = x.y * inv, uz = x.z * inv;
+            const float lu = (rk + a.rmax + fabsf(a.coh)) * inv;
+            const float lu2 = lu * lu * 1.0001f + 1e-6f;
+            auto gap2 = [&](float u, int c) -> float {
+                if (c + kGridOrigin < 0) return 0.0f;
+                const float lo = float(c), hi = float(c) + 1.0f;
+                float g = lo > u ? lo - u : (u >= hi ? u - hi : 0.0f);
+                g -= 1e-5f * (fabsf(u) + 1.0f);
+                return g > 0.0f ? g * g : 0.0f;
+            };
             for (int cz = zs; cz <= ze; cz++)
                 for (int cy = ys; cy <= ye; cy++)
                     for (int cx = xs; cx <= xe; cx++) {
+                        if (gap2(ux, cx) + gap2(uy, cy) + gap2(uz, cz) > lu2) continue;
                         const uint32_t cell = base + uint32_t(cell_index(cx, cy, cz));
                         int2 rg = range[size_t(cell)];
                         if (VALIDATE && !(rg.x >= 0 && rg.x < a.n && rg.y > rg.x && a.keys[rg.x] == cell)) continue;

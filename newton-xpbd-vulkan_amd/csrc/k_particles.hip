@@ -339,7 +339,7 @@ __global__ __launch_bounds__(256) void k_pp_apply(PPArgs a) {
         return !((dx * dx + dy * dy) + dz * dz > lim * lim * 1.0001f + 1e-30f);
     };
     f3 acc = {0.f, 0.f, 0.f};
-    for (int cz = zs; cz <= ze; cz++)
+    for (int cz = zs; cz <= ze; cz++) {
         for (int cy = ys; cy <= ye; cy++) {
             int cx = xs;
             while (cx <= xe) {
@@ -382,6 +382,7 @@ __global__ __launch_bounds__(256) void k_pp_apply(PPArgs a) {
                 cx += len;
             }
         }
+    }
     const f3 dw = {fma_(acc.x, w1, 0.f), fma_(acc.y, w1, 0.f), fma_(acc.z, w1, 0.f)};
     const f3 tot = {fma_(dw.x, a.relax, 0.f), fma_(dw.y, a.relax, 0.f), fma_(dw.z, a.relax, 0.f)};
     const f3 dl = ds + tot;

@@ -427,6 +427,8 @@ void Solver::set_particles(const float* q, const float* qd, const float* w, cons
     HIPCHECK(hipStreamSynchronize(stream_));
     if (w) std::memcpy(invm_h_.data(), w, n * sizeof(float));
     if (r) std::memcpy(radius_h_.data(), r, n * sizeof(float));
+    rmax_ = 0.0f;
+    for (size_t i = 0; i < n; i++) rmax_ = std::max(rmax_, std::fabs(radius_h_[i]));
     std::vector<float4> P(n), V(n);
     for (size_t i = 0; i < n; i++) {
         P[i] = make_float4(q[3 * i], q[3 * i + 1], q[3 * i + 2], radius_h_[i]);
@@ -647,6 +649,7 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
         pa.Vs_next = more ? Vs_[xs ^ 1].p : nullptr;
         pa.mu = cfg_.particle_mu; pa.coh = cfg_.particle_cohesion; pa.max_r = cfg_.particle_max_radius;
         pa.dt = sdt; pa.relax = relax_; pa.inv_cell = inv_cell; pa.vmax = cfg_.particle_v_max;
+        pa.rmax = rmax_;
         pa.iter_index = uint32_t(it); pa.max_diag_pairs = max_pairs_;
         pa.diag_header = diag_header_.p; pa.diag_pairs = diag_pairs_.p;
         stage_begin(kPP);

@@ -175,6 +175,7 @@ private:
     int s_ = 0, p_ = 1, o_ = 2;
     DevBuf<int32_t> flags_;
     std::vector<float> radius_h_, invm_h_;
+    float rmax_ = 0.f;   // largest particle radius (host copy of the radii)
     // grid and sort
     DevBuf<uint32_t> keysA_, valsA_, keysB_, valsB_, hist_, bintot_;
     DevBuf<float4> Xs_[2], Vs_[2];   // predicted state in sorted order (ping-pong across iterations)

@@ -87,6 +87,7 @@ struct PPArgs {
     float4* Po; float4* Vo;                         // output state, original order
     float4* Xs_next; float4* Vs_next;               // output state, sorted order (when another iteration follows)
     float mu, coh, max_r, dt, relax, inv_cell, vmax;
+    float rmax;                                     // largest particle radius (bounds every accepted pair)
     uint32_t iter_index, max_diag_pairs;
     uint32_t* diag_header; float* diag_pairs;
 };
@@ -190,6 +191,7 @@ struct AdjArgs {
     float* shx; float* shv;
     double* relax_part;
     float mu, coh, max_r, dt, relax, inv_cell, vmax, gx, gy, gz;
+    float rmax;
 };
 struct AdjShapeArgs {
     int cmax; const int* cc;
