@@ -57,6 +57,84 @@ def stage_bytes(stage, N, C, Cm, U, tiles, mesh_bytes):
     return 0
 
 
+# profiler stage -> kernels it brackets (for the measured-traffic lookup)
+STAGE_KERNELS = {
+    "mesh_contacts": ["k_mesh_count", "k_mesh_scan", "k_mesh_enqueue", "k_mesh_walk"],
+    "integrate": ["k_integrate"],
+    "sort_upsweep": ["k_sort_upsweep"],
+    "sort_binscan": ["k_sort_binscan"],
+    "sort_scatter": ["k_sort_scatter"],
+    "cell_ranges": ["k_fill_zero", "k_cell_ranges"],
+    "shape_solve": ["k_shape_solve"],
+    "pp_solve_apply": ["k_pp_apply"],
+}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_step_pile1M_icosphere.txt")
+
+
+def summarize_pmc(fetch_dir, write_dir, cfetch_dir, cwrite_dir):
+    """Per-dispatch HBM traffic from two rocprofv3 --pmc runs of the workload
+    (FETCH_SIZE, WRITE_SIZE; KiB per dispatch), each scaled by what the same
+    counter reads for a plain 256 MiB device copy.  Prints the summary that
+    profiles/ keeps and bench.py reads."""
+    import csv
+    from collections import defaultdict
+
+    def per_kernel(d, counter):
+        acc = defaultdict(list)
+        with open(os.path.join(d, "pmc_counter_collection.csv")) as f:
+            for r in csv.DictReader(f):
+                if r["Counter_Name"] == counter:
+                    name = r["Kernel_Name"].split("(")[0].replace("void ", "").strip()
+                    acc[name].append(float(r["Counter_Value"]))
+        return acc
+
+    copy_kib = 256 * 1024
+    cf = per_kernel(cfetch_dir, "FETCH_SIZE")
+    cw = per_kernel(cwrite_dir, "WRITE_SIZE")
+    ck = max(cf, key=lambda k: sum(cf[k]) / len(cf[k]))
+    f_factor = copy_kib / (sum(cf[ck]) / len(cf[ck]))
+    w_factor = copy_kib / (sum(cw[ck]) / len(cw[ck]))
+    fe = per_kernel(fetch_dir, "FETCH_SIZE")
+    wr = per_kernel(write_dir, "WRITE_SIZE")
+    print(f"fetch_factor {f_factor:.4f}")
+    print(f"write_factor {w_factor:.4f}")
+    for k in sorted(fe):
+        if not k.startswith("xpbd::"):
+            continue
+        fk = sum(fe[k]) / len(fe[k])
+        wk = sum(wr.get(k, [0.0])) / max(len(wr.get(k, [0.0])), 1)
+        print(f"kernel {k.replace(' ', '')} fetch_kib {fk:.1f} write_kib {wk:.1f} dispatches {len(fe[k])}")
+
+
+def measured_traffic(stage):
+    """HBM bytes per launch of a profiler stage from the committed --pmc
+    summary, or None when it does not cover the stage."""
+    if not os.path.exists(TRAFFIC_FILE) or stage not in STAGE_KERNELS:
+        return None
+    ff = wf = None
+    rows = []
+    with open(TRAFFIC_FILE) as f:
+        for line in f:
+            t = line.split()
+            if len(t) == 2 and t[0] == "fetch_factor":
+                ff = float(t[1])
+            elif len(t) == 2 and t[0] == "write_factor":
+                wf = float(t[1])
+            elif len(t) >= 6 and t[0] == "kernel":
+                rows.append((t[1], float(t[3]), float(t[5])))
+    if ff is None or wf is None:
+        return None
+    total, hits = 0.0, 0
+    for name, fk, wk in rows:
+        if any(("::" + k + "<") in name or name.endswith("::" + k) for k in STAGE_KERNELS[stage]):
+            total += (ff * fk + wf * wk) * 1024.0
+            hits += 1
+    if not hits:
+        return None
+    # the scatter stage is one launch of one of three pass variants: their mean
+    return total / hits if stage == "sort_scatter" else total
+
+
 def reset(s, sc):
     p = sc["particles"]
     s.set_particles(p["q"], p["qd"], p["inv_mass"], p["radii"], p["flags"])
@@ -144,10 +222,15 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--batched-frames", type=int, default=10)
+    ap.add_argument("--summarize-pmc", nargs=4, metavar=("FETCH_DIR", "WRITE_DIR", "COPY_FETCH_DIR", "COPY_WRITE_DIR"),
+                    help="print the per-kernel traffic summary from rocprofv3 --pmc output directories")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy (3 times): the reference kernel for scaling "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE counts to bytes")
     args = ap.parse_args()
+    if args.summarize_pmc:
+        summarize_pmc(*args.summarize_pmc)
+        return
     if args.copy_calibration:
         import torch
         n = 64 * 1024 * 1024
@@ -230,7 +313,7 @@ def main():
         dom = max(kern, key=lambda k: kern[k]["avg_us"] * kern[k]["launches"])
         d = kern[dom]
         out["roofline"] = {"bound": "hbm", "kernel": dom, "achieved": d["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                           "frac": d["gbs"] / HBM_PEAK_GBS, "traffic": None,
+                           "frac": d["gbs"] / HBM_PEAK_GBS, "traffic": measured_traffic(dom),
                            "alg_bytes_per_launch": d["alg_bytes"], "avg_us": d["avg_us"]}
         out["kernels"] = {k: {"avg_us": round(v["avg_us"], 3), "launches": v["launches"],
                               "GBs": round(v["gbs"], 1)} for k, v in kern.items()}

@@ -158,7 +158,15 @@ __device__ __forceinline__ uint32_t parity_hits(const MeshArgs& a, const MeshInf
     const uint32_t b = a.pg_start[cell], e = a.pg_start[cell + 1];
     tested += e - b;
     uint32_t hits = 0;
-    for (uint32_t k = b; k < e; k++) {
+    uint32_t k = b;
+    for (; k + 3 < e; k += 4) {   // four triangles' loads in flight together
+        float4 T[12];
+#pragma unroll
+        for (int q = 0; q < 12; q++) T[q] = a.pg_soup[3 * k + q];
+#pragma unroll
+        for (int q = 0; q < 4; q++) hits += ray_tri(ray, xyz(T[3 * q]), xyz(T[3 * q + 1]), xyz(T[3 * q + 2])) ? 1u : 0u;
+    }
+    for (; k < e; k++) {
         const float4 A = a.pg_soup[3 * k], B = a.pg_soup[3 * k + 1], C = a.pg_soup[3 * k + 2];
         if (ray_tri(ray, xyz(A), xyz(B), xyz(C))) hits++;
     }
@@ -195,8 +203,12 @@ __device__ bool closest_face(const MeshArgs& a, const MeshInfo& mi, uint32_t sta
             if (wt) { if (nds > min_dist_sq) continue; }
             else if (nds > (min_dist + eps) * (min_dist + eps)) continue;
             const uint32_t li = __float_as_uint(L.w) & 0x3FFFFFFFu, ri = __float_as_uint(U.w) & 0x3FFFFFFFu;
+            // the next triangle's corners load while this one is tested
+            float4 nA = make_float4(0.f, 0.f, 0.f, 0.f), nB = nA, nC = nA;
+            if (li < ri) { nA = a.tsoup[3 * li]; nB = a.tsoup[3 * li + 1]; nC = a.tsoup[3 * li + 2]; }
             for (uint32_t pc = li; pc < ri; pc++) {
-                const float4 A = a.tsoup[3 * pc], B = a.tsoup[3 * pc + 1], C = a.tsoup[3 * pc + 2];
+                const float4 A = nA, B = nB, C = nC;
+                if (pc + 1 < ri) { nA = a.tsoup[3 * pc + 3]; nB = a.tsoup[3 * pc + 4]; nC = a.tsoup[3 * pc + 5]; }
                 const uint32_t t = __float_as_uint(A.w);
                 if (STATS) tests++;
                 const f3 va = xyz(A), vb = xyz(B), vc = xyz(C);
