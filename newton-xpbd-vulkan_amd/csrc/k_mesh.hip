@@ -195,17 +195,30 @@ __device__ bool closest_face(const MeshArgs& a, const MeshInfo& mi, uint32_t sta
     float min_dist_sq = max_dist * max_dist;
     while (sp > 0) {
         const uint32_t word = stk[--sp][lt];
-        const uint32_t ni = word & 0x7FFFFFFFu;
         if (STATS) pops++;
         if (word >> 31) {   // leaf: the reference's pop-time box test, then its triangles
-            const float4 L = node_lo(a.nodes, ni), U = node_hi(a.nodes, ni);
+            // short leaves carry their primitive range in the word, so the
+            // box and the first triangle load together; others go through
+            // their node
+            const uint32_t cnt = (word >> 27) & 15u;
+            uint32_t li, ri;
+            float4 L, U;
+            float4 nA = make_float4(0.f, 0.f, 0.f, 0.f), nB = nA, nC = nA;
+            if (cnt) {
+                li = word & 0x07FFFFFFu;
+                ri = li + cnt;
+                nA = a.tsoup[3 * li]; nB = a.tsoup[3 * li + 1]; nC = a.tsoup[3 * li + 2];
+                L = a.lbox[2 * size_t(li)]; U = a.lbox[2 * size_t(li) + 1];
+            } else {
+                const uint32_t ni = word & 0x07FFFFFFu;
+                L = node_lo(a.nodes, ni); U = node_hi(a.nodes, ni);
+                li = __float_as_uint(L.w) & 0x3FFFFFFFu; ri = __float_as_uint(U.w) & 0x3FFFFFFFu;
+                if (li < ri) { nA = a.tsoup[3 * li]; nB = a.tsoup[3 * li + 1]; nC = a.tsoup[3 * li + 2]; }
+            }
             const float nds = dist_sq_aabb(p, xyz(L), xyz(U));
             if (wt) { if (nds > min_dist_sq) continue; }
             else if (nds > (min_dist + eps) * (min_dist + eps)) continue;
-            const uint32_t li = __float_as_uint(L.w) & 0x3FFFFFFFu, ri = __float_as_uint(U.w) & 0x3FFFFFFFu;
             // the next triangle's corners load while this one is tested
-            float4 nA = make_float4(0.f, 0.f, 0.f, 0.f), nB = nA, nC = nA;
-            if (li < ri) { nA = a.tsoup[3 * li]; nB = a.tsoup[3 * li + 1]; nC = a.tsoup[3 * li + 2]; }
             for (uint32_t pc = li; pc < ri; pc++) {
                 const float4 A = nA, B = nB, C = nC;
                 if (pc + 1 < ri) { nA = a.tsoup[3 * pc + 3]; nB = a.tsoup[3 * pc + 4]; nC = a.tsoup[3 * pc + 5]; }
@@ -255,24 +268,52 @@ __device__ bool closest_face(const MeshArgs& a, const MeshInfo& mi, uint32_t sta
                 }
             }
         } else {
-            // internal node: one read of both children's boxes.  Its own
-            // pop-time box test is not repeated: a node it would prune has
-            // children at least as far (their boxes lie inside its box), so
-            // neither is pushed and nothing changes.
-            const float4* rec = a.cnodes + 4 * size_t(ni);
-            const float4 LL = rec[0], LU = rec[1], RL = rec[2], RU = rec[3];
-            const float lds = dist_sq_aabb(p, xyz(LL), xyz(LU));
-            const float rds = dist_sq_aabb(p, xyz(RL), xyz(RU));
+            // internal node: one read of a record holding both children's
+            // boxes and, for internal children, their children's boxes.  The
+            // node's own pop-time box test is not repeated: a node it would
+            // prune has children at least as far (their boxes lie inside its
+            // box), so neither is pushed and nothing changes.  The child the
+            // reference pops next (the last one pushed) is, when internal,
+            // expanded right away from the same record: no triangle is tested
+            // between its push and its pop, so its pop-time test passes and
+            // the threshold is unchanged.
+            const float4* rec = a.cnodes + 12 * size_t(word);
+            float4 R[12];
+#pragma unroll
+            for (int q = 0; q < 12; q++) R[q] = rec[q];
             const float th = wt ? min_dist_sq : (min_dist + eps) * (min_dist + eps);
-            const uint32_t lw = __float_as_uint(LL.w), rw = __float_as_uint(RL.w);
-            if (!(lds < th) && !(rds < th)) continue;
+            const float lds = dist_sq_aabb(p, xyz(R[0]), xyz(R[1]));
+            const float rds = dist_sq_aabb(p, xyz(R[2]), xyz(R[3]));
+            const bool pl = lds < th, pr = rds < th;
+            if (!pl && !pr) continue;
             if (sp + 2 > kStack) { atomicOr(a.err, 1u); continue; }
+            const uint32_t lw = __float_as_uint(R[0].w), rw = __float_as_uint(R[2].w);
+            // push order of the reference: (far, near) with near = the smaller
+            // distance, left on ties going first (so right is popped first)
+            bool near_left;
             if (lds < rds) {
-                if (rds < th) stk[sp++][lt] = rw;
-                if (lds < th) stk[sp++][lt] = lw;
+                if (pl && pr) stk[sp++][lt] = rw;
+                near_left = pl;
             } else {
-                if (lds < th) stk[sp++][lt] = lw;
-                if (rds < th) stk[sp++][lt] = rw;
+                if (pl && pr) stk[sp++][lt] = lw;
+                near_left = !pr;
+            }
+            const uint32_t nw = near_left ? lw : rw;
+            if (nw >> 31) { stk[sp++][lt] = nw; continue; }
+            const int g = near_left ? 4 : 8;
+            const float cl = dist_sq_aabb(p, xyz(R[g]), xyz(R[g + 1]));
+            const float cr = dist_sq_aabb(p, xyz(R[g + 2]), xyz(R[g + 3]));
+            const bool ql = cl < th, qr = cr < th;
+            if (STATS) pops++;
+            if (!ql && !qr) continue;
+            if (sp + 2 > kStack) { atomicOr(a.err, 1u); continue; }
+            const uint32_t clw = __float_as_uint(R[g].w), crw = __float_as_uint(R[g + 2].w);
+            if (cl < cr) {
+                if (qr) stk[sp++][lt] = crw;
+                if (ql) stk[sp++][lt] = clw;
+            } else {
+                if (ql) stk[sp++][lt] = clw;
+                if (qr) stk[sp++][lt] = crw;
             }
         }
     }
@@ -327,7 +368,7 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
 // (the root box against the search distance) and the distance-bound early
 // out.  Returns the search start node word or kNoNode.
 __device__ __forceinline__ uint32_t mesh_candidate(const MeshArgs& a, const MeshInfo& mi, int i) {
-    if (i >= a.n || (a.flags[i] & 1) == 0) return kNoNode;
+    if (mi.prim_cnt == 0u || i >= a.n || (a.flags[i] & 1) == 0) return kNoNode;   // an empty mesh has no nodes
     const float4 P = a.P[i];
     const f3 p = xyz(P);
     const float max_dist = (a.margin + mi.margin) + P.w;

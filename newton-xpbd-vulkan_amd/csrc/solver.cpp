@@ -57,7 +57,7 @@ void Solver::free_all() {
     dfree(range_); dfree(tbl_); dfree(cpart_); dfree(cshape_); dfree(cc_); dfree(cbpos_); dfree(cbvel_); dfree(cnorm_);
     dfree(dshape_); dfree(delta_); dfree(body_q_); dfree(body_qd_); dfree(body_com_); dfree(body_invI_);
     dfree(body_invm_); dfree(body_delta_); dfree(mat_mu_); dfree(body_flags_); dfree(shape_body_);
-    dfree(verts_d_); dfree(vvel_d_); dfree(tris_d_); dfree(prim_d_); dfree(err_d_); dfree(nodes_d_); dfree(cnodes_d_); dfree(pg_start_); dfree(lb_); dfree(lb_node_); dfree(tsoup_); dfree(pg_soup_); dfree(mq_count_); dfree(mq_);
+    dfree(verts_d_); dfree(vvel_d_); dfree(tris_d_); dfree(prim_d_); dfree(err_d_); dfree(nodes_d_); dfree(cnodes_d_); dfree(lbox_d_); dfree(pg_start_); dfree(lb_); dfree(lb_node_); dfree(tsoup_); dfree(pg_soup_); dfree(mq_count_); dfree(mq_);
     dfree(info_d_); dfree(diag_header_); dfree(diag_pairs_); dfree(mesh_stats_);
     for (hipEvent_t e : ev_pool_) (void)hipEventDestroy(e);
     ev_pool_.clear();
@@ -206,31 +206,58 @@ void Solver::upload_nodes() {
     std::vector<float4> inter(2 * total);
     for (size_t i = 0; i < total; i++) { inter[2 * i] = nl[i]; inter[2 * i + 1] = nu[i]; }
     upload(nodes_d_, inter.data(), inter.size());
-    // child-box records: per internal node both children's boxes, each lower
-    // corner's w = child index | leaf flag << 31 (one 64-byte read per expansion)
-    std::vector<float4> rec(4 * total, make_float4(0.f, 0.f, 0.f, 0.f));
+    // Walk records: per internal node its children's boxes and, for internal
+    // children, their children's boxes (12 float4); each lower corner's w is
+    // the child's stack word.  Leaves whose range is short carry it in the
+    // word; their boxes are also stored by first primitive position.
+    std::vector<float4> rec(12 * total, make_float4(0.f, 0.f, 0.f, 0.f));
+    size_t nprim = 0;
+    for (auto& b : bvh_) nprim += b.prim.size();
+    std::vector<float4> lbox(2 * std::max<size_t>(nprim, 1), make_float4(0.f, 0.f, 0.f, 0.f));
+    auto put = [](float4* dst, const NodeHalf& L, const NodeHalf& U, uint32_t w) {
+        float fw;
+        std::memcpy(&fw, &w, 4);
+        dst[0] = make_float4(L.x, L.y, L.z, fw);
+        dst[1] = make_float4(U.x, U.y, U.z, 0.f);
+    };
     for (size_t c = 0; c < bvh_.size(); c++) {
         const uint32_t nb = uint32_t(node_off_[c]);
         const MeshBvhHost& b = bvh_[c];
         for (size_t i = 0; i < b.lo.size(); i++) {
-            if ((b.lo[i].packed >> 30) & 3u) continue;
+            if ((b.lo[i].packed >> 30) & 3u) {   // leaf: box by first primitive
+                const uint32_t li = (b.lo[i].packed & 0x3FFFFFFFu) + uint32_t(prim_off_[c]);
+                if (li < nprim) put(&lbox[2 * size_t(li)], b.lo[i], b.hi[i], 0u);
+                continue;
+            }
             const uint32_t ch[2] = {b.lo[i].packed & 0x3FFFFFFFu, b.hi[i].packed & 0x3FFFFFFFu};
             for (int k = 0; k < 2; k++) {
-                const uint32_t g = (nb + ch[k]) | ((((b.lo[ch[k]].packed >> 30) & 3u) != 0u) ? 0x80000000u : 0u);
-                float gw;
-                std::memcpy(&gw, &g, 4);
-                const NodeHalf L = b.lo[ch[k]], U = b.hi[ch[k]];
-                rec[4 * (nb + i) + 2 * k] = make_float4(L.x, L.y, L.z, gw);
-                rec[4 * (nb + i) + 2 * k + 1] = make_float4(U.x, U.y, U.z, 0.f);
+                put(&rec[12 * (nb + i) + 2 * k], b.lo[ch[k]], b.hi[ch[k]], node_word(c, ch[k]));
+                if ((b.lo[ch[k]].packed >> 30) & 3u) continue;
+                const uint32_t gc[2] = {b.lo[ch[k]].packed & 0x3FFFFFFFu, b.hi[ch[k]].packed & 0x3FFFFFFFu};
+                for (int m = 0; m < 2; m++)
+                    put(&rec[12 * (nb + i) + 4 + 4 * k + 2 * m], b.lo[gc[m]], b.hi[gc[m]], node_word(c, gc[m]));
             }
         }
     }
     upload(cnodes_d_, rec.data(), rec.size());
-    for (size_t c = 0; c < bvh_.size() && c < info_h_.size(); c++) {
-        const MeshBvhHost& b = bvh_[c];
-        const bool leaf = !b.lo.empty() && ((b.lo[b.root].packed >> 30) & 3u) != 0u;
-        info_h_[c].root_word = uint32_t(node_off_[c] + b.root) | (leaf ? 0x80000000u : 0u);
-    }
+    upload(lbox_d_, lbox.data(), lbox.size());
+    for (size_t c = 0; c < bvh_.size() && c < info_h_.size(); c++)
+        info_h_[c].root_word = bvh_[c].lo.empty() ? 0u : node_word(c, bvh_[c].root);
+}
+
+// Stack word of node v of mesh c for the tree walk: an internal node's global
+// index; a leaf with 1..15 primitives as 1 << 31 | count << 27 | first
+// primitive position; any other leaf as 1 << 31 | global node index.
+uint32_t Solver::node_word(size_t c, uint32_t v) const {
+    const MeshBvhHost& b = bvh_[c];
+    const uint32_t g = uint32_t(node_off_[c]) + v;
+    if (g >= (1u << 27)) throw std::runtime_error("mesh BVH exceeds 2^27 nodes");
+    if (!((b.lo[v].packed >> 30) & 3u)) return g;
+    const uint32_t li = (b.lo[v].packed & 0x3FFFFFFFu) + uint32_t(prim_off_[c]);
+    const uint32_t ri = (b.hi[v].packed & 0x3FFFFFFFu) + uint32_t(prim_off_[c]);
+    const uint32_t cnt = ri - li;
+    if (cnt >= 1 && cnt <= 15 && li < (1u << 27)) return 0x80000000u | (cnt << 27) | li;
+    return 0x80000000u | g;
 }
 
 // Host-side query accelerators of one mesh: the projected parity grid
@@ -275,11 +302,8 @@ void Solver::upload_query_grids() {
         info_h_[c].lb_off = uint32_t(lb.size());
         info_h_[c].lb_cap = d.lb.empty() ? 0.0f : d.cap;   // cap 0: no early out
         lb.insert(lb.end(), d.lb.begin(), d.lb.end());
-        for (size_t k = 0; k < d.lb.size(); k++) {   // start nodes, re-based, with the leaf flag
-            const uint32_t v = k < d.start.size() ? d.start[k] : bvh_[c].root;
-            const bool leaf = ((bvh_[c].lo[v].packed >> 30) & 3u) != 0u;
-            lbn.push_back((uint32_t(node_off_[c]) + v) | (leaf ? 0x80000000u : 0u));
-        }
+        for (size_t k = 0; k < d.lb.size(); k++)   // start nodes as stack words
+            lbn.push_back(node_word(c, k < d.start.size() ? d.start[k] : bvh_[c].root));
     }
     start.push_back(uint32_t(tri.size()));
     upload(pg_start_, start.data(), start.size());
@@ -558,7 +582,7 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
     if (collide && !meshes_.empty()) {
         MeshArgs m;
         m.n = n; m.P = P_[s_].p; m.flags = flags_.p;
-        m.verts = verts_d_.p; m.tris = tris_d_.p; m.nodes = nodes_d_.p; m.cnodes = cnodes_d_.p; m.prim = prim_d_.p;
+        m.verts = verts_d_.p; m.tris = tris_d_.p; m.nodes = nodes_d_.p; m.cnodes = cnodes_d_.p; m.lbox = lbox_d_.p; m.prim = prim_d_.p;
         m.pg_start = pg_start_.p; m.pg_soup = pg_soup_.p; m.tsoup = tsoup_.p; m.lb = lb_.p; m.lb_node = lb_node_.p;
         m.mq_count = mq_count_.p; m.mq = mq_.p;
         m.vvel = vvel_d_.p; m.info = info_d_.p; m.nmesh = M;
@@ -868,7 +892,7 @@ void Solver::collide_only() {
     if (!meshes_.empty()) {
         MeshArgs m;
         m.n = int(n_); m.P = P_[s_].p; m.flags = flags_.p;
-        m.verts = verts_d_.p; m.tris = tris_d_.p; m.nodes = nodes_d_.p; m.cnodes = cnodes_d_.p; m.prim = prim_d_.p;
+        m.verts = verts_d_.p; m.tris = tris_d_.p; m.nodes = nodes_d_.p; m.cnodes = cnodes_d_.p; m.lbox = lbox_d_.p; m.prim = prim_d_.p;
         m.pg_start = pg_start_.p; m.pg_soup = pg_soup_.p; m.tsoup = tsoup_.p; m.lb = lb_.p; m.lb_node = lb_node_.p;
         m.mq_count = mq_count_.p; m.mq = mq_.p;
         m.vvel = vvel_d_.p; m.info = info_d_.p; m.nmesh = int(meshes_.size());

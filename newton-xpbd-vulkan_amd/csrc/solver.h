@@ -119,6 +119,7 @@ private:
     void ensure_tape();
     void build_query_grids(size_t mesh);
     void upload_tri_soup();
+    uint32_t node_word(size_t mesh, uint32_t node) const;
     void upload_query_grids();
     BodyArgs body_args() const;
     void substep(float sdt, int iterations, bool collide, int slot, bool zero_next);
@@ -209,7 +210,8 @@ private:
     DevBuf<float> verts_d_, vvel_d_;
     DevBuf<uint32_t> tris_d_, prim_d_, err_d_;
     DevBuf<float4> nodes_d_;   // interleaved (lower, upper) node boxes, re-based
-    DevBuf<float4> cnodes_d_;  // per internal node: both children's boxes and tagged indices
+    DevBuf<float4> cnodes_d_;  // per internal node: walk record (children and grandchildren boxes, stack words)
+    DevBuf<float4> lbox_d_;    // leaf boxes by first primitive position
     std::vector<ParityGrid> pgrid_;        // per mesh (empty for non-watertight)
     DevBuf<uint32_t> pg_start_;            // all meshes' parity-grid cell starts, concatenated
     std::vector<DistGrid> dgrid_;          // per mesh distance lower bounds

@@ -116,7 +116,7 @@ struct MeshInfo {
     float lb_ox, lb_oy, lb_oz, lb_inv;
     uint32_t lb_nx, lb_ny, lb_nz, lb_off;
     float lb_cap;
-    uint32_t root_word;   // root index | leaf flag << 31
+    uint32_t root_word;   // stack word of the root
     uint32_t pad[4];
 };
 static_assert(sizeof(MeshInfo) == 96, "MeshInfo layout");
@@ -125,7 +125,9 @@ struct MeshArgs {
     int n; const float4* P; const int* flags;
     const float* verts; const uint32_t* tris;
     const float4* nodes;   // interleaved (lower, upper) per node
-    const float4* cnodes;  // per internal node: (lower, upper) of both children; lower.w = child | leaf << 31
+    const float4* cnodes;  // per internal node, 12 float4: (lower, upper) of both children, then of the left
+                           // child's children, then of the right child's; lower.w = the child's stack word
+    const float4* lbox;    // (lower, upper) of the leaf starting at each leaf-order primitive position
     const uint32_t* prim;
     const float* vvel; const MeshInfo* info; int nmesh;
     float margin, eps;

@@ -190,6 +190,49 @@ def test_mesh_contacts_collide_only(vk, watertight):
     assert np.all(np.sum(rel * gc["normals"][m], axis=1) > 0.0)
 
 
+def test_mesh_contacts_several_meshes_and_large_particles(vk):
+    """Two watertight spheres (one small and offset, with a shape margin) and an
+    empty mesh; a tenth of the particles are 6x larger than
+    particle_max_radius, which takes their search beyond the distance-bound
+    grid's cap (the search then starts at the root).  Contacts equal the
+    oracle's brute-force narrowphase bit for bit."""
+    v1, t1 = scenes.icosphere(3)
+    v2, t2 = scenes.icosphere(2, 0.2, (0.45, 0.1, 0.95))
+    q, r = ico_shell(3, 3000, seed=5, band=0.04)
+    rng = np.random.default_rng(6)
+    d = rng.normal(size=(1500, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    q2 = (d * (0.2 + rng.uniform(-0.03, 0.03, (1500, 1))) + [0.45, 0.1, 0.95]).astype(np.float32)
+    q = np.concatenate([q, q2])
+    radii = np.full(q.shape[0], r, np.float32)
+    radii[::10] = 6 * r
+    empty_v, empty_t = np.zeros((0, 3), np.float32), np.zeros((0, 3), np.uint32)
+    meshes = [(v1, t1, 0.0), (empty_v, empty_t, 0.0), (v2, t2, 0.004)]
+    cfg = scenes.solver_config(vk, r)
+    g = vk.XpbdSolver()
+    g.init(cfg, q.shape[0])
+    o = O.OracleSolver()
+    o.init(cfg, q.shape[0])
+    for v, t, m in meshes:
+        g.add_mesh(v.reshape(-1), t.reshape(-1), shape_margin=m)
+        o.add_mesh(v, t, shape_margin=m)
+    g.finalize_meshes()
+    o.finalize_meshes()
+    n = q.shape[0]
+    args = (q, np.zeros_like(q), np.full(n, scenes.PARTICLE_INV_MASS, np.float32), radii, np.ones(n, np.int32))
+    g.set_particles(*args)
+    o.set_particles(*args)
+    g.collide_only()
+    o.collide_only()
+    gc, oc = gpu_contacts(g), o.get_contacts()
+    log(f"two meshes + empty mesh, large particles: contacts gpu={g.last_contact_count} oracle={o.last_contact_count}")
+    assert g.last_contact_count == o.last_contact_count > 0
+    assert set(np.unique(gc["shapes"])) >= {1, 3}
+    big = np.isin(gc["particles"], np.arange(0, n, 10))
+    assert np.count_nonzero(big) > 0
+    assert_contacts_equal(gc, oc)
+
+
 def test_mesh_pile_frames(vk):
     """A pile falling onto a watertight icosphere: the config-1 path at small size."""
     rng = np.random.default_rng(2)
