@@ -223,7 +223,18 @@ void Solver::upload_nodes() {
     for (size_t c = 0; c < bvh_.size(); c++) {
         const uint32_t nb = uint32_t(node_off_[c]);
         const MeshBvhHost& b = bvh_[c];
-        for (size_t i = 0; i < b.lo.size(); i++) {
+        if (b.lo.empty()) continue;
+        // only nodes reachable from the root: a collapsed leaf's former
+        // descendants are leaves too and share its first primitive
+        std::vector<uint32_t> reach{b.root};
+        for (size_t k = 0; k < reach.size(); k++) {
+            const uint32_t v = reach[k];
+            if (!((b.lo[v].packed >> 30) & 3u)) {
+                reach.push_back(b.lo[v].packed & 0x3FFFFFFFu);
+                reach.push_back(b.hi[v].packed & 0x3FFFFFFFu);
+            }
+        }
+        for (const uint32_t i : reach) {
             if ((b.lo[i].packed >> 30) & 3u) {   // leaf: box by first primitive
                 const uint32_t li = (b.lo[i].packed & 0x3FFFFFFFu) + uint32_t(prim_off_[c]);
                 if (li < nprim) put(&lbox[2 * size_t(li)], b.lo[i], b.hi[i], 0u);

@@ -195,7 +195,11 @@ def test_mesh_contacts_several_meshes_and_large_particles(vk):
     empty mesh; a tenth of the particles are 6x larger than
     particle_max_radius, which takes their search beyond the distance-bound
     grid's cap (the search then starts at the root).  Contacts equal the
-    oracle's brute-force narrowphase bit for bit."""
+    oracle's brute-force narrowphase bit for bit, except for the empty mesh:
+    the reference points an empty mesh's root at the next mesh's node 0
+    (scene.cpp:291, root = node_base + 0 with no nodes of its own), so it and
+    the oracle report phantom contacts with that leaf's triangle under the
+    empty mesh's shape; this build reports none (a deliberate deviation)."""
     v1, t1 = scenes.icosphere(3)
     v2, t2 = scenes.icosphere(2, 0.2, (0.45, 0.1, 0.95))
     q, r = ico_shell(3, 3000, seed=5, band=0.04)
@@ -226,7 +230,11 @@ def test_mesh_contacts_several_meshes_and_large_particles(vk):
     o.collide_only()
     gc, oc = gpu_contacts(g), o.get_contacts()
     log(f"two meshes + empty mesh, large particles: contacts gpu={g.last_contact_count} oracle={o.last_contact_count}")
-    assert g.last_contact_count == o.last_contact_count > 0
+    assert not np.any(gc["shapes"] == 2)
+    phantom = np.asarray(oc["shapes"]) == 2
+    log(f"  oracle phantom contacts under the empty mesh: {int(np.count_nonzero(phantom))}")
+    oc = {k: np.asarray(v)[~phantom] for k, v in oc.items()}
+    assert g.last_contact_count == len(oc["shapes"]) > 0
     assert set(np.unique(gc["shapes"])) >= {1, 3}
     big = np.isin(gc["particles"], np.arange(0, n, 10))
     assert np.count_nonzero(big) > 0
