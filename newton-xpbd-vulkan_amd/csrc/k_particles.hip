@@ -294,8 +294,11 @@ __device__ __forceinline__ void pp_pair(const PPArgs& a, int t, int k, float4 Pj
 // cell ids, hence one contiguous run of sorted slots in the same order.  A row
 // is split where the x index wraps at 128; a virtual x below the clamp (the
 // reference maps every such cell to x = 0) is visited cell by cell.
-template <bool DIAG, bool VALIDATE>
-__global__ __launch_bounds__(256) void k_pp_apply(PPArgs a) {
+constexpr int kSurv = 24;   // recorded survivors per lane before a drain
+
+template <bool DIAG, bool VALIDATE, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_pp_apply(PPArgs a) {
+    __shared__ uint32_t s_surv[kSurv][BLOCK];
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= a.n) return;
     const int i = int(a.vals[t]);
@@ -339,6 +342,25 @@ __global__ __launch_bounds__(256) void k_pp_apply(PPArgs a) {
         return !((dx * dx + dy * dy) + dz * dz > lim * lim * 1.0001f + 1e-30f);
     };
     f3 acc = {0.f, 0.f, 0.f};
+    // Two phases keep the lanes of a wave together: the scan applies the
+    // cheap reject to every candidate and records the survivors' slots (in
+    // visiting order) in this lane's LDS column; the projection then runs over
+    // the recorded slots, where each lane's j-th survivor is processed at the
+    // same step.  A full column is drained before the scan continues.
+    int cnt = 0;
+    auto drain = [&]() {
+        for (int m = 0; m < cnt; m++) {
+            const int k = int(s_surv[m][threadIdx.x]);
+            pp_pair<DIAG>(a, t, k, a.Xs[k], i, proxy, x, v, rad, w1, R, acc);
+        }
+        cnt = 0;
+    };
+    auto consider = [&](int k, float4 Pj) {
+        if (k != t && near(Pj)) {
+            s_surv[cnt][threadIdx.x] = uint32_t(k);
+            if (++cnt == kSurv) drain();
+        }
+    };
     for (int cz = zs; cz <= ze; cz++) {
         for (int cy = ys; cy <= ye; cy++) {
             int cx = xs;
@@ -346,10 +368,7 @@ __global__ __launch_bounds__(256) void k_pp_apply(PPArgs a) {
                 const int vx = cx + kGridOrigin;
                 if (vx < 0) {   // clamped virtual cell: one physical cell per step
                     const int2 rg = lookup(cell_index(cx, cy, cz));
-                    for (int k = rg.x; k < rg.y; k++) {
-                        const float4 Pj = a.Xs[k];
-                        if (near(Pj)) pp_pair<DIAG>(a, t, k, Pj, i, proxy, x, v, rad, w1, R, acc);
-                    }
+                    for (int k = rg.x; k < rg.y; k++) consider(k, a.Xs[k]);
                     cx++;
                     continue;
                 }
@@ -366,23 +385,18 @@ __global__ __launch_bounds__(256) void k_pp_apply(PPArgs a) {
                         any = true;
                     }
                 }
-                // two positions per step: both loads issue before the cheap
-                // reject; survivors are projected in slot order
                 int k = s;
-                for (; k + 1 < e; k += 2) {
+                for (; k + 1 < e; k += 2) {   // both loads issue before the tests
                     const float4 P0 = a.Xs[k], P1 = a.Xs[k + 1];
-                    const bool n0 = near(P0), n1 = near(P1);
-                    if (n0) pp_pair<DIAG>(a, t, k, P0, i, proxy, x, v, rad, w1, R, acc);
-                    if (n1) pp_pair<DIAG>(a, t, k + 1, P1, i, proxy, x, v, rad, w1, R, acc);
+                    consider(k, P0);
+                    consider(k + 1, P1);
                 }
-                for (; k < e; k++) {
-                    const float4 Pj = a.Xs[k];
-                    if (near(Pj)) pp_pair<DIAG>(a, t, k, Pj, i, proxy, x, v, rad, w1, R, acc);
-                }
+                if (k < e) consider(k, a.Xs[k]);
                 cx += len;
             }
         }
     }
+    drain();
     const f3 dw = {fma_(acc.x, w1, 0.f), fma_(acc.y, w1, 0.f), fma_(acc.z, w1, 0.f)};
     const f3 tot = {fma_(dw.x, a.relax, 0.f), fma_(dw.y, a.relax, 0.f), fma_(dw.z, a.relax, 0.f)};
     const f3 dl = ds + tot;
@@ -484,12 +498,16 @@ void launch_pp_apply(const PPArgs& a, bool diag, bool validate, int block, hipSt
     if (a.n <= 0) return;
     const int bs = (block == 64 || block == 128) ? block : 256;
     const dim3 g(blocks_for(a.n, bs)), b(bs);
-    if (diag) {
-        if (validate) hipLaunchKernelGGL((k_pp_apply<true, true>), g, b, 0, s, a);
-        else hipLaunchKernelGGL((k_pp_apply<true, false>), g, b, 0, s, a);
+    auto go = [&](auto kernel) { hipLaunchKernelGGL(kernel, g, b, 0, s, a); };
+    if (bs == 64) {
+        if (diag) { if (validate) go(k_pp_apply<true, true, 64>); else go(k_pp_apply<true, false, 64>); }
+        else { if (validate) go(k_pp_apply<false, true, 64>); else go(k_pp_apply<false, false, 64>); }
+    } else if (bs == 128) {
+        if (diag) { if (validate) go(k_pp_apply<true, true, 128>); else go(k_pp_apply<true, false, 128>); }
+        else { if (validate) go(k_pp_apply<false, true, 128>); else go(k_pp_apply<false, false, 128>); }
     } else {
-        if (validate) hipLaunchKernelGGL((k_pp_apply<false, true>), g, b, 0, s, a);
-        else hipLaunchKernelGGL((k_pp_apply<false, false>), g, b, 0, s, a);
+        if (diag) { if (validate) go(k_pp_apply<true, true, 256>); else go(k_pp_apply<true, false, 256>); }
+        else { if (validate) go(k_pp_apply<false, true, 256>); else go(k_pp_apply<false, false, 256>); }
     }
 }
 void launch_restitution(const RestArgs& a, int grid, hipStream_t s) {
