@@ -167,8 +167,11 @@ __global__ __launch_bounds__(256) void k_adj_shape(AdjShapeArgs a) {
 
 // Particle-pair adjoint gathered per sorted slot, then the integrate adjoint
 // (integrate_particles.comp:44-57); writes the substep-start cotangents.
+constexpr int kSurvAdj = 24;   // recorded candidates per lane before a drain
+
 template <bool VALIDATE>
 __global__ __launch_bounds__(256) void k_adj_pp(AdjArgs a) {
+    __shared__ uint32_t s_surv[kSurvAdj][256];
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     double racc = 0.0;
     if (t < a.n) {
@@ -205,6 +208,42 @@ __global__ __launch_bounds__(256) void k_adj_pp(AdjArgs a) {
                 g -= 1e-5f * (fabsf(u) + 1.0f);
                 return g > 0.0f ? g * g : 0.0f;
             };
+            // two phases as in the forward solve: a cheap-reject scan records
+            // candidate slots per lane, the reverse projection runs over them
+            const float lim0 = rk + fabsf(a.coh);
+            int cnt = 0;
+            auto drain = [&]() {
+                for (int q = 0; q < cnt; q++) {
+                    const int m = int(s_surv[q][threadIdx.x]);
+                    const int nf = a.Fs[m];
+                    if ((nf & 1) == 0) continue;
+                    const float4 Pj = a.Xs[m], Vj = a.Vs[m];
+                    const f3 rd = x - xyz(Pj);
+                    const float ax = fmaxf(fmaxf(fabsf(rd.x), fabsf(rd.y)), fabsf(rd.z));
+                    const float rj = Pj.w, wj = Vj.w;
+                    const float Rj = rj + a.max_r + a.coh;
+                    if (ax > R && ax > Rj) continue;
+                    const float d = exact_length(rd);
+                    if (!(d > 0.0f)) continue;
+                    // particle k's solve and particle j's solve, each with its own skip rules
+                    const float err_kj = d - rk - rj, err_jk = d - rj - rk;
+                    const bool kj = !(proxy_k != 0 && ((nf & 2) != 0 || wj <= 0.0f)) && !(ax > R) &&
+                                    !(err_kj > a.coh) && (wk + wj) > 0.0f;
+                    const bool jk = !((nf & 2) != 0 && (proxy_k != 0 || wk <= 0.0f)) && !(ax > Rj) &&
+                                    !(err_jk > a.coh) && (wj + wk) > 0.0f;
+                    if (!kj && !jk) continue;
+                    const float den = wk + wj;
+                    f3 C = {0.f, 0.f, 0.f};
+                    if (kj) C = ak * (wk * a.relax / den);
+                    if (jk) C = C - xyz(a.As[m]) * (wj * a.relax / den);
+                    f3 g_rd, g_vr;
+                    const f3 g = pair_vjp(rd, v - xyz(Vj), d, err_kj, a.mu, a.dt, C, g_rd, g_vr);
+                    gx = gx + g_rd;
+                    gv = gv + g_vr;
+                    if (kj) rsum += dot3(ak, g) * (wk / den);
+                }
+                cnt = 0;
+            };
             for (int cz = zs; cz <= ze; cz++)
                 for (int cy = ys; cy <= ye; cy++)
                     for (int cx = xs; cx <= xe; cx++) {
@@ -213,37 +252,15 @@ __global__ __launch_bounds__(256) void k_adj_pp(AdjArgs a) {
                         int2 rg = range[size_t(cell)];
                         if (VALIDATE && !(rg.x >= 0 && rg.x < a.n && rg.y > rg.x && a.keys[rg.x] == cell)) continue;
                         for (int m = rg.x; m < rg.y; m++) {
-                            if (m == t) continue;
-                            const int nf = a.Fs[m];
-                            if ((nf & 1) == 0) continue;
-                            const float4 Pj = a.Xs[m], Vj = a.Vs[m];
-                            const f3 rd = x - xyz(Pj);
-                            const float ax = fmaxf(fmaxf(fabsf(rd.x), fabsf(rd.y)), fabsf(rd.z));
-                            const float rj = Pj.w, wj = Vj.w;
-                            const float Rj = rj + a.max_r + a.coh;
-                            if (ax > R && ax > Rj) continue;
-                            const float lim = (rk + rj) + fabsf(a.coh);   // as the forward's cheap reject
-                            if ((rd.x * rd.x + rd.y * rd.y) + rd.z * rd.z > lim * lim * 1.0001f + 1e-30f) continue;
-                            const float d = exact_length(rd);
-                            if (!(d > 0.0f)) continue;
-                            // particle k's solve and particle j's solve, each with its own skip rules
-                            const float err_kj = d - rk - rj, err_jk = d - rj - rk;
-                            const bool kj = !(proxy_k != 0 && ((nf & 2) != 0 || wj <= 0.0f)) && !(ax > R) &&
-                                            !(err_kj > a.coh) && (wk + wj) > 0.0f;
-                            const bool jk = !((nf & 2) != 0 && (proxy_k != 0 || wk <= 0.0f)) && !(ax > Rj) &&
-                                            !(err_jk > a.coh) && (wj + wk) > 0.0f;
-                            if (!kj && !jk) continue;
-                            const float den = wk + wj;
-                            f3 C = {0.f, 0.f, 0.f};
-                            if (kj) C = ak * (wk * a.relax / den);
-                            if (jk) C = C - xyz(a.As[m]) * (wj * a.relax / den);
-                            f3 g_rd, g_vr;
-                            const f3 g = pair_vjp(rd, v - xyz(Vj), d, err_kj, a.mu, a.dt, C, g_rd, g_vr);
-                            gx = gx + g_rd;
-                            gv = gv + g_vr;
-                            if (kj) rsum += dot3(ak, g) * (wk / den);
+                            const float4 Pj = a.Xs[m];
+                            const float dx = x.x - Pj.x, dy = x.y - Pj.y, dz = x.z - Pj.z;
+                            const float lim = lim0 + Pj.w;   // as the forward's cheap reject
+                            if (m == t || (dx * dx + dy * dy) + dz * dz > lim * lim * 1.0001f + 1e-30f) continue;
+                            s_surv[cnt][threadIdx.x] = uint32_t(m);
+                            if (++cnt == kSurvAdj) drain();
                         }
                     }
+            drain();
             racc = double(rsum);
             const f3 sx = {a.shx[3 * k], a.shx[3 * k + 1], a.shx[3 * k + 2]};
             const f3 sv = {a.shv[3 * k], a.shv[3 * k + 1], a.shv[3 * k + 2]};
