@@ -167,7 +167,7 @@ __global__ __launch_bounds__(256) void k_adj_shape(AdjShapeArgs a) {
 
 // Particle-pair adjoint gathered per sorted slot, then the integrate adjoint
 // (integrate_particles.comp:44-57); writes the substep-start cotangents.
-constexpr int kSurvAdj = 24;   // recorded candidates per lane before a drain
+constexpr int kSurvAdj = 16;   // recorded candidates per lane before a drain
 
 template <bool VALIDATE>
 __global__ __launch_bounds__(256) void k_adj_pp(AdjArgs a) {

@@ -294,7 +294,7 @@ __device__ __forceinline__ void pp_pair(const PPArgs& a, int t, int k, float4 Pj
 // cell ids, hence one contiguous run of sorted slots in the same order.  A row
 // is split where the x index wraps at 128; a virtual x below the clamp (the
 // reference maps every such cell to x = 0) is visited cell by cell.
-constexpr int kSurv = 24;   // recorded survivors per lane before a drain
+constexpr int kSurv = 16;   // recorded survivors per lane before a drain
 
 template <bool DIAG, bool VALIDATE, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_pp_apply(PPArgs a) {
