@@ -103,13 +103,36 @@ def pile(n, r, rng, z0=None):
     return q[:n]
 
 
+def random_quats(rng, n):
+    """Uniform random unit quaternions (Shoemake), xyzw, float32."""
+    u1, u2, u3 = rng.uniform(size=(3, n))
+    q = np.stack([np.sqrt(1 - u1) * np.sin(2 * np.pi * u2), np.sqrt(1 - u1) * np.cos(2 * np.pi * u2),
+                  np.sqrt(u1) * np.sin(2 * np.pi * u3), np.sqrt(u1) * np.cos(2 * np.pi * u3)], 1)
+    return q.astype(np.float32)
+
+
+def config2(seed=2, nb=512, n1=64):
+    """nb box bodies (half-extent 0.05, density 1000: 1 kg) with uniform
+    random orientations, centres uniform in [-1,1] x [-1,1] x [0.1,1.1], plus
+    n1^3 particles (r = 0.005, spacing 0.01) with the bottom layer at z = 1.3."""
+    rng = np.random.default_rng(seed)
+    r = 0.005
+    pos = np.stack([rng.uniform(-1, 1, nb), rng.uniform(-1, 1, nb), rng.uniform(0.1, 1.1, nb)], 1).astype(np.float32)
+    quats = random_quats(rng, nb)
+    s = 2 * r
+    q = lattice(n1, n1, n1, s, (-(n1 - 1) * s / 2, -(n1 - 1) * s / 2, 1.3), 0.1 * r, rng)
+    boxes = [(pos[k], quats[k], np.float32([0.05, 0.05, 0.05]), 1000.0) for k in range(nb)]
+    return dict(name=f"boxes_{nb}_particles_{q.shape[0]}", r=r, particles=_particles(q, r), meshes=[], boxes=boxes,
+                frames=60, dt=1.0 / 60.0, substeps=8, iterations=1)
+
+
 def config3(seed=3, num_scenes=256, lo=1024, hi=65536):
     """num_scenes independent piles, particle counts log-uniform on [lo, hi]
     (seeded), r = 0.005, each over its own ground plane and hash grid."""
     rng = np.random.default_rng(seed)
     r = 0.005
     counts = np.exp(rng.uniform(np.log(lo), np.log(hi), num_scenes)).astype(np.int64)
-    parts = [pile(int(c), r, rng) for c in counts]
+    parts = [pile(int(c), r, rng, z0=0.02) for c in counts]
     q = np.concatenate(parts).astype(np.float32)
     return dict(name=f"batch_{num_scenes}_scenes_{q.shape[0]}", r=r, particles=_particles(q, r), meshes=[],
                 counts=counts, frames=60, dt=1.0 / 60.0, substeps=8, iterations=1)
@@ -156,6 +179,8 @@ def build_solver(vkxpbd_module, scene, max_contacts=0, **cfg_overrides):
     s.init(cfg, p["q"].shape[0], max_contacts)
     for verts, tris in scene["meshes"]:
         s.add_mesh(verts.reshape(-1), tris.reshape(-1))
+    for pos, quat, half, density in scene.get("boxes", []):
+        s.add_box_body(pos, quat, half, density)
     s.finalize_meshes()
     s.set_particles(p["q"], p["qd"], p["inv_mass"], p["radii"], p["flags"])
     return s

@@ -21,6 +21,8 @@ def make_pair(vk, scene, max_contacts=0, fp64=False, **cfg_overrides):
     o.init(cfg, p["q"].shape[0], max_contacts)
     for verts, tris in scene["meshes"]:
         o.add_mesh(verts, tris)
+    for pos, quat, half, density in scene.get("boxes", []):
+        o.add_box_body(pos, quat, half, density)
     o.finalize_meshes()
     o.set_particles(p["q"], p["qd"], p["inv_mass"], p["radii"], p["flags"])
     return g, o

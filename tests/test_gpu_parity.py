@@ -356,6 +356,39 @@ def test_boxes_and_particles(vk):
     np.testing.assert_allclose(np.linalg.norm(gq[:, 3:], axis=1), 1.0, atol=1e-5)
 
 
+def test_config2_boxes_and_particles_full_size(vk):
+    """The rigid-body configuration at full size: 512 boxes with random
+    orientations falling onto the plane and 262,144 particles falling onto
+    them.  After the first frame the state is within 1e-5 m of the oracle
+    (body deltas are float atomics summed in arbitrary order); later frames'
+    drift is reported, and the boxes stay normalised and above the plane."""
+    import time
+    sc = scenes.config2()
+    g, o = make_pair(vk, sc)
+    t0 = time.perf_counter()
+    g.step(1.0 / 60.0, 8, 1)
+    t1 = time.perf_counter()
+    o.step(1.0 / 60.0, 8, 1)
+    log(f"config 2 frame 1: gpu {1e3 * (t1 - t0):.1f} ms (first frame, includes capture); oracle {time.perf_counter() - t1:.1f} s")
+    dx = compare_state(g, o, exact=False, what="config 2 frame 1")
+    db = float(np.max(np.abs(g.get_body_q().astype(np.float64) - o.get_body_q())))
+    log(f"  bodies frame 1: max|dq_body|={db:.3e}")
+    assert db <= 1e-5 and dx <= 1e-5
+    for f in range(2, 6):
+        g.step(1.0 / 60.0, 8, 1)
+        o.step(1.0 / 60.0, 8, 1)
+        gp, op = g.get_positions().astype(np.float64), o.get_positions()
+        db = float(np.max(np.abs(g.get_body_q().astype(np.float64) - o.get_body_q())))
+        log(f"  frame {f}: particle drift {float(np.max(np.abs(gp - op))):.3e} m, body drift {db:.3e}")
+    bq = g.get_body_q()
+    assert np.allclose(np.linalg.norm(bq[:, 3:], axis=1), 1.0, atol=1e-5)
+    assert np.all(bq[:, 2] > 0.0)
+    t0 = time.perf_counter()
+    for _ in range(10):
+        g.step(1.0 / 60.0, 8, 1)
+    log(f"  gpu {1e3 * (time.perf_counter() - t0) / 10:.2f} ms per frame (frames 7-16)")
+
+
 def test_graph_replay_equals_eager(vk):
     """Captured frames (hipGraph replay) give the same bits as eager launches,
     across an update_mesh (which drops the captured frames) and a change of
