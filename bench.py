@@ -315,8 +315,20 @@ def main():
         out["roofline"] = {"bound": "hbm", "kernel": dom, "achieved": d["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                            "frac": d["gbs"] / HBM_PEAK_GBS, "traffic": measured_traffic(dom),
                            "alg_bytes_per_launch": d["alg_bytes"], "avg_us": d["avg_us"]}
-        out["kernels"] = {k: {"avg_us": round(v["avg_us"], 3), "launches": v["launches"],
-                              "GBs": round(v["gbs"], 1)} for k, v in kern.items()}
+        out["kernels"] = {}
+        for k, v in kern.items():
+            e = {"avg_us": round(v["avg_us"], 3), "launches": v["launches"], "alg_GBs": round(v["gbs"], 1)}
+            tr = measured_traffic(k)
+            if tr is not None and v["avg_us"] > 0:   # measured HBM bytes (rocprofv3 --pmc summary) per launch
+                e["traffic_MB"] = round(tr / 1e6, 2)
+                e["traffic_GBs"] = round(tr / (v["avg_us"] * 1e-6) / 1e9, 1)
+                e["traffic_frac_of_peak"] = round(tr / (v["avg_us"] * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)
+            out["kernels"][k] = e
+        # whole-step roofline with the reference-layout byte model per substep,
+        # N(376 + 24m) + 156C + 16U + 8U' (mean contacts, final occupied cells)
+        b_sub = N * (376 + 24) + 156 * C + 16 * U + 8 * U
+        out["step_roofline"] = {"bytes_per_substep": b_sub, "achieved_GBs": b_sub * value / N / 1e9,
+                                "frac": b_sub * value / N / 1e9 / HBM_PEAK_GBS}
         out["gpu_ms_per_frame_profiled"] = sum(v[0] for v in prof.values()) / args.steps
         out["contacts_mean"] = C
         out["occupied_cells"] = U
