@@ -355,11 +355,13 @@ __global__ __launch_bounds__(BLOCK) void k_pp_apply(PPArgs a) {
         }
         cnt = 0;
     };
+    // a lane about to overflow drains together with every lane of the wave
+    // at the same loop point, so the projections stay in lockstep
     auto consider = [&](int k, float4 Pj) {
-        if (k != t && near(Pj)) {
-            s_surv[cnt][threadIdx.x] = uint32_t(k);
-            if (++cnt == kSurv) drain();
-        }
+        if (k != t && near(Pj)) s_surv[cnt++][threadIdx.x] = uint32_t(k);
+    };
+    auto room = [&](int need) {
+        if (__any(cnt > kSurv - need)) drain();
     };
     for (int cz = zs; cz <= ze; cz++) {
         for (int cy = ys; cy <= ye; cy++) {
@@ -368,7 +370,7 @@ __global__ __launch_bounds__(BLOCK) void k_pp_apply(PPArgs a) {
                 const int vx = cx + kGridOrigin;
                 if (vx < 0) {   // clamped virtual cell: one physical cell per step
                     const int2 rg = lookup(cell_index(cx, cy, cz));
-                    for (int k = rg.x; k < rg.y; k++) consider(k, a.Xs[k]);
+                    for (int k = rg.x; k < rg.y; k++) { room(1); consider(k, a.Xs[k]); }
                     cx++;
                     continue;
                 }
@@ -386,12 +388,15 @@ __global__ __launch_bounds__(BLOCK) void k_pp_apply(PPArgs a) {
                     }
                 }
                 int k = s;
-                for (; k + 1 < e; k += 2) {   // both loads issue before the tests
-                    const float4 P0 = a.Xs[k], P1 = a.Xs[k + 1];
+                for (; k + 3 < e; k += 4) {   // four loads issue before the tests
+                    const float4 P0 = a.Xs[k], P1 = a.Xs[k + 1], P2 = a.Xs[k + 2], P3 = a.Xs[k + 3];
+                    room(4);
                     consider(k, P0);
                     consider(k + 1, P1);
+                    consider(k + 2, P2);
+                    consider(k + 3, P3);
                 }
-                if (k < e) consider(k, a.Xs[k]);
+                for (; k < e; k++) { room(1); consider(k, a.Xs[k]); }
                 cx += len;
             }
         }
