@@ -233,6 +233,20 @@ int xpbd_mesh_query_stats(xpbd_solver* h, uint64_t* out6);
 int xpbd_init_batch(xpbd_solver* h, const xpbd_config* cfg, const uint64_t* counts, size_t num_scenes,
                     size_t max_contacts);
 int xpbd_scene_count(const xpbd_solver* h, size_t* out);
+/* Wrap planes: the hash grid wraps every 128 cells, so particles 128 cells
+ * apart share a cell and every neighbour query scans them.  With wrap planes a
+ * plain solver sorts by wrap plane << 21 | cell (two bits per axis of which
+ * wrap a cell lies in) and queries read only their own wrap's run: fewer
+ * candidates for scenes wider than 128 cells, one more radix pass (27-bit
+ * keys).  xpbd_set_wrap_planes asks for the mode (default off; environment
+ * VKXPBD_WRAP_PLANES=1 turns it on at init); xpbd_wrap_planes reports whether
+ * it is in effect: never for a batched solver, nor while a pair could reach
+ * 126 cells (3 r_max + particle_max_radius + 2|cohesion| >= 126 search_radius,
+ * re-checked by set_particles).  Results are identical either way; the
+ * sorted-cell and cell-table getters report the reference's order in both.
+ * No reference counterpart (src/scene.cpp:885-914 sorts plain cell keys). */
+int xpbd_set_wrap_planes(xpbd_solver* h, int on);
+int xpbd_wrap_planes(const xpbd_solver* h, int* out);
 /* Frame capture: when on (default; environment VKXPBD_GRAPHS=0 turns it off),
  * xpbd_step records each distinct frame once as a hipGraph and replays it.
  * Results are identical either way.  No reference counterpart (the reference

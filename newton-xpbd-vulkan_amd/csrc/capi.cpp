@@ -237,6 +237,12 @@ int xpbd_scene_count(const xpbd_solver* h, size_t* out) {
     *out = h->s.scene_count();
     return 0;
 }
+int xpbd_set_wrap_planes(xpbd_solver* h, int on) { return guarded(h, [&] { h->s.set_wrap_planes(on != 0); }); }
+int xpbd_wrap_planes(const xpbd_solver* h, int* out) {
+    if (!h || !out) return 1;
+    *out = h->s.wrap_planes() ? 1 : 0;
+    return 0;
+}
 int xpbd_set_graph_mode(xpbd_solver* h, int on) { return guarded(h, [&] { h->s.set_graph_mode(on != 0); }); }
 int xpbd_autotune(xpbd_solver* h, int frames, float dt, int substeps, int iterations, int* block) {
     return guarded(h, [&] {

@@ -106,6 +106,7 @@ __global__ __launch_bounds__(256) void k_integrate(IntegrateArgs a) {
         a.Vp[idx] = Vn;
         uint32_t key = cell_of(Pn.x, Pn.y, Pn.z, a.inv_cell);
         if (a.scene != nullptr) key |= uint32_t(a.scene[idx]) << kCellBits;
+        else if (a.planes) key |= plane_of(Pn.x, Pn.y, Pn.z, a.inv_cell) << kCellBits;
         a.keys[idx] = key;
         atomicAdd(&h[key & (kSortBins - 1)], 1u);
     }
@@ -135,13 +136,26 @@ __global__ __launch_bounds__(256) void k_plane_contacts(PlaneArgs a) {
 // grid_offsets.comp:16-35 into one interleaved (start, end) table; the table
 // is zeroed by k_fill_zero first (the two 8 MiB fills of scene.cpp:918-923
 // become one 16 MiB vectorised fill), so empty cells read (0, 0).
-__global__ __launch_bounds__(256) void k_cell_ranges(int n, const uint32_t* __restrict__ keys, int* __restrict__ range) {
+// rkeys (when non-null) records the keys whose rows were written, so that
+// k_clear_ranges can reset exactly those rows before the next build.
+__global__ __launch_bounds__(256) void k_cell_ranges(int n, const uint32_t* __restrict__ keys, int* __restrict__ range,
+                                                     uint32_t* __restrict__ rkeys) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const uint32_t c = keys[t];
     const size_t row = size_t(2) * c;   // batched keys reach 2^32 - 1
     if (t == 0 || keys[t - 1] != c) range[row] = t;
     if (t == n - 1 || keys[t + 1] != c) range[row + 1] = t + 1;
+    if (rkeys) rkeys[t] = c;
+}
+
+// Zeroes the rows k_cell_ranges wrote from the sorted keys rkeys: with wrap
+// planes the table has 2^27 rows, too many to clear whole every substep.
+__global__ __launch_bounds__(256) void k_clear_ranges(int n, const uint32_t* __restrict__ rkeys, int* __restrict__ range) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const uint32_t c = rkeys[t];
+    if (t == 0 || rkeys[t - 1] != c) *reinterpret_cast<int2*>(range + size_t(2) * c) = make_int2(0, 0);
 }
 
 // Dense tables exactly as the reference leaves them after grid_offsets
@@ -207,9 +221,10 @@ __global__ __launch_bounds__(256) void k_shape_solve(ShapeArgs a) {
         float l2 = vt.y * vt.y;
         l2 = fma_(vt.x, vt.x, l2);
         l2 = fma_(vt.z, vt.z, l2);
-        float vt_len = exact_sqrt(l2);
+        const RootInv vl = exact_sqrt_inv(l2);
+        const float vt_len = vl.d;
         float lambda_f = gmax(mu * lambda_n, -vt_len * a.dt);
-        f3 vt_dir = vt_len > 0.0f ? exact_div(vt, vt_len) : f3{0.f, 0.f, 0.f};
+        f3 vt_dir = vt_len > 0.0f ? exact_div_by(vt, vt_len, vl.inv) : f3{0.f, 0.f, 0.f};
         f3 diff = {fma_(vt_dir.x, lambda_f, -delta_n.x), fma_(vt_dir.y, lambda_f, -delta_n.y),
                    fma_(vt_dir.z, lambda_f, -delta_n.z)};
         f3 dtot = exact_div(diff, denom) * a.relax;
@@ -233,29 +248,29 @@ __global__ __launch_bounds__(256) void k_shape_solve(ShapeArgs a) {
 // (self, inactive, proxy, box, refined distance).  Every test is a pure
 // reject, so testing the distance first changes nothing.
 template <bool DIAG>
-__device__ __forceinline__ void pp_pair(const PPArgs& a, int t, int k, float4 Pj, int i, int proxy, f3 x, f3 v, float rad,
-                                        float w1, float R, f3& acc) {
+__device__ __forceinline__ void pp_pair(const PPArgs& a, int t, int k, float4 Pj, float4 Vj, int nf, int i, int proxy, f3 x,
+                                        f3 v, float rad, float w1, float R, f3& acc) {
     if (k == t) return;   // j == i
-    const int nf = a.Fs[k];
     if ((nf & 1) == 0) return;
-    const float4 Vj = a.Vs[k];
     if (proxy != 0 && (((nf & 2) != 0) || (Vj.w <= 0.0f))) return;
     const f3 rd = x - xyz(Pj);
     if (fabsf(rd.x) > R || fabsf(rd.y) > R || fabsf(rd.z) > R) return;
-    const float d = exact_length(rd);
+    const RootInv dl = exact_length_inv(rd);
+    const float d = dl.d;
     const float err = d - rad - Pj.w;
     if (err > a.coh || d <= 0.0f) return;
     const float w2 = Vj.w;
     const float denom = w1 + w2;
     if (denom <= 0.0f) return;
-    const f3 nn = exact_div(rd, d);
+    const f3 nn = exact_div_by(rd, d, dl.inv);
     const f3 vrel = v - xyz(Vj);
     const float lambda_n = err;
     const float vn = exact_dot(nn, vrel);
     const f3 vt = {fma_(vn, -nn.x, vrel.x), fma_(vn, -nn.y, vrel.y), fma_(vn, -nn.z, vrel.z)};
-    const float vt_len = exact_length(vt);
+    const RootInv vl = exact_length_inv(vt);
+    const float vt_len = vl.d;
     const float lambda_f = gmax(a.mu * lambda_n, -vt_len * a.dt);
-    const f3 vt_dir = vt_len > 0.0f ? exact_div(vt, vt_len) : f3{0.f, 0.f, 0.f};
+    const f3 vt_dir = vt_len > 0.0f ? exact_div_by(vt, vt_len, vl.inv) : f3{0.f, 0.f, 0.f};
     const f3 p2 = nn * err;
     const f3 df = {fma_(vt_dir.x, lambda_f, -p2.x), fma_(vt_dir.y, lambda_f, -p2.y), fma_(vt_dir.z, lambda_f, -p2.z)};
     const f3 pair_d = exact_div(df, denom);
@@ -297,7 +312,9 @@ __device__ __forceinline__ void pp_pair(const PPArgs& a, int t, int k, float4 Pj
 constexpr int kSurv = 16;   // recorded survivors per lane before a drain
 
 template <bool DIAG, bool VALIDATE, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_pp_apply(PPArgs a) {
+// Six waves per SIMD measured faster than the five that 88 VGPRs would give,
+// despite a few spilled registers.
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_pp_apply(PPArgs a) {
     __shared__ uint32_t s_surv[kSurv][BLOCK];
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= a.n) return;
@@ -324,81 +341,123 @@ __global__ __launch_bounds__(BLOCK) void k_pp_apply(PPArgs a) {
     const int ze = min((int)floorf((x.z + R) * inv), zs + kGridDim - 1);
     const int2* range = reinterpret_cast<const int2*>(a.range);
     // batched mode: this particle's scene selects its own 2^21-cell table
-    // (keys reach 2^32 - 1 with 2048 scenes: row arithmetic stays unsigned)
-    const uint32_t base = VALIDATE ? (a.keys[t] & ~((1u << kCellBits) - 1u)) : 0u;
-    auto lookup = [&](int cell) -> int2 {
-        int2 rg = range[size_t(base) + size_t(cell)];
+    // (keys reach 2^32 - 1 with 2048 scenes: row arithmetic stays unsigned);
+    // with wrap planes each visited virtual cell selects its plane's table
+    const uint32_t sbase = VALIDATE ? (a.keys[t] & ~((1u << kCellBits) - 1u)) : 0u;
+    auto table = [&](int cx, int cy, int cz) -> uint32_t {
+        return VALIDATE ? sbase : (a.planes ? wrap_plane(cx, cy, cz) << kCellBits : 0u);
+    };
+    auto raw = [&](uint32_t base, int cell) -> int2 { return range[size_t(base) + size_t(cell)]; };
+    auto check = [&](uint32_t base, int cell, int2 rg) -> int2 {
         if (VALIDATE && !(rg.x >= 0 && rg.x < a.n && rg.y > rg.x && a.keys[rg.x] == base + uint32_t(cell)))
             rg = make_int2(0, 0);
         return rg;
     };
     // cheap reject before the refined length: a pair the exact test accepts
-    // has |rd| <= (r_i + r_j + coh)(1 + 5 ulp), and the plain squared length is
-    // within 3 ulp of |rd|^2, far inside the 1e-4 relative slack
-    const float lim0 = rad + fabsf(a.coh);
-    auto near = [&](float4 Pj) -> bool {
+    // has |rd| <= (r_i + r_j + coh)(1 + 5 ulp) <= (r_i + r_max + |coh|)(1 + 5 ulp)
+    // (r_max bounds every radius), and the plain squared length is within
+    // 3 ulp of |rd|^2, far inside the 1e-4 relative slack
+    const float lim = rad + a.rmax + fabsf(a.coh);
+    const float lim2 = lim * lim * 1.0001f + 1e-30f;
+    auto near = [&](f3 Pj) -> bool {
         const float dx = x.x - Pj.x, dy = x.y - Pj.y, dz = x.z - Pj.z;
-        const float lim = lim0 + Pj.w;
-        return !((dx * dx + dy * dy) + dz * dz > lim * lim * 1.0001f + 1e-30f);
+        return !((dx * dx + dy * dy) + dz * dz > lim2);
+    };
+    // the scan needs positions only (12-byte loads; the radius enters
+    // through r_max)
+    auto pos = [&](int k) -> f3 {
+        const float3 p = *reinterpret_cast<const float3*>(a.Xs + k);
+        return f3{p.x, p.y, p.z};
     };
     f3 acc = {0.f, 0.f, 0.f};
     // Two phases keep the lanes of a wave together: the scan applies the
     // cheap reject to every candidate and records the survivors' slots (in
     // visiting order) in this lane's LDS column; the projection then runs over
     // the recorded slots, where each lane's j-th survivor is processed at the
-    // same step.  A full column is drained before the scan continues.
+    // same step, its three loads issued together.  A full column is drained
+    // before the scan continues.
     int cnt = 0;
     auto drain = [&]() {
         for (int m = 0; m < cnt; m++) {
             const int k = int(s_surv[m][threadIdx.x]);
-            pp_pair<DIAG>(a, t, k, a.Xs[k], i, proxy, x, v, rad, w1, R, acc);
+            const float4 Pj = a.Xs[k], Vj = a.Vs[k];
+            const int nf = a.Fs[k];
+            pp_pair<DIAG>(a, t, k, Pj, Vj, nf, i, proxy, x, v, rad, w1, R, acc);
         }
         cnt = 0;
     };
     // a lane about to overflow drains together with every lane of the wave
     // at the same loop point, so the projections stay in lockstep
-    auto consider = [&](int k, float4 Pj) {
+    auto consider = [&](int k, f3 Pj) {
         if (k != t && near(Pj)) s_surv[cnt++][threadIdx.x] = uint32_t(k);
     };
     auto room = [&](int need) {
         if (__any(cnt > kSurv - need)) drain();
     };
+    auto scan = [&](int s0, int e0) {
+        int k = s0;
+        for (; k + 3 < e0; k += 4) {   // four loads issue before the tests
+            const f3 P0 = pos(k), P1 = pos(k + 1), P2 = pos(k + 2), P3 = pos(k + 3);
+            room(4);
+            consider(k, P0);
+            consider(k + 1, P1);
+            consider(k + 2, P2);
+            consider(k + 3, P3);
+        }
+        for (; k < e0; k++) { room(1); consider(k, pos(k)); }
+    };
+    // One x-run of a row: cells [cx, cx + len) inside one wrap, hence
+    // consecutive cell ids and one contiguous run of sorted slots.  The first
+    // (up to) three cells' table rows may come in already loaded.
+    auto run = [&](int cx, int cy, int cz, int len, int2 p0, int2 p1, int2 p2, bool pre) {
+        const int c0 = cell_index(cx, cy, cz);
+        const uint32_t base = table(cx, cy, cz);
+        int s0 = 0, e0 = 0;
+        bool any = false;
+        for (int m = 0; m < len; m++) {
+            int2 rg;
+            if (pre && m < 3) rg = m == 0 ? p0 : (m == 1 ? p1 : p2);
+            else rg = raw(base, c0 + m);
+            rg = check(base, c0 + m, rg);
+            if (rg.y > rg.x) {
+                if (!any) s0 = rg.x;
+                e0 = rg.y;
+                any = true;
+            }
+        }
+        scan(s0, e0);
+    };
+    // Rows (cz, cy) in the reference's order.  Every row's first x-run starts
+    // at xs, so it has the same length in every row; its first three table
+    // rows are requested one row ahead, while the current row is scanned.
+    const int vx0 = xs + kGridOrigin;
+    const bool clamp0 = vx0 < 0;   // clamped virtual cells: one physical cell per run
+    const int len0 = clamp0 ? 1 : min(xe - xs + 1, kGridDim - vx0 % kGridDim);
+    const int npre = min(len0, 3);
+    auto prefetch = [&](int cy, int cz, int2& q0, int2& q1, int2& q2) {
+        const int c0 = cell_index(xs, cy, cz);
+        const uint32_t base = table(xs, cy, cz);
+        q0 = raw(base, c0);
+        q1 = npre > 1 ? raw(base, c0 + 1) : make_int2(0, 0);
+        q2 = npre > 2 ? raw(base, c0 + 2) : make_int2(0, 0);
+    };
+    int2 r0, r1, r2;
+    prefetch(ys, zs, r0, r1, r2);
     for (int cz = zs; cz <= ze; cz++) {
         for (int cy = ys; cy <= ye; cy++) {
-            int cx = xs;
-            while (cx <= xe) {
+            int ny = cy + 1, nz = cz;
+            if (ny > ye) { ny = ys; nz++; }
+            int2 n0 = make_int2(0, 0), n1 = n0, n2 = n0;
+            if (nz <= ze) prefetch(ny, nz, n0, n1, n2);
+            run(xs, cy, cz, len0, r0, r1, r2, true);
+            int cx = xs + len0;
+            while (cx <= xe) {   // the rest of the row: past an x wrap, or clamped cells
                 const int vx = cx + kGridOrigin;
-                if (vx < 0) {   // clamped virtual cell: one physical cell per step
-                    const int2 rg = lookup(cell_index(cx, cy, cz));
-                    for (int k = rg.x; k < rg.y; k++) { room(1); consider(k, a.Xs[k]); }
-                    cx++;
-                    continue;
-                }
-                const int px = vx % kGridDim;
-                const int len = min(xe - cx + 1, kGridDim - px);
-                const int c0 = cell_index(cx, cy, cz);
-                int s = 0, e = 0;
-                bool any = false;
-                for (int m = 0; m < len; m++) {
-                    const int2 rg = lookup(c0 + m);
-                    if (rg.y > rg.x) {
-                        if (!any) s = rg.x;
-                        e = rg.y;
-                        any = true;
-                    }
-                }
-                int k = s;
-                for (; k + 3 < e; k += 4) {   // four loads issue before the tests
-                    const float4 P0 = a.Xs[k], P1 = a.Xs[k + 1], P2 = a.Xs[k + 2], P3 = a.Xs[k + 3];
-                    room(4);
-                    consider(k, P0);
-                    consider(k + 1, P1);
-                    consider(k + 2, P2);
-                    consider(k + 3, P3);
-                }
-                for (; k < e; k++) { room(1); consider(k, a.Xs[k]); }
+                const int len = vx < 0 ? 1 : min(xe - cx + 1, kGridDim - vx % kGridDim);
+                run(cx, cy, cz, len, r0, r1, r2, false);
                 cx += len;
             }
+            r0 = n0; r1 = n1; r2 = n2;
         }
     }
     drain();
@@ -490,8 +549,11 @@ void launch_integrate(const IntegrateArgs& a, hipStream_t s) {
 void launch_plane_contacts(const PlaneArgs& a, hipStream_t s) {
     if (a.n > 0) hipLaunchKernelGGL(k_plane_contacts, dim3(blocks_for(a.n, 256)), dim3(256), 0, s, a);
 }
-void launch_cell_ranges(int n, const uint32_t* keys, int* range, hipStream_t s) {
-    if (n > 0) hipLaunchKernelGGL(k_cell_ranges, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, keys, range);
+void launch_cell_ranges(int n, const uint32_t* keys, int* range, uint32_t* rkeys, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_cell_ranges, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, keys, range, rkeys);
+}
+void launch_clear_ranges(int n, const uint32_t* rkeys, int* range, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_clear_ranges, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, rkeys, range);
 }
 void launch_cell_tables(int n, const uint32_t* keys, int* starts, int* ends, hipStream_t s) {
     if (n > 0) hipLaunchKernelGGL(k_cell_tables, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, keys, starts, ends);

@@ -154,8 +154,7 @@ void Solver::rollout_backward(const float* gx, const float* gv, float* gx0, floa
             // reverse the segment
             for (int j = s1 - s0 - 1; j >= 0; j--) {
                 const int nxt = cur ^ 1;
-                if (num_scenes_ == 1) launch_fill_zero(range_.p, size_t(2) * kNumCells, stream_);
-                launch_cell_ranges(n, t.keys[j].p, range_.p, stream_);
+                build_ranges(t.keys[j].p);
                 AdjArgs a;
                 a.n = n; a.keys = t.keys[j].p; a.vals = t.vals[j].p; a.range = range_.p; a.Fs = t.Fs[j].p;
                 a.Xs = t.Xs[j].p; a.Vs = t.Vs[j].p; a.P0 = t.P0[j].p; a.V0 = t.V0[j].p; a.delta = t.delta[j].p;
@@ -166,6 +165,7 @@ void Solver::rollout_backward(const float* gx, const float* gv, float* gx0, floa
                 a.dt = t.sdt; a.relax = t.relax; a.inv_cell = inv_cell; a.vmax = cfg_.particle_v_max;
                 a.gx = cfg_.gravity[0]; a.gy = cfg_.gravity[1]; a.gz = cfg_.gravity[2];
                 a.rmax = rmax_;
+                a.planes = planes_ ? 1 : 0;
                 launch_adj_prep(a, stream_);
                 AdjShapeArgs sa;
                 sa.cmax = int(cmax_); sa.cc = t.counts.p + j;

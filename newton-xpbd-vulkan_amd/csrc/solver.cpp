@@ -54,7 +54,7 @@ void Solver::free_all() {
     for (int k = 0; k < 2; k++) { dfree(Xs_[k]); dfree(Vs_[k]); }
     dfree(Fs_); dfree(body_half_); dfree(body_I_); dfree(body_invw_); dfree(mat_mu_t_); dfree(mat_mu_r_);
     dfree(rc_s0_); dfree(rc_s1_); dfree(rc_p0_); dfree(rc_p1_); dfree(rc_n_); dfree(rc_impulse_);
-    dfree(range_); dfree(tbl_); dfree(cpart_); dfree(cshape_); dfree(cc_); dfree(cbpos_); dfree(cbvel_); dfree(cnorm_);
+    dfree(range_); dfree(rkeys_); dfree(tbl_); dfree(cpart_); dfree(cshape_); dfree(cc_); dfree(cbpos_); dfree(cbvel_); dfree(cnorm_);
     dfree(dshape_); dfree(delta_); dfree(body_q_); dfree(body_qd_); dfree(body_com_); dfree(body_invI_);
     dfree(body_invm_); dfree(body_delta_); dfree(mat_mu_); dfree(body_flags_); dfree(shape_body_);
     dfree(verts_d_); dfree(vvel_d_); dfree(tris_d_); dfree(prim_d_); dfree(err_d_); dfree(nodes_d_); dfree(cnodes_d_); dfree(lbox_d_); dfree(pg_start_); dfree(lb_); dfree(lb_node_); dfree(tsoup_); dfree(pg_soup_); dfree(mq_count_); dfree(mq_);
@@ -74,6 +74,7 @@ void Solver::alloc_all() {
     dalloc(flags_, n_);
     radius_h_.assign(n_, 0.f);
     invm_h_.assign(n_, 0.f);
+    rmax_ = 0.f;
     dalloc(keysA_, n_); dalloc(valsA_, n_); dalloc(keysB_, n_); dalloc(valsB_, n_);
     for (int k = 0; k < 2; k++) { dalloc(Xs_[k], n_); dalloc(Vs_[k], n_); }
     dalloc(Fs_, n_);
@@ -117,6 +118,10 @@ void Solver::init_common(const xpbd_config& cfg, size_t n, size_t max_contacts) 
     mesh_stats_on_ = senv && (std::strcmp(senv, "1") == 0 || std::strcmp(senv, "true") == 0);
     const char* genv = std::getenv("VKXPBD_GRAPHS");
     use_graphs_ = !(genv && std::strcmp(genv, "0") == 0);
+    const char* penv = std::getenv("VKXPBD_WRAP_PLANES");
+    planes_allowed_ = penv && std::strcmp(penv, "1") == 0;
+    planes_ = false;
+    update_planes();
     grid_valid_ = false;
     contact_slot_ = 0;
     rigid_slot_ = 0;
@@ -464,6 +469,7 @@ void Solver::set_particles(const float* q, const float* qd, const float* w, cons
     if (r) std::memcpy(radius_h_.data(), r, n * sizeof(float));
     rmax_ = 0.0f;
     for (size_t i = 0; i < n; i++) rmax_ = std::max(rmax_, std::fabs(radius_h_[i]));
+    update_planes();
     std::vector<float4> P(n), V(n);
     for (size_t i = 0; i < n; i++) {
         P[i] = make_float4(q[3 * i], q[3 * i + 1], q[3 * i + 2], radius_h_[i]);
@@ -536,6 +542,47 @@ void Solver::sync_profile() {
     }
     pending_.clear();
     ev_used_ = 0;
+}
+
+// ----------------------------------------------------------- wrap planes ----
+// The hash grid wraps every 128 cells, so particles far apart can share a
+// cell; every neighbour query then scans them and rejects them on distance.
+// A plain solver sorts by wrap plane << 21 | cell instead (two bits per axis of
+// which wrap a cell lies in, 2^27 rows): a query reads only its own wrap's
+// run, which holds every particle the reference would accept, in the same
+// order.  That holds while no accepted pair, seen from anywhere in its query
+// box, can reach 126 cells: then particles of one physical cell that lie in
+// different wraps never interact.  The check uses the largest radius set so
+// far, so set_particles can switch the mode (and drops captured frames).
+void Solver::update_planes() {
+    const float rm = std::max(rmax_, cfg_.particle_max_radius);
+    const float reach = 3.0f * rm + cfg_.particle_max_radius + 2.0f * std::fabs(cfg_.particle_cohesion);
+    const bool want = planes_allowed_ && num_scenes_ == 1 && reach < 126.0f * cfg_.search_radius;
+    if (want == planes_ && range_.p) return;
+    DeviceGuard g(device_);
+    HIPCHECK(hipStreamSynchronize(stream_));
+    invalidate_graphs();
+    planes_ = want;
+    if (num_scenes_ == 1) sort_passes_ = planes_ ? (kCellBits + kPlaneBits + kSortBits - 1) / kSortBits : kSortPasses;
+    dalloc(range_, size_t(2) * (size_t(kNumCells) << (planes_ ? kPlaneBits : 0)) * num_scenes_);
+    if (planes_) dalloc(rkeys_, n_);   // zero keys: the first reset touches row 0 only
+    else dfree(rkeys_);
+    HIPCHECK(hipDeviceSynchronize());
+    grid_valid_ = false;
+}
+
+// Cell ranges of the sorted keys.  Batched tables are validated on lookup;
+// the plain 2^21-row table is zeroed whole; the 2^27-row table of wrap planes
+// has the previous build's rows reset instead.
+void Solver::build_ranges(const uint32_t* sorted_keys) {
+    const int n = int(n_);
+    if (planes_) {
+        launch_clear_ranges(n, rkeys_.p, range_.p, stream_);
+        launch_cell_ranges(n, sorted_keys, range_.p, rkeys_.p, stream_);
+    } else {
+        if (num_scenes_ == 1) launch_fill_zero(range_.p, size_t(2) * kNumCells, stream_);
+        launch_cell_ranges(n, sorted_keys, range_.p, nullptr, stream_);
+    }
 }
 
 // ----------------------------------------------------------------- step ----
@@ -642,6 +689,7 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
         a.rcc_next = zero_next ? cc_.p + 4 + (slot ^ 1) : nullptr;
         a.mesh_next = zero_next ? cc_.p + 6 : nullptr;
         a.scene = num_scenes_ > 1 ? scene_d_.p : nullptr;
+        a.planes = planes_ ? 1 : 0;
         a.cpart = cpart_.p; a.cshape = cshape_.p; a.cbpos = cbpos_.p; a.cbvel = cbvel_.p; a.cnorm = cnorm_.p;
         stage_begin(kIntegrate);
         launch_integrate(a, stream_);
@@ -656,8 +704,7 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
     if (n > 0) {
         run_sort(tiles);
         stage_begin(kRanges);
-        if (num_scenes_ == 1) launch_fill_zero(range_.p, size_t(2) * kNumCells, stream_);
-        launch_cell_ranges(n, keysA_.p, range_.p, stream_);
+        build_ranges(keysA_.p);
         stage_end(kRanges);
         grid_valid_ = true;
     }
@@ -685,6 +732,7 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
         pa.mu = cfg_.particle_mu; pa.coh = cfg_.particle_cohesion; pa.max_r = cfg_.particle_max_radius;
         pa.dt = sdt; pa.relax = relax_; pa.inv_cell = inv_cell; pa.vmax = cfg_.particle_v_max;
         pa.rmax = rmax_;
+        pa.planes = planes_ ? 1 : 0;
         pa.iter_index = uint32_t(it); pa.max_diag_pairs = max_pairs_;
         pa.diag_header = diag_header_.p; pa.diag_pairs = diag_pairs_.p;
         stage_begin(kPP);
@@ -983,10 +1031,31 @@ void Solver::set_contact_data(const int32_t* p, const int32_t* s, const float* b
     }
     HIPCHECK(hipMemcpy(cc_.p + contact_slot_, &dev_count, sizeof(int32_t), hipMemcpyHostToDevice));
 }
+// With wrap planes the device order is (plane, cell, id); the getters present
+// the reference's order instead: cells ascending, particle ids ascending
+// within a cell (what a stable sort of plain cell keys in id order gives).
+std::vector<uint64_t> Solver::reference_sorted() {
+    std::vector<uint32_t> k(n_), v(n_);
+    if (n_) {
+        HIPCHECK(hipMemcpyAsync(k.data(), keysA_.p, n_ * sizeof(uint32_t), hipMemcpyDeviceToHost, stream_));
+        HIPCHECK(hipMemcpyAsync(v.data(), valsA_.p, n_ * sizeof(uint32_t), hipMemcpyDeviceToHost, stream_));
+    }
+    HIPCHECK(hipStreamSynchronize(stream_));
+    std::vector<uint64_t> kv(n_);
+    for (size_t t = 0; t < n_; t++) kv[t] = (uint64_t(k[t] & ((1u << kCellBits) - 1u)) << 32) | v[t];
+    std::sort(kv.begin(), kv.end());
+    return kv;
+}
+
 void Solver::get_sorted(bool ids, uint32_t* out, size_t n) {
     require_init();
     DeviceGuard g(device_);
     n = std::min(n, n_);
+    if (planes_) {
+        const std::vector<uint64_t> kv = reference_sorted();
+        for (size_t t = 0; t < n; t++) out[t] = ids ? uint32_t(kv[t]) : uint32_t(kv[t] >> 32);
+        return;
+    }
     if (n) HIPCHECK(hipMemcpyAsync(out, ids ? valsA_.p : keysA_.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost, stream_));
     HIPCHECK(hipStreamSynchronize(stream_));
 }
@@ -996,6 +1065,17 @@ void Solver::get_cell_table(bool ends, int32_t* out, size_t n) {
     DeviceGuard g(device_);
     n = std::min<size_t>(n, kNumCells);
     if (!grid_valid_ || n_ == 0) { std::memset(out, 0, n * sizeof(int32_t)); return; }
+    if (planes_) {   // dense tables of the reference's order (k_cell_tables on the host)
+        const std::vector<uint64_t> kv = reference_sorted();
+        std::memset(out, 0, n * sizeof(int32_t));
+        for (size_t t = 0; t < n_; t++) {
+            const uint32_t c = uint32_t(kv[t] >> 32);
+            if (c >= n) continue;
+            if (!ends && (t == 0 || uint32_t(kv[t - 1] >> 32) != c)) out[c] = int32_t(t);
+            if (ends && (t + 1 == n_ || uint32_t(kv[t + 1] >> 32) != c)) out[c] = int32_t(t + 1);
+        }
+        return;
+    }
     if (!tbl_.p) dalloc(tbl_, size_t(2) * kNumCells);
     HIPCHECK(hipMemsetAsync(tbl_.p, 0, size_t(2) * kNumCells * sizeof(int32_t), stream_));
     launch_cell_tables(int(n_), keysA_.p, tbl_.p, tbl_.p + kNumCells, stream_);

@@ -31,8 +31,10 @@ public:
     // ground plane, advanced together (one launch per stage for all scenes).
     void init_batch(const xpbd_config& cfg, const size_t* counts, size_t num_scenes, size_t max_contacts);
     size_t scene_count() const { return num_scenes_; }
+    bool wrap_planes() const { return planes_; }
     // hipGraph capture of whole frames in step() (default on; VKXPBD_GRAPHS=0 disables)
     void set_graph_mode(bool on) { use_graphs_ = on; invalidate_graphs(); }
+    void set_wrap_planes(bool on) { planes_allowed_ = on; update_planes(); }
     // Times candidate launch shapes on the current state (then restores it) and
     // keeps the fastest; returns the chosen particle-particle block size.
     int autotune(int frames, float dt, int substeps, int iterations);
@@ -126,6 +128,9 @@ private:
     BodyStepArgs body_step_args(float sdt);
     void run_sort(int tiles);
     uint32_t* first_keys();
+    void update_planes();
+    std::vector<uint64_t> reference_sorted();
+    void build_ranges(const uint32_t* sorted_keys);
     void read_count(int slot);
     void stage_begin(Stage s);
     void stage_end(Stage s);
@@ -142,6 +147,8 @@ private:
     std::vector<size_t> scene_off_;
     DevBuf<int32_t> scene_d_;
     int sort_passes_ = kSortPasses;
+    // plain solver: sort keys carry the wrap plane above the cell (see update_planes)
+    bool planes_ = false, planes_allowed_ = false;
     int pp_block_ = 256;
     // captured frames: rotation state and parameters -> executable graph
     struct FrameGraph {
@@ -181,7 +188,8 @@ private:
     DevBuf<uint32_t> keysA_, valsA_, keysB_, valsB_, hist_, bintot_;
     DevBuf<float4> Xs_[2], Vs_[2];   // predicted state in sorted order (ping-pong across iterations)
     DevBuf<int32_t> Fs_;
-    DevBuf<int32_t> range_;   // int2 per cell (start, end), zero-filled every substep
+    DevBuf<int32_t> range_;   // int2 per key (start, end); rows of the last build are reset before the next
+    DevBuf<uint32_t> rkeys_;  // with wrap planes: the sorted keys whose rows range_ holds
     DevBuf<int32_t> tbl_;     // 2 x cells scratch for the dense getters
     // contacts: SoA of float4 payloads, counter slots
     DevBuf<int32_t> cpart_, cshape_, cc_;

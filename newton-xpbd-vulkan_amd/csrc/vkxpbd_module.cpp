@@ -457,6 +457,16 @@ PYBIND11_MODULE(_vkxpbd, m) {
                                    return v;
                                })
         .def(
+            "set_wrap_planes", [](PySolver& s, bool on) { s.check(xpbd_set_wrap_planes(s.h, on ? 1 : 0)); },
+            py::arg("enabled"),
+            "Sort by wrap plane and cell, so queries skip particles 128 cells away (default off; same results).")
+        .def_property_readonly("wrap_planes",
+                               [](const PySolver& s) {
+                                   int v = 0;
+                                   xpbd_wrap_planes(s.h, &v);
+                                   return v != 0;
+                               })
+        .def(
             "set_graph_mode", [](PySolver& s, bool on) { s.check(xpbd_set_graph_mode(s.h, on ? 1 : 0)); },
             py::arg("enabled"), "Capture and replay whole frames as hipGraphs (default on).")
         .def(

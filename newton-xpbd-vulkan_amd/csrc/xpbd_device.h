@@ -56,11 +56,39 @@ __device__ __forceinline__ float exact_dot(f3 a, f3 b) {
     float xy = fma_(a.x, b.x, y);
     return fma_(a.z, b.z, xy);
 }
-__device__ __forceinline__ f3 exact_div(f3 v, float d) {
-    float inv = 1.0f / d;
+__device__ __forceinline__ f3 exact_div_by(f3 v, float d, float inv) {   // inv = 1.0f / d
     f3 q = v * inv;
     f3 rem = {fma_(-d, q.x, v.x), fma_(-d, q.y, v.y), fma_(-d, q.z, v.z)};
     return {fma_(rem.x, inv, q.x), fma_(rem.y, inv, q.y), fma_(rem.z, inv, q.z)};
+}
+__device__ __forceinline__ f3 exact_div(f3 v, float d) { return exact_div_by(v, d, 1.0f / d); }
+// exact_sqrt together with 1.0f / result, for an exact_div by the root.  The
+// correction's 0.5f / d0 equals 0.5f * (1.0f / d0) exactly: d0, the root of a
+// float, lies in [2^-75, 2^64], so neither quotient is subnormal or overflows
+// and halving is exact (0, inf and NaN give the same values too).  One
+// correctly rounded division then serves the correction and, whenever the
+// corrected root is d0 itself, the reciprocal of the root.
+struct RootInv {
+    float d, inv;
+};
+__device__ __forceinline__ RootInv exact_sqrt_inv(float a) {
+    const float d0 = sqrtf(a);
+    const float i0 = 1.0f / d0;
+    const float r = fma_(-d0, d0, a);
+    const float d = fma_(r, 0.5f * i0, d0);
+    float inv = i0;
+    if (d != d0) {   // a near-tie moved the root (about 2^-23 of roots): divide again
+        float dd = d;
+        asm volatile("" : "+v"(dd));   // keeps the division inside the rarely taken branch
+        inv = 1.0f / dd;
+    }
+    return {d, inv};
+}
+__device__ __forceinline__ RootInv exact_length_inv(f3 a) {
+    float yy = a.y * a.y;
+    float xx_yy = fma_(a.x, a.x, yy);
+    float sq = fma_(a.z, a.z, xx_yy);
+    return exact_sqrt_inv(sq);
 }
 __device__ __forceinline__ float exact_div1(float v, float d) {
     float inv = 1.0f / d;
@@ -110,6 +138,25 @@ __device__ __forceinline__ int cell_index(int x, int y, int z) {
     x += kGridOrigin; y += kGridOrigin; z += kGridOrigin;
     x = x < 0 ? 0 : x; y = y < 0 ? 0 : y; z = z < 0 ? 0 : z;
     return (z % kGridDim) * (kGridDim * kGridDim) + (y % kGridDim) * kGridDim + (x % kGridDim);
+}
+
+// Wrap plane of a virtual cell: which 128-cell wrap of the hash grid it lies
+// in, two bits per axis, after the same origin offset and clamp as
+// cell_index.  Particles that share a physical cell but lie in different
+// wraps are at least 128 - 3 cells apart; when no accepted pair can be that
+// long, such particles never interact, and a sort key of plane << 21 | cell
+// keeps each wrap's particles in a run of their own (in particle order, as
+// the plain key does).
+constexpr int kPlaneBits = 6;
+__device__ __forceinline__ uint32_t wrap_plane(int x, int y, int z) {
+    x += kGridOrigin; y += kGridOrigin; z += kGridOrigin;
+    x = x < 0 ? 0 : x; y = y < 0 ? 0 : y; z = z < 0 ? 0 : z;
+    return (uint32_t((z >> 7) & 3) << 4) | (uint32_t((y >> 7) & 3) << 2) | uint32_t((x >> 7) & 3);
+}
+static_assert(kGridDim == 128, "wrap_plane shifts by log2(kGridDim)");
+// Wrap plane of the cell holding a position (wrap_plane of its virtual cell).
+__device__ __forceinline__ uint32_t plane_of(float px, float py, float pz, float inv) {
+    return wrap_plane((int)floorf(px * inv), (int)floorf(py * inv), (int)floorf(pz * inv));
 }
 
 // Wave-aggregated slot reservation on a global counter: one atomic per wave

@@ -42,6 +42,7 @@ struct IntegrateArgs {
     int has_plane; float margin, altitude;
     int cmax; int* cc; int* cc_next; int* rcc_next; int* mesh_next;
     const int* scene;   // batched mode: scene of each particle (key = scene << 21 | cell); else null
+    int planes;         // key = wrap plane << 21 | cell (plain mode only)
     int* cpart; int* cshape; float4* cbpos; float4* cbvel; float4* cnorm;
 };
 
@@ -88,6 +89,7 @@ struct PPArgs {
     float4* Xs_next; float4* Vs_next;               // output state, sorted order (when another iteration follows)
     float mu, coh, max_r, dt, relax, inv_cell, vmax;
     float rmax;                                     // largest particle radius (bounds every accepted pair)
+    int planes;                                     // keys carry the wrap plane above the cell
     uint32_t iter_index, max_diag_pairs;
     uint32_t* diag_header; float* diag_pairs;
 };
@@ -194,6 +196,7 @@ struct AdjArgs {
     double* relax_part;
     float mu, coh, max_r, dt, relax, inv_cell, vmax, gx, gy, gz;
     float rmax;
+    int planes;
 };
 struct AdjShapeArgs {
     int cmax; const int* cc;
@@ -219,7 +222,8 @@ void launch_body_contact_solve(const BodySolveArgs& a, hipStream_t s);
 
 void launch_integrate(const IntegrateArgs& a, hipStream_t s);
 void launch_plane_contacts(const PlaneArgs& a, hipStream_t s);
-void launch_cell_ranges(int n, const uint32_t* keys, int* range, hipStream_t s);
+void launch_cell_ranges(int n, const uint32_t* keys, int* range, uint32_t* rkeys, hipStream_t s);
+void launch_clear_ranges(int n, const uint32_t* rkeys, int* range, hipStream_t s);
 void launch_cell_tables(int n, const uint32_t* keys, int* starts, int* ends, hipStream_t s);
 void launch_shape_solve(const ShapeArgs& a, int grid, hipStream_t s);
 // validate: the cell table is not cleared (batched mode); an entry is used

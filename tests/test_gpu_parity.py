@@ -303,6 +303,67 @@ def test_large_grid_properties(vk):
     np.testing.assert_array_equal(keys[st[u]], u)
 
 
+def aliased_piles(r=0.01, side=8, layers=4, seed=11):
+    """Copies of one small pile placed whole wraps of the 128-cell hash grid
+    apart (128 cells along x, y, z and combinations; 512 cells along x, which
+    lands in the same x wrap plane; -256 cells along y), so that they share
+    hash cells without ever touching, with random velocities."""
+    rng = np.random.default_rng(seed)
+    cell = 2 * r
+    base = scenes.lattice(side, side, layers, 2 * r, (-(side - 1) * r, -(side - 1) * r, 1.2 * r), 0.1 * r, rng)
+    shifts = [(0, 0, 0), (128, 0, 0), (0, 128, 0), (0, 0, 128), (128, 128, 0), (-128, 128, 128), (512, 0, 0),
+              (0, -256, 0)]
+    q = np.concatenate([base + np.float32(np.array(sh, np.float64) * cell) for sh in shifts]).astype(np.float32)
+    qd = rng.normal(0.0, 0.2, size=q.shape).astype(np.float32)
+    return scene_from(q, r, qd)
+
+
+def test_wrap_planes_aliased_piles(vk, monkeypatch):
+    """Piles that share hash cells across wraps of the grid: the wrap-plane
+    keys give the oracle's bits, the same state as plain cell keys, and the
+    reference's sorted order and cell tables from the getters."""
+    sc = aliased_piles()
+    monkeypatch.setenv("VKXPBD_WRAP_PLANES", "1")
+    g, o = make_pair(vk, sc)
+    monkeypatch.delenv("VKXPBD_WRAP_PLANES")
+    h = scenes.build_solver(vk, sc)
+    assert g.wrap_planes and not h.wrap_planes
+    for f in range(6):
+        for s in (g, o, h):
+            s.step(1.0 / 60.0, 8, 1)
+        compare_state(g, o, exact=True, what=f"aliased piles frame {f + 1}")
+        np.testing.assert_array_equal(g.get_positions(), h.get_positions())
+        np.testing.assert_array_equal(g.get_velocities(), h.get_velocities())
+        assert g.last_contact_count == o.last_contact_count == h.last_contact_count
+    compare_grid(g, o)
+    np.testing.assert_array_equal(g.get_sorted_point_ids(), h.get_sorted_point_ids())
+    np.testing.assert_array_equal(g.get_cell_starts(), h.get_cell_starts())
+
+
+def test_wrap_planes_follow_the_radii(vk):
+    """set_particles with a radius that could reach across a wrap turns the
+    planes off, a later small radius turns them on again; every frame keeps
+    the oracle's bits."""
+    sc = aliased_piles(side=6, layers=3)
+    g, o = make_pair(vk, sc)
+    p = sc["particles"]
+    assert not g.wrap_planes
+    g.set_wrap_planes(True)
+    assert g.wrap_planes
+    big = p["radii"].copy()
+    big[5] = 0.9   # 3 * 0.9 + 0.01 >= 126 * 0.02
+    for f, radii in enumerate([None, big, None, p["radii"]]):
+        if radii is not None:
+            x, v = g.get_positions(), g.get_velocities()
+            for s in (g, o):
+                s.set_particles(x, v, p["inv_mass"], radii, p["flags"])
+            assert g.wrap_planes == (radii is p["radii"])
+        for s in (g, o):
+            s.step(1.0 / 60.0, 4, 1)
+        compare_state(g, o, exact=True, what=f"radii phase {f} planes={g.wrap_planes}")
+        compare_grid(g, o)
+
+
 def random_quats(rng, n):
     """Uniform random unit quaternions (Shoemake), xyzw."""
     u1, u2, u3 = rng.uniform(size=(3, n))
