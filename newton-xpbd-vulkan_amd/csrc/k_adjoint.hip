@@ -78,25 +78,25 @@ __device__ __forceinline__ void block_accumulate(double v, double* slot) {
     }
 }
 
-// Apply adjoint (apply_particle_deltas.comp:44-57) per sorted slot: the
-// cotangent a of the unclamped x1 + D (written in particle and sorted order)
-// and the apply part of the x0 cotangent; also the particle -> slot rank.
+// Apply adjoint (apply_particle_deltas.comp:44-57) per sorted slot t: the
+// cotangent a of the unclamped x1 + D and the apply part of the x0 cotangent.
+// A substep moves the state from the previous order (slot vals[t]: x0, v0 and
+// their cotangents) to the sorted order (slot t: the end state, its
+// cotangents, deltas and contacts).
 __global__ __launch_bounds__(256) void k_adj_prep(AdjArgs a) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= a.n) return;
     const int i = int(a.vals[t]);
-    a.rank[i] = uint32_t(t);
     if ((a.Fs[t] & 1) == 0) {
-        a.A[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-        a.As[t] = a.A[i];
+        a.As[t] = make_float4(0.f, 0.f, 0.f, 0.f);
         return;
     }
     const f3 x1 = xyz(a.Xs[t]), x0 = xyz(a.P0[i]);
-    const f3 dl = {a.delta[3 * i], a.delta[3 * i + 1], a.delta[3 * i + 2]};
+    const f3 dl = {a.delta[3 * t], a.delta[3 * t + 1], a.delta[3 * t + 2]};
     const f3 xn = x1 + dl;
     const f3 u = markstein_div(xn - x0, a.dt);
     const float m = len3(u);
-    const f3 lx = xyz(a.lx_in[i]), lv = xyz(a.lv_in[i]);
+    const f3 lx = xyz(a.lx_in[t]), lv = xyz(a.lv_in[t]);
     f3 av, lxa;
     if (m > a.vmax) {
         const f3 uh = u * (1.0f / m);
@@ -108,9 +108,7 @@ __global__ __launch_bounds__(256) void k_adj_prep(AdjArgs a) {
         av = lx + lv * (1.0f / a.dt);
         lxa = lv * (-1.0f / a.dt);
     }
-    const float4 A4 = make_float4(av.x, av.y, av.z, 0.f);
-    a.A[i] = A4;
-    a.As[t] = A4;
+    a.As[t] = make_float4(av.x, av.y, av.z, 0.f);
     a.lx_out[i] = make_float4(lxa.x, lxa.y, lxa.z, 0.f);
 }
 
@@ -130,8 +128,7 @@ __global__ __launch_bounds__(256) void k_adj_shape(AdjShapeArgs a) {
             if (a.body.flags[body] & 4) continue;
             if (a.body.invm[body] == 0.0f) continue;
         }
-        const uint32_t r = a.rank[pi];
-        const float4 P = a.Xs[r], Vv = a.Vs[r];
+        const float4 P = a.Xs[pi], Vv = a.Vs[pi];   // contacts name sorted slots
         const f3 px = xyz(P), pv = xyz(Vv);
         const BodyFrame F = load_body(a.body, body);
         const f3 bx = F.pos + quat_rotate_sc(F.rot, xyz(a.cbpos[t]));
@@ -155,7 +152,7 @@ __global__ __launch_bounds__(256) void k_adj_shape(AdjShapeArgs a) {
         const float mu = 0.5f * (a.pmu + a.body.mat_mu[s]);
         const f3 bv = F.v + cross3(F.w, rr) + quat_rotate_sc(F.rot, xyz(a.cbvel[t]));
         const float sc = w1 / denom;
-        const f3 av = xyz(a.A[pi]);
+        const f3 av = xyz(a.As[pi]);
         f3 g_x, g_v;
         const f3 g = shape_vjp(c, pv - bv, n, mu, a.dt, av * (sc * a.relax), g_x, g_v);
         atomicAdd(&a.shx[3 * pi + 0], g_x.x); atomicAdd(&a.shx[3 * pi + 1], g_x.y); atomicAdd(&a.shx[3 * pi + 2], g_x.z);
@@ -178,8 +175,8 @@ __global__ __launch_bounds__(256) void k_adj_pp(AdjArgs a) {
         const int k = int(a.vals[t]);
         const int fk = a.Fs[t];
         if ((fk & 1) == 0) {
-            a.lx_out[k] = a.lx_in[k];
-            a.lv_out[k] = a.lv_in[k];
+            a.lx_out[k] = a.lx_in[t];
+            a.lv_out[k] = a.lv_in[t];
         } else {
             const float4 Pk = a.Xs[t], Vk = a.Vs[t];
             const f3 x = xyz(Pk), v = xyz(Vk), ak = xyz(a.As[t]);
@@ -263,10 +260,10 @@ __global__ __launch_bounds__(256) void k_adj_pp(AdjArgs a) {
                     }
             drain();
             racc = double(rsum);
-            const f3 sx = {a.shx[3 * k], a.shx[3 * k + 1], a.shx[3 * k + 2]};
-            const f3 sv = {a.shv[3 * k], a.shv[3 * k + 1], a.shv[3 * k + 2]};
-            a.shx[3 * k] = 0.f; a.shx[3 * k + 1] = 0.f; a.shx[3 * k + 2] = 0.f;
-            a.shv[3 * k] = 0.f; a.shv[3 * k + 1] = 0.f; a.shv[3 * k + 2] = 0.f;
+            const f3 sx = {a.shx[3 * t], a.shx[3 * t + 1], a.shx[3 * t + 2]};
+            const f3 sv = {a.shv[3 * t], a.shv[3 * t + 1], a.shv[3 * t + 2]};
+            a.shx[3 * t] = 0.f; a.shx[3 * t + 1] = 0.f; a.shx[3 * t + 2] = 0.f;
+            a.shv[3 * t] = 0.f; a.shv[3 * t + 1] = 0.f; a.shv[3 * t + 2] = 0.f;
             const f3 lx1 = ak + sx + gx;
             const f3 lv1 = sv + gv;
             const f3 lx0 = xyz(a.lx_out[k]) + lx1;

@@ -31,20 +31,20 @@ __device__ __forceinline__ int tile_item(int tile, int it) {
 // ---------------------------------------------------------------------------
 // integrate_particles.comp:26-60 fused with generate_plane_contacts.comp:27-54
 // (evaluated on the pre-integration position, as the reference orders them:
-// scene.cpp:853-871), grid_cell_indices.comp:17-32 on the predicted position,
-// and the per-tile digit-0 histogram of the first radix pass.
+// scene.cpp:853-871) and grid_cell_indices.comp:17-32 on the predicted
+// position.  The state is stored in slot order (the previous substep's sorted
+// order); the radix input is written in particle-id order, (key, slot) at
+// index id, so the stable sort orders equal cells by particle id exactly as
+// the reference's does.  Plane contacts name the slot.
 // The force buffer and world index of the reference have no setter and are
 // always zero (scene.cpp:82-85), so v1 = v0 + (0*w + g*[w>0])*dt.
 __global__ __launch_bounds__(256) void k_integrate(IntegrateArgs a) {
-    __shared__ uint32_t h[kSortBins];
     const int tile = blockIdx.x;
-    if (threadIdx.x < kSortBins) h[threadIdx.x] = 0;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         if (a.cc_next != nullptr) *a.cc_next = 0;
         if (a.rcc_next != nullptr) *a.rcc_next = 0;
         if (a.mesh_next != nullptr) { a.mesh_next[0] = 0; a.mesh_next[1] = 0; }
     }
-    __syncthreads();
     float4 P[kSortItems], V[kSortItems];
     int f[kSortItems];
 #pragma unroll
@@ -105,13 +105,12 @@ __global__ __launch_bounds__(256) void k_integrate(IntegrateArgs a) {
         a.Pp[idx] = Pn;
         a.Vp[idx] = Vn;
         uint32_t key = cell_of(Pn.x, Pn.y, Pn.z, a.inv_cell);
-        if (a.scene != nullptr) key |= uint32_t(a.scene[idx]) << kCellBits;
+        if (a.scene != nullptr) key |= uint32_t(a.scene[idx]) << kCellBits;   // scenes keep their slot ranges
         else if (a.planes) key |= plane_of(Pn.x, Pn.y, Pn.z, a.inv_cell) << kCellBits;
-        a.keys[idx] = key;
-        atomicAdd(&h[key & (kSortBins - 1)], 1u);
+        const uint32_t id = a.ids[idx];
+        a.keys[id] = key;
+        a.vals[id] = uint32_t(idx);
     }
-    __syncthreads();
-    if (threadIdx.x < kSortBins) a.hist[threadIdx.x * a.tiles + tile] = h[threadIdx.x];
 }
 
 // generate_plane_contacts.comp:27-54 on the current positions (collide_only).
@@ -282,7 +281,7 @@ __device__ __forceinline__ void pp_pair(const PPArgs& a, int t, int k, float4 Pj
             const f3 di = {fma_(dw.x, a.relax, 0.f), fma_(dw.y, a.relax, 0.f), fma_(dw.z, a.relax, 0.f)};
             float* rec = a.diag_pairs + 20 * size_t(slot);
             reinterpret_cast<uint32_t*>(rec)[0] = uint32_t(i);
-            reinterpret_cast<uint32_t*>(rec)[1] = a.vals[k];
+            reinterpret_cast<uint32_t*>(rec)[1] = a.ids[k];
             rec[2] = d; rec[3] = err;
             rec[4] = nn.x; rec[5] = nn.y; rec[6] = nn.z;
             rec[7] = vrel.x; rec[8] = vrel.y; rec[9] = vrel.z;
@@ -298,7 +297,9 @@ __device__ __forceinline__ void pp_pair(const PPArgs& a, int t, int k, float4 Pj
 
 // solve_particle_particle_contacts.comp:107-218 fused with
 // apply_particle_deltas.comp:38-58.  One thread per sorted slot t (the
-// reference's thread order), particle i = vals[t].  Particle i's delta is
+// reference's thread order), particle i = ids[t]; the output state, deltas
+// and shape deltas live in sorted order, the substep-start positions at slot
+// vals[t] of the previous order.  Particle i's delta is
 // (shape contributions) + (its own particle-particle sum): the reference
 // accumulates exactly these with float atomics into a zeroed buffer, and with
 // at most two shape contacts per particle the sum is order independent, so a
@@ -318,18 +319,17 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6, 8))) v
     __shared__ uint32_t s_surv[kSurv][BLOCK];
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= a.n) return;
-    const int i = int(a.vals[t]);
     const int pf = a.Fs[t];
-    const f3 ds = {a.dshape[3 * i], a.dshape[3 * i + 1], a.dshape[3 * i + 2]};
-    a.dshape[3 * i] = 0.0f; a.dshape[3 * i + 1] = 0.0f; a.dshape[3 * i + 2] = 0.0f;
+    const f3 ds = {a.dshape[3 * t], a.dshape[3 * t + 1], a.dshape[3 * t + 2]};
+    a.dshape[3 * t] = 0.0f; a.dshape[3 * t + 1] = 0.0f; a.dshape[3 * t + 2] = 0.0f;
     const float4 Pi = a.Xs[t], Vi = a.Vs[t];
     if ((pf & 1) == 0) {   // inactive: passthrough (the reference leaves x_out unwritten)
-        a.delta[3 * i] = ds.x; a.delta[3 * i + 1] = ds.y; a.delta[3 * i + 2] = ds.z;
-        a.Po[i] = Pi;
-        a.Vo[i] = Vi;
-        if (a.Xs_next) { a.Xs_next[t] = Pi; a.Vs_next[t] = Vi; }
+        a.delta[3 * t] = ds.x; a.delta[3 * t + 1] = ds.y; a.delta[3 * t + 2] = ds.z;
+        a.Po[t] = Pi;
+        a.Vo[t] = Vi;
         return;
     }
+    const int i = DIAG ? int(a.ids[t]) : 0;
     const int proxy = pf & 2;
     const f3 x = xyz(Pi), v = xyz(Vi);
     const float rad = Pi.w, w1 = Vi.w;
@@ -464,9 +464,9 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6, 8))) v
     const f3 dw = {fma_(acc.x, w1, 0.f), fma_(acc.y, w1, 0.f), fma_(acc.z, w1, 0.f)};
     const f3 tot = {fma_(dw.x, a.relax, 0.f), fma_(dw.y, a.relax, 0.f), fma_(dw.z, a.relax, 0.f)};
     const f3 dl = ds + tot;
-    a.delta[3 * i] = dl.x; a.delta[3 * i + 1] = dl.y; a.delta[3 * i + 2] = dl.z;
+    a.delta[3 * t] = dl.x; a.delta[3 * t + 1] = dl.y; a.delta[3 * t + 2] = dl.z;
     // apply_particle_deltas.comp:44-57
-    const f3 x0 = xyz(a.Ps[i]);
+    const f3 x0 = xyz(a.Ps[a.vals[t]]);
     f3 xn = x + dl;
     f3 vnew = markstein_div(xn - x0, a.dt);
     const float mag = len3(vnew);
@@ -475,9 +475,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6, 8))) v
         xn = x0 + vnew * a.dt;
     }
     const float4 Po = make_float4(xn.x, xn.y, xn.z, rad), Vo = make_float4(vnew.x, vnew.y, vnew.z, w1);
-    a.Po[i] = Po;
-    a.Vo[i] = Vo;
-    if (a.Xs_next) { a.Xs_next[t] = Po; a.Vs_next[t] = Vo; }
+    a.Po[t] = Po;
+    a.Vo[t] = Vo;
 }
 
 // apply_particle_shape_restitution.comp:48-116.  Body previous pose/velocity
@@ -491,9 +490,10 @@ __global__ __launch_bounds__(256) void k_restitution(RestArgs a) {
         if ((a.flags[pi] & 1) == 0) continue;
         float4 Vn = a.V[pi];
         f3 vnew = xyz(Vn);
-        float4 X0 = a.P0[pi];
+        const uint32_t po = a.old[pi];
+        float4 X0 = a.P0[po];
         f3 px = xyz(X0);
-        f3 vold = xyz(a.V0[pi]);
+        f3 vold = xyz(a.V0[po]);
         BodyFrame F = load_body(a.body, body);
         f3 cb = xyz(a.cbpos[t]);
         f3 bx = F.pos + quat_rotate_sc(F.rot, cb);
@@ -551,6 +551,48 @@ void launch_plane_contacts(const PlaneArgs& a, hipStream_t s) {
 }
 void launch_cell_ranges(int n, const uint32_t* keys, int* range, uint32_t* rkeys, hipStream_t s) {
     if (n > 0) hipLaunchKernelGGL(k_cell_ranges, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, keys, range, rkeys);
+}
+// Contact particle indices through a slot map (previous slot -> new slot
+// after a sort, or slot -> particle id when the layout is reset).
+__global__ __launch_bounds__(256) void k_remap_contacts(int* __restrict__ cpart, const int* __restrict__ cc, int cmax,
+                                                        const uint32_t* __restrict__ map) {
+    const int count = min(cmax, *cc);
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < count; t += gridDim.x * blockDim.x)
+        cpart[t] = int(map[cpart[t]]);
+}
+// dst[ids[t]] = src[t].xyz: slot order -> particle-order float3 rows.
+__global__ __launch_bounds__(256) void k_scatter_f3(float* __restrict__ dst, const float4* __restrict__ src,
+                                                    const uint32_t* __restrict__ ids, int n) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const uint32_t i = ids[t];
+    const float4 v = src[t];
+    dst[3 * i] = v.x; dst[3 * i + 1] = v.y; dst[3 * i + 2] = v.z;
+}
+__global__ __launch_bounds__(256) void k_iota(uint32_t* __restrict__ p, int n) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = uint32_t(t);
+}
+// dst[t].xyz = src[ids[t]] (w kept): particle-order float3 rows into slot order.
+__global__ __launch_bounds__(256) void k_gather_f3(float4* __restrict__ dst, const float* __restrict__ src,
+                                                   const uint32_t* __restrict__ ids, int n) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const uint32_t i = ids[t];
+    const float4 o = dst[t];
+    dst[t] = make_float4(src[3 * i], src[3 * i + 1], src[3 * i + 2], o.w);
+}
+void launch_remap_contacts(int* cpart, const int* cc, int cmax, const uint32_t* map, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(k_remap_contacts, dim3(grid), dim3(256), 0, s, cpart, cc, cmax, map);
+}
+void launch_scatter_f3(float* dst, const float4* src, const uint32_t* ids, int n, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_scatter_f3, dim3(blocks_for(n, 256)), dim3(256), 0, s, dst, src, ids, n);
+}
+void launch_iota(uint32_t* p, int n, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_iota, dim3(blocks_for(n, 256)), dim3(256), 0, s, p, n);
+}
+void launch_gather_f3(float4* dst, const float* src, const uint32_t* ids, int n, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_gather_f3, dim3(blocks_for(n, 256)), dim3(256), 0, s, dst, src, ids, n);
 }
 void launch_clear_ranges(int n, const uint32_t* rkeys, int* range, hipStream_t s) {
     if (n > 0) hipLaunchKernelGGL(k_clear_ranges, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, rkeys, range);

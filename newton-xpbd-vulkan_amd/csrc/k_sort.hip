@@ -13,7 +13,8 @@
 //               per key give each lane its same-digit peers), per-wave bin
 //               counters in LDS, an LDS-staged reorder of the tile, then
 //               bin-run-contiguous global stores.
-// The first pass's upsweep is fused into k_integrate, which produces the keys.
+// The input is in particle-id order (k_integrate writes it so), which makes
+// equal keys come out in increasing particle id.
 #include "xpbd_kernels.h"
 
 namespace xpbd {
@@ -70,9 +71,10 @@ __global__ __launch_bounds__(256) void k_sort_binscan(uint32_t* __restrict__ his
     if (threadIdx.x == 0) bintot[b] = carry;
 }
 
-// GATHER (last pass only): also copy each particle's predicted position,
-// velocity and flags into sorted order, so the particle-particle solve reads
-// neighbours as contiguous runs.
+// GATHER (last pass only): the values are slots of the previous order; the
+// particle's predicted position, velocity, flags and id move to the sorted
+// slot (the new order of the state: the particle-particle solve reads
+// neighbours as contiguous runs), and rank records previous slot -> new slot.
 template <bool IDENT, bool GATHER>
 __global__ __launch_bounds__(256) void k_sort_scatter(int n, const uint32_t* __restrict__ keys_in,
                                                       const uint32_t* __restrict__ vals_in,
@@ -159,6 +161,8 @@ __global__ __launch_bounds__(256) void k_sort_scatter(int n, const uint32_t* __r
                 g.Xs[dst] = g.P[v];
                 g.Vs[dst] = g.V[v];
                 g.Fs[dst] = g.flags[v];
+                g.ids_out[dst] = g.ids[v];
+                g.rank[v] = dst;
             }
         }
     }

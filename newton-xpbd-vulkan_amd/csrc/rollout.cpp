@@ -15,11 +15,11 @@ namespace xpbd {
 void Solver::free_tape() {
     Tape& t = tape_;
     auto fv = [](auto& v) { for (auto& b : v) dfree(b); v.clear(); };
-    fv(t.ckP); fv(t.ckV); fv(t.P0); fv(t.V0); fv(t.Xs); fv(t.Vs); fv(t.cbpos); fv(t.cbvel); fv(t.cnorm);
+    fv(t.ckP); fv(t.ckV); fv(t.ckF); fv(t.ckI); fv(t.P0); fv(t.V0); fv(t.Xs); fv(t.Vs); fv(t.cbpos); fv(t.cbvel); fv(t.cnorm);
     fv(t.keys); fv(t.vals); fv(t.Fs); fv(t.cpart); fv(t.cshape); fv(t.delta);
     dfree(t.counts);
     for (int k = 0; k < 2; k++) { dfree(t.lx[k]); dfree(t.lv[k]); }
-    dfree(t.A); dfree(t.As); dfree(t.finP); dfree(t.finV); dfree(t.rank); dfree(t.shx); dfree(t.shv);
+    dfree(t.As); dfree(t.finP); dfree(t.finV); dfree(t.finF); dfree(t.finI); dfree(t.shx); dfree(t.shv);
     dfree(t.relax_part);
     t.valid = false;
 }
@@ -36,19 +36,20 @@ void Solver::ensure_tape() {
     grow(t.keys, n_); grow(t.vals, n_); grow(t.Fs, n_); grow(t.cpart, cmax_); grow(t.cshape, cmax_);
     grow(t.delta, 3 * n_);
     if (t.counts.n < k) dalloc(t.counts, k);
-    if (t.A.n != n_) {
+    if (t.As.n != n_) {
         for (int q = 0; q < 2; q++) { dalloc(t.lx[q], n_); dalloc(t.lv[q], n_); }
-        dalloc(t.A, n_); dalloc(t.As, n_); dalloc(t.finP, n_); dalloc(t.finV, n_);
-        dalloc(t.rank, n_); dalloc(t.shx, 3 * n_); dalloc(t.shv, 3 * n_);
+        dalloc(t.As, n_); dalloc(t.finP, n_); dalloc(t.finV, n_); dalloc(t.finF, n_); dalloc(t.finI, n_);
+        dalloc(t.shx, 3 * n_); dalloc(t.shv, 3 * n_);
     }
 }
 
 // Exchanges the live stage buffers with segment record j: a substep run in
-// between writes its sorted arrays, contacts and deltas into the record.
+// between writes its sorted arrays, contacts and deltas into the record (its
+// sorted flags are copied after it).
 void Solver::swap_tape(int j) {
     Tape& t = tape_;
     std::swap(keysA_, t.keys[j]); std::swap(valsA_, t.vals[j]);
-    std::swap(Xs_[0], t.Xs[j]); std::swap(Vs_[0], t.Vs[j]); std::swap(Fs_, t.Fs[j]);
+    std::swap(Xs_[0], t.Xs[j]); std::swap(Vs_[0], t.Vs[j]);
     std::swap(delta_, t.delta[j]);
     std::swap(cpart_, t.cpart[j]); std::swap(cshape_, t.cshape[j]);
     std::swap(cbpos_, t.cbpos[j]); std::swap(cbvel_, t.cbvel[j]); std::swap(cnorm_, t.cnorm[j]);
@@ -66,8 +67,9 @@ void Solver::rollout_forward(const float* x0, const float* v0, float relax, floa
     DeviceGuard g(device_);
     HIPCHECK(hipStreamSynchronize(stream_));
     const int n = int(n_);
-    if (x0) launch_xyz_in(P_[s_].p, x0, n, stream_);
-    if (v0) launch_xyz_in(V_[s_].p, v0, n, stream_);
+    // the state is in slot order: slot t holds particle ids[t]
+    if (x0) launch_gather_f3(P_[s_].p, x0, ids_cur(), n, stream_);
+    if (v0) launch_gather_f3(V_[s_].p, v0, ids_cur(), n, stream_);
     Tape& t = tape_;
     t.valid = false;
     const int G = (substeps + every - 1) / every;
@@ -78,6 +80,8 @@ void Solver::rollout_forward(const float* x0, const float* v0, float relax, floa
     while (int(t.ckP.size()) < G) {
         t.ckP.emplace_back(); dalloc(t.ckP.back(), n_);
         t.ckV.emplace_back(); dalloc(t.ckV.back(), n_);
+        t.ckF.emplace_back(); dalloc(t.ckF.back(), n_);
+        t.ckI.emplace_back(); dalloc(t.ckI.back(), n_);
     }
     const float keep = relax_;
     relax_ = relax;
@@ -88,6 +92,8 @@ void Solver::rollout_forward(const float* x0, const float* v0, float relax, floa
                 const int c = sub / every;
                 HIPCHECK(hipMemcpyAsync(t.ckP[c].p, P_[s_].p, n_ * sizeof(float4), hipMemcpyDeviceToDevice, stream_));
                 HIPCHECK(hipMemcpyAsync(t.ckV[c].p, V_[s_].p, n_ * sizeof(float4), hipMemcpyDeviceToDevice, stream_));
+                HIPCHECK(hipMemcpyAsync(t.ckF[c].p, flags_cur(), n_ * sizeof(int32_t), hipMemcpyDeviceToDevice, stream_));
+                HIPCHECK(hipMemcpyAsync(t.ckI[c].p, ids_cur(), n_ * sizeof(uint32_t), hipMemcpyDeviceToDevice, stream_));
             }
             substep(sdt, 1, true, sub & 1, true);
         }
@@ -97,8 +103,8 @@ void Solver::rollout_forward(const float* x0, const float* v0, float relax, floa
     }
     relax_ = keep;
     contact_slot_ = rigid_slot_ = (substeps - 1) & 1;
-    if (x_out) launch_xyz_out(x_out, P_[s_].p, n, stream_);
-    if (v_out) launch_xyz_out(v_out, V_[s_].p, n, stream_);
+    if (x_out) launch_scatter_f3(x_out, P_[s_].p, ids_cur(), n, stream_);
+    if (v_out) launch_scatter_f3(v_out, V_[s_].p, ids_cur(), n, stream_);
     read_count(contact_slot_);
     t.valid = true;
 }
@@ -119,14 +125,16 @@ void Solver::rollout_backward(const float* gx, const float* gv, float* gx0, floa
     HIPCHECK(hipMemsetAsync(t.relax_part.p, 0, t.relax_part.n * sizeof(double), stream_));
     HIPCHECK(hipMemsetAsync(t.shx.p, 0, t.shx.n * sizeof(float), stream_));
     HIPCHECK(hipMemsetAsync(t.shv.p, 0, t.shv.n * sizeof(float), stream_));
-    // keep the forward's final state
+    // keep the forward's final state (and its slot order)
     HIPCHECK(hipMemcpyAsync(t.finP.p, P_[s_].p, n_ * sizeof(float4), hipMemcpyDeviceToDevice, stream_));
     HIPCHECK(hipMemcpyAsync(t.finV.p, V_[s_].p, n_ * sizeof(float4), hipMemcpyDeviceToDevice, stream_));
+    HIPCHECK(hipMemcpyAsync(t.finF.p, flags_cur(), n_ * sizeof(int32_t), hipMemcpyDeviceToDevice, stream_));
+    HIPCHECK(hipMemcpyAsync(t.finI.p, ids_cur(), n_ * sizeof(uint32_t), hipMemcpyDeviceToDevice, stream_));
     int cur = 0;
     launch_fill_f4(t.lx[cur].p, n, stream_);
     launch_fill_f4(t.lv[cur].p, n, stream_);
-    launch_xyz_in(t.lx[cur].p, gx, n, stream_);
-    if (gv) launch_xyz_in(t.lv[cur].p, gv, n, stream_);
+    launch_gather_f3(t.lx[cur].p, gx, ids_cur(), n, stream_);   // final cotangents into the final slot order
+    if (gv) launch_gather_f3(t.lv[cur].p, gv, ids_cur(), n, stream_);
     const float keep = relax_;
     relax_ = t.relax;
     const float inv_cell = 1.0f / cfg_.search_radius;
@@ -136,6 +144,8 @@ void Solver::rollout_backward(const float* gx, const float* gv, float* gx0, floa
             // recompute the segment from its checkpoint, recording each substep
             HIPCHECK(hipMemcpyAsync(P_[s_].p, t.ckP[seg].p, n_ * sizeof(float4), hipMemcpyDeviceToDevice, stream_));
             HIPCHECK(hipMemcpyAsync(V_[s_].p, t.ckV[seg].p, n_ * sizeof(float4), hipMemcpyDeviceToDevice, stream_));
+            HIPCHECK(hipMemcpyAsync(flags_cur(), t.ckF[seg].p, n_ * sizeof(int32_t), hipMemcpyDeviceToDevice, stream_));
+            HIPCHECK(hipMemcpyAsync(ids_cur(), t.ckI[seg].p, n_ * sizeof(uint32_t), hipMemcpyDeviceToDevice, stream_));
             launch_frame_begin(cc_.p, stream_);
             for (int j = 0; j < s1 - s0; j++) {
                 HIPCHECK(hipMemcpyAsync(t.P0[j].p, P_[s_].p, n_ * sizeof(float4), hipMemcpyDeviceToDevice, stream_));
@@ -148,6 +158,7 @@ void Solver::rollout_backward(const float* gx, const float* gv, float* gx0, floa
                     throw;
                 }
                 swap_tape(j);
+                HIPCHECK(hipMemcpyAsync(t.Fs[j].p, flags_cur(), n_ * sizeof(int32_t), hipMemcpyDeviceToDevice, stream_));
                 HIPCHECK(hipMemcpyAsync(t.counts.p + j, cc_.p + (j & 1), sizeof(int32_t), hipMemcpyDeviceToDevice,
                                         stream_));
             }
@@ -159,7 +170,7 @@ void Solver::rollout_backward(const float* gx, const float* gv, float* gx0, floa
                 a.n = n; a.keys = t.keys[j].p; a.vals = t.vals[j].p; a.range = range_.p; a.Fs = t.Fs[j].p;
                 a.Xs = t.Xs[j].p; a.Vs = t.Vs[j].p; a.P0 = t.P0[j].p; a.V0 = t.V0[j].p; a.delta = t.delta[j].p;
                 a.lx_in = t.lx[cur].p; a.lv_in = t.lv[cur].p; a.lx_out = t.lx[nxt].p; a.lv_out = t.lv[nxt].p;
-                a.A = t.A.p; a.As = t.As.p; a.rank = t.rank.p; a.shx = t.shx.p; a.shv = t.shv.p;
+                a.As = t.As.p; a.shx = t.shx.p; a.shv = t.shv.p;
                 a.relax_part = t.relax_part.p;
                 a.mu = cfg_.particle_mu; a.coh = cfg_.particle_cohesion; a.max_r = cfg_.particle_max_radius;
                 a.dt = t.sdt; a.relax = t.relax; a.inv_cell = inv_cell; a.vmax = cfg_.particle_v_max;
@@ -170,8 +181,8 @@ void Solver::rollout_backward(const float* gx, const float* gv, float* gx0, floa
                 AdjShapeArgs sa;
                 sa.cmax = int(cmax_); sa.cc = t.counts.p + j;
                 sa.cpart = t.cpart[j].p; sa.cshape = t.cshape[j].p; sa.cbpos = t.cbpos[j].p; sa.cbvel = t.cbvel[j].p;
-                sa.cnorm = t.cnorm[j].p; sa.flags = flags_.p; sa.rank = t.rank.p; sa.Xs = t.Xs[j].p; sa.Vs = t.Vs[j].p;
-                sa.A = t.A.p; sa.body = body_args();
+                sa.cnorm = t.cnorm[j].p; sa.flags = t.Fs[j].p; sa.Xs = t.Xs[j].p; sa.Vs = t.Vs[j].p;
+                sa.As = t.As.p; sa.body = body_args();
                 sa.pmu = cfg_.soft_contact_mu; sa.ka = cfg_.particle_adhesion; sa.dt = t.sdt; sa.relax = t.relax;
                 sa.shx = t.shx.p; sa.shv = t.shv.p; sa.relax_part = t.relax_part.p + pp_grid;
                 launch_adj_shape(sa, shape_grid, stream_);
@@ -185,12 +196,15 @@ void Solver::rollout_backward(const float* gx, const float* gv, float* gx0, floa
         throw;
     }
     relax_ = keep;
-    if (gx0) launch_xyz_out(gx0, t.lx[cur].p, n, stream_);
-    if (gv0) launch_xyz_out(gv0, t.lv[cur].p, n, stream_);
+    // start cotangents are in the slot order of the first checkpoint
+    if (gx0) launch_scatter_f3(gx0, t.lx[cur].p, t.ckI[0].p, n, stream_);
+    if (gv0) launch_scatter_f3(gv0, t.lv[cur].p, t.ckI[0].p, n, stream_);
     std::vector<double> parts(t.relax_part.n);
     HIPCHECK(hipMemcpyAsync(parts.data(), t.relax_part.p, parts.size() * sizeof(double), hipMemcpyDeviceToHost, stream_));
     HIPCHECK(hipMemcpyAsync(P_[s_].p, t.finP.p, n_ * sizeof(float4), hipMemcpyDeviceToDevice, stream_));
     HIPCHECK(hipMemcpyAsync(V_[s_].p, t.finV.p, n_ * sizeof(float4), hipMemcpyDeviceToDevice, stream_));
+    HIPCHECK(hipMemcpyAsync(flags_cur(), t.finF.p, n_ * sizeof(int32_t), hipMemcpyDeviceToDevice, stream_));
+    HIPCHECK(hipMemcpyAsync(ids_cur(), t.finI.p, n_ * sizeof(uint32_t), hipMemcpyDeviceToDevice, stream_));
     HIPCHECK(hipStreamSynchronize(stream_));
     double sum = 0.0;
     for (double v : parts) sum += v;

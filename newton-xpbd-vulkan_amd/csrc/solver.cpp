@@ -50,9 +50,10 @@ void Solver::require_init() const {
 
 void Solver::free_all() {
     for (int k = 0; k < 3; k++) { dfree(P_[k]); dfree(V_[k]); }
-    dfree(flags_); dfree(keysA_); dfree(valsA_); dfree(keysB_); dfree(valsB_); dfree(hist_); dfree(bintot_);
-    for (int k = 0; k < 2; k++) { dfree(Xs_[k]); dfree(Vs_[k]); }
-    dfree(Fs_); dfree(body_half_); dfree(body_I_); dfree(body_invw_); dfree(mat_mu_t_); dfree(mat_mu_r_);
+    for (int k = 0; k < 2; k++) { dfree(flags_[k]); dfree(ids_[k]); }
+    dfree(rank_); dfree(keysA_); dfree(valsA_); dfree(keysB_); dfree(valsB_); dfree(hist_); dfree(bintot_);
+    dfree(Xs_[0]); dfree(Vs_[0]);
+    dfree(body_half_); dfree(body_I_); dfree(body_invw_); dfree(mat_mu_t_); dfree(mat_mu_r_);
     dfree(rc_s0_); dfree(rc_s1_); dfree(rc_p0_); dfree(rc_p1_); dfree(rc_n_); dfree(rc_impulse_);
     dfree(range_); dfree(rkeys_); dfree(tbl_); dfree(cpart_); dfree(cshape_); dfree(cc_); dfree(cbpos_); dfree(cbvel_); dfree(cnorm_);
     dfree(dshape_); dfree(delta_); dfree(body_q_); dfree(body_qd_); dfree(body_com_); dfree(body_invI_);
@@ -71,13 +72,14 @@ void Solver::free_all() {
 void Solver::alloc_all() {
     for (int k = 0; k < 3; k++) { dalloc(P_[k], n_); dalloc(V_[k], n_); }
     s_ = 0; p_ = 1; o_ = 2;
-    dalloc(flags_, n_);
+    for (int k = 0; k < 2; k++) { dalloc(flags_[k], n_); dalloc(ids_[k], n_); }
+    lay_ = 0;
+    dalloc(rank_, n_);
     radius_h_.assign(n_, 0.f);
     invm_h_.assign(n_, 0.f);
     rmax_ = 0.f;
     dalloc(keysA_, n_); dalloc(valsA_, n_); dalloc(keysB_, n_); dalloc(valsB_, n_);
-    for (int k = 0; k < 2; k++) { dalloc(Xs_[k], n_); dalloc(Vs_[k], n_); }
-    dalloc(Fs_, n_);
+    dalloc(Xs_[0], n_); dalloc(Vs_[0], n_);
     const size_t tiles = (n_ + kSortTile - 1) / kSortTile;
     dalloc(hist_, size_t(kSortBins) * std::max<size_t>(tiles, 1));
     dalloc(bintot_, kSortBins);
@@ -94,6 +96,8 @@ void Solver::alloc_all() {
     max_pairs_ = cfg_.max_diag_pairs ? cfg_.max_diag_pairs : 100000u;
     dalloc(diag_pairs_, cfg_.enable_diagnostics ? size_t(20) * max_pairs_ : 20);
     HIPCHECK(hipHostMalloc(reinterpret_cast<void**>(&count_host_), 4 * sizeof(int32_t), hipHostMallocDefault));
+    launch_iota(ids_[0].p, int(n_), stream_);   // slot order starts as particle order
+    HIPCHECK(hipStreamSynchronize(stream_));
 }
 
 void Solver::init_common(const xpbd_config& cfg, size_t n, size_t max_contacts) {
@@ -470,15 +474,20 @@ void Solver::set_particles(const float* q, const float* qd, const float* w, cons
     rmax_ = 0.0f;
     for (size_t i = 0; i < n; i++) rmax_ = std::max(rmax_, std::fabs(radius_h_[i]));
     update_planes();
+    // written into the current slot order: slot t holds particle ids[t]
+    const std::vector<uint32_t> ids = host_ids();
     std::vector<float4> P(n), V(n);
-    for (size_t i = 0; i < n; i++) {
-        P[i] = make_float4(q[3 * i], q[3 * i + 1], q[3 * i + 2], radius_h_[i]);
-        V[i] = qd ? make_float4(qd[3 * i], qd[3 * i + 1], qd[3 * i + 2], invm_h_[i]) : make_float4(0.f, 0.f, 0.f, invm_h_[i]);
+    std::vector<int32_t> F(f ? n : 0);
+    for (size_t t = 0; t < n; t++) {
+        const size_t i = ids[t];
+        P[t] = make_float4(q[3 * i], q[3 * i + 1], q[3 * i + 2], radius_h_[i]);
+        V[t] = qd ? make_float4(qd[3 * i], qd[3 * i + 1], qd[3 * i + 2], invm_h_[i]) : make_float4(0.f, 0.f, 0.f, invm_h_[i]);
+        if (f) F[t] = f[i];
     }
     if (n) {
         HIPCHECK(hipMemcpy(P_[s_].p, P.data(), n * sizeof(float4), hipMemcpyHostToDevice));
         HIPCHECK(hipMemcpy(V_[s_].p, V.data(), n * sizeof(float4), hipMemcpyHostToDevice));
-        if (f) HIPCHECK(hipMemcpy(flags_.p, f, n * sizeof(int32_t), hipMemcpyHostToDevice));
+        if (f) HIPCHECK(hipMemcpy(flags_cur(), F.data(), n * sizeof(int32_t), hipMemcpyHostToDevice));
     }
 }
 
@@ -590,30 +599,31 @@ void Solver::build_ranges(const uint32_t* sorted_keys) {
 // buffers the getters read), so k_integrate writes its keys into A when the
 // pass count is even and into B when it is odd.
 uint32_t* Solver::first_keys() { return (sort_passes_ & 1) ? keysB_.p : keysA_.p; }
+uint32_t* Solver::first_vals() { return (sort_passes_ & 1) ? valsB_.p : valsA_.p; }
 
 void Solver::run_sort(int tiles) {
     const int n = int(n_);
     uint32_t* kb[2] = {keysA_.p, keysB_.p};
     uint32_t* vb[2] = {valsA_.p, valsB_.p};
     int src = (sort_passes_ & 1) ? 1 : 0;
-    const SortGather g{P_[p_].p, V_[p_].p, flags_.p, Xs_[0].p, Vs_[0].p, Fs_.p};
+    const SortGather g{P_[p_].p, V_[p_].p, flags_cur(), ids_cur(), Xs_[0].p, Vs_[0].p,
+                       flags_[lay_ ^ 1].p, ids_[lay_ ^ 1].p, rank_.p};
     for (int pass = 0; pass < sort_passes_; pass++) {
         const int shift = pass * kSortBits;
         const int dst = src ^ 1;
-        if (pass > 0) {   // pass 0's tile histograms came from k_integrate
-            stage_begin(kSortUp);
-            launch_sort_upsweep(n, kb[src], shift, hist_.p, tiles, stream_);
-            stage_end(kSortUp);
-        }
+        stage_begin(kSortUp);
+        launch_sort_upsweep(n, kb[src], shift, hist_.p, tiles, stream_);
+        stage_end(kSortUp);
         stage_begin(kSortScan);
         launch_sort_binscan(hist_.p, bintot_.p, tiles, stream_);
         stage_end(kSortScan);
         stage_begin(kSortScatter);
-        launch_sort_scatter(n, kb[src], pass == 0 ? nullptr : vb[src], kb[dst], vb[dst], shift, hist_.p, bintot_.p,
-                            tiles, pass == sort_passes_ - 1 ? &g : nullptr, stream_);
+        launch_sort_scatter(n, kb[src], vb[src], kb[dst], vb[dst], shift, hist_.p, bintot_.p, tiles,
+                            pass == sort_passes_ - 1 ? &g : nullptr, stream_);
         stage_end(kSortScatter);
         src = dst;
     }
+    lay_ ^= 1;   // flags and ids of the sorted order are current from here on
 }
 
 BodyStepArgs Solver::body_step_args(float sdt) {
@@ -639,7 +649,7 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
     const float inv_cell = 1.0f / cfg_.search_radius;
     if (collide && !meshes_.empty()) {
         MeshArgs m;
-        m.n = n; m.P = P_[s_].p; m.flags = flags_.p;
+        m.n = n; m.P = P_[s_].p; m.flags = flags_cur();
         m.verts = verts_d_.p; m.tris = tris_d_.p; m.nodes = nodes_d_.p; m.cnodes = cnodes_d_.p; m.lbox = lbox_d_.p; m.prim = prim_d_.p;
         m.pg_start = pg_start_.p; m.pg_soup = pg_soup_.p; m.tsoup = tsoup_.p; m.lb = lb_.p; m.lb_node = lb_node_.p;
         m.mq_count = mq_count_.p; m.mq = mq_.p;
@@ -656,7 +666,7 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
     if (collide && nbox > 0) {
         stage_begin(kBodyContacts);
         ParticleBoxArgs pb;
-        pb.n = n; pb.P = P_[s_].p; pb.flags = flags_.p;
+        pb.n = n; pb.P = P_[s_].p; pb.flags = flags_cur();
         pb.nbox = nbox; pb.b0 = M; pb.shape0 = M + 1;
         pb.q = body_q_.p; pb.half = body_half_.p; pb.margin = cfg_.soft_contact_margin;
         pb.cmax = int(cmax_); pb.cc = cc_gen;
@@ -677,8 +687,8 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
     if (n > 0) {
         IntegrateArgs a;
         a.n = n; a.tiles = tiles;
-        a.Ps = P_[s_].p; a.Vs = V_[s_].p; a.flags = flags_.p; a.Pp = P_[p_].p; a.Vp = V_[p_].p;
-        a.keys = first_keys(); a.hist = hist_.p;
+        a.Ps = P_[s_].p; a.Vs = V_[s_].p; a.flags = flags_cur(); a.Pp = P_[p_].p; a.Vp = V_[p_].p;
+        a.ids = ids_cur(); a.keys = first_keys(); a.vals = first_vals();
         a.dt = sdt; a.vmax = cfg_.particle_v_max;
         a.gx = cfg_.gravity[0]; a.gy = cfg_.gravity[1]; a.gz = cfg_.gravity[2];
         a.inv_cell = inv_cell;
@@ -701,21 +711,30 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
         launch_integrate_bodies(bs, stream_);
         stage_end(kBodyIntegrate);
     }
+    const int shape_grid = int(std::max<size_t>(1, std::min<size_t>((cmax_ + 255) / 256, 2048)));
+    int cur = p_, nxt = o_;
     if (n > 0) {
         run_sort(tiles);
         stage_begin(kRanges);
         build_ranges(keysA_.p);
+        // contacts were generated in the previous slot order
+        launch_remap_contacts(cpart_.p, cc_solve, int(cmax_), rank_.p, shape_grid, stream_);
         stage_end(kRanges);
         grid_valid_ = true;
+        if (iterations == 0) {   // no solve: the predicted state still moves to the sorted order
+            HIPCHECK(hipMemcpyAsync(P_[cur].p, Xs_[0].p, n_ * sizeof(float4), hipMemcpyDeviceToDevice, stream_));
+            HIPCHECK(hipMemcpyAsync(V_[cur].p, Vs_[0].p, n_ * sizeof(float4), hipMemcpyDeviceToDevice, stream_));
+        }
     }
-    int cur = p_, nxt = o_;
-    int xs = 0;   // sorted copy holding the current iteration's input
-    const int shape_grid = int(std::max<size_t>(1, std::min<size_t>((cmax_ + 255) / 256, 2048)));
     for (int it = 0; it < iterations; it++) {
+        // iteration input: the gathered predicted state, then the previous
+        // iteration's output (both in sorted order)
+        const float4* Xin = it == 0 ? Xs_[0].p : P_[cur].p;
+        const float4* Vin = it == 0 ? Vs_[0].p : V_[cur].p;
         ShapeArgs sa;
         sa.cmax = int(cmax_); sa.cc = cc_solve;
         sa.cpart = cpart_.p; sa.cshape = cshape_.p; sa.cbpos = cbpos_.p; sa.cbvel = cbvel_.p; sa.cnorm = cnorm_.p;
-        sa.P = P_[cur].p; sa.V = V_[cur].p; sa.flags = flags_.p;
+        sa.P = Xin; sa.V = Vin; sa.flags = flags_cur();
         sa.body = body_args();
         sa.pmu = cfg_.soft_contact_mu; sa.ka = cfg_.particle_adhesion; sa.dt = sdt; sa.relax = relax_;
         sa.dshape = dshape_.p;
@@ -723,12 +742,9 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
         launch_shape_solve(sa, shape_grid, stream_);
         stage_end(kShape);
         PPArgs pa;
-        pa.n = n; pa.keys = keysA_.p; pa.vals = valsA_.p; pa.range = range_.p;
-        pa.Ps = P_[s_].p; pa.Xs = Xs_[xs].p; pa.Vs = Vs_[xs].p; pa.Fs = Fs_.p;
+        pa.n = n; pa.keys = keysA_.p; pa.vals = valsA_.p; pa.ids = ids_cur(); pa.range = range_.p;
+        pa.Ps = P_[s_].p; pa.Xs = Xin; pa.Vs = Vin; pa.Fs = flags_cur();
         pa.dshape = dshape_.p; pa.delta = delta_.p; pa.Po = P_[nxt].p; pa.Vo = V_[nxt].p;
-        const bool more = it + 1 < iterations;
-        pa.Xs_next = more ? Xs_[xs ^ 1].p : nullptr;
-        pa.Vs_next = more ? Vs_[xs ^ 1].p : nullptr;
         pa.mu = cfg_.particle_mu; pa.coh = cfg_.particle_cohesion; pa.max_r = cfg_.particle_max_radius;
         pa.dt = sdt; pa.relax = relax_; pa.inv_cell = inv_cell; pa.vmax = cfg_.particle_v_max;
         pa.rmax = rmax_;
@@ -739,7 +755,6 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
         launch_pp_apply(pa, cfg_.enable_diagnostics != 0, num_scenes_ > 1, pp_block_, stream_);
         stage_end(kPP);
         std::swap(cur, nxt);
-        xs ^= 1;
         if (nbox > 0) {
             BodySolveArgs bv;
             bv.rcmax = 8 * nbox; bv.rcc = rcc_solve;
@@ -761,7 +776,7 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
         RestArgs ra;
         ra.cmax = int(cmax_); ra.cc = cc_solve;
         ra.cpart = cpart_.p; ra.cshape = cshape_.p; ra.cbpos = cbpos_.p; ra.cbvel = cbvel_.p; ra.cnorm = cnorm_.p;
-        ra.P0 = P_[s_].p; ra.V0 = V_[s_].p; ra.flags = flags_.p; ra.V = V_[cur].p;
+        ra.old = valsA_.p; ra.P0 = P_[s_].p; ra.V0 = V_[s_].p; ra.flags = flags_cur(); ra.V = V_[cur].p;
         ra.body = body_args();
         ra.ka = cfg_.particle_adhesion; ra.restitution = cfg_.soft_contact_restitution;
         stage_begin(kRest);
@@ -792,7 +807,8 @@ void Solver::read_count(int slot) {
 
 // ---------------------------------------------------------- frame graphs ----
 // A frame (counter reset + all substeps) is captured once per rotation state of
-// the three particle buffers and per (dt, substeps, iterations), then replayed
+// the three particle buffers and of the flags/ids pair, and per (dt, substeps,
+// iterations), then replayed
 // with one hipGraphLaunch.  Every device pointer and scalar a frame bakes in is
 // fixed between invalidations: the methods that reallocate buffers or change
 // kernel arguments (finalize_meshes, update_mesh, body parameters, init) drop
@@ -806,14 +822,15 @@ void Solver::invalidate_graphs() {
 void Solver::record_frame(float dt, int substeps, int iterations) {
     const float sdt = dt / float(substeps);
     for (const FrameGraph& f : graphs_)
-        if (f.s == s_ && f.p == p_ && f.o == o_ && f.dt == dt && f.substeps == substeps && f.iterations == iterations) {
+        if (f.s == s_ && f.p == p_ && f.o == o_ && f.lay == lay_ && f.dt == dt && f.substeps == substeps &&
+            f.iterations == iterations) {
             HIPCHECK(hipGraphLaunch(f.exec, stream_));
-            s_ = f.s2; p_ = f.p2; o_ = f.o2;
+            s_ = f.s2; p_ = f.p2; o_ = f.o2; lay_ = f.lay2;
             grid_valid_ = true;
             return;
         }
     if (graphs_.size() >= 8) invalidate_graphs();
-    FrameGraph f{s_, p_, o_, dt, substeps, iterations, 0, 0, 0, nullptr};
+    FrameGraph f{s_, p_, o_, dt, substeps, iterations, lay_, 0, 0, 0, 0, nullptr};
     HIPCHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
     hipGraph_t graph = nullptr;
     try {
@@ -822,14 +839,14 @@ void Solver::record_frame(float dt, int substeps, int iterations) {
     } catch (...) {
         (void)hipStreamEndCapture(stream_, &graph);
         if (graph) (void)hipGraphDestroy(graph);
-        s_ = f.s; p_ = f.p; o_ = f.o;
+        s_ = f.s; p_ = f.p; o_ = f.o; lay_ = f.lay;
         throw;
     }
     HIPCHECK(hipStreamEndCapture(stream_, &graph));
     const hipError_t e = hipGraphInstantiate(&f.exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     HIPCHECK(e);
-    f.s2 = s_; f.p2 = p_; f.o2 = o_;
+    f.s2 = s_; f.p2 = p_; f.o2 = o_; f.lay2 = lay_;
     graphs_.push_back(f);
     HIPCHECK(hipGraphLaunch(f.exec, stream_));
 }
@@ -840,11 +857,15 @@ int Solver::autotune(int frames, float dt, int substeps, int iterations) {
     DeviceGuard g(device_);
     if (n_ == 0 || substeps <= 0) return pp_block_;
     frames = std::max(frames, 1);
-    // snapshot of everything a frame advances
+    // snapshot of everything a frame advances (the state and its slot order)
     DevBuf<float4> P0, V0, bq, bqd;
-    dalloc(P0, n_); dalloc(V0, n_);
+    DevBuf<int32_t> F0;
+    DevBuf<uint32_t> I0;
+    dalloc(P0, n_); dalloc(V0, n_); dalloc(F0, n_); dalloc(I0, n_);
     HIPCHECK(hipMemcpyAsync(P0.p, P_[s_].p, n_ * sizeof(float4), hipMemcpyDeviceToDevice, stream_));
     HIPCHECK(hipMemcpyAsync(V0.p, V_[s_].p, n_ * sizeof(float4), hipMemcpyDeviceToDevice, stream_));
+    HIPCHECK(hipMemcpyAsync(F0.p, flags_cur(), n_ * sizeof(int32_t), hipMemcpyDeviceToDevice, stream_));
+    HIPCHECK(hipMemcpyAsync(I0.p, ids_cur(), n_ * sizeof(uint32_t), hipMemcpyDeviceToDevice, stream_));
     const size_t nb = body_q_.n;
     if (nb) {
         dalloc(bq, nb); dalloc(bqd, nb);
@@ -854,6 +875,8 @@ int Solver::autotune(int frames, float dt, int substeps, int iterations) {
     auto restore = [&] {
         HIPCHECK(hipMemcpyAsync(P_[s_].p, P0.p, n_ * sizeof(float4), hipMemcpyDeviceToDevice, stream_));
         HIPCHECK(hipMemcpyAsync(V_[s_].p, V0.p, n_ * sizeof(float4), hipMemcpyDeviceToDevice, stream_));
+        HIPCHECK(hipMemcpyAsync(flags_cur(), F0.p, n_ * sizeof(int32_t), hipMemcpyDeviceToDevice, stream_));
+        HIPCHECK(hipMemcpyAsync(ids_cur(), I0.p, n_ * sizeof(uint32_t), hipMemcpyDeviceToDevice, stream_));
         if (nb) {
             HIPCHECK(hipMemcpyAsync(body_q_.p, bq.p, nb * sizeof(float4), hipMemcpyDeviceToDevice, stream_));
             HIPCHECK(hipMemcpyAsync(body_qd_.p, bqd.p, nb * sizeof(float4), hipMemcpyDeviceToDevice, stream_));
@@ -876,10 +899,10 @@ int Solver::autotune(int frames, float dt, int substeps, int iterations) {
             if (ms < best_ms) { best_ms = ms; best = c; }
         }
     } catch (...) {
-        dfree(P0); dfree(V0); dfree(bq); dfree(bqd);
+        dfree(P0); dfree(V0); dfree(F0); dfree(I0); dfree(bq); dfree(bqd);
         throw;
     }
-    dfree(P0); dfree(V0); dfree(bq); dfree(bqd);
+    dfree(P0); dfree(V0); dfree(F0); dfree(I0); dfree(bq); dfree(bqd);
     pp_block_ = best;
     invalidate_graphs();
     return best;
@@ -934,14 +957,14 @@ void Solver::collide_only() {
     launch_frame_begin(cc_.p, stream_);
     contact_slot_ = 0;
     PlaneArgs pl;
-    pl.n = int(n_); pl.P = P_[s_].p; pl.flags = flags_.p;
+    pl.n = int(n_); pl.P = P_[s_].p; pl.flags = flags_cur();
     pl.margin = cfg_.soft_contact_margin; pl.altitude = cfg_.ground_plane_altitude;
     pl.cmax = int(cmax_); pl.cc = cc_.p;
     pl.cpart = cpart_.p; pl.cshape = cshape_.p; pl.cbpos = cbpos_.p; pl.cbvel = cbvel_.p; pl.cnorm = cnorm_.p;
     if (cfg_.has_ground_plane) launch_plane_contacts(pl, stream_);
     if (!boxes_.empty()) {
         ParticleBoxArgs pb;
-        pb.n = int(n_); pb.P = P_[s_].p; pb.flags = flags_.p;
+        pb.n = int(n_); pb.P = P_[s_].p; pb.flags = flags_cur();
         pb.nbox = int(boxes_.size()); pb.b0 = int(meshes_.size()); pb.shape0 = int(meshes_.size()) + 1;
         pb.q = body_q_.p; pb.half = body_half_.p; pb.margin = cfg_.soft_contact_margin;
         pb.cmax = int(cmax_); pb.cc = cc_.p;
@@ -950,7 +973,7 @@ void Solver::collide_only() {
     }
     if (!meshes_.empty()) {
         MeshArgs m;
-        m.n = int(n_); m.P = P_[s_].p; m.flags = flags_.p;
+        m.n = int(n_); m.P = P_[s_].p; m.flags = flags_cur();
         m.verts = verts_d_.p; m.tris = tris_d_.p; m.nodes = nodes_d_.p; m.cnodes = cnodes_d_.p; m.lbox = lbox_d_.p; m.prim = prim_d_.p;
         m.pg_start = pg_start_.p; m.pg_soup = pg_soup_.p; m.tsoup = tsoup_.p; m.lb = lb_.p; m.lb_node = lb_node_.p;
         m.mq_count = mq_count_.p; m.mq = mq_.p;
@@ -968,23 +991,36 @@ void Solver::collide_only() {
 }
 
 // -------------------------------------------------------------- getters ----
+// Slot -> particle id of the current order (host copy).
+std::vector<uint32_t> Solver::host_ids() {
+    std::vector<uint32_t> ids(n_);
+    if (n_) HIPCHECK(hipMemcpyAsync(ids.data(), ids_cur(), n_ * sizeof(uint32_t), hipMemcpyDeviceToHost, stream_));
+    HIPCHECK(hipStreamSynchronize(stream_));
+    return ids;
+}
+// Particle rows of a slot-order float4 buffer (xyz), in particle-id order.
+static void rows_by_id(const std::vector<float4>& t, const std::vector<uint32_t>& ids, float* out, size_t n) {
+    for (size_t s = 0; s < t.size(); s++) {
+        const size_t i = ids[s];
+        if (i >= n) continue;
+        out[3 * i] = t[s].x; out[3 * i + 1] = t[s].y; out[3 * i + 2] = t[s].z;
+    }
+}
 void Solver::get_positions(float* out, size_t n) {
     require_init();
     DeviceGuard g(device_);
     n = std::min(n, n_);
-    std::vector<float4> t(n);
-    if (n) HIPCHECK(hipMemcpyAsync(t.data(), P_[s_].p, n * sizeof(float4), hipMemcpyDeviceToHost, stream_));
-    HIPCHECK(hipStreamSynchronize(stream_));
-    for (size_t i = 0; i < n; i++) { out[3 * i] = t[i].x; out[3 * i + 1] = t[i].y; out[3 * i + 2] = t[i].z; }
+    std::vector<float4> t(n_);
+    if (n_) HIPCHECK(hipMemcpyAsync(t.data(), P_[s_].p, n_ * sizeof(float4), hipMemcpyDeviceToHost, stream_));
+    rows_by_id(t, host_ids(), out, n);
 }
 void Solver::get_velocities(float* out, size_t n) {
     require_init();
     DeviceGuard g(device_);
     n = std::min(n, n_);
-    std::vector<float4> t(n);
-    if (n) HIPCHECK(hipMemcpyAsync(t.data(), V_[s_].p, n * sizeof(float4), hipMemcpyDeviceToHost, stream_));
-    HIPCHECK(hipStreamSynchronize(stream_));
-    for (size_t i = 0; i < n; i++) { out[3 * i] = t[i].x; out[3 * i + 1] = t[i].y; out[3 * i + 2] = t[i].z; }
+    std::vector<float4> t(n_);
+    if (n_) HIPCHECK(hipMemcpyAsync(t.data(), V_[s_].p, n_ * sizeof(float4), hipMemcpyDeviceToHost, stream_));
+    rows_by_id(t, host_ids(), out, n);
 }
 void Solver::get_contact_ints(bool shapes, int32_t* out, size_t cap) {
     require_init();
@@ -992,6 +1028,11 @@ void Solver::get_contact_ints(bool shapes, int32_t* out, size_t cap) {
     const size_t c = std::min<size_t>(size_t(std::max(last_count_, 0)), cap);
     if (c) HIPCHECK(hipMemcpyAsync(out, shapes ? cshape_.p : cpart_.p, c * sizeof(int32_t), hipMemcpyDeviceToHost, stream_));
     HIPCHECK(hipStreamSynchronize(stream_));
+    if (!shapes && c) {   // contacts name slots of the current order
+        const std::vector<uint32_t> ids = host_ids();
+        for (size_t k = 0; k < c; k++)
+            if (out[k] >= 0 && size_t(out[k]) < n_) out[k] = int32_t(ids[size_t(out[k])]);
+    }
 }
 void Solver::get_contact_vec(int which, float* out, size_t cap) {
     require_init();
@@ -1024,7 +1065,12 @@ void Solver::set_contact_data(const int32_t* p, const int32_t* s, const float* b
             b[i] = make_float4(bp[3 * i], bp[3 * i + 1], bp[3 * i + 2], 0.f);
             nn[i] = make_float4(nrm[3 * i], nrm[3 * i + 1], nrm[3 * i + 2], 0.f);
         }
-        HIPCHECK(hipMemcpy(cpart_.p, p, c * sizeof(int32_t), hipMemcpyHostToDevice));
+        // particle ids -> slots of the current order
+        const std::vector<uint32_t> ids = host_ids();
+        std::vector<int32_t> slot_of(n_), cp(c);
+        for (size_t t = 0; t < n_; t++) slot_of[ids[t]] = int32_t(t);
+        for (size_t i = 0; i < c; i++) cp[i] = slot_of[size_t(p[i])];
+        HIPCHECK(hipMemcpy(cpart_.p, cp.data(), c * sizeof(int32_t), hipMemcpyHostToDevice));
         HIPCHECK(hipMemcpy(cshape_.p, s, c * sizeof(int32_t), hipMemcpyHostToDevice));
         HIPCHECK(hipMemcpy(cbpos_.p, b.data(), c * sizeof(float4), hipMemcpyHostToDevice));
         HIPCHECK(hipMemcpy(cnorm_.p, nn.data(), c * sizeof(float4), hipMemcpyHostToDevice));
@@ -1038,7 +1084,7 @@ std::vector<uint64_t> Solver::reference_sorted() {
     std::vector<uint32_t> k(n_), v(n_);
     if (n_) {
         HIPCHECK(hipMemcpyAsync(k.data(), keysA_.p, n_ * sizeof(uint32_t), hipMemcpyDeviceToHost, stream_));
-        HIPCHECK(hipMemcpyAsync(v.data(), valsA_.p, n_ * sizeof(uint32_t), hipMemcpyDeviceToHost, stream_));
+        HIPCHECK(hipMemcpyAsync(v.data(), ids_cur(), n_ * sizeof(uint32_t), hipMemcpyDeviceToHost, stream_));
     }
     HIPCHECK(hipStreamSynchronize(stream_));
     std::vector<uint64_t> kv(n_);
@@ -1056,7 +1102,8 @@ void Solver::get_sorted(bool ids, uint32_t* out, size_t n) {
         for (size_t t = 0; t < n; t++) out[t] = ids ? uint32_t(kv[t]) : uint32_t(kv[t] >> 32);
         return;
     }
-    if (n) HIPCHECK(hipMemcpyAsync(out, ids ? valsA_.p : keysA_.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost, stream_));
+    // after a sort the current slot order is the sorted order: its ids are the sorted ids
+    if (n) HIPCHECK(hipMemcpyAsync(out, ids ? ids_cur() : keysA_.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost, stream_));
     HIPCHECK(hipStreamSynchronize(stream_));
 }
 void Solver::get_cell_table(bool ends, int32_t* out, size_t n) {
@@ -1087,8 +1134,13 @@ void Solver::get_deltas(float* out, size_t n) {
     require_init();
     DeviceGuard g(device_);
     n = std::min(n, n_);
-    if (n) HIPCHECK(hipMemcpyAsync(out, delta_.p, 3 * n * sizeof(float), hipMemcpyDeviceToHost, stream_));
-    HIPCHECK(hipStreamSynchronize(stream_));
+    std::vector<float> d(3 * n_);
+    if (n_) HIPCHECK(hipMemcpyAsync(d.data(), delta_.p, 3 * n_ * sizeof(float), hipMemcpyDeviceToHost, stream_));
+    const std::vector<uint32_t> ids = host_ids();   // deltas are in sorted (current) order
+    for (size_t t = 0; t < n_; t++) {
+        const size_t i = ids[t];
+        if (i < n) { out[3 * i] = d[3 * t]; out[3 * i + 1] = d[3 * t + 1]; out[3 * i + 2] = d[3 * t + 2]; }
+    }
 }
 uint32_t Solver::diag_word(int k) {
     if (!initialized_) return 0;

@@ -128,6 +128,7 @@ private:
     BodyStepArgs body_step_args(float sdt);
     void run_sort(int tiles);
     uint32_t* first_keys();
+    uint32_t* first_vals();
     void update_planes();
     std::vector<uint64_t> reference_sorted();
     void build_ranges(const uint32_t* sorted_keys);
@@ -155,7 +156,8 @@ private:
         int s, p, o;
         float dt;
         int substeps, iterations;
-        int s2, p2, o2;
+        int lay;
+        int s2, p2, o2, lay2;
         hipGraphExec_t exec;
     };
     std::vector<FrameGraph> graphs_;
@@ -166,14 +168,17 @@ private:
         bool valid = false;
         float sdt = 0.f, relax = 0.f;
         int substeps = 0, every = 0;
-        std::vector<DevBuf<float4>> ckP, ckV;
+        std::vector<DevBuf<float4>> ckP, ckV;      // checkpoints: state, flags and ids (slot order)
+        std::vector<DevBuf<int32_t>> ckF;
+        std::vector<DevBuf<uint32_t>> ckI;
         std::vector<DevBuf<float4>> P0, V0, Xs, Vs, cbpos, cbvel, cnorm;
         std::vector<DevBuf<uint32_t>> keys, vals;
         std::vector<DevBuf<int32_t>> Fs, cpart, cshape;
         std::vector<DevBuf<float>> delta;
         DevBuf<int32_t> counts;
-        DevBuf<float4> lx[2], lv[2], A, As, finP, finV;
-        DevBuf<uint32_t> rank;
+        DevBuf<float4> lx[2], lv[2], As, finP, finV;
+        DevBuf<int32_t> finF;
+        DevBuf<uint32_t> finI;
         DevBuf<float> shx, shv;
         DevBuf<double> relax_part;
     } tape_;
@@ -181,13 +186,22 @@ private:
     // particle state: 3 rotating float4 buffers each (state / predicted / out)
     DevBuf<float4> P_[3], V_[3];
     int s_ = 0, p_ = 1, o_ = 2;
-    DevBuf<int32_t> flags_;
+    // The particle state lives in slot order: after every substep's sort, the
+    // slots follow that sort (neighbours adjacent, writes coalesced).  ids_
+    // maps slot -> particle id; flags_ and ids_ are double-buffered (lay_ is
+    // current) because the last radix pass writes the next order's copies.
+    DevBuf<int32_t> flags_[2];
+    DevBuf<uint32_t> ids_[2];
+    int lay_ = 0;
+    DevBuf<uint32_t> rank_;    // previous slot -> sorted slot, from the last sort
+    int32_t* flags_cur() { return flags_[lay_].p; }
+    uint32_t* ids_cur() { return ids_[lay_].p; }
+    std::vector<uint32_t> host_ids();
     std::vector<float> radius_h_, invm_h_;
     float rmax_ = 0.f;   // largest particle radius (host copy of the radii)
     // grid and sort
     DevBuf<uint32_t> keysA_, valsA_, keysB_, valsB_, hist_, bintot_;
-    DevBuf<float4> Xs_[2], Vs_[2];   // predicted state in sorted order (ping-pong across iterations)
-    DevBuf<int32_t> Fs_;
+    DevBuf<float4> Xs_[1], Vs_[1];   // predicted state in sorted order (the first iteration's input)
     DevBuf<int32_t> range_;   // int2 per key (start, end); rows of the last build are reset before the next
     DevBuf<uint32_t> rkeys_;  // with wrap planes: the sorted keys whose rows range_ holds
     DevBuf<int32_t> tbl_;     // 2 x cells scratch for the dense getters

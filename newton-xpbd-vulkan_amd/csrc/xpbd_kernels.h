@@ -16,9 +16,12 @@ constexpr int kSortTile = 256 * kSortItems;
 
 // Sorted-order copies of the predicted particle state, written by the last
 // radix pass (positions with radius, velocities with inverse mass, flags).
+// Last radix pass: the particle state moves into the new sorted order.  The
+// sort values are slots of the previous order; ids follow their particles and
+// rank maps each previous slot to its new one.
 struct SortGather {
-    const float4* P; const float4* V; const int* flags;
-    float4* Xs; float4* Vs; int* Fs;
+    const float4* P; const float4* V; const int* flags; const uint32_t* ids;
+    float4* Xs; float4* Vs; int* Fs; uint32_t* ids_out; uint32_t* rank;
 };
 
 struct BodyArgs {
@@ -35,9 +38,10 @@ struct BodyArgs {
 
 struct IntegrateArgs {
     int n, tiles;
-    const float4* Ps; const float4* Vs; const int* flags;
+    const float4* Ps; const float4* Vs; const int* flags;   // slot order
     float4* Pp; float4* Vp;
-    uint32_t* keys; uint32_t* hist;
+    const uint32_t* ids;                // particle id of each slot
+    uint32_t* keys; uint32_t* vals;     // radix input in particle-id order: cell key, slot
     float dt, vmax, gx, gy, gz, inv_cell;
     int has_plane; float margin, altitude;
     int cmax; int* cc; int* cc_next; int* rcc_next; int* mesh_next;
@@ -81,12 +85,13 @@ struct ShapeArgs {
 struct PPArgs {
     int n;
     const uint32_t* keys;   // sorted keys (batched mode: scene << 21 | cell)
-    const uint32_t* vals; const int* range;
-    const float4* Ps;                               // substep-start positions, original order
-    const float4* Xs; const float4* Vs; const int* Fs;   // predicted state, sorted order
-    float* dshape; float* delta;
-    float4* Po; float4* Vo;                         // output state, original order
-    float4* Xs_next; float4* Vs_next;               // output state, sorted order (when another iteration follows)
+    const uint32_t* vals;   // sorted slot -> slot of the substep-start order
+    const uint32_t* ids;    // sorted slot -> particle id
+    const int* range;
+    const float4* Ps;                               // substep-start positions (substep-start order)
+    const float4* Xs; const float4* Vs; const int* Fs;   // iteration input state, sorted order
+    float* dshape; float* delta;                    // sorted order
+    float4* Po; float4* Vo;                         // output state, sorted order
     float mu, coh, max_r, dt, relax, inv_cell, vmax;
     float rmax;                                     // largest particle radius (bounds every accepted pair)
     int planes;                                     // keys carry the wrap plane above the cell
@@ -97,6 +102,7 @@ struct PPArgs {
 struct RestArgs {
     int cmax; const int* cc;
     const int* cpart; const int* cshape; const float4* cbpos; const float4* cbvel; const float4* cnorm;
+    const uint32_t* old;                    // sorted slot -> substep-start slot (P0, V0)
     const float4* P0; const float4* V0; const int* flags;
     float4* V;
     BodyArgs body;
@@ -187,11 +193,11 @@ struct AdjArgs {
     int n;
     const uint32_t* keys; const uint32_t* vals; const int* range; const int* Fs;
     const float4* Xs; const float4* Vs;     // sorted predicted state (w: radius, inverse mass)
-    const float4* P0; const float4* V0;     // substep-start state, particle order
-    const float* delta;                     // final deltas, particle order
-    const float4* lx_in; const float4* lv_in;
-    float4* lx_out; float4* lv_out;
-    float4* A; float4* As; uint32_t* rank;
+    const float4* P0; const float4* V0;     // substep-start state, previous order (slot vals[t])
+    const float* delta;                     // final deltas, sorted order
+    const float4* lx_in; const float4* lv_in;   // end-state cotangents, sorted order
+    float4* lx_out; float4* lv_out;             // start-state cotangents, previous order
+    float4* As;                             // apply cotangent per sorted slot
     float* shx; float* shv;
     double* relax_part;
     float mu, coh, max_r, dt, relax, inv_cell, vmax, gx, gy, gz;
@@ -201,7 +207,7 @@ struct AdjArgs {
 struct AdjShapeArgs {
     int cmax; const int* cc;
     const int* cpart; const int* cshape; const float4* cbpos; const float4* cbvel; const float4* cnorm;
-    const int* flags; const uint32_t* rank; const float4* Xs; const float4* Vs; const float4* A;
+    const int* flags; const float4* Xs; const float4* Vs; const float4* As;   // sorted order
     BodyArgs body;
     float pmu, ka, dt, relax;
     float* shx; float* shv; double* relax_part;
@@ -224,6 +230,10 @@ void launch_integrate(const IntegrateArgs& a, hipStream_t s);
 void launch_plane_contacts(const PlaneArgs& a, hipStream_t s);
 void launch_cell_ranges(int n, const uint32_t* keys, int* range, uint32_t* rkeys, hipStream_t s);
 void launch_clear_ranges(int n, const uint32_t* rkeys, int* range, hipStream_t s);
+void launch_remap_contacts(int* cpart, const int* cc, int cmax, const uint32_t* map, int grid, hipStream_t s);
+void launch_scatter_f3(float* dst, const float4* src, const uint32_t* ids, int n, hipStream_t s);
+void launch_iota(uint32_t* p, int n, hipStream_t s);
+void launch_gather_f3(float4* dst, const float* src, const uint32_t* ids, int n, hipStream_t s);
 void launch_cell_tables(int n, const uint32_t* keys, int* starts, int* ends, hipStream_t s);
 void launch_shape_solve(const ShapeArgs& a, int grid, hipStream_t s);
 // validate: the cell table is not cleared (batched mode); an entry is used
