@@ -107,9 +107,7 @@ __global__ __launch_bounds__(256) void k_integrate(IntegrateArgs a) {
         uint32_t key = cell_of(Pn.x, Pn.y, Pn.z, a.inv_cell);
         if (a.scene != nullptr) key |= uint32_t(a.scene[idx]) << kCellBits;   // scenes keep their slot ranges
         else if (a.planes) key |= plane_of(Pn.x, Pn.y, Pn.z, a.inv_cell) << kCellBits;
-        const uint32_t id = a.ids[idx];
-        a.keys[id] = key;
-        a.vals[id] = uint32_t(idx);
+        a.kv[a.ids[idx]] = make_uint2(key, uint32_t(idx));
     }
 }
 

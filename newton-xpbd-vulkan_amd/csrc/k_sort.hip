@@ -13,8 +13,8 @@
 //               per key give each lane its same-digit peers), per-wave bin
 //               counters in LDS, an LDS-staged reorder of the tile, then
 //               bin-run-contiguous global stores.
-// The input is in particle-id order (k_integrate writes it so), which makes
-// equal keys come out in increasing particle id.
+// The first pass reads (key, slot) pairs in particle-id order (k_integrate
+// writes them so), which makes equal keys come out in increasing particle id.
 #include "xpbd_kernels.h"
 
 namespace xpbd {
@@ -29,7 +29,9 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
     return x;
 }
 
-__global__ __launch_bounds__(256) void k_sort_upsweep(int n, const uint32_t* __restrict__ keys, int shift,
+// keys, or the x of (key, value) pairs when pairs is non-null (the first pass)
+__global__ __launch_bounds__(256) void k_sort_upsweep(int n, const uint32_t* __restrict__ keys,
+                                                      const uint2* __restrict__ pairs, int shift,
                                                       uint32_t* __restrict__ hist, int tiles) {
     __shared__ uint32_t h[kSortBins];
     const int tile = blockIdx.x, w = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -38,7 +40,7 @@ __global__ __launch_bounds__(256) void k_sort_upsweep(int n, const uint32_t* __r
 #pragma unroll
     for (int it = 0; it < kSortItems; it++) {
         int idx = tile * kSortTile + w * (kSortItems * kWave) + it * kWave + lane;
-        if (idx < n) atomicAdd(&h[(keys[idx] >> shift) & (kSortBins - 1)], 1u);
+        if (idx < n) atomicAdd(&h[((pairs ? pairs[idx].x : keys[idx]) >> shift) & (kSortBins - 1)], 1u);
     }
     __syncthreads();
     if (threadIdx.x < kSortBins) hist[threadIdx.x * tiles + tile] = h[threadIdx.x];
@@ -75,9 +77,10 @@ __global__ __launch_bounds__(256) void k_sort_binscan(uint32_t* __restrict__ his
 // particle's predicted position, velocity, flags and id move to the sorted
 // slot (the new order of the state: the particle-particle solve reads
 // neighbours as contiguous runs), and rank records previous slot -> new slot.
-template <bool IDENT, bool GATHER>
+template <bool PAIRS, bool GATHER>
 __global__ __launch_bounds__(256) void k_sort_scatter(int n, const uint32_t* __restrict__ keys_in,
                                                       const uint32_t* __restrict__ vals_in,
+                                                      const uint2* __restrict__ pairs_in,
                                                       uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out,
                                                       int shift, const uint32_t* __restrict__ hist,
                                                       const uint32_t* __restrict__ bintot, int tiles, SortGather g) {
@@ -97,8 +100,14 @@ __global__ __launch_bounds__(256) void k_sort_scatter(int n, const uint32_t* __r
     for (int it = 0; it < kSortItems; it++) {
         const int idx = tile * kSortTile + w * (kSortItems * kWave) + it * kWave + lane;
         const bool valid = idx < n;
-        key[it] = valid ? keys_in[idx] : 0xFFFFFFFFu;
-        val[it] = valid ? (IDENT ? uint32_t(idx) : vals_in[idx]) : 0u;
+        if (PAIRS) {
+            const uint2 kv = valid ? pairs_in[idx] : make_uint2(0xFFFFFFFFu, 0u);
+            key[it] = kv.x;
+            val[it] = kv.y;
+        } else {
+            key[it] = valid ? keys_in[idx] : 0xFFFFFFFFu;
+            val[it] = valid ? vals_in[idx] : 0u;
+        }
         const uint32_t d = (key[it] >> shift) & (kSortBins - 1);
         uint64_t peers = __ballot(valid);
 #pragma unroll
@@ -168,30 +177,31 @@ __global__ __launch_bounds__(256) void k_sort_scatter(int n, const uint32_t* __r
     }
 }
 
-void launch_sort_upsweep(int n, const uint32_t* keys, int shift, uint32_t* hist, int tiles, hipStream_t s) {
-    hipLaunchKernelGGL(k_sort_upsweep, dim3(tiles), dim3(256), 0, s, n, keys, shift, hist, tiles);
+void launch_sort_upsweep(int n, const uint32_t* keys, const uint2* pairs, int shift, uint32_t* hist, int tiles,
+                         hipStream_t s) {
+    hipLaunchKernelGGL(k_sort_upsweep, dim3(tiles), dim3(256), 0, s, n, keys, pairs, shift, hist, tiles);
 }
 void launch_sort_binscan(uint32_t* hist, uint32_t* bintot, int tiles, hipStream_t s) {
     hipLaunchKernelGGL(k_sort_binscan, dim3(kSortBins), dim3(256), 0, s, hist, bintot, tiles);
 }
-void launch_sort_scatter(int n, const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out, uint32_t* vals_out,
-                         int shift, const uint32_t* hist, const uint32_t* bintot, int tiles, const SortGather* gather,
-                         hipStream_t s) {
+void launch_sort_scatter(int n, const uint32_t* keys_in, const uint32_t* vals_in, const uint2* pairs_in,
+                         uint32_t* keys_out, uint32_t* vals_out, int shift, const uint32_t* hist, const uint32_t* bintot,
+                         int tiles, const SortGather* gather, hipStream_t s) {
     const SortGather g = gather ? *gather : SortGather{};
-    if (vals_in == nullptr) {
+    if (pairs_in) {
         if (gather)
-            hipLaunchKernelGGL((k_sort_scatter<true, true>), dim3(tiles), dim3(256), 0, s, n, keys_in, vals_in, keys_out,
-                               vals_out, shift, hist, bintot, tiles, g);
+            hipLaunchKernelGGL((k_sort_scatter<true, true>), dim3(tiles), dim3(256), 0, s, n, keys_in, vals_in, pairs_in,
+                               keys_out, vals_out, shift, hist, bintot, tiles, g);
         else
-            hipLaunchKernelGGL((k_sort_scatter<true, false>), dim3(tiles), dim3(256), 0, s, n, keys_in, vals_in, keys_out,
-                               vals_out, shift, hist, bintot, tiles, g);
+            hipLaunchKernelGGL((k_sort_scatter<true, false>), dim3(tiles), dim3(256), 0, s, n, keys_in, vals_in, pairs_in,
+                               keys_out, vals_out, shift, hist, bintot, tiles, g);
     } else {
         if (gather)
-            hipLaunchKernelGGL((k_sort_scatter<false, true>), dim3(tiles), dim3(256), 0, s, n, keys_in, vals_in, keys_out,
-                               vals_out, shift, hist, bintot, tiles, g);
+            hipLaunchKernelGGL((k_sort_scatter<false, true>), dim3(tiles), dim3(256), 0, s, n, keys_in, vals_in, pairs_in,
+                               keys_out, vals_out, shift, hist, bintot, tiles, g);
         else
-            hipLaunchKernelGGL((k_sort_scatter<false, false>), dim3(tiles), dim3(256), 0, s, n, keys_in, vals_in, keys_out,
-                               vals_out, shift, hist, bintot, tiles, g);
+            hipLaunchKernelGGL((k_sort_scatter<false, false>), dim3(tiles), dim3(256), 0, s, n, keys_in, vals_in, pairs_in,
+                               keys_out, vals_out, shift, hist, bintot, tiles, g);
     }
 }
 

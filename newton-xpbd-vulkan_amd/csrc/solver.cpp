@@ -51,7 +51,7 @@ void Solver::require_init() const {
 void Solver::free_all() {
     for (int k = 0; k < 3; k++) { dfree(P_[k]); dfree(V_[k]); }
     for (int k = 0; k < 2; k++) { dfree(flags_[k]); dfree(ids_[k]); }
-    dfree(rank_); dfree(keysA_); dfree(valsA_); dfree(keysB_); dfree(valsB_); dfree(hist_); dfree(bintot_);
+    dfree(rank_); dfree(kv_); dfree(keysA_); dfree(valsA_); dfree(keysB_); dfree(valsB_); dfree(hist_); dfree(bintot_);
     dfree(Xs_[0]); dfree(Vs_[0]);
     dfree(body_half_); dfree(body_I_); dfree(body_invw_); dfree(mat_mu_t_); dfree(mat_mu_r_);
     dfree(rc_s0_); dfree(rc_s1_); dfree(rc_p0_); dfree(rc_p1_); dfree(rc_n_); dfree(rc_impulse_);
@@ -75,6 +75,7 @@ void Solver::alloc_all() {
     for (int k = 0; k < 2; k++) { dalloc(flags_[k], n_); dalloc(ids_[k], n_); }
     lay_ = 0;
     dalloc(rank_, n_);
+    dalloc(kv_, n_);
     radius_h_.assign(n_, 0.f);
     invm_h_.assign(n_, 0.f);
     rmax_ = 0.f;
@@ -595,33 +596,33 @@ void Solver::build_ranges(const uint32_t* sorted_keys) {
 }
 
 // ----------------------------------------------------------------- step ----
-// Key buffers alternate between passes and the result must land in A (the
-// buffers the getters read), so k_integrate writes its keys into A when the
-// pass count is even and into B when it is odd.
-uint32_t* Solver::first_keys() { return (sort_passes_ & 1) ? keysB_.p : keysA_.p; }
-uint32_t* Solver::first_vals() { return (sort_passes_ & 1) ? valsB_.p : valsA_.p; }
 
 void Solver::run_sort(int tiles) {
     const int n = int(n_);
     uint32_t* kb[2] = {keysA_.p, keysB_.p};
     uint32_t* vb[2] = {valsA_.p, valsB_.p};
-    int src = (sort_passes_ & 1) ? 1 : 0;
+    // pass 0 reads k_integrate's (key, slot) pairs; the passes then alternate
+    // between the A and B buffers and the last one must land in A (the
+    // buffers the getters read)
+    int dst = (sort_passes_ & 1) ? 0 : 1;
     const SortGather g{P_[p_].p, V_[p_].p, flags_cur(), ids_cur(), Xs_[0].p, Vs_[0].p,
                        flags_[lay_ ^ 1].p, ids_[lay_ ^ 1].p, rank_.p};
     for (int pass = 0; pass < sort_passes_; pass++) {
         const int shift = pass * kSortBits;
-        const int dst = src ^ 1;
+        const uint32_t* kin = pass == 0 ? nullptr : kb[dst ^ 1];
+        const uint32_t* vin = pass == 0 ? nullptr : vb[dst ^ 1];
+        const uint2* pin = pass == 0 ? kv_.p : nullptr;
         stage_begin(kSortUp);
-        launch_sort_upsweep(n, kb[src], shift, hist_.p, tiles, stream_);
+        launch_sort_upsweep(n, kin, pin, shift, hist_.p, tiles, stream_);
         stage_end(kSortUp);
         stage_begin(kSortScan);
         launch_sort_binscan(hist_.p, bintot_.p, tiles, stream_);
         stage_end(kSortScan);
         stage_begin(kSortScatter);
-        launch_sort_scatter(n, kb[src], vb[src], kb[dst], vb[dst], shift, hist_.p, bintot_.p, tiles,
+        launch_sort_scatter(n, kin, vin, pin, kb[dst], vb[dst], shift, hist_.p, bintot_.p, tiles,
                             pass == sort_passes_ - 1 ? &g : nullptr, stream_);
         stage_end(kSortScatter);
-        src = dst;
+        dst ^= 1;
     }
     lay_ ^= 1;   // flags and ids of the sorted order are current from here on
 }
@@ -688,7 +689,7 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
         IntegrateArgs a;
         a.n = n; a.tiles = tiles;
         a.Ps = P_[s_].p; a.Vs = V_[s_].p; a.flags = flags_cur(); a.Pp = P_[p_].p; a.Vp = V_[p_].p;
-        a.ids = ids_cur(); a.keys = first_keys(); a.vals = first_vals();
+        a.ids = ids_cur(); a.kv = kv_.p;
         a.dt = sdt; a.vmax = cfg_.particle_v_max;
         a.gx = cfg_.gravity[0]; a.gy = cfg_.gravity[1]; a.gz = cfg_.gravity[2];
         a.inv_cell = inv_cell;

@@ -127,8 +127,6 @@ private:
     void substep(float sdt, int iterations, bool collide, int slot, bool zero_next);
     BodyStepArgs body_step_args(float sdt);
     void run_sort(int tiles);
-    uint32_t* first_keys();
-    uint32_t* first_vals();
     void update_planes();
     std::vector<uint64_t> reference_sorted();
     void build_ranges(const uint32_t* sorted_keys);
@@ -194,6 +192,7 @@ private:
     DevBuf<uint32_t> ids_[2];
     int lay_ = 0;
     DevBuf<uint32_t> rank_;    // previous slot -> sorted slot, from the last sort
+    DevBuf<uint2> kv_;         // radix input: (cell key, slot) at each particle id
     int32_t* flags_cur() { return flags_[lay_].p; }
     uint32_t* ids_cur() { return ids_[lay_].p; }
     std::vector<uint32_t> host_ids();

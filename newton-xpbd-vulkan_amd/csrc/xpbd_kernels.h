@@ -41,7 +41,7 @@ struct IntegrateArgs {
     const float4* Ps; const float4* Vs; const int* flags;   // slot order
     float4* Pp; float4* Vp;
     const uint32_t* ids;                // particle id of each slot
-    uint32_t* keys; uint32_t* vals;     // radix input in particle-id order: cell key, slot
+    uint2* kv;                          // radix input in particle-id order: (cell key, slot)
     float dt, vmax, gx, gy, gz, inv_cell;
     int has_plane; float margin, altitude;
     int cmax; int* cc; int* cc_next; int* rcc_next; int* mesh_next;
@@ -245,11 +245,12 @@ void launch_frame_begin(int* cc, hipStream_t s);
 void launch_mesh_contacts(const MeshArgs& a, hipStream_t s);
 
 // Radix sort: one pass = upsweep (tile histograms) -> bin-row scan -> scatter.
-void launch_sort_upsweep(int n, const uint32_t* keys, int shift, uint32_t* hist, int tiles, hipStream_t s);
-void launch_sort_binscan(uint32_t* hist, uint32_t* bintot, int tiles, hipStream_t s);
-void launch_sort_scatter(int n, const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out, uint32_t* vals_out,
-                         int shift, const uint32_t* hist, const uint32_t* bintot, int tiles, const SortGather* gather,
+void launch_sort_upsweep(int n, const uint32_t* keys, const uint2* pairs, int shift, uint32_t* hist, int tiles,
                          hipStream_t s);
+void launch_sort_binscan(uint32_t* hist, uint32_t* bintot, int tiles, hipStream_t s);
+void launch_sort_scatter(int n, const uint32_t* keys_in, const uint32_t* vals_in, const uint2* pairs_in,
+                         uint32_t* keys_out, uint32_t* vals_out, int shift, const uint32_t* hist, const uint32_t* bintot,
+                         int tiles, const SortGather* gather, hipStream_t s);
 void launch_fill_zero(int* p, size_t count, hipStream_t s);
 
 }  // namespace xpbd
