@@ -1,17 +1,19 @@
-// k_sort.hip -- stable LSD radix sort of (cell key, particle id) pairs for
-// gfx950: the replacement of sort_histogram / sort_scan_a/b/c / sort_scatter
+// k_sort.hip -- stable LSD radix sort of (cell key, slot) pairs for gfx950:
+// the replacement of sort_histogram / sort_scan_a/b/c / sort_scatter
 // (shaders/sort_*.comp, driven from src/gpu_sort.h:110-129 and
-// src/scene.cpp:885-914).  Same contract: a stable sort by key, so equal cells
-// keep increasing particle ids; the result lands in the "A" pair.
+// src/scene.cpp:885-914).  Same contract: a stable sort by key, so equal
+// cells keep increasing particle ids; the result lands in the "A" pair.
 //
-// Per 7-bit pass (three passes cover the 21-bit cell id):
-//   upsweep  -- per-tile 128-bin histogram in LDS, written bin-major
-//               hist[bin][tile] (no global atomics);
+// Per pass of BITS-wide digits (7 to 10 bits, 2^BITS bins; the host picks
+// the fewest passes of at most 10 bits that cover the key, e.g. 3 x 7 for the
+// 21-bit cell id, 3 x 9 with wrap planes):
+//   upsweep  -- per-tile histogram in LDS, written bin-major hist[bin][tile]
+//               (no global atomics);
 //   binscan  -- one workgroup per bin scans its row over tiles in place and
 //               writes the bin total;
-//   scatter  -- stable in-tile ranks from wave64 ballot matching (7 ballots
-//               per key give each lane its same-digit peers), per-wave bin
-//               counters in LDS, an LDS-staged reorder of the tile, then
+//   scatter  -- stable in-tile ranks from wave64 ballot matching (BITS
+//               ballots per key give each lane its same-digit peers), per-wave
+//               bin counters in LDS, an LDS-staged reorder of the tile, then
 //               bin-run-contiguous global stores.
 // The first pass reads (key, slot) pairs in particle-id order (k_integrate
 // writes them so), which makes equal keys come out in increasing particle id.
@@ -30,20 +32,22 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
 }
 
 // keys, or the x of (key, value) pairs when pairs is non-null (the first pass)
+template <int BITS>
 __global__ __launch_bounds__(256) void k_sort_upsweep(int n, const uint32_t* __restrict__ keys,
                                                       const uint2* __restrict__ pairs, int shift,
                                                       uint32_t* __restrict__ hist, int tiles) {
-    __shared__ uint32_t h[kSortBins];
+    constexpr int B = 1 << BITS;
+    __shared__ uint32_t h[B];
     const int tile = blockIdx.x, w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    if (threadIdx.x < kSortBins) h[threadIdx.x] = 0;
+    for (int b = threadIdx.x; b < B; b += 256) h[b] = 0;
     __syncthreads();
 #pragma unroll
     for (int it = 0; it < kSortItems; it++) {
         int idx = tile * kSortTile + w * (kSortItems * kWave) + it * kWave + lane;
-        if (idx < n) atomicAdd(&h[((pairs ? pairs[idx].x : keys[idx]) >> shift) & (kSortBins - 1)], 1u);
+        if (idx < n) atomicAdd(&h[((pairs ? pairs[idx].x : keys[idx]) >> shift) & (B - 1)], 1u);
     }
     __syncthreads();
-    if (threadIdx.x < kSortBins) hist[threadIdx.x * tiles + tile] = h[threadIdx.x];
+    for (int b = threadIdx.x; b < B; b += 256) hist[size_t(b) * tiles + tile] = h[b];
 }
 
 // One workgroup per bin: exclusive scan of hist[bin][0..tiles) in place.
@@ -77,22 +81,24 @@ __global__ __launch_bounds__(256) void k_sort_binscan(uint32_t* __restrict__ his
 // particle's predicted position, velocity, flags and id move to the sorted
 // slot (the new order of the state: the particle-particle solve reads
 // neighbours as contiguous runs), and rank records previous slot -> new slot.
-template <bool PAIRS, bool GATHER>
+template <int BITS, bool PAIRS, bool GATHER>
 __global__ __launch_bounds__(256) void k_sort_scatter(int n, const uint32_t* __restrict__ keys_in,
                                                       const uint32_t* __restrict__ vals_in,
                                                       const uint2* __restrict__ pairs_in,
                                                       uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out,
                                                       int shift, const uint32_t* __restrict__ hist,
                                                       const uint32_t* __restrict__ bintot, int tiles, SortGather g) {
-    __shared__ uint32_t s_cnt[4][kSortBins];    // per-wave bin counters, then per-wave bin offsets
-    __shared__ uint32_t s_excl[kSortBins];      // tile-local exclusive bin starts (staging order)
-    __shared__ uint32_t s_gbase[kSortBins];     // global destination of the tile's first key per bin
+    constexpr int B = 1 << BITS;
+    constexpr int BPT = B > 256 ? B / 256 : 1;   // bins per thread in the tile-level scans
+    __shared__ uint32_t s_cnt[4][B];    // per-wave bin counters, then per-wave bin offsets
+    __shared__ uint32_t s_excl[B];      // tile-local exclusive bin starts (staging order)
+    __shared__ uint32_t s_gbase[B];     // global destination of the tile's first key per bin
     __shared__ uint32_t s_keys[kSortTile];
     __shared__ uint32_t s_vals[kSortTile];
-    __shared__ uint32_t s_part[2];
+    __shared__ uint32_t s_wsum[2][4];
     const int tile = blockIdx.x, w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint64_t lt = (1ull << lane) - 1ull;
-    for (int k = threadIdx.x; k < 4 * kSortBins; k += 256) (&s_cnt[0][0])[k] = 0;
+    for (int k = threadIdx.x; k < 4 * B; k += 256) (&s_cnt[0][0])[k] = 0;
     __syncthreads();
 
     uint32_t key[kSortItems], val[kSortItems], rank[kSortItems];
@@ -108,10 +114,10 @@ __global__ __launch_bounds__(256) void k_sort_scatter(int n, const uint32_t* __r
             key[it] = valid ? keys_in[idx] : 0xFFFFFFFFu;
             val[it] = valid ? vals_in[idx] : 0u;
         }
-        const uint32_t d = (key[it] >> shift) & (kSortBins - 1);
+        const uint32_t d = (key[it] >> shift) & (B - 1);
         uint64_t peers = __ballot(valid);
 #pragma unroll
-        for (int b = 0; b < kSortBits; b++) {
+        for (int b = 0; b < BITS; b++) {
             const bool bit = (d >> b) & 1u;
             const uint64_t m = __ballot(bit);
             peers &= bit ? m : ~m;
@@ -123,32 +129,46 @@ __global__ __launch_bounds__(256) void k_sort_scatter(int n, const uint32_t* __r
         if (valid && (peers & lt) == 0) s_cnt[w][d] = before + uint32_t(__popcll(peers));
     }
     __syncthreads();
-    // per-bin offsets of each wave inside the tile, tile totals, staging starts
-    uint32_t total = 0;
-    if (threadIdx.x < kSortBins) {
-        const int b = threadIdx.x;
-        const uint32_t c0 = s_cnt[0][b], c1 = s_cnt[1][b], c2 = s_cnt[2][b], c3 = s_cnt[3][b];
-        s_cnt[0][b] = 0; s_cnt[1][b] = c0; s_cnt[2][b] = c0 + c1; s_cnt[3][b] = c0 + c1 + c2;
-        total = c0 + c1 + c2 + c3;
+    // per-bin offsets of each wave inside the tile; each thread then owns
+    // BPT consecutive bins for the block-wide scans of the tile totals (the
+    // staging starts) and of the bin totals (the global starts)
+    uint32_t tot[BPT], bt[BPT];
+    uint32_t tsum = 0, bsum = 0;
+#pragma unroll
+    for (int j = 0; j < BPT; j++) {
+        const int b = threadIdx.x * BPT + j;
+        tot[j] = 0;
+        bt[j] = 0;
+        if (b < B) {
+            const uint32_t c0 = s_cnt[0][b], c1 = s_cnt[1][b], c2 = s_cnt[2][b], c3 = s_cnt[3][b];
+            s_cnt[0][b] = 0; s_cnt[1][b] = c0; s_cnt[2][b] = c0 + c1; s_cnt[3][b] = c0 + c1 + c2;
+            tot[j] = c0 + c1 + c2 + c3;
+            bt[j] = bintot[b];
+        }
+        tsum += tot[j];
+        bsum += bt[j];
     }
-    // exclusive scan of 128 tile totals (threads 0..127 = waves 0,1) and of bin totals
-    uint32_t bt = threadIdx.x < kSortBins ? bintot[threadIdx.x] : 0u;
-    uint32_t inc_t = wave_incl_scan(total);
-    uint32_t inc_b = wave_incl_scan(bt);
-    if (threadIdx.x == 63) { s_part[0] = inc_t; s_part[1] = inc_b; }
+    const uint32_t inc_t = wave_incl_scan(tsum), inc_b = wave_incl_scan(bsum);
+    if (lane == 63) { s_wsum[0][w] = inc_t; s_wsum[1][w] = inc_b; }
     __syncthreads();
-    if (threadIdx.x < kSortBins) {
-        const int b = threadIdx.x;
-        const uint32_t add_t = (b >= 64) ? s_part[0] : 0u, add_b = (b >= 64) ? s_part[1] : 0u;
-        s_excl[b] = add_t + inc_t - total;
-        s_gbase[b] = (add_b + inc_b - bt) + hist[size_t(b) * tiles + tile];
+    uint32_t et = inc_t - tsum, eb = inc_b - bsum;
+    for (int k = 0; k < w; k++) { et += s_wsum[0][k]; eb += s_wsum[1][k]; }
+#pragma unroll
+    for (int j = 0; j < BPT; j++) {
+        const int b = threadIdx.x * BPT + j;
+        if (b < B) {
+            s_excl[b] = et;
+            s_gbase[b] = eb + hist[size_t(b) * tiles + tile];
+        }
+        et += tot[j];
+        eb += bt[j];
     }
     __syncthreads();
 #pragma unroll
     for (int it = 0; it < kSortItems; it++) {
         const int idx = tile * kSortTile + w * (kSortItems * kWave) + it * kWave + lane;
         if (idx < n) {
-            const uint32_t d = (key[it] >> shift) & (kSortBins - 1);
+            const uint32_t d = (key[it] >> shift) & (B - 1);
             const uint32_t pos = s_excl[d] + s_cnt[w][d] + rank[it];
             if (pos < uint32_t(kSortTile)) {
                 s_keys[pos] = key[it];
@@ -160,7 +180,7 @@ __global__ __launch_bounds__(256) void k_sort_scatter(int n, const uint32_t* __r
     const int cnt = min(kSortTile, n - tile * kSortTile);
     for (int e = threadIdx.x; e < cnt; e += 256) {
         const uint32_t k = s_keys[e];
-        const uint32_t d = (k >> shift) & (kSortBins - 1);
+        const uint32_t d = (k >> shift) & (B - 1);
         const uint32_t dst = s_gbase[d] + (uint32_t(e) - s_excl[d]);
         if (dst < uint32_t(n)) {
             const uint32_t v = s_vals[e];
@@ -177,31 +197,47 @@ __global__ __launch_bounds__(256) void k_sort_scatter(int n, const uint32_t* __r
     }
 }
 
-void launch_sort_upsweep(int n, const uint32_t* keys, const uint2* pairs, int shift, uint32_t* hist, int tiles,
-                         hipStream_t s) {
-    hipLaunchKernelGGL(k_sort_upsweep, dim3(tiles), dim3(256), 0, s, n, keys, pairs, shift, hist, tiles);
+template <int BITS, bool PAIRS, bool GATHER>
+static void scatter(int n, const uint32_t* keys_in, const uint32_t* vals_in, const uint2* pairs_in, uint32_t* keys_out,
+                    uint32_t* vals_out, int shift, const uint32_t* hist, const uint32_t* bintot, int tiles,
+                    const SortGather& g, hipStream_t s) {
+    hipLaunchKernelGGL((k_sort_scatter<BITS, PAIRS, GATHER>), dim3(tiles), dim3(256), 0, s, n, keys_in, vals_in, pairs_in,
+                       keys_out, vals_out, shift, hist, bintot, tiles, g);
 }
-void launch_sort_binscan(uint32_t* hist, uint32_t* bintot, int tiles, hipStream_t s) {
-    hipLaunchKernelGGL(k_sort_binscan, dim3(kSortBins), dim3(256), 0, s, hist, bintot, tiles);
-}
-void launch_sort_scatter(int n, const uint32_t* keys_in, const uint32_t* vals_in, const uint2* pairs_in,
+template <int BITS>
+static void scatter_bits(int n, const uint32_t* keys_in, const uint32_t* vals_in, const uint2* pairs_in,
                          uint32_t* keys_out, uint32_t* vals_out, int shift, const uint32_t* hist, const uint32_t* bintot,
                          int tiles, const SortGather* gather, hipStream_t s) {
     const SortGather g = gather ? *gather : SortGather{};
     if (pairs_in) {
-        if (gather)
-            hipLaunchKernelGGL((k_sort_scatter<true, true>), dim3(tiles), dim3(256), 0, s, n, keys_in, vals_in, pairs_in,
-                               keys_out, vals_out, shift, hist, bintot, tiles, g);
-        else
-            hipLaunchKernelGGL((k_sort_scatter<true, false>), dim3(tiles), dim3(256), 0, s, n, keys_in, vals_in, pairs_in,
-                               keys_out, vals_out, shift, hist, bintot, tiles, g);
+        if (gather) scatter<BITS, true, true>(n, keys_in, vals_in, pairs_in, keys_out, vals_out, shift, hist, bintot, tiles, g, s);
+        else scatter<BITS, true, false>(n, keys_in, vals_in, pairs_in, keys_out, vals_out, shift, hist, bintot, tiles, g, s);
     } else {
-        if (gather)
-            hipLaunchKernelGGL((k_sort_scatter<false, true>), dim3(tiles), dim3(256), 0, s, n, keys_in, vals_in, pairs_in,
-                               keys_out, vals_out, shift, hist, bintot, tiles, g);
-        else
-            hipLaunchKernelGGL((k_sort_scatter<false, false>), dim3(tiles), dim3(256), 0, s, n, keys_in, vals_in, pairs_in,
-                               keys_out, vals_out, shift, hist, bintot, tiles, g);
+        if (gather) scatter<BITS, false, true>(n, keys_in, vals_in, pairs_in, keys_out, vals_out, shift, hist, bintot, tiles, g, s);
+        else scatter<BITS, false, false>(n, keys_in, vals_in, pairs_in, keys_out, vals_out, shift, hist, bintot, tiles, g, s);
+    }
+}
+
+void launch_sort_upsweep(int bits, int n, const uint32_t* keys, const uint2* pairs, int shift, uint32_t* hist, int tiles,
+                         hipStream_t s) {
+    switch (bits) {
+        case 7: hipLaunchKernelGGL(k_sort_upsweep<7>, dim3(tiles), dim3(256), 0, s, n, keys, pairs, shift, hist, tiles); break;
+        case 8: hipLaunchKernelGGL(k_sort_upsweep<8>, dim3(tiles), dim3(256), 0, s, n, keys, pairs, shift, hist, tiles); break;
+        case 9: hipLaunchKernelGGL(k_sort_upsweep<9>, dim3(tiles), dim3(256), 0, s, n, keys, pairs, shift, hist, tiles); break;
+        default: hipLaunchKernelGGL(k_sort_upsweep<10>, dim3(tiles), dim3(256), 0, s, n, keys, pairs, shift, hist, tiles);
+    }
+}
+void launch_sort_binscan(int bits, uint32_t* hist, uint32_t* bintot, int tiles, hipStream_t s) {
+    hipLaunchKernelGGL(k_sort_binscan, dim3(1 << bits), dim3(256), 0, s, hist, bintot, tiles);
+}
+void launch_sort_scatter(int bits, int n, const uint32_t* keys_in, const uint32_t* vals_in, const uint2* pairs_in,
+                         uint32_t* keys_out, uint32_t* vals_out, int shift, const uint32_t* hist, const uint32_t* bintot,
+                         int tiles, const SortGather* gather, hipStream_t s) {
+    switch (bits) {
+        case 7: scatter_bits<7>(n, keys_in, vals_in, pairs_in, keys_out, vals_out, shift, hist, bintot, tiles, gather, s); break;
+        case 8: scatter_bits<8>(n, keys_in, vals_in, pairs_in, keys_out, vals_out, shift, hist, bintot, tiles, gather, s); break;
+        case 9: scatter_bits<9>(n, keys_in, vals_in, pairs_in, keys_out, vals_out, shift, hist, bintot, tiles, gather, s); break;
+        default: scatter_bits<10>(n, keys_in, vals_in, pairs_in, keys_out, vals_out, shift, hist, bintot, tiles, gather, s);
     }
 }
 

@@ -82,8 +82,8 @@ void Solver::alloc_all() {
     dalloc(keysA_, n_); dalloc(valsA_, n_); dalloc(keysB_, n_); dalloc(valsB_, n_);
     dalloc(Xs_[0], n_); dalloc(Vs_[0], n_);
     const size_t tiles = (n_ + kSortTile - 1) / kSortTile;
-    dalloc(hist_, size_t(kSortBins) * std::max<size_t>(tiles, 1));
-    dalloc(bintot_, kSortBins);
+    dalloc(hist_, size_t(kSortMaxBins) * std::max<size_t>(tiles, 1));
+    dalloc(bintot_, kSortMaxBins);
     // one 2^21-cell table per scene; the batched key scene << 21 | cell is the row
     dalloc(range_, size_t(2) * kNumCells * num_scenes_);
     dalloc(cpart_, cmax_); dalloc(cshape_, cmax_);
@@ -138,7 +138,7 @@ void Solver::init_common(const xpbd_config& cfg, size_t n, size_t max_contacts) 
 void Solver::init(const xpbd_config& cfg, size_t n, size_t max_contacts) {
     num_scenes_ = 1;
     scene_off_.assign({0, n});
-    sort_passes_ = kSortPasses;
+    set_sort_shape(kCellBits);
     init_common(cfg, n, max_contacts);
 }
 
@@ -151,7 +151,7 @@ void Solver::init_batch(const xpbd_config& cfg, const size_t* counts, size_t num
     num_scenes_ = num_scenes;
     int bits = kCellBits;
     while ((size_t(1) << (bits - kCellBits)) < num_scenes) bits++;
-    sort_passes_ = (bits + kSortBits - 1) / kSortBits;
+    set_sort_shape(bits);
     init_common(cfg, total, max_contacts);
     std::vector<int32_t> sc(total);
     for (size_t k = 0; k < num_scenes; k++)
@@ -573,7 +573,7 @@ void Solver::update_planes() {
     HIPCHECK(hipStreamSynchronize(stream_));
     invalidate_graphs();
     planes_ = want;
-    if (num_scenes_ == 1) sort_passes_ = planes_ ? (kCellBits + kPlaneBits + kSortBits - 1) / kSortBits : kSortPasses;
+    if (num_scenes_ == 1) set_sort_shape(planes_ ? kCellBits + kPlaneBits : kCellBits);
     dalloc(range_, size_t(2) * (size_t(kNumCells) << (planes_ ? kPlaneBits : 0)) * num_scenes_);
     if (planes_) dalloc(rkeys_, n_);   // zero keys: the first reset touches row 0 only
     else dfree(rkeys_);
@@ -596,6 +596,13 @@ void Solver::build_ranges(const uint32_t* sorted_keys) {
 }
 
 // ----------------------------------------------------------------- step ----
+// The fewest radix passes of at most kSortUseBits that cover the key, with
+// the digit width spread evenly over them (never below 7 bits).
+void Solver::set_sort_shape(int key_bits) {
+    sort_passes_ = (key_bits + kSortUseBits - 1) / kSortUseBits;
+    sort_bits_ = std::max(7, (key_bits + sort_passes_ - 1) / sort_passes_);
+}
+
 
 void Solver::run_sort(int tiles) {
     const int n = int(n_);
@@ -608,18 +615,18 @@ void Solver::run_sort(int tiles) {
     const SortGather g{P_[p_].p, V_[p_].p, flags_cur(), ids_cur(), Xs_[0].p, Vs_[0].p,
                        flags_[lay_ ^ 1].p, ids_[lay_ ^ 1].p, rank_.p};
     for (int pass = 0; pass < sort_passes_; pass++) {
-        const int shift = pass * kSortBits;
+        const int shift = pass * sort_bits_;
         const uint32_t* kin = pass == 0 ? nullptr : kb[dst ^ 1];
         const uint32_t* vin = pass == 0 ? nullptr : vb[dst ^ 1];
         const uint2* pin = pass == 0 ? kv_.p : nullptr;
         stage_begin(kSortUp);
-        launch_sort_upsweep(n, kin, pin, shift, hist_.p, tiles, stream_);
+        launch_sort_upsweep(sort_bits_, n, kin, pin, shift, hist_.p, tiles, stream_);
         stage_end(kSortUp);
         stage_begin(kSortScan);
-        launch_sort_binscan(hist_.p, bintot_.p, tiles, stream_);
+        launch_sort_binscan(sort_bits_, hist_.p, bintot_.p, tiles, stream_);
         stage_end(kSortScan);
         stage_begin(kSortScatter);
-        launch_sort_scatter(n, kin, vin, pin, kb[dst], vb[dst], shift, hist_.p, bintot_.p, tiles,
+        launch_sort_scatter(sort_bits_, n, kin, vin, pin, kb[dst], vb[dst], shift, hist_.p, bintot_.p, tiles,
                             pass == sort_passes_ - 1 ? &g : nullptr, stream_);
         stage_end(kSortScatter);
         dst ^= 1;

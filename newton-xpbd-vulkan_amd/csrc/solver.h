@@ -145,7 +145,8 @@ private:
     size_t num_scenes_ = 1;
     std::vector<size_t> scene_off_;
     DevBuf<int32_t> scene_d_;
-    int sort_passes_ = kSortPasses;
+    int sort_passes_ = 3, sort_bits_ = 7;
+    void set_sort_shape(int key_bits);
     // plain solver: sort keys carry the wrap plane above the cell (see update_planes)
     bool planes_ = false, planes_allowed_ = false;
     int pp_block_ = 256;

@@ -5,11 +5,14 @@
 
 namespace xpbd {
 
-// Radix sort geometry: 21-bit cell keys sorted in three 7-bit LSD passes over
-// tiles of 256 threads x 4 keys.
-constexpr int kSortBits = 7;
-constexpr int kSortBins = 1 << kSortBits;
-constexpr int kSortPasses = 3;      // plain solver: 21-bit cell keys
+// Radix sort geometry: LSD passes of 7- to 10-bit digits over tiles of 256
+// threads x 4 keys.  The host uses the fewest passes of at most kSortUseBits
+// that cover the key (three 7-bit passes for the 21-bit cell id): wider
+// digits leave shorter same-bin runs per tile and measured slower (3 x 9 bits
+// took longer than 4 x 7).
+constexpr int kSortMaxBits = 10;
+constexpr int kSortMaxBins = 1 << kSortMaxBits;
+constexpr int kSortUseBits = 8;
 constexpr int kCellBits = 21;       // batched solver: key = scene << 21 | cell
 constexpr int kSortItems = 4;
 constexpr int kSortTile = 256 * kSortItems;
@@ -245,10 +248,10 @@ void launch_frame_begin(int* cc, hipStream_t s);
 void launch_mesh_contacts(const MeshArgs& a, hipStream_t s);
 
 // Radix sort: one pass = upsweep (tile histograms) -> bin-row scan -> scatter.
-void launch_sort_upsweep(int n, const uint32_t* keys, const uint2* pairs, int shift, uint32_t* hist, int tiles,
+void launch_sort_upsweep(int bits, int n, const uint32_t* keys, const uint2* pairs, int shift, uint32_t* hist, int tiles,
                          hipStream_t s);
-void launch_sort_binscan(uint32_t* hist, uint32_t* bintot, int tiles, hipStream_t s);
-void launch_sort_scatter(int n, const uint32_t* keys_in, const uint32_t* vals_in, const uint2* pairs_in,
+void launch_sort_binscan(int bits, uint32_t* hist, uint32_t* bintot, int tiles, hipStream_t s);
+void launch_sort_scatter(int bits, int n, const uint32_t* keys_in, const uint32_t* vals_in, const uint2* pairs_in,
                          uint32_t* keys_out, uint32_t* vals_out, int shift, const uint32_t* hist, const uint32_t* bintot,
                          int tiles, const SortGather* gather, hipStream_t s);
 void launch_fill_zero(int* p, size_t count, hipStream_t s);

@@ -474,19 +474,22 @@ def test_graph_replay_equals_eager(vk):
     log(f"graph replay == eager over 12 frames, contacts {a.last_contact_count}")
 
 
-def test_batched_scenes_match_sequential(vk):
+@pytest.mark.parametrize("num_scenes,lo,hi,frames", [(7, 64, 3000, 20), (300, 16, 64, 4)])
+def test_batched_scenes_match_sequential(vk, num_scenes, lo, hi, frames):
     """A batched solver over uneven scenes equals one plain solver per scene,
-    bit for bit (each scene keeps its own grid, neighbour order and plane)."""
-    sc = scenes.config3(seed=9, num_scenes=7, lo=64, hi=3000)
+    bit for bit (each scene keeps its own grid, neighbour order and plane).
+    7 scenes sort 24-bit keys in three 8-bit digits, 300 scenes 30-bit keys
+    in four."""
+    sc = scenes.config3(seed=9, num_scenes=num_scenes, lo=lo, hi=hi)
     sc["counts"][3] = 1   # a single-particle scene
     off = np.concatenate([[0], np.cumsum(sc["counts"])])
-    sc["particles"] = {k: np.concatenate([scenes.scene_slice(sc, j)["particles"][k] for j in range(7)])
+    sc["particles"] = {k: np.concatenate([scenes.scene_slice(sc, j)["particles"][k] for j in range(num_scenes)])
                        for k in sc["particles"]}
-    log(f"batched scenes: counts {list(sc['counts'])}")
+    log(f"batched scenes: {num_scenes}, {int(off[-1])} particles")
     bs = scenes.build_batch_solver(vk, sc)
-    assert bs.scene_count == 7
-    singles = [scenes.build_solver(vk, scenes.scene_slice(sc, k)) for k in range(7)]
-    for f in range(20):
+    assert bs.scene_count == num_scenes
+    singles = [scenes.build_solver(vk, scenes.scene_slice(sc, k)) for k in range(num_scenes)]
+    for f in range(frames):
         bs.step(1.0 / 60.0, 8, 1)
         tot = 0
         for s in singles:
@@ -501,7 +504,7 @@ def test_batched_scenes_match_sequential(vk):
     # scenes overlap in space but never interact: keys carry the scene index
     keys = bs.get_sorted_cells()
     assert np.all(np.diff(keys.astype(np.int64)) >= 0)
-    assert int(keys.max()) >> 21 == 6
+    assert int(keys.max()) >> 21 == num_scenes - 1
 
 
 def test_autotune_keeps_state(vk):
