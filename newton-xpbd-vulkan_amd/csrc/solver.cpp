@@ -66,6 +66,9 @@ void Solver::free_all() {
     free_tape();
     dfree(scene_d_);
     if (count_host_) { (void)hipHostFree(count_host_); count_host_ = nullptr; }
+    if (fork_ev_) { (void)hipEventDestroy(fork_ev_); fork_ev_ = nullptr; }
+    if (join_ev_) { (void)hipEventDestroy(join_ev_); join_ev_ = nullptr; }
+    if (side_) { (void)hipStreamDestroy(side_); side_ = nullptr; }
     if (stream_) { (void)hipStreamDestroy(stream_); stream_ = nullptr; }
 }
 
@@ -114,6 +117,9 @@ void Solver::init_common(const xpbd_config& cfg, size_t n, size_t max_contacts) 
     device_ = (cfg.device_index >= 0 && cfg.device_index < count) ? cfg.device_index : 0;
     DeviceGuard g(device_);
     HIPCHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    HIPCHECK(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking));
+    HIPCHECK(hipEventCreateWithFlags(&fork_ev_, hipEventDisableTiming));
+    HIPCHECK(hipEventCreateWithFlags(&join_ev_, hipEventDisableTiming));
     alloc_all();
     const char* env = std::getenv("VKXPBD_PROFILE");
     bool env_on = env && (std::strcmp(env, "1") == 0 || std::strcmp(env, "true") == 0);
@@ -655,6 +661,13 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
     int* cc_solve = collide ? cc_.p + slot : cc_.p + contact_slot_;
     int* rcc_solve = collide ? cc_.p + 4 + slot : cc_.p + 4 + rigid_slot_;
     const float inv_cell = 1.0f / cfg_.search_radius;
+    // the mesh walk is latency bound and fills few SIMDs: it overlaps the
+    // bandwidth-bound integrate and sort (unless stage times are measured)
+    const bool fork = collide && !meshes_.empty() && n > 0 && !profiling_;
+    if (fork) {
+        HIPCHECK(hipEventRecord(fork_ev_, stream_));
+        HIPCHECK(hipStreamWaitEvent(side_, fork_ev_, 0));
+    }
     if (collide && !meshes_.empty()) {
         MeshArgs m;
         m.n = n; m.P = P_[s_].p; m.flags = flags_cur();
@@ -668,8 +681,9 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
         m.err = reinterpret_cast<uint32_t*>(cc_.p + 2);
         m.stats = mesh_stats_on_ ? mesh_stats_.p : nullptr;
         stage_begin(kMesh);
-        launch_mesh_contacts(m, stream_);
+        launch_mesh_contacts(m, fork ? side_ : stream_);
         stage_end(kMesh);
+        if (fork) HIPCHECK(hipEventRecord(join_ev_, side_));
     }
     if (collide && nbox > 0) {
         stage_begin(kBodyContacts);
@@ -725,6 +739,7 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
         run_sort(tiles);
         stage_begin(kRanges);
         build_ranges(keysA_.p);
+        if (fork) HIPCHECK(hipStreamWaitEvent(stream_, join_ev_, 0));
         // contacts were generated in the previous slot order
         launch_remap_contacts(cpart_.p, cc_solve, int(cmax_), rank_.p, shape_grid, stream_);
         stage_end(kRanges);

@@ -139,6 +139,11 @@ private:
     bool initialized_ = false, finalized_ = false, grid_valid_ = false;
     int device_ = 0;
     hipStream_t stream_ = nullptr;
+    // mesh contacts of a substep run on side_ (default priority, like stream_)
+    // alongside integrate, sort and cell ranges, which do not touch them; the
+    // contact remap after the sort waits for them
+    hipStream_t side_ = nullptr;
+    hipEvent_t fork_ev_ = nullptr, join_ev_ = nullptr;
     size_t n_ = 0, cmax_ = 0;
 
     // batched scenes
