@@ -34,24 +34,25 @@ SUBSTEPS = 8
 DT = 1.0 / 60.0
 
 
-def stage_bytes(stage, N, C, Cm, U, tiles, mesh_bytes):
+def stage_bytes(stage, N, C, Cm, U, tiles, mesh_bytes, passes):
     """Algorithmic HBM bytes of ONE launch of a stage: every array the kernel
-    must touch, counted once (neighbour re-reads are cache hits)."""
+    must touch, counted once (neighbour re-reads are cache hits).  The state
+    lives in sorted slot order; the sort runs `passes` LSD passes."""
     hist = tiles * 128 * 4
-    if stage == "integrate":      # P,V in; P,V out; flags; key out; hist; plane contacts
-        return N * (16 + 16 + 4 + 16 + 16 + 4) + hist + 56 * (C - Cm)
-    if stage == "sort_upsweep":
-        return 4 * N + hist
+    if stage == "integrate":      # P,V,flags,ids in; P,V out; (key, slot) pair out; plane contacts
+        return N * (16 + 16 + 4 + 4 + 16 + 16 + 8) + 56 * (C - Cm)
+    if stage == "sort_upsweep":   # pass 0 reads the pairs, the others the keys
+        return (8 * N + 4 * N * (passes - 1)) / passes + hist
     if stage == "sort_binscan":
         return 2 * hist
-    if stage == "sort_scatter":   # mean over the 3 passes: pass 0 reads keys only
-        return (12 * N + 16 * N + 16 * N) / 3.0 + hist
-    if stage == "cell_ranges":
-        return 4 * N + 8 * U
+    if stage == "sort_scatter":   # mean over the passes; the last also moves P,V,flags,id and writes rank
+        return (16 * N * passes + (16 + 16 + 4 + 4) * 2 * N + 4 * N) / passes + hist
+    if stage == "cell_ranges":    # reset of the last rows, new rows, key record, contact remap
+        return 4 * N + 8 * U + 4 * N + 8 * U + 4 * N + 12 * C
     if stage == "shape_solve":    # contact record, particle P,V gather, delta RMW
         return C * (56 + 32 + 24)
-    if stage == "pp_solve_apply":  # vals, keys, flags, dshape rw, Pp, Vp, Ps, delta, P/V out; cell ranges
-        return N * (4 + 4 + 4 + 24 + 16 + 16 + 16 + 12 + 32) + 8 * U
+    if stage == "pp_solve_apply":  # vals, flags, dshape rw, Xs/Vs, x0, delta, P/V out; cell ranges
+        return N * (4 + 4 + 24 + 32 + 16 + 12 + 32) + 8 * U
     if stage == "mesh_contacts":
         return N * (16 + 4) + 56 * Cm + mesh_bytes
     return 0
@@ -64,7 +65,7 @@ STAGE_KERNELS = {
     "sort_upsweep": ["k_sort_upsweep"],
     "sort_binscan": ["k_sort_binscan"],
     "sort_scatter": ["k_sort_scatter"],
-    "cell_ranges": ["k_fill_zero", "k_cell_ranges"],
+    "cell_ranges": ["k_fill_zero", "k_clear_ranges", "k_cell_ranges", "k_remap_contacts"],
     "shape_solve": ["k_shape_solve"],
     "pp_solve_apply": ["k_pp_apply"],
 }
@@ -120,19 +121,22 @@ def measured_traffic(stage):
                 ff = float(t[1])
             elif len(t) == 2 and t[0] == "write_factor":
                 wf = float(t[1])
-            elif len(t) >= 6 and t[0] == "kernel":
-                rows.append((t[1], float(t[3]), float(t[5])))
+            elif len(t) >= 8 and t[0] == "kernel":
+                rows.append((t[1], float(t[3]), float(t[5]), float(t[7])))
     if ff is None or wf is None:
         return None
-    total, hits = 0.0, 0
-    for name, fk, wk in rows:
+    total, wsum, launches, hits = 0.0, 0.0, 0.0, 0
+    for name, fk, wk, nd in rows:
         if any(("::" + k + "<") in name or name.endswith("::" + k) for k in STAGE_KERNELS[stage]):
-            total += (ff * fk + wf * wk) * 1024.0
+            b = (ff * fk + wf * wk) * 1024.0
+            total += b
+            wsum += b * nd
+            launches += nd
             hits += 1
     if not hits:
         return None
-    # the scatter stage is one launch of one of three pass variants: their mean
-    return total / hits if stage == "sort_scatter" else total
+    # a sort stage launches one of several pass variants: their dispatch-weighted mean
+    return wsum / launches if stage.startswith("sort_") else total
 
 
 def reset(s, sc):
@@ -297,6 +301,7 @@ def main():
         shapes = sp.get_contact_shapes()
         cm.append(int(np.count_nonzero(shapes > 0)))
         prof = sp.profile_stages()
+        passes = 4 if sp.wrap_planes else 3   # 27-bit keys with wrap planes, else 21
         del sp
         C = float(np.mean(contacts))
         Cm = float(cm[-1]) * (C / max(contacts[-1], 1))
@@ -307,7 +312,7 @@ def main():
             if launches == 0 or name == "readback":
                 continue
             avg_ms = ms / launches
-            b = stage_bytes(name, N, C, Cm, U, tiles, mesh_bytes)
+            b = stage_bytes(name, N, C, Cm, U, tiles, mesh_bytes, passes)
             kern[name] = {"avg_us": 1e3 * avg_ms, "launches": int(launches), "alg_bytes": b,
                           "gbs": b / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0}
         dom = max(kern, key=lambda k: kern[k]["avg_us"] * kern[k]["launches"])

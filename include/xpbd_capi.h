@@ -238,9 +238,10 @@ int xpbd_scene_count(const xpbd_solver* h, size_t* out);
  * plain solver sorts by wrap plane << 21 | cell (two bits per axis of which
  * wrap a cell lies in) and queries read only their own wrap's run: fewer
  * candidates for scenes wider than 128 cells, one more radix pass (27-bit
- * keys).  xpbd_set_wrap_planes asks for the mode (default off; environment
- * VKXPBD_WRAP_PLANES=1 turns it on at init); xpbd_wrap_planes reports whether
- * it is in effect: never for a batched solver, nor while a pair could reach
+ * keys) and a 1 GiB cell table.  xpbd_set_wrap_planes asks for the mode
+ * (default: on from 262,144 particles; environment VKXPBD_WRAP_PLANES=1 or 0
+ * forces it at init); xpbd_wrap_planes reports whether it is in effect: never
+ * for a batched solver, nor while a pair could reach
  * 126 cells (3 r_max + particle_max_radius + 2|cohesion| >= 126 search_radius,
  * re-checked by set_particles).  Results are identical either way; the
  * sorted-cell and cell-table getters report the reference's order in both.

@@ -153,6 +153,7 @@ private:
     int sort_passes_ = 3, sort_bits_ = 7;
     void set_sort_shape(int key_bits);
     // plain solver: sort keys carry the wrap plane above the cell (see update_planes)
+    static constexpr size_t kPlanesMinParticles = size_t(1) << 18;
     bool planes_ = false, planes_allowed_ = false;
     int pp_block_ = 256;
     // captured frames: rotation state and parameters -> executable graph

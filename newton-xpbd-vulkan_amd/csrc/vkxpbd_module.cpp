@@ -459,7 +459,8 @@ PYBIND11_MODULE(_vkxpbd, m) {
         .def(
             "set_wrap_planes", [](PySolver& s, bool on) { s.check(xpbd_set_wrap_planes(s.h, on ? 1 : 0)); },
             py::arg("enabled"),
-            "Sort by wrap plane and cell, so queries skip particles 128 cells away (default off; same results).")
+            "Sort by wrap plane and cell, so queries skip particles 128 cells away (default on from 262,144 "
+            "particles; same results).")
         .def_property_readonly("wrap_planes",
                                [](const PySolver& s) {
                                    int v = 0;
