@@ -1,19 +1,21 @@
 // k_mesh.hip -- particle-vs-triangle-mesh narrowphase over the packed LBVH,
 // the replacement of shaders/generate_mesh_contacts.comp:181-414.
 //
-// One kernel, one particle per thread, one block-level slot reservation per
-// mesh for the contacts it emits (same-address global atomics serialise, so
-// per-wave reservations throttle the kernel):
+// Five launches per mesh, one block-level slot reservation per block for the
+// contacts emitted (same-address global atomics serialise, so per-wave
+// reservations throttle the kernels):
 //   * early outs: the root box and a grid of distance lower bounds; the
-//     particles left are compacted within the block before the tree walk;
+//     particles left are queued (count, scan, enqueue) for the tree walk;
 //   * closest triangle: depth-first search with the DFS stack in LDS (one
 //     32-deep column per thread, bank = thread), the reference's pruning, push
 //     order and strict-less tie rule, so the chosen face is the same;
-//   * watertight sign: parity of ray/triangle hits along the reference's fixed
-//     ray.  The reference tests every triangle of the mesh (:372-388); here
-//     the same watertight ray test runs only on the triangles registered in the
-//     origin's cell of a grid of triangle projections along that ray
-//     direction.  Registration is padded far beyond the test's rounding error,
+//   * watertight sign (a separate pass, eight lanes per contact): parity of
+//     ray/triangle hits along the reference's fixed ray.  The reference tests
+//     every triangle of the mesh (:372-388); here the same watertight ray test
+//     runs only on the triangles registered in the origin's cell of a grid of
+//     triangle projections along that ray direction.  Every ray has the
+//     reference's direction, so a triangle can only be hit by a ray whose
+//     origin projects into the triangle's projection.  Registration is padded far beyond the test's rounding error,
 //     so no triangle the test would report is skipped and the hit count, hence
 //     the sign, is unchanged;
 //   * non-watertight sign: the angle-weighted pseudo-normal (:266-313, :390).
@@ -141,37 +143,6 @@ __device__ __forceinline__ Ray make_ray(f3 p) {
 
 __device__ __forceinline__ float4 node_lo(const float4* nodes, uint32_t i) { return nodes[2 * i]; }
 __device__ __forceinline__ float4 node_hi(const float4* nodes, uint32_t i) { return nodes[2 * i + 1]; }
-
-// Ray parity through the projected grid of a watertight mesh: every ray has
-// the reference's direction, so a triangle can only be hit by a ray whose
-// origin projects (along that direction) into the triangle's projection.
-// The host registers each triangle in every grid cell its padded projected
-// box touches; the query tests the triangles of its cell with the same
-// watertight ray test, so the hit count equals the all-triangle loop's.
-__device__ __forceinline__ uint32_t parity_hits(const MeshArgs& a, const MeshInfo& mi, const Ray& ray, uint32_t& tested) {
-    const f3 p = ray.p;
-    const float u = comp3(p, ray.kx) - ray.Sx * comp3(p, ray.kz);
-    const float w = comp3(p, ray.ky) - ray.Sy * comp3(p, ray.kz);
-    const float fu = floorf((u - mi.pg_u0) * mi.pg_inv), fw = floorf((w - mi.pg_w0) * mi.pg_inv);
-    if (!(fu >= 0.0f && fw >= 0.0f && fu < float(mi.pg_dim) && fw < float(mi.pg_dim))) return 0u;
-    const uint32_t cell = mi.pg_cell_off + uint32_t(fw) * mi.pg_dim + uint32_t(fu);
-    const uint32_t b = a.pg_start[cell], e = a.pg_start[cell + 1];
-    tested += e - b;
-    uint32_t hits = 0;
-    uint32_t k = b;
-    for (; k + 3 < e; k += 4) {   // four triangles' loads in flight together
-        float4 T[12];
-#pragma unroll
-        for (int q = 0; q < 12; q++) T[q] = a.pg_soup[3 * k + q];
-#pragma unroll
-        for (int q = 0; q < 4; q++) hits += ray_tri(ray, xyz(T[3 * q]), xyz(T[3 * q + 1]), xyz(T[3 * q + 2])) ? 1u : 0u;
-    }
-    for (; k < e; k++) {
-        const float4 A = a.pg_soup[3 * k], B = a.pg_soup[3 * k + 1], C = a.pg_soup[3 * k + 2];
-        if (ray_tri(ray, xyz(A), xyz(B), xyz(C))) hits++;
-    }
-    return hits;
-}
 
 // Closest triangle of generate_mesh_contacts.comp:201-331 for one particle and
 // mesh: LDS stack column lt, the reference's pruning, push order and
@@ -393,7 +364,7 @@ __global__ __launch_bounds__(256) void k_mesh_count(MeshArgs a, int ci) {
 }
 
 // Pass 2 (one block): exclusive scan of the block counts in place; the total
-// goes to mq_count[nb].
+// goes to mq_count[nb], and the parity-queue count mq_count[nb + 1] is reset.
 __global__ __launch_bounds__(1024) void k_mesh_scan(uint32_t* cnt, int nb) {
     __shared__ uint32_t part[1024];
     const int t = threadIdx.x;
@@ -411,7 +382,7 @@ __global__ __launch_bounds__(1024) void k_mesh_scan(uint32_t* cnt, int nb) {
     }
     uint32_t run = part[t] - sum;
     for (int k = b0; k < b1; k++) { const uint32_t c = cnt[k]; cnt[k] = run; run += c; }
-    if (t == 1023) cnt[nb] = part[1023];
+    if (t == 1023) { cnt[nb] = part[1023]; cnt[nb + 1] = 0u; }   // total; the parity queue starts empty
 }
 
 // Pass 3: candidates into the queue in particle order.
@@ -428,7 +399,10 @@ __global__ __launch_bounds__(256) void k_mesh_enqueue(MeshArgs a, int ci) {
 // Pass 4: one queued particle per thread, one wave per block so the walks
 // spread over every SIMD; contact slots reserved once per block, then the
 // payloads (generate_mesh_contacts.comp:333-411).  Grid-stride over the
-// queue with a block-uniform trip count.
+// queue with a block-uniform trip count.  A watertight mesh's contacts are
+// written with the outside sign; those with a nonzero offset are queued for
+// k_mesh_parity, which flips the normal of the inside ones (the contact point
+// does not depend on the sign: (-n)(-d) = n d exactly).
 template <bool STATS>
 __global__ __launch_bounds__(kMeshBlock) void k_mesh_walk(MeshArgs a, int ci, int nb) {
     __shared__ uint32_t stk[kStack][kMeshBlock];
@@ -451,10 +425,10 @@ __global__ __launch_bounds__(kMeshBlock) void k_mesh_walk(MeshArgs a, int ci, in
             const f3 p = xyz(P);
             if (STATS) st[1]++;
             hit = closest_face<STATS>(a, mi, uint32_t(e.y), p, P.w, stk, lt, face, sv, sw, sg, st[2], st[3]);
-            if (hit && mi.watertight) sg = (parity_hits(a, mi, make_ray(p), st[4]) & 1u) ? -1.0f : 1.0f;
         }
         if (STATS) st[5] += hit ? 1u : 0u;
         const int slot = block_reserve(a.cc, hit ? 1 : 0, s_tmp);
+        bool parity = false;
         if (hit && slot < a.cmax) {
             const f3 p = xyz(P);
             const float u_m = (1.0f - sv) - sw, v_m = sv;
@@ -478,7 +452,10 @@ __global__ __launch_bounds__(kMeshBlock) void k_mesh_walk(MeshArgs a, int ci, in
             a.cbpos[slot] = make_float4(bp.x, bp.y, bp.z, 0.f);
             a.cbvel[slot] = make_float4(svel.x, svel.y, svel.z, 0.f);
             a.cnorm[slot] = make_float4(nn.x, nn.y, nn.z, 0.f);
+            parity = mi.watertight && dlen > 0.0f;
         }
+        const int pslot = block_reserve(reinterpret_cast<int*>(a.mq_count + nb + 1), parity ? 1 : 0, s_tmp);
+        if (parity) a.pq[pslot] = uint32_t(slot);
     }
     if (STATS) {
 #pragma unroll
@@ -489,10 +466,56 @@ __global__ __launch_bounds__(kMeshBlock) void k_mesh_walk(MeshArgs a, int ci, in
     }
 }
 
+// Pass 5: the watertight sign of each queued contact, eight lanes per
+// contact over the triangles registered in the ray origin's projected cell
+// (the hit count, hence its parity, is the all-triangle loop's); an odd count
+// flips the stored normal (generate_mesh_contacts.comp:372-392).
+constexpr int kParityLanes = 8;
+template <bool STATS>
+__global__ __launch_bounds__(256) void k_mesh_parity(MeshArgs a, int ci, int nb) {
+    const MeshInfo mi = a.info[ci];
+    const uint32_t total = a.mq_count[nb + 1];
+    const int j = threadIdx.x % kParityLanes;
+    const uint32_t per_block = 256 / kParityLanes;
+    uint64_t tested = 0;
+    for (uint32_t k = blockIdx.x * per_block + threadIdx.x / kParityLanes; k < total; k += gridDim.x * per_block) {
+        const uint32_t slot = a.pq[k];
+        const f3 p = xyz(a.P[a.cpart[slot]]);
+        const Ray ray = make_ray(p);
+        const float u = comp3(p, ray.kx) - ray.Sx * comp3(p, ray.kz);
+        const float w = comp3(p, ray.ky) - ray.Sy * comp3(p, ray.kz);
+        const float fu = floorf((u - mi.pg_u0) * mi.pg_inv), fw = floorf((w - mi.pg_w0) * mi.pg_inv);
+        uint32_t b = 0, e = 0;
+        if (fu >= 0.0f && fw >= 0.0f && fu < float(mi.pg_dim) && fw < float(mi.pg_dim)) {
+            const uint32_t cell = mi.pg_cell_off + uint32_t(fw) * mi.pg_dim + uint32_t(fu);
+            b = a.pg_start[cell];
+            e = a.pg_start[cell + 1];
+        }
+        uint32_t hits = 0;
+        for (uint32_t t = b + uint32_t(j); t < e; t += kParityLanes) {
+            const float4 A = a.pg_soup[3 * t], B = a.pg_soup[3 * t + 1], C = a.pg_soup[3 * t + 2];
+            if (ray_tri(ray, xyz(A), xyz(B), xyz(C))) hits++;
+        }
+        hits += __shfl_xor(hits, 1);
+        hits += __shfl_xor(hits, 2);
+        hits += __shfl_xor(hits, 4);
+        if (STATS && j == 0) tested += e - b;
+        if (j == 0 && (hits & 1u)) {
+            const float4 n = a.cnorm[slot];
+            a.cnorm[slot] = make_float4(-n.x, -n.y, -n.z, n.w);
+        }
+    }
+    if (STATS) {
+        const uint64_t v = wave_sum_u64(tested);
+        if (__lane_id() == 0 && v) atomicAdd(reinterpret_cast<unsigned long long*>(a.stats + 4), (unsigned long long)v);
+    }
+}
+
 void launch_mesh_contacts(const MeshArgs& a, hipStream_t s) {
     if (a.n <= 0 || a.nmesh <= 0) return;
     const int nb = (a.n + 255) / 256;
     const unsigned walk_grid = unsigned(std::min(4096, (a.n + kMeshBlock - 1) / kMeshBlock));
+    const unsigned parity_grid = unsigned(std::min(2048, (a.n + 255 / kParityLanes) / (256 / kParityLanes)));
     for (int ci = 0; ci < a.nmesh; ci++) {
         if (a.stats) hipLaunchKernelGGL(k_mesh_count<true>, dim3(nb), dim3(256), 0, s, a, ci);
         else hipLaunchKernelGGL(k_mesh_count<false>, dim3(nb), dim3(256), 0, s, a, ci);
@@ -500,6 +523,8 @@ void launch_mesh_contacts(const MeshArgs& a, hipStream_t s) {
         hipLaunchKernelGGL(k_mesh_enqueue, dim3(nb), dim3(256), 0, s, a, ci);
         if (a.stats) hipLaunchKernelGGL(k_mesh_walk<true>, dim3(walk_grid), dim3(kMeshBlock), 0, s, a, ci, nb);
         else hipLaunchKernelGGL(k_mesh_walk<false>, dim3(walk_grid), dim3(kMeshBlock), 0, s, a, ci, nb);
+        if (a.stats) hipLaunchKernelGGL(k_mesh_parity<true>, dim3(parity_grid), dim3(256), 0, s, a, ci, nb);
+        else hipLaunchKernelGGL(k_mesh_parity<false>, dim3(parity_grid), dim3(256), 0, s, a, ci, nb);
     }
 }
 

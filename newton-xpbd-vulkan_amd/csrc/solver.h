@@ -250,6 +250,7 @@ private:
     DevBuf<float4> pg_soup_;               // triangle corners of every parity-grid entry
     DevBuf<uint32_t> mq_count_;            // mesh candidates per 256-particle block, scanned
     DevBuf<int2> mq_;                      // mesh candidate queue (particle, start node)
+    DevBuf<uint32_t> pq_;                  // watertight mesh contacts awaiting their sign
     DevBuf<MeshInfo> info_d_;
     std::vector<MeshInfo> info_h_;
     float avg_edge_ = 0.f;

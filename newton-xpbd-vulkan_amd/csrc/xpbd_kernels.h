@@ -151,8 +151,10 @@ struct MeshArgs {
     const float* lb;            // distance lower bounds of all meshes' grids
     const uint32_t* lb_node;    // per bound cell: node the closest-face search starts at
     uint64_t* stats;            // optional query counters (6 words), null = off
-    uint32_t* mq_count;         // per 256-particle block candidate counts (+ total), scanned in place
+    uint32_t* mq_count;         // per 256-particle block candidate counts, then the total and the
+                                // parity-queue count (nb + 2 words)
     int2* mq;                   // candidate queue: (particle, search start node word)
+    uint32_t* pq;               // watertight contacts whose normal sign the parity pass decides
 };
 
 // Rigid bodies (dynamic bodies occupy indices [b0, b1) of the body arrays).
