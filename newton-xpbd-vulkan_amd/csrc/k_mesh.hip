@@ -1,11 +1,11 @@
 // k_mesh.hip -- particle-vs-triangle-mesh narrowphase over the packed LBVH,
 // the replacement of shaders/generate_mesh_contacts.comp:181-414.
 //
-// Five launches per mesh, one block-level slot reservation per block for the
+// Three launches per mesh, one block-level slot reservation per block for the
 // contacts emitted (same-address global atomics serialise, so per-wave
 // reservations throttle the kernels):
 //   * early outs: the root box and a grid of distance lower bounds; the
-//     particles left are queued (count, scan, enqueue) for the tree walk;
+//     particles left are queued for the tree walk;
 //   * closest triangle: depth-first search with the DFS stack in LDS (one
 //     32-deep column per thread, bank = thread), the reference's pruning, push
 //     order and strict-less tie rule, so the chosen face is the same;
@@ -28,6 +28,7 @@
 namespace xpbd {
 
 constexpr int kMeshBlock = 64;   // tree-walk workgroup: one wave
+constexpr uint32_t kXcds = 8;    // MI355X: 8 XCDs, each with its own L2
 constexpr int kStack = 32;
 
 __device__ __forceinline__ f3 ldv(const float* v, uint32_t k) { return {v[3 * k], v[3 * k + 1], v[3 * k + 2]}; }
@@ -349,54 +350,35 @@ __device__ __forceinline__ uint32_t mesh_candidate(const MeshArgs& a, const Mesh
     return search_start(a, mi, p, thr);
 }
 
-// Pass 1: candidates per 256-particle block.
+// Queue counters of mesh ci for this substep: (queue length, parity-queue
+// length).  Substeps alternate between two pairs; each substep's enqueue
+// resets the other pair for the next one (the frame start resets both).
+__device__ __forceinline__ uint32_t* mesh_counters(const MeshArgs& a, int ci, int slot) {
+    return a.mcnt + 2 * (size_t(slot) * size_t(a.nmesh) + size_t(ci));
+}
+
+// Pass 1: candidates into the queue, one slot reservation per block (queue
+// order is arbitrary; every queued particle's result is its own).
 template <bool STATS>
-__global__ __launch_bounds__(256) void k_mesh_count(MeshArgs a, int ci) {
+__global__ __launch_bounds__(256) void k_mesh_enqueue(MeshArgs a, int ci) {
+    __shared__ int s_tmp[256 / 64 + 1];
     const MeshInfo mi = a.info[ci];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        uint32_t* o = mesh_counters(a, ci, a.slot ^ 1);
+        o[0] = 0u;
+        o[1] = 0u;
+    }
     const int i = blockIdx.x * 256 + threadIdx.x;
-    const bool cand = mesh_candidate(a, mi, i) != kNoNode;
-    const int c = __syncthreads_count(cand);
-    if (threadIdx.x == 0) a.mq_count[blockIdx.x] = uint32_t(c);
+    const uint32_t start = mesh_candidate(a, mi, i);
+    const int pos = block_reserve(reinterpret_cast<int*>(mesh_counters(a, ci, a.slot)), start != kNoNode ? 1 : 0, s_tmp);
+    if (start != kNoNode) a.mq[pos] = make_int2(i, int(start));
     if (STATS) {
         const uint64_t act = wave_sum_u64((i < a.n && (a.flags[i] & 1)) ? 1u : 0u);
         if (__lane_id() == 0 && act) atomicAdd(reinterpret_cast<unsigned long long*>(a.stats), (unsigned long long)act);
     }
 }
 
-// Pass 2 (one block): exclusive scan of the block counts in place; the total
-// goes to mq_count[nb], and the parity-queue count mq_count[nb + 1] is reset.
-__global__ __launch_bounds__(1024) void k_mesh_scan(uint32_t* cnt, int nb) {
-    __shared__ uint32_t part[1024];
-    const int t = threadIdx.x;
-    const int per = (nb + 1023) / 1024;
-    const int b0 = min(nb, t * per), b1 = min(nb, b0 + per);
-    uint32_t sum = 0;
-    for (int k = b0; k < b1; k++) sum += cnt[k];
-    part[t] = sum;
-    __syncthreads();
-    for (int o = 1; o < 1024; o <<= 1) {   // Hillis-Steele over 1024 partial sums
-        const uint32_t v = t >= o ? part[t - o] : 0u;
-        __syncthreads();
-        part[t] += v;
-        __syncthreads();
-    }
-    uint32_t run = part[t] - sum;
-    for (int k = b0; k < b1; k++) { const uint32_t c = cnt[k]; cnt[k] = run; run += c; }
-    if (t == 1023) { cnt[nb] = part[1023]; cnt[nb + 1] = 0u; }   // total; the parity queue starts empty
-}
-
-// Pass 3: candidates into the queue in particle order.
-__global__ __launch_bounds__(256) void k_mesh_enqueue(MeshArgs a, int ci) {
-    __shared__ int s_tmp[256 / 64 + 1];
-    const MeshInfo mi = a.info[ci];
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    const uint32_t start = mesh_candidate(a, mi, i);
-    int tot = 0;
-    const int pos = block_exclusive_scan(start != kNoNode ? 1 : 0, s_tmp, &tot);
-    if (start != kNoNode) a.mq[a.mq_count[blockIdx.x] + uint32_t(pos)] = make_int2(i, int(start));
-}
-
-// Pass 4: one queued particle per thread, one wave per block so the walks
+// Pass 2: one queued particle per thread, one wave per block so the walks
 // spread over every SIMD; contact slots reserved once per block, then the
 // payloads (generate_mesh_contacts.comp:333-411).  Grid-stride over the
 // queue with a block-uniform trip count.  A watertight mesh's contacts are
@@ -404,21 +386,28 @@ __global__ __launch_bounds__(256) void k_mesh_enqueue(MeshArgs a, int ci) {
 // k_mesh_parity, which flips the normal of the inside ones (the contact point
 // does not depend on the sign: (-n)(-d) = n d exactly).
 template <bool STATS>
-__global__ __launch_bounds__(kMeshBlock) void k_mesh_walk(MeshArgs a, int ci, int nb) {
+__global__ __launch_bounds__(kMeshBlock) void k_mesh_walk(MeshArgs a, int ci) {
     __shared__ uint32_t stk[kStack][kMeshBlock];
     __shared__ int s_tmp[kMeshBlock / 64 + 1];
     uint32_t st[6] = {0, 0, 0, 0, 0, 0};
     const MeshInfo mi = a.info[ci];
     const int lt = threadIdx.x;
-    const uint32_t total = a.mq_count[nb];
-    for (uint32_t base = blockIdx.x * kMeshBlock; base < total; base += gridDim.x * kMeshBlock) {
+    uint32_t* const cnt = mesh_counters(a, ci, a.slot);
+    const uint32_t total = cnt[0];
+    // Workgroups are dealt to the eight XCDs round-robin; give each XCD one
+    // contiguous eighth of the queue (spatially sorted particles), so the
+    // tree nodes its walks share stay in that XCD's L2.
+    const uint32_t xcd = blockIdx.x % kXcds, nbx = gridDim.x / kXcds;
+    const uint32_t chunk = (total + kXcds - 1) / kXcds;
+    const uint32_t lo = min(total, xcd * chunk), hi = min(total, lo + chunk);
+    for (uint32_t base = lo + (blockIdx.x / kXcds) * kMeshBlock; base < hi; base += nbx * kMeshBlock) {
         const uint32_t q = base + uint32_t(lt);
         bool hit = false;
         uint32_t face = 0;
         float sv = 0.f, sw = 0.f, sg = 1.0f;
         int pi = 0;
         float4 P = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (q < total) {
+        if (q < hi) {
             const int2 e = a.mq[q];
             pi = e.x;
             P = a.P[pi];
@@ -454,7 +443,7 @@ __global__ __launch_bounds__(kMeshBlock) void k_mesh_walk(MeshArgs a, int ci, in
             a.cnorm[slot] = make_float4(nn.x, nn.y, nn.z, 0.f);
             parity = mi.watertight && dlen > 0.0f;
         }
-        const int pslot = block_reserve(reinterpret_cast<int*>(a.mq_count + nb + 1), parity ? 1 : 0, s_tmp);
+        const int pslot = block_reserve(reinterpret_cast<int*>(cnt + 1), parity ? 1 : 0, s_tmp);
         if (parity) a.pq[pslot] = uint32_t(slot);
     }
     if (STATS) {
@@ -466,15 +455,15 @@ __global__ __launch_bounds__(kMeshBlock) void k_mesh_walk(MeshArgs a, int ci, in
     }
 }
 
-// Pass 5: the watertight sign of each queued contact, eight lanes per
+// Pass 3: the watertight sign of each queued contact, eight lanes per
 // contact over the triangles registered in the ray origin's projected cell
 // (the hit count, hence its parity, is the all-triangle loop's); an odd count
 // flips the stored normal (generate_mesh_contacts.comp:372-392).
 constexpr int kParityLanes = 8;
 template <bool STATS>
-__global__ __launch_bounds__(256) void k_mesh_parity(MeshArgs a, int ci, int nb) {
+__global__ __launch_bounds__(256) void k_mesh_parity(MeshArgs a, int ci) {
     const MeshInfo mi = a.info[ci];
-    const uint32_t total = a.mq_count[nb + 1];
+    const uint32_t total = mesh_counters(a, ci, a.slot)[1];
     const int j = threadIdx.x % kParityLanes;
     const uint32_t per_block = 256 / kParityLanes;
     uint64_t tested = 0;
@@ -514,17 +503,19 @@ __global__ __launch_bounds__(256) void k_mesh_parity(MeshArgs a, int ci, int nb)
 void launch_mesh_contacts(const MeshArgs& a, hipStream_t s) {
     if (a.n <= 0 || a.nmesh <= 0) return;
     const int nb = (a.n + 255) / 256;
-    const unsigned walk_grid = unsigned(std::min(4096, (a.n + kMeshBlock - 1) / kMeshBlock));
+    // a multiple of the XCD count, so every XCD gets the same number of workgroups
+    const unsigned walk_grid = unsigned(std::max(1, std::min(4096, (a.n + kMeshBlock - 1) / kMeshBlock) / 8)) * 8u;
     const unsigned parity_grid = unsigned(std::min(2048, (a.n + 255 / kParityLanes) / (256 / kParityLanes)));
     for (int ci = 0; ci < a.nmesh; ci++) {
-        if (a.stats) hipLaunchKernelGGL(k_mesh_count<true>, dim3(nb), dim3(256), 0, s, a, ci);
-        else hipLaunchKernelGGL(k_mesh_count<false>, dim3(nb), dim3(256), 0, s, a, ci);
-        hipLaunchKernelGGL(k_mesh_scan, dim3(1), dim3(1024), 0, s, a.mq_count, nb);
-        hipLaunchKernelGGL(k_mesh_enqueue, dim3(nb), dim3(256), 0, s, a, ci);
-        if (a.stats) hipLaunchKernelGGL(k_mesh_walk<true>, dim3(walk_grid), dim3(kMeshBlock), 0, s, a, ci, nb);
-        else hipLaunchKernelGGL(k_mesh_walk<false>, dim3(walk_grid), dim3(kMeshBlock), 0, s, a, ci, nb);
-        if (a.stats) hipLaunchKernelGGL(k_mesh_parity<true>, dim3(parity_grid), dim3(256), 0, s, a, ci, nb);
-        else hipLaunchKernelGGL(k_mesh_parity<false>, dim3(parity_grid), dim3(256), 0, s, a, ci, nb);
+        if (a.stats) {
+            hipLaunchKernelGGL(k_mesh_enqueue<true>, dim3(nb), dim3(256), 0, s, a, ci);
+            hipLaunchKernelGGL(k_mesh_walk<true>, dim3(walk_grid), dim3(kMeshBlock), 0, s, a, ci);
+            hipLaunchKernelGGL(k_mesh_parity<true>, dim3(parity_grid), dim3(256), 0, s, a, ci);
+        } else {
+            hipLaunchKernelGGL(k_mesh_enqueue<false>, dim3(nb), dim3(256), 0, s, a, ci);
+            hipLaunchKernelGGL(k_mesh_walk<false>, dim3(walk_grid), dim3(kMeshBlock), 0, s, a, ci);
+            hipLaunchKernelGGL(k_mesh_parity<false>, dim3(parity_grid), dim3(256), 0, s, a, ci);
+        }
     }
 }
 

@@ -175,9 +175,13 @@ __global__ __launch_bounds__(256) void k_cell_tables(int n, const uint32_t* __re
 __global__ __launch_bounds__(256) void k_shape_solve(ShapeArgs a) {
     const int count = min(a.cmax, *a.cc);
     for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < count; t += gridDim.x * blockDim.x) {
+        int pi = a.cpart[t];
+        if (a.remap) {   // contacts were generated in the previous slot order
+            pi = int(a.remap[pi]);
+            a.cpart[t] = pi;
+        }
         int s = a.cshape[t];
         int body = a.body.shape_body[s];
-        int pi = a.cpart[t];
         int pf = a.flags[pi];
         if ((pf & 1) == 0) continue;
         if (pf & 2) {
@@ -534,8 +538,9 @@ __global__ void k_zero_int(int* p, int count) {
 
 // Zeroes the counters a frame's first substep uses: particle contacts cc[0],
 // rigid contacts cc[4], mesh narrowphase queues cc[6], cc[7].
-__global__ void k_frame_begin(int* cc) {
+__global__ void k_frame_begin(int* cc, uint32_t* mcnt, int nmcnt) {
     if (threadIdx.x == 0) { cc[0] = 0; cc[4] = 0; cc[6] = 0; cc[7] = 0; }
+    for (int k = threadIdx.x; k < nmcnt; k += blockDim.x) mcnt[k] = 0u;
 }
 
 // ------------------------------------------------------------- launchers --
@@ -627,7 +632,9 @@ void launch_fill_zero(int* p, size_t count, hipStream_t s) {
     const unsigned grid = unsigned(std::min<size_t>((n4 + 255) / 256, 4096));
     hipLaunchKernelGGL(k_fill_zero, dim3(grid), dim3(256), 0, s, reinterpret_cast<int4*>(p), n4);
 }
-void launch_frame_begin(int* cc, hipStream_t s) { hipLaunchKernelGGL(k_frame_begin, dim3(1), dim3(64), 0, s, cc); }
+void launch_frame_begin(int* cc, uint32_t* mcnt, int nmcnt, hipStream_t s) {
+    hipLaunchKernelGGL(k_frame_begin, dim3(1), dim3(64), 0, s, cc, mcnt, nmcnt);
+}
 void launch_zero_int(int* p, int count, hipStream_t s) {
     if (count > 0) hipLaunchKernelGGL(k_zero_int, dim3(blocks_for(count, 256)), dim3(256), 0, s, p, count);
 }

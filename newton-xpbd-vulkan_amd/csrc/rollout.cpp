@@ -86,7 +86,7 @@ void Solver::rollout_forward(const float* x0, const float* v0, float relax, floa
     const float keep = relax_;
     relax_ = relax;
     try {
-        launch_frame_begin(cc_.p, stream_);
+        launch_frame_begin(cc_.p, mcnt_.p, int(mcnt_.n), stream_);
         for (int sub = 0; sub < substeps; sub++) {
             if (sub % every == 0) {
                 const int c = sub / every;
@@ -146,7 +146,7 @@ void Solver::rollout_backward(const float* gx, const float* gv, float* gx0, floa
             HIPCHECK(hipMemcpyAsync(V_[s_].p, t.ckV[seg].p, n_ * sizeof(float4), hipMemcpyDeviceToDevice, stream_));
             HIPCHECK(hipMemcpyAsync(flags_cur(), t.ckF[seg].p, n_ * sizeof(int32_t), hipMemcpyDeviceToDevice, stream_));
             HIPCHECK(hipMemcpyAsync(ids_cur(), t.ckI[seg].p, n_ * sizeof(uint32_t), hipMemcpyDeviceToDevice, stream_));
-            launch_frame_begin(cc_.p, stream_);
+            launch_frame_begin(cc_.p, mcnt_.p, int(mcnt_.n), stream_);
             for (int j = 0; j < s1 - s0; j++) {
                 HIPCHECK(hipMemcpyAsync(t.P0[j].p, P_[s_].p, n_ * sizeof(float4), hipMemcpyDeviceToDevice, stream_));
                 HIPCHECK(hipMemcpyAsync(t.V0[j].p, V_[s_].p, n_ * sizeof(float4), hipMemcpyDeviceToDevice, stream_));

@@ -58,7 +58,7 @@ void Solver::free_all() {
     dfree(range_); dfree(rkeys_); dfree(tbl_); dfree(cpart_); dfree(cshape_); dfree(cc_); dfree(cbpos_); dfree(cbvel_); dfree(cnorm_);
     dfree(dshape_); dfree(delta_); dfree(body_q_); dfree(body_qd_); dfree(body_com_); dfree(body_invI_);
     dfree(body_invm_); dfree(body_delta_); dfree(mat_mu_); dfree(body_flags_); dfree(shape_body_);
-    dfree(verts_d_); dfree(vvel_d_); dfree(tris_d_); dfree(prim_d_); dfree(err_d_); dfree(nodes_d_); dfree(cnodes_d_); dfree(lbox_d_); dfree(pg_start_); dfree(lb_); dfree(lb_node_); dfree(tsoup_); dfree(pg_soup_); dfree(mq_count_); dfree(mq_); dfree(pq_);
+    dfree(verts_d_); dfree(vvel_d_); dfree(tris_d_); dfree(prim_d_); dfree(err_d_); dfree(nodes_d_); dfree(cnodes_d_); dfree(lbox_d_); dfree(pg_start_); dfree(lb_); dfree(lb_node_); dfree(tsoup_); dfree(pg_soup_); dfree(mcnt_); dfree(mq_); dfree(pq_);
     dfree(info_d_); dfree(diag_header_); dfree(diag_pairs_); dfree(mesh_stats_);
     for (hipEvent_t e : ev_pool_) (void)hipEventDestroy(e);
     ev_pool_.clear();
@@ -95,8 +95,7 @@ void Solver::alloc_all() {
     dalloc(dshape_, 3 * n_); dalloc(delta_, 3 * n_);
     dalloc(diag_header_, 2);
     dalloc(mesh_stats_, 8);
-    dalloc(mq_count_, (n_ + 255) / 256 + 2);   // mesh candidate queue (+ total, parity count)
-    dalloc(mq_, n_);
+    dalloc(mq_, n_);   // mesh candidate queue (its counters are sized by finalize_meshes)
     dalloc(pq_, n_);
     max_pairs_ = cfg_.max_diag_pairs ? cfg_.max_diag_pairs : 100000u;
     dalloc(diag_pairs_, cfg_.enable_diagnostics ? size_t(20) * max_pairs_ : 20);
@@ -471,6 +470,7 @@ void Solver::finalize_meshes() {
     dgrid_.assign(M, DistGrid{});
     for (size_t c = 0; c < M; c++) build_query_grids(c);
     upload_query_grids();
+    dalloc(mcnt_, 4 * std::max<size_t>(M, 1));
     finalized_ = true;
 }
 
@@ -676,7 +676,8 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
         m.n = n; m.P = P_[s_].p; m.flags = flags_cur();
         m.verts = verts_d_.p; m.tris = tris_d_.p; m.nodes = nodes_d_.p; m.cnodes = cnodes_d_.p; m.lbox = lbox_d_.p; m.prim = prim_d_.p;
         m.pg_start = pg_start_.p; m.pg_soup = pg_soup_.p; m.tsoup = tsoup_.p; m.lb = lb_.p; m.lb_node = lb_node_.p;
-        m.mq_count = mq_count_.p; m.mq = mq_.p; m.pq = pq_.p;
+        m.mcnt = mcnt_.p; m.mq = mq_.p; m.pq = pq_.p;
+        m.slot = slot;
         m.vvel = vvel_d_.p; m.info = info_d_.p; m.nmesh = M;
         m.margin = cfg_.soft_contact_margin; m.eps = avg_edge_ * 1e-3f;
         m.cmax = int(cmax_); m.cc = cc_gen;
@@ -743,11 +744,12 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
         stage_begin(kRanges);
         build_ranges(keysA_.p);
         if (fork) HIPCHECK(hipStreamWaitEvent(stream_, join_ev_, 0));
-        // contacts were generated in the previous slot order
-        launch_remap_contacts(cpart_.p, cc_solve, int(cmax_), rank_.p, shape_grid, stream_);
         stage_end(kRanges);
         grid_valid_ = true;
+        // contacts were generated in the previous slot order: the first shape
+        // solve maps them through rank; with no iteration a kernel does
         if (iterations == 0) {   // no solve: the predicted state still moves to the sorted order
+            launch_remap_contacts(cpart_.p, cc_solve, int(cmax_), rank_.p, shape_grid, stream_);
             HIPCHECK(hipMemcpyAsync(P_[cur].p, Xs_[0].p, n_ * sizeof(float4), hipMemcpyDeviceToDevice, stream_));
             HIPCHECK(hipMemcpyAsync(V_[cur].p, Vs_[0].p, n_ * sizeof(float4), hipMemcpyDeviceToDevice, stream_));
         }
@@ -759,6 +761,7 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
         const float4* Vin = it == 0 ? Vs_[0].p : V_[cur].p;
         ShapeArgs sa;
         sa.cmax = int(cmax_); sa.cc = cc_solve;
+        sa.remap = (it == 0 && n > 0) ? rank_.p : nullptr;
         sa.cpart = cpart_.p; sa.cshape = cshape_.p; sa.cbpos = cbpos_.p; sa.cbvel = cbvel_.p; sa.cnorm = cnorm_.p;
         sa.P = Xin; sa.V = Vin; sa.flags = flags_cur();
         sa.body = body_args();
@@ -860,7 +863,7 @@ void Solver::record_frame(float dt, int substeps, int iterations) {
     HIPCHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
     hipGraph_t graph = nullptr;
     try {
-        launch_frame_begin(cc_.p, stream_);
+        launch_frame_begin(cc_.p, mcnt_.p, int(mcnt_.n), stream_);
         for (int sub = 0; sub < substeps; sub++) substep(sdt, iterations, true, sub & 1, true);
     } catch (...) {
         (void)hipStreamEndCapture(stream_, &graph);
@@ -943,7 +946,7 @@ void Solver::step(float dt, int substeps, int iterations) {
     if (use_graphs_ && !profiling_ && substeps > 0) {
         record_frame(dt, substeps, iterations);
     } else {
-        launch_frame_begin(cc_.p, stream_);
+        launch_frame_begin(cc_.p, mcnt_.p, int(mcnt_.n), stream_);
         for (int sub = 0; sub < substeps; sub++) substep(sdt, iterations, true, sub & 1, true);
     }
     if (substeps > 0) contact_slot_ = rigid_slot_ = (substeps - 1) & 1;
@@ -980,7 +983,7 @@ void Solver::collide_only() {
     require_init();
     if (!finalized_) finalize_meshes();
     DeviceGuard g(device_);
-    launch_frame_begin(cc_.p, stream_);
+    launch_frame_begin(cc_.p, mcnt_.p, int(mcnt_.n), stream_);
     contact_slot_ = 0;
     PlaneArgs pl;
     pl.n = int(n_); pl.P = P_[s_].p; pl.flags = flags_cur();
@@ -1002,7 +1005,8 @@ void Solver::collide_only() {
         m.n = int(n_); m.P = P_[s_].p; m.flags = flags_cur();
         m.verts = verts_d_.p; m.tris = tris_d_.p; m.nodes = nodes_d_.p; m.cnodes = cnodes_d_.p; m.lbox = lbox_d_.p; m.prim = prim_d_.p;
         m.pg_start = pg_start_.p; m.pg_soup = pg_soup_.p; m.tsoup = tsoup_.p; m.lb = lb_.p; m.lb_node = lb_node_.p;
-        m.mq_count = mq_count_.p; m.mq = mq_.p; m.pq = pq_.p;
+        m.mcnt = mcnt_.p; m.mq = mq_.p; m.pq = pq_.p;
+        m.slot = 0;
         m.vvel = vvel_d_.p; m.info = info_d_.p; m.nmesh = int(meshes_.size());
         m.margin = cfg_.soft_contact_margin; m.eps = avg_edge_ * 1e-3f;
         m.cmax = int(cmax_); m.cc = cc_.p;

@@ -248,7 +248,7 @@ private:
     std::vector<uint32_t> prim_h_;         // global triangle id per leaf position
     DevBuf<float4> tsoup_;                 // triangle corners in leaf order
     DevBuf<float4> pg_soup_;               // triangle corners of every parity-grid entry
-    DevBuf<uint32_t> mq_count_;            // mesh candidates per 256-particle block, scanned
+    DevBuf<uint32_t> mcnt_;                // mesh queue counters: 2 substep parities x meshes x 2
     DevBuf<int2> mq_;                      // mesh candidate queue (particle, start node)
     DevBuf<uint32_t> pq_;                  // watertight mesh contacts awaiting their sign
     DevBuf<MeshInfo> info_d_;

@@ -78,7 +78,8 @@ __device__ __forceinline__ BodyFrame load_body(const BodyArgs& b, int body) {
 
 struct ShapeArgs {
     int cmax; const int* cc;
-    const int* cpart; const int* cshape; const float4* cbpos; const float4* cbvel; const float4* cnorm;
+    const uint32_t* remap;      // non-null: map each contact's slot through it first (and store it)
+    int* cpart; const int* cshape; const float4* cbpos; const float4* cbvel; const float4* cnorm;
     const float4* P; const float4* V; const int* flags;
     BodyArgs body;
     float pmu, ka, dt, relax;
@@ -151,8 +152,8 @@ struct MeshArgs {
     const float* lb;            // distance lower bounds of all meshes' grids
     const uint32_t* lb_node;    // per bound cell: node the closest-face search starts at
     uint64_t* stats;            // optional query counters (6 words), null = off
-    uint32_t* mq_count;         // per 256-particle block candidate counts, then the total and the
-                                // parity-queue count (nb + 2 words)
+    uint32_t* mcnt;             // per (substep parity, mesh): queue length and parity-queue length
+    int slot;                   // substep parity: which counter pair this substep uses
     int2* mq;                   // candidate queue: (particle, search start node word)
     uint32_t* pq;               // watertight contacts whose normal sign the parity pass decides
 };
@@ -246,7 +247,7 @@ void launch_shape_solve(const ShapeArgs& a, int grid, hipStream_t s);
 void launch_pp_apply(const PPArgs& a, bool diag, bool validate, int block, hipStream_t s);
 void launch_restitution(const RestArgs& a, int grid, hipStream_t s);
 void launch_zero_int(int* p, int count, hipStream_t s);
-void launch_frame_begin(int* cc, hipStream_t s);
+void launch_frame_begin(int* cc, uint32_t* mcnt, int nmcnt, hipStream_t s);
 void launch_mesh_contacts(const MeshArgs& a, hipStream_t s);
 
 // Radix sort: one pass = upsweep (tile histograms) -> bin-row scan -> scatter.
