@@ -152,7 +152,7 @@ __device__ __forceinline__ float4 node_hi(const float4* nodes, uint32_t i) { ret
 template <bool STATS>
 __device__ bool closest_face(const MeshArgs& a, const MeshInfo& mi, uint32_t start, f3 p, float rad,
                              uint32_t (*stk)[kMeshBlock], int lt, uint32_t& face, float& sv_out, float& sw_out,
-                             float& sign, uint32_t& pops, uint32_t& tests) {
+                             float& sign, float& dsq_out, uint32_t& pops, uint32_t& tests) {
     const float eps = a.eps, eps2 = eps * eps;
     const float PI = 3.14159265359f;
     const float max_dist = (a.margin + mi.margin) + rad;
@@ -289,6 +289,7 @@ __device__ bool closest_face(const MeshArgs& a, const MeshInfo& mi, uint32_t sta
             }
         }
     }
+    dsq_out = min_dist_sq;
     if (wt) min_dist = exact_sqrt(min_dist_sq);
     if (!(found && min_dist < max_dist)) return false;
     face = min_face;
@@ -378,33 +379,98 @@ __global__ __launch_bounds__(256) void k_mesh_enqueue(MeshArgs a, int ci) {
     }
 }
 
-// Pass 2: one queued particle per thread, one wave per block so the walks
-// spread over every SIMD; contact slots reserved once per block, then the
-// payloads (generate_mesh_contacts.comp:333-411).  Grid-stride over the
-// queue with a block-uniform trip count.  A watertight mesh's contacts are
+// Up to four subtree roots below `start`, in the order the depth-first
+// search of closest_face reaches them with threshold th (the search's
+// initial one): the near child's children (expanded from this record, near
+// first), then the far child's children (popped later and expanded by the
+// same rule as any popped node).  A leaf child is a root itself.
+__device__ int split_roots(const MeshArgs& a, uint32_t start, f3 p, float th, uint32_t roots[4]) {
+    if (start >> 31) { roots[0] = start; return 1; }
+    const float4* rec = a.cnodes + 12 * size_t(start);
+    float4 R[12];
+#pragma unroll
+    for (int q = 0; q < 12; q++) R[q] = rec[q];
+    const float lds = dist_sq_aabb(p, xyz(R[0]), xyz(R[1]));
+    const float rds = dist_sq_aabb(p, xyz(R[2]), xyz(R[3]));
+    const bool pl = lds < th, pr = rds < th;
+    if (!pl && !pr) return 0;
+    const bool near_left = lds < rds ? pl : !pr;
+    int n = 0;
+    // the near child, expanded at once from this record
+    {
+        const uint32_t w = __float_as_uint(near_left ? R[0].w : R[2].w);
+        if (w >> 31) {
+            roots[n++] = w;
+        } else {
+            const int g = near_left ? 4 : 8;
+            const float cl = dist_sq_aabb(p, xyz(R[g]), xyz(R[g + 1]));
+            const float cr = dist_sq_aabb(p, xyz(R[g + 2]), xyz(R[g + 3]));
+            const uint32_t clw = __float_as_uint(R[g].w), crw = __float_as_uint(R[g + 2].w);
+            if (cl < cr) {
+                if (cl < th) roots[n++] = clw;
+                if (cr < th) roots[n++] = crw;
+            } else {
+                if (cr < th) roots[n++] = crw;
+                if (cl < th) roots[n++] = clw;
+            }
+        }
+    }
+    // the far child, when both were pushed: a popped node's rule on its children
+    if (pl && pr) {
+        const uint32_t w = __float_as_uint(near_left ? R[2].w : R[0].w);
+        if (w >> 31) {
+            roots[n++] = w;
+        } else {
+            const int g = near_left ? 8 : 4;
+            const float l2 = dist_sq_aabb(p, xyz(R[g]), xyz(R[g + 1]));
+            const float r2 = dist_sq_aabb(p, xyz(R[g + 2]), xyz(R[g + 3]));
+            const bool ql = l2 < th, qr = r2 < th;
+            const uint32_t lw2 = __float_as_uint(R[g].w), rw2 = __float_as_uint(R[g + 2].w);
+            if (ql || qr) {
+                const bool nl = l2 < r2 ? ql : !qr;
+                roots[n++] = nl ? lw2 : rw2;
+                if (ql && qr) roots[n++] = nl ? rw2 : lw2;
+            }
+        }
+    }
+    return n;
+}
+
+// Pass 2: the tree walk, one wave per workgroup so the walks spread over
+// every SIMD; contact slots reserved once per block, then the payloads
+// (generate_mesh_contacts.comp:333-411).  Grid-stride over the queue with a
+// block-uniform trip count.  G = 1: one queued particle per lane.  G = 4
+// (watertight meshes): four lanes per particle, each searching one of the
+// subtree roots of split_roots with its own threshold; the group keeps the
+// smallest squared distance, ties to the earliest root.  A lane prunes only
+// what the single search would prune (its threshold is never below the
+// single search's at the same node), and the single search keeps the first
+// face in its visiting order among those at the minimum, which is the
+// earliest root's: the face is the same.  A watertight mesh's contacts are
 // written with the outside sign; those with a nonzero offset are queued for
 // k_mesh_parity, which flips the normal of the inside ones (the contact point
 // does not depend on the sign: (-n)(-d) = n d exactly).
-template <bool STATS>
+template <bool STATS, int G>
 __global__ __launch_bounds__(kMeshBlock) void k_mesh_walk(MeshArgs a, int ci) {
     __shared__ uint32_t stk[kStack][kMeshBlock];
     __shared__ int s_tmp[kMeshBlock / 64 + 1];
+    constexpr uint32_t PER = kMeshBlock / G;   // queue entries per block step
     uint32_t st[6] = {0, 0, 0, 0, 0, 0};
     const MeshInfo mi = a.info[ci];
-    const int lt = threadIdx.x;
+    const int lt = threadIdx.x, r = lt % G;
     uint32_t* const cnt = mesh_counters(a, ci, a.slot);
     const uint32_t total = cnt[0];
     // Workgroups are dealt to the eight XCDs round-robin; give each XCD one
-    // contiguous eighth of the queue (spatially sorted particles), so the
+    // contiguous eighth of the queue (spatially coherent particles), so the
     // tree nodes its walks share stay in that XCD's L2.
     const uint32_t xcd = blockIdx.x % kXcds, nbx = gridDim.x / kXcds;
     const uint32_t chunk = (total + kXcds - 1) / kXcds;
     const uint32_t lo = min(total, xcd * chunk), hi = min(total, lo + chunk);
-    for (uint32_t base = lo + (blockIdx.x / kXcds) * kMeshBlock; base < hi; base += nbx * kMeshBlock) {
-        const uint32_t q = base + uint32_t(lt);
+    for (uint32_t base = lo + (blockIdx.x / kXcds) * PER; base < hi; base += nbx * PER) {
+        const uint32_t q = base + uint32_t(lt / G);
         bool hit = false;
         uint32_t face = 0;
-        float sv = 0.f, sw = 0.f, sg = 1.0f;
+        float sv = 0.f, sw = 0.f, sg = 1.0f, dsq = 0.f;
         int pi = 0;
         float4 P = make_float4(0.f, 0.f, 0.f, 0.f);
         if (q < hi) {
@@ -412,8 +478,31 @@ __global__ __launch_bounds__(kMeshBlock) void k_mesh_walk(MeshArgs a, int ci) {
             pi = e.x;
             P = a.P[pi];
             const f3 p = xyz(P);
-            if (STATS) st[1]++;
-            hit = closest_face<STATS>(a, mi, uint32_t(e.y), p, P.w, stk, lt, face, sv, sw, sg, st[2], st[3]);
+            if (STATS && r == 0) st[1]++;
+            if (G == 1) {
+                hit = closest_face<STATS>(a, mi, uint32_t(e.y), p, P.w, stk, lt, face, sv, sw, sg, dsq, st[2], st[3]);
+            } else {
+                const float max_dist = (a.margin + mi.margin) + P.w;
+                uint32_t roots[4];
+                const int nr = split_roots(a, uint32_t(e.y), p, max_dist * max_dist, roots);
+                if (r < nr)
+                    hit = closest_face<STATS>(a, mi, roots[r], p, P.w, stk, lt, face, sv, sw, sg, dsq, st[2], st[3]);
+            }
+        }
+        if (G > 1) {   // butterfly over the group: (hit, dsq, rank) order, payload carried along
+            int rank = r;
+#pragma unroll
+            for (int o = 1; o < G; o <<= 1) {
+                const bool h2 = __shfl_xor(int(hit), o) != 0;
+                const float d2 = __shfl_xor(dsq, o);
+                const int r2 = __shfl_xor(rank, o);
+                const uint32_t f2 = uint32_t(__shfl_xor(int(face), o));
+                const float sv2 = __shfl_xor(sv, o), sw2 = __shfl_xor(sw, o);
+                if (h2 && (!hit || d2 < dsq || (d2 == dsq && r2 < rank))) {
+                    hit = true; dsq = d2; rank = r2; face = f2; sv = sv2; sw = sw2;
+                }
+            }
+            hit = hit && r == 0;   // the group's first lane writes the contact
         }
         if (STATS) st[5] += hit ? 1u : 0u;
         const int slot = block_reserve(a.cc, hit ? 1 : 0, s_tmp);
@@ -500,20 +589,25 @@ __global__ __launch_bounds__(256) void k_mesh_parity(MeshArgs a, int ci) {
     }
 }
 
-void launch_mesh_contacts(const MeshArgs& a, hipStream_t s) {
+void launch_mesh_contacts(const MeshArgs& a, const uint8_t* watertight, hipStream_t s) {
     if (a.n <= 0 || a.nmesh <= 0) return;
     const int nb = (a.n + 255) / 256;
-    // a multiple of the XCD count, so every XCD gets the same number of workgroups
-    const unsigned walk_grid = unsigned(std::max(1, std::min(4096, (a.n + kMeshBlock - 1) / kMeshBlock) / 8)) * 8u;
+    // walk grids are multiples of the XCD count, so every XCD gets the same
+    // number of workgroups
+    const unsigned walk_grid1 = unsigned(std::max(1, std::min(4096, (a.n + kMeshBlock - 1) / kMeshBlock) / 8)) * 8u;
+    const unsigned walk_grid4 = unsigned(std::max(1, std::min(4096, (a.n + kMeshBlock / 4 - 1) / (kMeshBlock / 4)) / 8)) * 8u;
     const unsigned parity_grid = unsigned(std::min(2048, (a.n + 255 / kParityLanes) / (256 / kParityLanes)));
     for (int ci = 0; ci < a.nmesh; ci++) {
+        const bool split = watertight[ci] != 0;
         if (a.stats) {
             hipLaunchKernelGGL(k_mesh_enqueue<true>, dim3(nb), dim3(256), 0, s, a, ci);
-            hipLaunchKernelGGL(k_mesh_walk<true>, dim3(walk_grid), dim3(kMeshBlock), 0, s, a, ci);
+            if (split) hipLaunchKernelGGL((k_mesh_walk<true, 4>), dim3(walk_grid4), dim3(kMeshBlock), 0, s, a, ci);
+            else hipLaunchKernelGGL((k_mesh_walk<true, 1>), dim3(walk_grid1), dim3(kMeshBlock), 0, s, a, ci);
             hipLaunchKernelGGL(k_mesh_parity<true>, dim3(parity_grid), dim3(256), 0, s, a, ci);
         } else {
             hipLaunchKernelGGL(k_mesh_enqueue<false>, dim3(nb), dim3(256), 0, s, a, ci);
-            hipLaunchKernelGGL(k_mesh_walk<false>, dim3(walk_grid), dim3(kMeshBlock), 0, s, a, ci);
+            if (split) hipLaunchKernelGGL((k_mesh_walk<false, 4>), dim3(walk_grid4), dim3(kMeshBlock), 0, s, a, ci);
+            else hipLaunchKernelGGL((k_mesh_walk<false, 1>), dim3(walk_grid1), dim3(kMeshBlock), 0, s, a, ci);
             hipLaunchKernelGGL(k_mesh_parity<false>, dim3(parity_grid), dim3(256), 0, s, a, ci);
         }
     }

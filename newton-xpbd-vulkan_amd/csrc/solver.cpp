@@ -471,6 +471,11 @@ void Solver::finalize_meshes() {
     for (size_t c = 0; c < M; c++) build_query_grids(c);
     upload_query_grids();
     dalloc(mcnt_, 4 * std::max<size_t>(M, 1));
+    // watertight meshes walk with four lanes per particle (VKXPBD_MESH_LANES=1: one)
+    const char* lenv = std::getenv("VKXPBD_MESH_LANES");
+    const bool split = !(lenv && std::strcmp(lenv, "1") == 0);
+    mesh_split_.assign(M, 0);
+    for (size_t c = 0; c < M; c++) mesh_split_[c] = (split && meshes_[c].watertight) ? 1 : 0;
     finalized_ = true;
 }
 
@@ -685,7 +690,7 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
         m.err = reinterpret_cast<uint32_t*>(cc_.p + 2);
         m.stats = mesh_stats_on_ ? mesh_stats_.p : nullptr;
         stage_begin(kMesh);
-        launch_mesh_contacts(m, fork ? side_ : stream_);
+        launch_mesh_contacts(m, mesh_split_.data(), fork ? side_ : stream_);
         stage_end(kMesh);
         if (fork) HIPCHECK(hipEventRecord(join_ev_, side_));
     }
@@ -1013,7 +1018,7 @@ void Solver::collide_only() {
         m.cpart = cpart_.p; m.cshape = cshape_.p; m.cbpos = cbpos_.p; m.cbvel = cbvel_.p; m.cnorm = cnorm_.p;
         m.err = reinterpret_cast<uint32_t*>(cc_.p + 2);
         m.stats = mesh_stats_on_ ? mesh_stats_.p : nullptr;
-        launch_mesh_contacts(m, stream_);
+        launch_mesh_contacts(m, mesh_split_.data(), stream_);
     }
     HIPCHECK(hipGetLastError());
     read_count(0);
