@@ -245,7 +245,7 @@ __global__ __launch_bounds__(256) void k_adj_pp(AdjArgs a) {
                 for (int cy = ys; cy <= ye; cy++)
                     for (int cx = xs; cx <= xe; cx++) {
                         if (gap2(ux, cx) + gap2(uy, cy) + gap2(uz, cz) > lu2) continue;
-                        const uint32_t tb = VALIDATE ? base : (a.planes ? wrap_plane(cx, cy, cz) << kCellBits : 0u);
+                        const uint32_t tb = VALIDATE ? base : (a.planes ? wrap_plane(cx, cy, cz, a.planes) << kCellBits : 0u);
                         const uint32_t cell = tb + uint32_t(cell_index(cx, cy, cz));
                         int2 rg = range[size_t(cell)];
                         if (VALIDATE && !(rg.x >= 0 && rg.x < a.n && rg.y > rg.x && a.keys[rg.x] == cell)) continue;

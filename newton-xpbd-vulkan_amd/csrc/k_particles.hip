@@ -20,12 +20,14 @@
 
 namespace xpbd {
 
-// Warp-striped tile mapping shared with the radix-sort kernels: wave w of a
-// 256-thread block owns keys [w*256, w*256+256) of the tile, 64 per step, so
-// item order within a tile is (wave, step, lane) = input order.
+// k_integrate's blocking: kIntItems slots per thread, wave-striped (wave w of
+// a 256-thread block owns slots [w*64*kIntItems, (w+1)*64*kIntItems) of the
+// block's range, 64 per step).
+constexpr int kIntItems = 4;
+constexpr int kIntTile = 256 * kIntItems;
 __device__ __forceinline__ int tile_item(int tile, int it) {
     int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    return tile * kSortTile + w * (kSortItems * kWave) + it * kWave + lane;
+    return tile * kIntTile + w * (kIntItems * kWave) + it * kWave + lane;
 }
 
 // ---------------------------------------------------------------------------
@@ -45,10 +47,10 @@ __global__ __launch_bounds__(256) void k_integrate(IntegrateArgs a) {
         if (a.rcc_next != nullptr) *a.rcc_next = 0;
         if (a.mesh_next != nullptr) { a.mesh_next[0] = 0; a.mesh_next[1] = 0; }
     }
-    float4 P[kSortItems], V[kSortItems];
-    int f[kSortItems];
+    float4 P[kIntItems], V[kIntItems];
+    int f[kIntItems];
 #pragma unroll
-    for (int it = 0; it < kSortItems; it++) {   // issue every load of the tile first
+    for (int it = 0; it < kIntItems; it++) {   // issue every load of the tile first
         const int idx = tile_item(tile, it);
         P[it] = make_float4(0.f, 0.f, 0.f, 0.f);
         V[it] = P[it];
@@ -57,10 +59,10 @@ __global__ __launch_bounds__(256) void k_integrate(IntegrateArgs a) {
     }
     // plane contacts of this thread's items (on the pre-integration position),
     // slots reserved once per block
-    bool want[kSortItems];
+    bool want[kIntItems];
     int mine = 0;
 #pragma unroll
-    for (int it = 0; it < kSortItems; it++) {
+    for (int it = 0; it < kIntItems; it++) {
         const int idx = tile_item(tile, it);
         want[it] = false;
         if (a.has_plane && idx < a.n && (f[it] & 1)) {
@@ -73,7 +75,7 @@ __global__ __launch_bounds__(256) void k_integrate(IntegrateArgs a) {
         __shared__ int s_tmp[256 / 64 + 1];
         int slot = block_reserve(a.cc, mine, s_tmp);
 #pragma unroll
-        for (int it = 0; it < kSortItems; it++) {
+        for (int it = 0; it < kIntItems; it++) {
             if (!want[it]) continue;
             if (slot < a.cmax) {
                 a.cpart[slot] = tile_item(tile, it);
@@ -86,7 +88,7 @@ __global__ __launch_bounds__(256) void k_integrate(IntegrateArgs a) {
         }
     }
 #pragma unroll
-    for (int it = 0; it < kSortItems; it++) {
+    for (int it = 0; it < kIntItems; it++) {
         const int idx = tile_item(tile, it);
         const bool valid = idx < a.n;
         if (!valid) continue;
@@ -106,7 +108,7 @@ __global__ __launch_bounds__(256) void k_integrate(IntegrateArgs a) {
         a.Vp[idx] = Vn;
         uint32_t key = cell_of(Pn.x, Pn.y, Pn.z, a.inv_cell);
         if (a.scene != nullptr) key |= uint32_t(a.scene[idx]) << kCellBits;   // scenes keep their slot ranges
-        else if (a.planes) key |= plane_of(Pn.x, Pn.y, Pn.z, a.inv_cell) << kCellBits;
+        else if (a.planes) key |= plane_of(Pn.x, Pn.y, Pn.z, a.inv_cell, a.planes) << kCellBits;
         a.kv[a.ids[idx]] = make_uint2(key, uint32_t(idx));
     }
 }
@@ -347,7 +349,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6, 8))) v
     // with wrap planes each visited virtual cell selects its plane's table
     const uint32_t sbase = VALIDATE ? (a.keys[t] & ~((1u << kCellBits) - 1u)) : 0u;
     auto table = [&](int cx, int cy, int cz) -> uint32_t {
-        return VALIDATE ? sbase : (a.planes ? wrap_plane(cx, cy, cz) << kCellBits : 0u);
+        return VALIDATE ? sbase : (a.planes ? wrap_plane(cx, cy, cz, a.planes) << kCellBits : 0u);
     };
     auto raw = [&](uint32_t base, int cell) -> int2 { return range[size_t(base) + size_t(cell)]; };
     auto check = [&](uint32_t base, int cell, int2 rg) -> int2 {
@@ -547,7 +549,7 @@ __global__ void k_frame_begin(int* cc, uint32_t* mcnt, int nmcnt) {
 static inline unsigned blocks_for(long long n, int bs) { return unsigned((n + bs - 1) / bs); }
 
 void launch_integrate(const IntegrateArgs& a, hipStream_t s) {
-    hipLaunchKernelGGL(k_integrate, dim3(a.tiles), dim3(256), 0, s, a);
+    if (a.n > 0) hipLaunchKernelGGL(k_integrate, dim3(blocks_for(a.n, kIntTile)), dim3(256), 0, s, a);
 }
 void launch_plane_contacts(const PlaneArgs& a, hipStream_t s) {
     if (a.n > 0) hipLaunchKernelGGL(k_plane_contacts, dim3(blocks_for(a.n, 256)), dim3(256), 0, s, a);

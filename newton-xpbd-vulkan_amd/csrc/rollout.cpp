@@ -176,7 +176,7 @@ void Solver::rollout_backward(const float* gx, const float* gv, float* gx0, floa
                 a.dt = t.sdt; a.relax = t.relax; a.inv_cell = inv_cell; a.vmax = cfg_.particle_v_max;
                 a.gx = cfg_.gravity[0]; a.gy = cfg_.gravity[1]; a.gz = cfg_.gravity[2];
                 a.rmax = rmax_;
-                a.planes = planes_ ? 1 : 0;
+                a.planes = planes_ ? plane_bits_ : 0;
                 launch_adj_prep(a, stream_);
                 AdjShapeArgs sa;
                 sa.cmax = int(cmax_); sa.cc = t.counts.p + j;

@@ -133,6 +133,8 @@ void Solver::init_common(const xpbd_config& cfg, size_t n, size_t max_contacts) 
     // scenes; the environment can force them either way
     const char* penv = std::getenv("VKXPBD_WRAP_PLANES");
     planes_allowed_ = penv ? std::strcmp(penv, "1") == 0 : n_ >= kPlanesMinParticles;
+    const char* benv = std::getenv("VKXPBD_PLANE_BITS");
+    plane_bits_ = (benv && std::strcmp(benv, "1") == 0) ? 1 : 2;
     planes_ = false;
     update_planes();
     grid_valid_ = false;
@@ -587,8 +589,8 @@ void Solver::update_planes() {
     HIPCHECK(hipStreamSynchronize(stream_));
     invalidate_graphs();
     planes_ = want;
-    if (num_scenes_ == 1) set_sort_shape(planes_ ? kCellBits + kPlaneBits : kCellBits);
-    dalloc(range_, size_t(2) * (size_t(kNumCells) << (planes_ ? kPlaneBits : 0)) * num_scenes_);
+    if (num_scenes_ == 1) set_sort_shape(planes_ ? kCellBits + 3 * plane_bits_ : kCellBits);
+    dalloc(range_, size_t(2) * (size_t(kNumCells) << (planes_ ? 3 * plane_bits_ : 0)) * num_scenes_);
     if (planes_) dalloc(rkeys_, n_);   // zero keys: the first reset touches row 0 only
     else dfree(rkeys_);
     HIPCHECK(hipDeviceSynchronize());
@@ -717,7 +719,7 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
     }
     if (n > 0) {
         IntegrateArgs a;
-        a.n = n; a.tiles = tiles;
+        a.n = n;
         a.Ps = P_[s_].p; a.Vs = V_[s_].p; a.flags = flags_cur(); a.Pp = P_[p_].p; a.Vp = V_[p_].p;
         a.ids = ids_cur(); a.kv = kv_.p;
         a.dt = sdt; a.vmax = cfg_.particle_v_max;
@@ -730,7 +732,7 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
         a.rcc_next = zero_next ? cc_.p + 4 + (slot ^ 1) : nullptr;
         a.mesh_next = zero_next ? cc_.p + 6 : nullptr;
         a.scene = num_scenes_ > 1 ? scene_d_.p : nullptr;
-        a.planes = planes_ ? 1 : 0;
+        a.planes = planes_ ? plane_bits_ : 0;
         a.cpart = cpart_.p; a.cshape = cshape_.p; a.cbpos = cbpos_.p; a.cbvel = cbvel_.p; a.cnorm = cnorm_.p;
         stage_begin(kIntegrate);
         launch_integrate(a, stream_);
@@ -782,7 +784,7 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
         pa.mu = cfg_.particle_mu; pa.coh = cfg_.particle_cohesion; pa.max_r = cfg_.particle_max_radius;
         pa.dt = sdt; pa.relax = relax_; pa.inv_cell = inv_cell; pa.vmax = cfg_.particle_v_max;
         pa.rmax = rmax_;
-        pa.planes = planes_ ? 1 : 0;
+        pa.planes = planes_ ? plane_bits_ : 0;
         pa.iter_index = uint32_t(it); pa.max_diag_pairs = max_pairs_;
         pa.diag_header = diag_header_.p; pa.diag_pairs = diag_pairs_.p;
         stage_begin(kPP);

@@ -155,6 +155,7 @@ private:
     // plain solver: sort keys carry the wrap plane above the cell (see update_planes)
     static constexpr size_t kPlanesMinParticles = size_t(1) << 18;
     bool planes_ = false, planes_allowed_ = false;
+    int plane_bits_ = 2;   // wrap-plane bits per axis (1 or 2)
     int pp_block_ = 256;
     // captured frames: rotation state and parameters -> executable graph
     struct FrameGraph {

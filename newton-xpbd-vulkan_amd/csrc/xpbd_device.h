@@ -141,22 +141,22 @@ __device__ __forceinline__ int cell_index(int x, int y, int z) {
 }
 
 // Wrap plane of a virtual cell: which 128-cell wrap of the hash grid it lies
-// in, two bits per axis, after the same origin offset and clamp as
+// in, pb bits per axis (1 or 2), after the same origin offset and clamp as
 // cell_index.  Particles that share a physical cell but lie in different
 // wraps are at least 128 - 3 cells apart; when no accepted pair can be that
 // long, such particles never interact, and a sort key of plane << 21 | cell
 // keeps each wrap's particles in a run of their own (in particle order, as
 // the plain key does).
-constexpr int kPlaneBits = 6;
-__device__ __forceinline__ uint32_t wrap_plane(int x, int y, int z) {
+__device__ __forceinline__ uint32_t wrap_plane(int x, int y, int z, int pb) {
     x += kGridOrigin; y += kGridOrigin; z += kGridOrigin;
     x = x < 0 ? 0 : x; y = y < 0 ? 0 : y; z = z < 0 ? 0 : z;
-    return (uint32_t((z >> 7) & 3) << 4) | (uint32_t((y >> 7) & 3) << 2) | uint32_t((x >> 7) & 3);
+    const int m = (1 << pb) - 1;
+    return (uint32_t((z >> 7) & m) << (2 * pb)) | (uint32_t((y >> 7) & m) << pb) | uint32_t((x >> 7) & m);
 }
 static_assert(kGridDim == 128, "wrap_plane shifts by log2(kGridDim)");
 // Wrap plane of the cell holding a position (wrap_plane of its virtual cell).
-__device__ __forceinline__ uint32_t plane_of(float px, float py, float pz, float inv) {
-    return wrap_plane((int)floorf(px * inv), (int)floorf(py * inv), (int)floorf(pz * inv));
+__device__ __forceinline__ uint32_t plane_of(float px, float py, float pz, float inv, int pb) {
+    return wrap_plane((int)floorf(px * inv), (int)floorf(py * inv), (int)floorf(pz * inv), pb);
 }
 
 // Wave-aggregated slot reservation on a global counter: one atomic per wave

@@ -40,7 +40,7 @@ struct BodyArgs {
 };
 
 struct IntegrateArgs {
-    int n, tiles;
+    int n;
     const float4* Ps; const float4* Vs; const int* flags;   // slot order
     float4* Pp; float4* Vp;
     const uint32_t* ids;                // particle id of each slot
