@@ -60,7 +60,7 @@ def stage_bytes(stage, N, C, Cm, U, tiles, mesh_bytes, passes):
 
 # profiler stage -> kernels it brackets (for the measured-traffic lookup)
 STAGE_KERNELS = {
-    "mesh_contacts": ["k_mesh_count", "k_mesh_scan", "k_mesh_enqueue", "k_mesh_walk"],
+    "mesh_contacts": ["k_mesh_enqueue", "k_mesh_walk", "k_mesh_parity"],
     "integrate": ["k_integrate"],
     "sort_upsweep": ["k_sort_upsweep"],
     "sort_binscan": ["k_sort_binscan"],
