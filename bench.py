@@ -51,8 +51,11 @@ def stage_bytes(stage, N, C, Cm, U, tiles, mesh_bytes, passes):
         return 4 * N + 8 * U + 4 * N + 8 * U + 4 * N + 12 * C
     if stage == "shape_solve":    # contact record, particle P,V gather, delta RMW
         return C * (56 + 32 + 24)
-    if stage == "pp_solve_apply":  # vals, flags, dshape rw, Xs/Vs, x0, delta, P/V out; cell ranges
-        return N * (4 + 4 + 24 + 32 + 16 + 12 + 32) + 8 * U
+    if stage == "pp_solve_apply":  # vals, flags, Xs/Vs, x0, delta, P/V out; cell ranges
+        return N * (4 + 4 + 32 + 16 + 12 + 32) + 8 * U
+    if stage == "shape_apply":     # per contact: particle slot, owner; per owner (taken as every
+        # contact): owner clear, shape delta rw, delta rw, Xs/Vs, slot + x0, P/V out
+        return C * (4 + 4 + 4 + 24 + 24 + 32 + 4 + 16 + 32)
     if stage == "mesh_contacts":
         return N * (16 + 4) + 56 * Cm + mesh_bytes
     return 0
@@ -68,6 +71,7 @@ STAGE_KERNELS = {
     "cell_ranges": ["k_fill_zero", "k_clear_ranges", "k_cell_ranges", "k_remap_contacts"],
     "shape_solve": ["k_shape_solve"],
     "pp_solve_apply": ["k_pp_apply"],
+    "shape_apply": ["k_shape_apply"],
 }
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_step_pile1M_icosphere.txt")
 

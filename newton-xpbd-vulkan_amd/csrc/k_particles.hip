@@ -12,7 +12,8 @@
 //   grid_offsets                            -> k_cell_ranges (no table clear)
 //   solve_particle_shape_contacts           -> k_shape_solve
 //   solve_particle_particle_contacts +
-//   apply_particle_deltas                   -> k_pp_apply
+//   apply_particle_deltas                   -> k_pp_apply (+ k_shape_apply for
+//                                              particles with shape deltas)
 //   apply_particle_shape_restitution        -> k_restitution
 #include "xpbd_kernels.h"
 
@@ -172,8 +173,9 @@ __global__ __launch_bounds__(256) void k_cell_tables(int n, const uint32_t* __re
     if (t == n - 1) ends[c] = t + 1;
 }
 
-// solve_particle_shape_contacts.comp:104-200.  Accumulates into dshape, which
-// k_pp_apply consumes and re-zeroes.
+// solve_particle_shape_contacts.comp:104-200.  Accumulates into dshape and
+// marks each receiving particle with its largest contact index (owner), which
+// k_shape_apply consumes and clears.
 __global__ __launch_bounds__(256) void k_shape_solve(ShapeArgs a) {
     const int count = min(a.cmax, *a.cc);
     for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < count; t += gridDim.x * blockDim.x) {
@@ -232,6 +234,7 @@ __global__ __launch_bounds__(256) void k_shape_solve(ShapeArgs a) {
                    fma_(vt_dir.z, lambda_f, -delta_n.z)};
         f3 dtot = exact_div(diff, denom) * a.relax;
         f3 fd = w1 * dtot;
+        atomicMax(&a.owner[pi], t);
         atomicAdd(&a.dshape[3 * pi + 0], fd.x);
         atomicAdd(&a.dshape[3 * pi + 1], fd.y);
         atomicAdd(&a.dshape[3 * pi + 2], fd.z);
@@ -299,15 +302,33 @@ __device__ __forceinline__ void pp_pair(const PPArgs& a, int t, int k, float4 Pj
     }
 }
 
+// apply_particle_deltas.comp:44-57 for sorted slot t: position x (radius
+// rad, inverse mass w1) moved by its accumulated delta dl.
+__device__ __forceinline__ void apply_delta(const float4* Ps, const uint32_t* vals, int t, f3 x, float rad, float w1,
+                                            f3 dl, float dt, float vmax, float4* Po, float4* Vo) {
+    const f3 x0 = xyz(Ps[vals[t]]);
+    f3 xn = x + dl;
+    f3 vnew = markstein_div(xn - x0, dt);
+    const float mag = len3(vnew);
+    if (mag > vmax) {
+        vnew = vnew * (vmax / mag);
+        xn = x0 + vnew * dt;
+    }
+    Po[t] = make_float4(xn.x, xn.y, xn.z, rad);
+    Vo[t] = make_float4(vnew.x, vnew.y, vnew.z, w1);
+}
+
 // solve_particle_particle_contacts.comp:107-218 fused with
 // apply_particle_deltas.comp:38-58.  One thread per sorted slot t (the
-// reference's thread order), particle i = ids[t]; the output state, deltas
-// and shape deltas live in sorted order, the substep-start positions at slot
-// vals[t] of the previous order.  Particle i's delta is
-// (shape contributions) + (its own particle-particle sum): the reference
-// accumulates exactly these with float atomics into a zeroed buffer, and with
-// at most two shape contacts per particle the sum is order independent, so a
-// single thread can finish the particle and write the applied state directly.
+// reference's thread order), particle i = ids[t]; the output state and deltas
+// live in sorted order, the substep-start positions at slot vals[t] of the
+// previous order.  Particle i's delta is (shape contributions) + (its own
+// particle-particle sum): the reference accumulates exactly these with float
+// atomics into a zeroed buffer, the shape dispatch first, and with at most two
+// shape contacts per particle their sum is order independent.  This kernel
+// finishes every particle as if its shape sum were +0 (the sum never rounds
+// to -0, so 0 + pp is what the reference adds), independently of the shape
+// solve; k_shape_apply then redoes the few particles whose sum is not.
 //
 // Neighbour state is read from the sorted copies: the query box visits cells z
 // -> y -> x -> sorted slot, and the x-adjacent cells of one row are consecutive
@@ -324,11 +345,9 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6, 8))) v
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= a.n) return;
     const int pf = a.Fs[t];
-    const f3 ds = {a.dshape[3 * t], a.dshape[3 * t + 1], a.dshape[3 * t + 2]};
-    a.dshape[3 * t] = 0.0f; a.dshape[3 * t + 1] = 0.0f; a.dshape[3 * t + 2] = 0.0f;
     const float4 Pi = a.Xs[t], Vi = a.Vs[t];
     if ((pf & 1) == 0) {   // inactive: passthrough (the reference leaves x_out unwritten)
-        a.delta[3 * t] = ds.x; a.delta[3 * t + 1] = ds.y; a.delta[3 * t + 2] = ds.z;
+        a.delta[3 * t] = 0.0f; a.delta[3 * t + 1] = 0.0f; a.delta[3 * t + 2] = 0.0f;
         a.Po[t] = Pi;
         a.Vo[t] = Vi;
         return;
@@ -467,20 +486,32 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6, 8))) v
     drain();
     const f3 dw = {fma_(acc.x, w1, 0.f), fma_(acc.y, w1, 0.f), fma_(acc.z, w1, 0.f)};
     const f3 tot = {fma_(dw.x, a.relax, 0.f), fma_(dw.y, a.relax, 0.f), fma_(dw.z, a.relax, 0.f)};
-    const f3 dl = ds + tot;
+    const f3 dl = f3{0.f, 0.f, 0.f} + tot;
     a.delta[3 * t] = dl.x; a.delta[3 * t + 1] = dl.y; a.delta[3 * t + 2] = dl.z;
-    // apply_particle_deltas.comp:44-57
-    const f3 x0 = xyz(a.Ps[a.vals[t]]);
-    f3 xn = x + dl;
-    f3 vnew = markstein_div(xn - x0, a.dt);
-    const float mag = len3(vnew);
-    if (mag > a.vmax) {
-        vnew = vnew * (a.vmax / mag);
-        xn = x0 + vnew * a.dt;
+    apply_delta(a.Ps, a.vals, t, x, rad, w1, dl, a.dt, a.vmax, a.Po, a.Vo);
+}
+
+// The shape half of the delta apply: runs after both k_shape_solve and
+// k_pp_apply.  Each particle that received a shape delta is redone by one
+// thread, its owner's (the largest contact index that added to it): delta =
+// (shape sum) + (the particle-particle delta k_pp_apply stored, 0 + pp, which
+// adds to any shape sum exactly as pp does, the sum never being -0), applied
+// as k_pp_apply applies it.  The owner clears the particle's shape sum and
+// mark for the next solve.
+__global__ __launch_bounds__(256) void k_shape_apply(ShapeApplyArgs a) {
+    const int count = min(a.cmax, *a.cc);
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < count; t += gridDim.x * blockDim.x) {
+        const int pi = a.cpart[t];
+        if (a.owner[pi] != t) continue;
+        a.owner[pi] = -1;
+        float* ds = a.dshape + 3 * pi;
+        float* dp = a.delta + 3 * pi;
+        const f3 dl = f3{ds[0], ds[1], ds[2]} + f3{dp[0], dp[1], dp[2]};
+        ds[0] = 0.0f; ds[1] = 0.0f; ds[2] = 0.0f;
+        dp[0] = dl.x; dp[1] = dl.y; dp[2] = dl.z;
+        const float4 Pi = a.Xs[pi], Vi = a.Vs[pi];
+        apply_delta(a.Ps, a.vals, pi, xyz(Pi), Pi.w, Vi.w, dl, a.dt, a.vmax, a.Po, a.Vo);
     }
-    const float4 Po = make_float4(xn.x, xn.y, xn.z, rad), Vo = make_float4(vnew.x, vnew.y, vnew.z, w1);
-    a.Po[t] = Po;
-    a.Vo[t] = Vo;
 }
 
 // apply_particle_shape_restitution.comp:48-116.  Body previous pose/velocity
@@ -607,6 +638,9 @@ void launch_cell_tables(int n, const uint32_t* keys, int* starts, int* ends, hip
 }
 void launch_shape_solve(const ShapeArgs& a, int grid, hipStream_t s) {
     hipLaunchKernelGGL(k_shape_solve, dim3(grid), dim3(256), 0, s, a);
+}
+void launch_shape_apply(const ShapeApplyArgs& a, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(k_shape_apply, dim3(grid), dim3(256), 0, s, a);
 }
 void launch_pp_apply(const PPArgs& a, bool diag, bool validate, int block, hipStream_t s) {
     if (a.n <= 0) return;

@@ -1,16 +1,19 @@
 // solver.cpp -- host side of the MI355X XPBD step (see solver.h).
 //
-// Substep pipeline on one HIP stream (K = iterations):
-//   [k_mesh_contacts]            particle/mesh narrowphase on x0      (if meshes)
-//   k_integrate                  plane contacts on x0, x0 -> x_pred, cell keys,
-//                                digit-0 tile histograms
-//   sort: binscan, scatter, (upsweep, binscan, scatter) x 2
-//   k_cell_ranges                (start, end) of every occupied cell
-//   K x { k_shape_solve, k_pp_apply }
+// Substep pipeline (K = iterations):
+//   [k_mesh_*]                   particle/mesh narrowphase on x0      (if meshes;
+//                                side stream, beside integrate and the sort)
+//   k_integrate                  plane contacts on x0, x0 -> x_pred, cell keys
+//   sort: (upsweep, binscan, scatter) per 7-bit digit, the last pass moving
+//         the particle state into cell order
+//   k_clear_ranges, k_cell_ranges  (start, end) of every occupied cell
+//   K x { k_shape_solve          (first one on the side stream, after the mesh)
+//         k_pp_apply             particle-particle solve + delta apply
+//         k_shape_apply }        shape deltas of the particles that have any
 //   [k_restitution]
 // Compared with src/scene.cpp:805-992 this drops the contact-counter fill,
 // the 16 MiB cell-table fills, the delta fill and the odd-K copy-back, and
-// replaces 15 sort dispatches by 8.
+// replaces 15 sort dispatches by 9 (12 with wrap planes).
 #include "solver.h"
 
 #include <algorithm>
@@ -30,7 +33,7 @@ namespace {
 constexpr const char* kStageNames[] = {"mesh_contacts", "integrate",      "sort_upsweep",   "sort_binscan",
                                        "sort_scatter",  "cell_ranges",    "shape_solve",    "pp_solve_apply",
                                        "restitution",   "readback",       "body_contacts",  "integrate_bodies",
-                                       "body_contact_solve", "apply_body_deltas"};
+                                       "body_contact_solve", "apply_body_deltas", "shape_apply"};
 
 }  // namespace
 
@@ -56,7 +59,7 @@ void Solver::free_all() {
     dfree(body_half_); dfree(body_I_); dfree(body_invw_); dfree(mat_mu_t_); dfree(mat_mu_r_);
     dfree(rc_s0_); dfree(rc_s1_); dfree(rc_p0_); dfree(rc_p1_); dfree(rc_n_); dfree(rc_impulse_);
     dfree(range_); dfree(rkeys_); dfree(tbl_); dfree(cpart_); dfree(cshape_); dfree(cc_); dfree(cbpos_); dfree(cbvel_); dfree(cnorm_);
-    dfree(dshape_); dfree(delta_); dfree(body_q_); dfree(body_qd_); dfree(body_com_); dfree(body_invI_);
+    dfree(dshape_); dfree(delta_); dfree(owner_); dfree(body_q_); dfree(body_qd_); dfree(body_com_); dfree(body_invI_);
     dfree(body_invm_); dfree(body_delta_); dfree(mat_mu_); dfree(body_flags_); dfree(shape_body_);
     dfree(verts_d_); dfree(vvel_d_); dfree(tris_d_); dfree(prim_d_); dfree(err_d_); dfree(nodes_d_); dfree(cnodes_d_); dfree(lbox_d_); dfree(pg_start_); dfree(lb_); dfree(lb_node_); dfree(tsoup_); dfree(pg_soup_); dfree(mcnt_); dfree(mq_); dfree(pq_);
     dfree(info_d_); dfree(diag_header_); dfree(diag_pairs_); dfree(mesh_stats_);
@@ -68,6 +71,7 @@ void Solver::free_all() {
     if (count_host_) { (void)hipHostFree(count_host_); count_host_ = nullptr; }
     if (fork_ev_) { (void)hipEventDestroy(fork_ev_); fork_ev_ = nullptr; }
     if (join_ev_) { (void)hipEventDestroy(join_ev_); join_ev_ = nullptr; }
+    if (sorted_ev_) { (void)hipEventDestroy(sorted_ev_); sorted_ev_ = nullptr; }
     if (side_) { (void)hipStreamDestroy(side_); side_ = nullptr; }
     if (stream_) { (void)hipStreamDestroy(stream_); stream_ = nullptr; }
 }
@@ -93,6 +97,8 @@ void Solver::alloc_all() {
     dalloc(cbpos_, cmax_); dalloc(cbvel_, cmax_); dalloc(cnorm_, cmax_);
     dalloc(cc_, 8);   // [particle slot 0, slot 1, error flags, spare, rigid slot 0, slot 1, spare, spare]
     dalloc(dshape_, 3 * n_); dalloc(delta_, 3 * n_);
+    dalloc(owner_, n_);
+    HIPCHECK(hipMemset(owner_.p, 0xff, owner_.n * sizeof(int)));   // -1: no shape delta pending
     dalloc(diag_header_, 2);
     dalloc(mesh_stats_, 8);
     dalloc(mq_, n_);   // mesh candidate queue (its counters are sized by finalize_meshes)
@@ -120,6 +126,7 @@ void Solver::init_common(const xpbd_config& cfg, size_t n, size_t max_contacts) 
     HIPCHECK(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking));
     HIPCHECK(hipEventCreateWithFlags(&fork_ev_, hipEventDisableTiming));
     HIPCHECK(hipEventCreateWithFlags(&join_ev_, hipEventDisableTiming));
+    HIPCHECK(hipEventCreateWithFlags(&sorted_ev_, hipEventDisableTiming));
     alloc_all();
     const char* env = std::getenv("VKXPBD_PROFILE");
     bool env_on = env && (std::strcmp(env, "1") == 0 || std::strcmp(env, "true") == 0);
@@ -694,7 +701,6 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
         stage_begin(kMesh);
         launch_mesh_contacts(m, mesh_split_.data(), fork ? side_ : stream_);
         stage_end(kMesh);
-        if (fork) HIPCHECK(hipEventRecord(join_ev_, side_));
     }
     if (collide && nbox > 0) {
         stage_begin(kBodyContacts);
@@ -748,14 +754,18 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
     int cur = p_, nxt = o_;
     if (n > 0) {
         run_sort(tiles);
+        if (fork && iterations > 0) HIPCHECK(hipEventRecord(sorted_ev_, stream_));
         stage_begin(kRanges);
         build_ranges(keysA_.p);
-        if (fork) HIPCHECK(hipStreamWaitEvent(stream_, join_ev_, 0));
         stage_end(kRanges);
         grid_valid_ = true;
         // contacts were generated in the previous slot order: the first shape
         // solve maps them through rank; with no iteration a kernel does
         if (iterations == 0) {   // no solve: the predicted state still moves to the sorted order
+            if (fork) {
+                HIPCHECK(hipEventRecord(join_ev_, side_));
+                HIPCHECK(hipStreamWaitEvent(stream_, join_ev_, 0));
+            }
             launch_remap_contacts(cpart_.p, cc_solve, int(cmax_), rank_.p, shape_grid, stream_);
             HIPCHECK(hipMemcpyAsync(P_[cur].p, Xs_[0].p, n_ * sizeof(float4), hipMemcpyDeviceToDevice, stream_));
             HIPCHECK(hipMemcpyAsync(V_[cur].p, Vs_[0].p, n_ * sizeof(float4), hipMemcpyDeviceToDevice, stream_));
@@ -773,14 +783,20 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
         sa.P = Xin; sa.V = Vin; sa.flags = flags_cur();
         sa.body = body_args();
         sa.pmu = cfg_.soft_contact_mu; sa.ka = cfg_.particle_adhesion; sa.dt = sdt; sa.relax = relax_;
-        sa.dshape = dshape_.p;
+        sa.dshape = dshape_.p; sa.owner = owner_.p;
+        // the first shape solve waits for the mesh contacts, on the side
+        // stream, while the particle-particle solve (which no longer reads
+        // the shape deltas) runs; k_shape_apply joins the two
+        const bool side = fork && it == 0;
+        if (side) HIPCHECK(hipStreamWaitEvent(side_, sorted_ev_, 0));
         stage_begin(kShape);
-        launch_shape_solve(sa, shape_grid, stream_);
+        launch_shape_solve(sa, shape_grid, side ? side_ : stream_);
         stage_end(kShape);
+        if (side) HIPCHECK(hipEventRecord(join_ev_, side_));
         PPArgs pa;
         pa.n = n; pa.keys = keysA_.p; pa.vals = valsA_.p; pa.ids = ids_cur(); pa.range = range_.p;
         pa.Ps = P_[s_].p; pa.Xs = Xin; pa.Vs = Vin; pa.Fs = flags_cur();
-        pa.dshape = dshape_.p; pa.delta = delta_.p; pa.Po = P_[nxt].p; pa.Vo = V_[nxt].p;
+        pa.delta = delta_.p; pa.Po = P_[nxt].p; pa.Vo = V_[nxt].p;
         pa.mu = cfg_.particle_mu; pa.coh = cfg_.particle_cohesion; pa.max_r = cfg_.particle_max_radius;
         pa.dt = sdt; pa.relax = relax_; pa.inv_cell = inv_cell; pa.vmax = cfg_.particle_v_max;
         pa.rmax = rmax_;
@@ -790,6 +806,16 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
         stage_begin(kPP);
         launch_pp_apply(pa, cfg_.enable_diagnostics != 0, num_scenes_ > 1, pp_block_, stream_);
         stage_end(kPP);
+        if (side) HIPCHECK(hipStreamWaitEvent(stream_, join_ev_, 0));
+        ShapeApplyArgs aa;
+        aa.cmax = int(cmax_); aa.cc = cc_solve; aa.cpart = cpart_.p;
+        aa.owner = owner_.p; aa.dshape = dshape_.p; aa.delta = delta_.p;
+        aa.Ps = P_[s_].p; aa.vals = valsA_.p; aa.Xs = Xin; aa.Vs = Vin;
+        aa.Po = P_[nxt].p; aa.Vo = V_[nxt].p;
+        aa.dt = sdt; aa.vmax = cfg_.particle_v_max;
+        stage_begin(kShapeApply);
+        launch_shape_apply(aa, shape_grid, stream_);
+        stage_end(kShapeApply);
         std::swap(cur, nxt);
         if (nbox > 0) {
             BodySolveArgs bv;

@@ -106,7 +106,7 @@ private:
     };
     enum Stage {
         kMesh, kIntegrate, kSortUp, kSortScan, kSortScatter, kRanges, kShape, kPP, kRest, kCopyOut,
-        kBodyContacts, kBodyIntegrate, kBodySolve, kBodyApply, kNumStages
+        kBodyContacts, kBodyIntegrate, kBodySolve, kBodyApply, kShapeApply, kNumStages
     };
 
     void require_init() const;
@@ -143,7 +143,7 @@ private:
     // alongside integrate, sort and cell ranges, which do not touch them; the
     // contact remap after the sort waits for them
     hipStream_t side_ = nullptr;
-    hipEvent_t fork_ev_ = nullptr, join_ev_ = nullptr;
+    hipEvent_t fork_ev_ = nullptr, join_ev_ = nullptr, sorted_ev_ = nullptr;
     size_t n_ = 0, cmax_ = 0;
 
     // batched scenes
@@ -218,6 +218,7 @@ private:
     int contact_slot_ = 0, last_count_ = 0;
     int32_t* count_host_ = nullptr;   // pinned
     DevBuf<float> dshape_, delta_;
+    DevBuf<int> owner_;   // per sorted slot: contact that finishes its shape delta, else -1
     // bodies (one static body per mesh, scene.cpp:220-238) and shapes
     DevBuf<float4> body_q_, body_qd_, body_half_;
     DevBuf<float> body_com_, body_I_, body_invI_, body_invm_, body_delta_, body_invw_, mat_mu_, mat_mu_t_, mat_mu_r_;

@@ -84,6 +84,19 @@ struct ShapeArgs {
     BodyArgs body;
     float pmu, ka, dt, relax;
     float* dshape;
+    int* owner;                 // per sorted slot: largest contact index that added to dshape, else -1
+};
+
+// Adds the shape deltas to the particles that received any, after k_pp_apply
+// wrote every particle with its particle-particle delta alone.
+struct ShapeApplyArgs {
+    int cmax; const int* cc;
+    const int* cpart;
+    int* owner; float* dshape; float* delta;        // sorted order; owner and dshape left cleared
+    const float4* Ps; const uint32_t* vals;         // substep-start positions, sorted slot -> their slot
+    const float4* Xs; const float4* Vs;             // iteration input state, sorted order
+    float4* Po; float4* Vo;                         // output state, sorted order
+    float dt, vmax;
 };
 
 struct PPArgs {
@@ -94,7 +107,7 @@ struct PPArgs {
     const int* range;
     const float4* Ps;                               // substep-start positions (substep-start order)
     const float4* Xs; const float4* Vs; const int* Fs;   // iteration input state, sorted order
-    float* dshape; float* delta;                    // sorted order
+    float* delta;                                   // sorted order
     float4* Po; float4* Vo;                         // output state, sorted order
     float mu, coh, max_r, dt, relax, inv_cell, vmax;
     float rmax;                                     // largest particle radius (bounds every accepted pair)
@@ -242,6 +255,7 @@ void launch_iota(uint32_t* p, int n, hipStream_t s);
 void launch_gather_f3(float4* dst, const float* src, const uint32_t* ids, int n, hipStream_t s);
 void launch_cell_tables(int n, const uint32_t* keys, int* starts, int* ends, hipStream_t s);
 void launch_shape_solve(const ShapeArgs& a, int grid, hipStream_t s);
+void launch_shape_apply(const ShapeApplyArgs& a, int grid, hipStream_t s);
 // validate: the cell table is not cleared (batched mode); an entry is used
 // only if keys[start] equals the looked-up key.  block: 64, 128 or 256.
 void launch_pp_apply(const PPArgs& a, bool diag, bool validate, int block, hipStream_t s);
