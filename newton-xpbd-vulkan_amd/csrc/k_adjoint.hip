@@ -169,7 +169,7 @@ constexpr int kSurvAdj = 16;   // recorded candidates per lane before a drain
 template <bool VALIDATE>
 __global__ __launch_bounds__(256) void k_adj_pp(AdjArgs a) {
     __shared__ uint32_t s_surv[kSurvAdj][256];
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     double racc = 0.0;
     if (t < a.n) {
         const int k = int(a.vals[t]);

@@ -42,7 +42,7 @@ __device__ __forceinline__ int tile_item(int tile, int it) {
 // The force buffer and world index of the reference have no setter and are
 // always zero (scene.cpp:82-85), so v1 = v0 + (0*w + g*[w>0])*dt.
 __global__ __launch_bounds__(256) void k_integrate(IntegrateArgs a) {
-    const int tile = blockIdx.x;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         if (a.cc_next != nullptr) *a.cc_next = 0;
         if (a.rcc_next != nullptr) *a.rcc_next = 0;
@@ -140,7 +140,7 @@ __global__ __launch_bounds__(256) void k_plane_contacts(PlaneArgs a) {
 // k_clear_ranges can reset exactly those rows before the next build.
 __global__ __launch_bounds__(256) void k_cell_ranges(int n, const uint32_t* __restrict__ keys, int* __restrict__ range,
                                                      uint32_t* __restrict__ rkeys) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int t = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const uint32_t c = keys[t];
     const size_t row = size_t(2) * c;   // batched keys reach 2^32 - 1
@@ -152,7 +152,7 @@ __global__ __launch_bounds__(256) void k_cell_ranges(int n, const uint32_t* __re
 // Zeroes the rows k_cell_ranges wrote from the sorted keys rkeys: with wrap
 // planes the table has 2^27 rows, too many to clear whole every substep.
 __global__ __launch_bounds__(256) void k_clear_ranges(int n, const uint32_t* __restrict__ rkeys, int* __restrict__ range) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int t = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const uint32_t c = rkeys[t];
     if (t == 0 || rkeys[t - 1] != c) *reinterpret_cast<int2*>(range + size_t(2) * c) = make_int2(0, 0);
@@ -342,7 +342,7 @@ template <bool DIAG, bool VALIDATE, int BLOCK>
 // despite a few spilled registers.
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_pp_apply(PPArgs a) {
     __shared__ uint32_t s_surv[kSurv][BLOCK];
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (t >= a.n) return;
     const int pf = a.Fs[t];
     const float4 Pi = a.Xs[t], Vi = a.Vs[t];

@@ -38,7 +38,7 @@ __global__ __launch_bounds__(256) void k_sort_upsweep(int n, const uint32_t* __r
                                                       uint32_t* __restrict__ hist, int tiles) {
     constexpr int B = 1 << BITS;
     __shared__ uint32_t h[B];
-    const int tile = blockIdx.x, w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x), w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     for (int b = threadIdx.x; b < B; b += 256) h[b] = 0;
     __syncthreads();
 #pragma unroll
@@ -96,7 +96,7 @@ __global__ __launch_bounds__(256) void k_sort_scatter(int n, const uint32_t* __r
     __shared__ uint32_t s_keys[kSortTile];
     __shared__ uint32_t s_vals[kSortTile];
     __shared__ uint32_t s_wsum[2][4];
-    const int tile = blockIdx.x, w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x), w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint64_t lt = (1ull << lane) - 1ull;
     for (int k = threadIdx.x; k < 4 * B; k += 256) (&s_cnt[0][0])[k] = 0;
     __syncthreads();

@@ -12,6 +12,18 @@
 
 namespace xpbd {
 
+// Block b of a grid of T blocks -> the tile it processes, chosen so that each
+// XCD (workgroups are dealt to the eight XCDs round-robin) works through one
+// contiguous range of tiles: writes that land in the same cache line (the
+// sort's bin runs, the cell table rows, the id-ordered radix input) then meet
+// in one L2 instead of leaving it as partial lines from several.  A bijection
+// of [0, T) for every T.
+__device__ __forceinline__ int xcd_tile(int b, int T) {
+    const int q = T >> 3, r = T & 7, x = b & 7, j = b >> 3;
+    return x * q + min(x, r) + j;
+}
+
+
 constexpr int kGridDim = 128;
 constexpr int kNumCells = kGridDim * kGridDim * kGridDim;
 constexpr int kGridOrigin = 1 << 20;
