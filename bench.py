@@ -39,18 +39,19 @@ def stage_bytes(stage, N, C, Cm, U, tiles, mesh_bytes, passes):
     must touch, counted once (neighbour re-reads are cache hits).  The state
     lives in sorted slot order; the sort runs `passes` LSD passes."""
     hist = tiles * 128 * 4
-    if stage == "integrate":      # P,V,flags,ids in; P,V out; (key, slot) pair out; plane contacts
-        return N * (16 + 16 + 4 + 4 + 16 + 16 + 8) + 56 * (C - Cm)
+    if stage == "integrate":      # P,V,flags,ids in; P,V out; (key, slot) pair out; plane contacts;
+        # reset of the last substep's cell rows (their keys, zeroed rows)
+        return N * (16 + 16 + 4 + 4 + 16 + 16 + 8) + 56 * (C - Cm) + 4 * N + 8 * U
     if stage == "sort_upsweep":   # pass 0 reads the pairs, the others the keys
         return (8 * N + 4 * N * (passes - 1)) / passes + hist
     if stage == "sort_binscan":
         return 2 * hist
     if stage == "sort_scatter":   # mean over the passes; the last also moves P,V,flags,id and writes rank
         return (16 * N * passes + (16 + 16 + 4 + 4) * 2 * N + 4 * N) / passes + hist
-    if stage == "cell_ranges":    # reset of the last rows, new rows, key record, contact remap
-        return 4 * N + 8 * U + 4 * N + 8 * U + 4 * N + 12 * C
-    if stage == "shape_solve":    # contact record, particle P,V gather, delta RMW
-        return C * (56 + 32 + 24)
+    if stage == "cell_ranges":    # sorted keys, new rows, key record
+        return 4 * N + 8 * U + 4 * N
+    if stage == "shape_solve":    # contact record, slot remap, particle P,V gather, delta RMW, owner
+        return C * (56 + 8 + 32 + 24 + 4)
     if stage == "pp_solve_apply":  # vals, flags, Xs/Vs, x0, delta, P/V out; cell ranges
         return N * (4 + 4 + 32 + 16 + 12 + 32) + 8 * U
     if stage == "shape_apply":     # per contact: particle slot, owner; per owner (taken as every

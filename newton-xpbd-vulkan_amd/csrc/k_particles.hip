@@ -107,6 +107,10 @@ __global__ __launch_bounds__(256) void k_integrate(IntegrateArgs a) {
         }
         a.Pp[idx] = Pn;
         a.Vp[idx] = Vn;
+        if (a.rkeys != nullptr) {   // k_clear_ranges, folded in
+            const uint32_t c = a.rkeys[idx];
+            if (idx == 0 || a.rkeys[idx - 1] != c) *reinterpret_cast<int2*>(a.range + size_t(2) * c) = make_int2(0, 0);
+        }
         uint32_t key = cell_of(Pn.x, Pn.y, Pn.z, a.inv_cell);
         if (a.scene != nullptr) key |= uint32_t(a.scene[idx]) << kCellBits;   // scenes keep their slot ranges
         else if (a.planes) key |= plane_of(Pn.x, Pn.y, Pn.z, a.inv_cell, a.planes) << kCellBits;

@@ -607,10 +607,10 @@ void Solver::update_planes() {
 // Cell ranges of the sorted keys.  Batched tables are validated on lookup;
 // the plain 2^21-row table is zeroed whole; the 2^27-row table of wrap planes
 // has the previous build's rows reset instead.
-void Solver::build_ranges(const uint32_t* sorted_keys) {
+void Solver::build_ranges(const uint32_t* sorted_keys, bool cleared) {
     const int n = int(n_);
     if (planes_) {
-        launch_clear_ranges(n, rkeys_.p, range_.p, stream_);
+        if (!cleared) launch_clear_ranges(n, rkeys_.p, range_.p, stream_);
         launch_cell_ranges(n, sorted_keys, range_.p, rkeys_.p, stream_);
     } else {
         if (num_scenes_ == 1) launch_fill_zero(range_.p, size_t(2) * kNumCells, stream_);
@@ -740,6 +740,7 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
         a.scene = num_scenes_ > 1 ? scene_d_.p : nullptr;
         a.planes = planes_ ? plane_bits_ : 0;
         a.cpart = cpart_.p; a.cshape = cshape_.p; a.cbpos = cbpos_.p; a.cbvel = cbvel_.p; a.cnorm = cnorm_.p;
+        a.rkeys = planes_ ? rkeys_.p : nullptr; a.range = range_.p;
         stage_begin(kIntegrate);
         launch_integrate(a, stream_);
         stage_end(kIntegrate);
@@ -756,7 +757,7 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
         run_sort(tiles);
         if (fork && iterations > 0) HIPCHECK(hipEventRecord(sorted_ev_, stream_));
         stage_begin(kRanges);
-        build_ranges(keysA_.p);
+        build_ranges(keysA_.p, true);
         stage_end(kRanges);
         grid_valid_ = true;
         // contacts were generated in the previous slot order: the first shape

@@ -129,7 +129,9 @@ private:
     void run_sort(int tiles);
     void update_planes();
     std::vector<uint64_t> reference_sorted();
-    void build_ranges(const uint32_t* sorted_keys);
+    // cell ranges of the sorted keys; cleared = the previous rows are already
+    // zero (k_integrate clears them when wrap planes are on)
+    void build_ranges(const uint32_t* sorted_keys, bool cleared = false);
     void read_count(int slot);
     void stage_begin(Stage s);
     void stage_end(Stage s);

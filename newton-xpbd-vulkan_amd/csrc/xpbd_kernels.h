@@ -51,6 +51,9 @@ struct IntegrateArgs {
     const int* scene;   // batched mode: scene of each particle (key = scene << 21 | cell); else null
     int planes;         // key = wrap plane << 21 | cell (plain mode only)
     int* cpart; int* cshape; float4* cbpos; float4* cbvel; float4* cnorm;
+    // non-null: zero the cell-table rows the last k_cell_ranges wrote (their
+    // keys, in slot order), before this substep's ranges are built
+    const uint32_t* rkeys; int* range;
 };
 
 struct PlaneArgs {
