@@ -337,12 +337,12 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
     return v;
 }
 
-// Candidate test of particle i against mesh mi: the reference's first pop
-// (the root box against the search distance) and the distance-bound early
-// out.  Returns the search start node word or kNoNode.
-__device__ __forceinline__ uint32_t mesh_candidate(const MeshArgs& a, const MeshInfo& mi, int i) {
-    if (mi.prim_cnt == 0u || i >= a.n || (a.flags[i] & 1) == 0) return kNoNode;   // an empty mesh has no nodes
-    const float4 P = a.P[i];
+// Candidate test of a particle (position and radius P, flags f) against mesh
+// mi: the reference's first pop (the root box against the search distance)
+// and the distance-bound early out.  Returns the search start node word or
+// kNoNode.
+__device__ __forceinline__ uint32_t mesh_candidate(const MeshArgs& a, const MeshInfo& mi, float4 P, int f) {
+    if (mi.prim_cnt == 0u || (f & 1) == 0) return kNoNode;   // an empty mesh has no nodes
     const f3 p = xyz(P);
     const float max_dist = (a.margin + mi.margin) + P.w;
     const float thr = mi.watertight ? max_dist : max_dist + a.eps;
@@ -359,7 +359,10 @@ __device__ __forceinline__ uint32_t* mesh_counters(const MeshArgs& a, int ci, in
 }
 
 // Pass 1: candidates into the queue, one slot reservation per block (queue
-// order is arbitrary; every queued particle's result is its own).
+// order is arbitrary; every queued particle's result is its own).  Each
+// thread tests kEnqItems particles of its block's range (wave-striped), their
+// loads issued together.
+constexpr int kEnqItems = 4;
 template <bool STATS>
 __global__ __launch_bounds__(256) void k_mesh_enqueue(MeshArgs a, int ci) {
     __shared__ int s_tmp[256 / 64 + 1];
@@ -369,13 +372,34 @@ __global__ __launch_bounds__(256) void k_mesh_enqueue(MeshArgs a, int ci) {
         o[0] = 0u;
         o[1] = 0u;
     }
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    const uint32_t start = mesh_candidate(a, mi, i);
-    const int pos = block_reserve(reinterpret_cast<int*>(mesh_counters(a, ci, a.slot)), start != kNoNode ? 1 : 0, s_tmp);
-    if (start != kNoNode) a.mq[pos] = make_int2(i, int(start));
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int i0 = blockIdx.x * (256 * kEnqItems) + w * (64 * kEnqItems) + lane;
+    float4 P[kEnqItems];
+    int f[kEnqItems];
+#pragma unroll
+    for (int it = 0; it < kEnqItems; it++) {
+        const int i = i0 + 64 * it;
+        P[it] = make_float4(0.f, 0.f, 0.f, 0.f);
+        f[it] = 0;
+        if (i < a.n) { P[it] = a.P[i]; f[it] = a.flags[i]; }
+    }
+    uint32_t start[kEnqItems];
+    int mine = 0;
+#pragma unroll
+    for (int it = 0; it < kEnqItems; it++) {
+        start[it] = mesh_candidate(a, mi, P[it], f[it]);
+        mine += start[it] != kNoNode ? 1 : 0;
+    }
+    int pos = block_reserve(reinterpret_cast<int*>(mesh_counters(a, ci, a.slot)), mine, s_tmp);
+#pragma unroll
+    for (int it = 0; it < kEnqItems; it++)
+        if (start[it] != kNoNode) a.mq[pos++] = make_int2(i0 + 64 * it, int(start[it]));
     if (STATS) {
-        const uint64_t act = wave_sum_u64((i < a.n && (a.flags[i] & 1)) ? 1u : 0u);
-        if (__lane_id() == 0 && act) atomicAdd(reinterpret_cast<unsigned long long*>(a.stats), (unsigned long long)act);
+        uint32_t act = 0;
+#pragma unroll
+        for (int it = 0; it < kEnqItems; it++) act += (f[it] & 1) ? 1u : 0u;
+        const uint64_t tot = wave_sum_u64(act);
+        if (__lane_id() == 0 && tot) atomicAdd(reinterpret_cast<unsigned long long*>(a.stats), (unsigned long long)tot);
     }
 }
 
@@ -591,7 +615,7 @@ __global__ __launch_bounds__(256) void k_mesh_parity(MeshArgs a, int ci) {
 
 void launch_mesh_contacts(const MeshArgs& a, const uint8_t* watertight, hipStream_t s) {
     if (a.n <= 0 || a.nmesh <= 0) return;
-    const int nb = (a.n + 255) / 256;
+    const int nb = (a.n + 256 * kEnqItems - 1) / (256 * kEnqItems);
     // walk grids are multiples of the XCD count, so every XCD gets the same
     // number of workgroups
     const unsigned walk_grid1 = unsigned(std::max(1, std::min(4096, (a.n + kMeshBlock - 1) / kMeshBlock) / 8)) * 8u;
