@@ -3,10 +3,11 @@
 // Substep pipeline (K = iterations):
 //   [k_mesh_*]                   particle/mesh narrowphase on x0      (if meshes;
 //                                side stream, beside integrate and the sort)
-//   k_integrate                  plane contacts on x0, x0 -> x_pred, cell keys
+//   k_integrate                  plane contacts on x0, x0 -> x_pred, cell keys;
+//                                resets the previous substep's cell rows
 //   sort: (upsweep, binscan, scatter) per 7-bit digit, the last pass moving
 //         the particle state into cell order
-//   k_clear_ranges, k_cell_ranges  (start, end) of every occupied cell
+//   k_cell_ranges                (start, end) of every occupied cell
 //   K x { k_shape_solve          (first one on the side stream, after the mesh)
 //         k_pp_apply             particle-particle solve + delta apply
 //         k_shape_apply }        shape deltas of the particles that have any
