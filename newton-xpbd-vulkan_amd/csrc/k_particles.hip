@@ -141,7 +141,8 @@ __global__ __launch_bounds__(256) void k_plane_contacts(PlaneArgs a) {
 // is zeroed by k_fill_zero first (the two 8 MiB fills of scene.cpp:918-923
 // become one 16 MiB vectorised fill), so empty cells read (0, 0).
 // rkeys (when non-null) records the keys whose rows were written, so that
-// k_clear_ranges can reset exactly those rows before the next build.
+// the next build (k_integrate, or k_clear_ranges) can reset exactly those
+// rows.
 __global__ __launch_bounds__(256) void k_cell_ranges(int n, const uint32_t* __restrict__ keys, int* __restrict__ range,
                                                      uint32_t* __restrict__ rkeys) {
     int t = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
