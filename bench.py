@@ -39,14 +39,14 @@ def stage_bytes(stage, N, C, Cm, U, tiles, mesh_bytes, passes):
     must touch, counted once (neighbour re-reads are cache hits).  The state
     lives in sorted slot order; the sort runs `passes` LSD passes."""
     hist = tiles * 128 * 4
-    if stage == "integrate":      # P,V,flags,ids in; P,V out; (key, slot) pair out; plane contacts;
+    if stage == "integrate":      # P,V,flags,ids in; (key, slot) pair out; plane contacts;
         # reset of the last substep's cell rows (their keys, zeroed rows)
-        return N * (16 + 16 + 4 + 4 + 16 + 16 + 8) + 56 * (C - Cm) + 4 * N + 8 * U
+        return N * (16 + 16 + 4 + 4 + 8) + 56 * (C - Cm) + 4 * N + 8 * U
     if stage == "sort_upsweep":   # pass 0 reads the pairs, the others the keys
         return (8 * N + 4 * N * (passes - 1)) / passes + hist
     if stage == "sort_binscan":
         return 2 * hist
-    if stage == "sort_scatter":   # mean over the passes; the last also moves P,V,flags,id and writes rank
+    if stage == "sort_scatter":   # mean over the passes; the last also integrates and stores P,V, moves flags,id, writes rank
         return (16 * N * passes + (16 + 16 + 4 + 4) * 2 * N + 4 * N) / passes + hist
     if stage == "cell_ranges":    # sorted keys, new rows, key record
         return 4 * N + 8 * U + 4 * N

@@ -2,7 +2,7 @@
 //
 // Layout in HBM (one solver): positions as float4 (x, y, z, radius) and
 // velocities as float4 (vx, vy, vz, inverse mass), three rotating buffers each
-// (state / predicted / output), so a neighbour read is one 16-byte load and no
+// (state and two iteration buffers), so a neighbour read is one 16-byte load and no
 // copy-back is ever needed.  Flags are a separate int32 array.
 //
 // Kernel map (reference shader -> here):
@@ -35,7 +35,8 @@ __device__ __forceinline__ int tile_item(int tile, int it) {
 // integrate_particles.comp:26-60 fused with generate_plane_contacts.comp:27-54
 // (evaluated on the pre-integration position, as the reference orders them:
 // scene.cpp:853-871) and grid_cell_indices.comp:17-32 on the predicted
-// position.  The state is stored in slot order (the previous substep's sorted
+// position, which only decides the key (the last radix pass integrates the
+// state again where it stores it in cell order).  The state is stored in slot order (the previous substep's sorted
 // order); the radix input is written in particle-id order, (key, slot) at
 // index id, so the stable sort orders equal cells by particle id exactly as
 // the reference's does.  Plane contacts name the slot.
@@ -93,20 +94,10 @@ __global__ __launch_bounds__(256) void k_integrate(IntegrateArgs a) {
         const int idx = tile_item(tile, it);
         const bool valid = idx < a.n;
         if (!valid) continue;
+        // the predicted state only decides the key here: the last radix
+        // pass integrates again where it stores the state in cell order
         float4 Pn = P[it], Vn = V[it];
-        if (f[it] & 1) {
-            const float w = V[it].w;
-            const float gs = w > 0.0f ? 1.0f : 0.0f;
-            const float fx = 0.0f * w, fy = 0.0f * w, fz = 0.0f * w;
-            f3 v1 = {V[it].x + (fx + a.gx * gs) * a.dt, V[it].y + (fy + a.gy * gs) * a.dt,
-                     V[it].z + (fz + a.gz * gs) * a.dt};
-            const float mag = len3(v1);
-            if (mag > a.vmax) v1 = v1 * (a.vmax / mag);
-            Pn = make_float4(P[it].x + v1.x * a.dt, P[it].y + v1.y * a.dt, P[it].z + v1.z * a.dt, P[it].w);
-            Vn = make_float4(v1.x, v1.y, v1.z, V[it].w);
-        }
-        a.Pp[idx] = Pn;
-        a.Vp[idx] = Vn;
+        integrate_particle(Pn, Vn, f[it], a.dt, a.vmax, a.gx, a.gy, a.gz);
         if (a.rkeys != nullptr) {   // k_clear_ranges, folded in
             const uint32_t c = a.rkeys[idx];
             if (idx == 0 || a.rkeys[idx - 1] != c) *reinterpret_cast<int2*>(a.range + size_t(2) * c) = make_int2(0, 0);

@@ -78,8 +78,9 @@ __global__ __launch_bounds__(256) void k_sort_binscan(uint32_t* __restrict__ his
 }
 
 // GATHER (last pass only): the values are slots of the previous order; the
-// particle's predicted position, velocity, flags and id move to the sorted
-// slot (the new order of the state: the particle-particle solve reads
+// particle's substep-start state is integrated (integrate_particle, the same
+// code k_integrate keyed it with) and stored with its flags and id at the
+// sorted slot (the new order of the state: the particle-particle solve reads
 // neighbours as contiguous runs), and rank records previous slot -> new slot.
 template <int BITS, bool PAIRS, bool GATHER>
 __global__ __launch_bounds__(256) void k_sort_scatter(int n, const uint32_t* __restrict__ keys_in,
@@ -187,9 +188,12 @@ __global__ __launch_bounds__(256) void k_sort_scatter(int n, const uint32_t* __r
             keys_out[dst] = k;
             vals_out[dst] = v;
             if (GATHER) {
-                g.Xs[dst] = g.P[v];
-                g.Vs[dst] = g.V[v];
-                g.Fs[dst] = g.flags[v];
+                float4 P = g.P[v], V = g.V[v];
+                const int f = g.flags[v];
+                integrate_particle(P, V, f, g.dt, g.vmax, g.gx, g.gy, g.gz);
+                g.Xs[dst] = P;
+                g.Vs[dst] = V;
+                g.Fs[dst] = f;
                 g.ids_out[dst] = g.ids[v];
                 g.rank[v] = dst;
             }

@@ -3,10 +3,10 @@
 // Substep pipeline (K = iterations):
 //   [k_mesh_*]                   particle/mesh narrowphase on x0      (if meshes;
 //                                side stream, beside integrate and the sort)
-//   k_integrate                  plane contacts on x0, x0 -> x_pred, cell keys;
+//   k_integrate                  plane contacts on x0, cell keys of x_pred;
 //                                resets the previous substep's cell rows
-//   sort: (upsweep, binscan, scatter) per 7-bit digit, the last pass moving
-//         the particle state into cell order
+//   sort: (upsweep, binscan, scatter) per 7-bit digit, the last pass storing
+//         the integrated state (x_pred, v_pred) in cell order
 //   k_cell_ranges                (start, end) of every occupied cell
 //   K x { k_shape_solve          (first one on the side stream, after the mesh)
 //         k_pp_apply             particle-particle solve + delta apply
@@ -628,7 +628,7 @@ void Solver::set_sort_shape(int key_bits) {
 }
 
 
-void Solver::run_sort(int tiles) {
+void Solver::run_sort(int tiles, float sdt) {
     const int n = int(n_);
     uint32_t* kb[2] = {keysA_.p, keysB_.p};
     uint32_t* vb[2] = {valsA_.p, valsB_.p};
@@ -636,8 +636,9 @@ void Solver::run_sort(int tiles) {
     // between the A and B buffers and the last one must land in A (the
     // buffers the getters read)
     int dst = (sort_passes_ & 1) ? 0 : 1;
-    const SortGather g{P_[p_].p, V_[p_].p, flags_cur(), ids_cur(), Xs_[0].p, Vs_[0].p,
-                       flags_[lay_ ^ 1].p, ids_[lay_ ^ 1].p, rank_.p};
+    const SortGather g{P_[s_].p, V_[s_].p, flags_cur(), ids_cur(), Xs_[0].p, Vs_[0].p,
+                       flags_[lay_ ^ 1].p, ids_[lay_ ^ 1].p, rank_.p,
+                       sdt, cfg_.particle_v_max, cfg_.gravity[0], cfg_.gravity[1], cfg_.gravity[2]};
     for (int pass = 0; pass < sort_passes_; pass++) {
         const int shift = pass * sort_bits_;
         const uint32_t* kin = pass == 0 ? nullptr : kb[dst ^ 1];
@@ -727,7 +728,7 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
     if (n > 0) {
         IntegrateArgs a;
         a.n = n;
-        a.Ps = P_[s_].p; a.Vs = V_[s_].p; a.flags = flags_cur(); a.Pp = P_[p_].p; a.Vp = V_[p_].p;
+        a.Ps = P_[s_].p; a.Vs = V_[s_].p; a.flags = flags_cur();
         a.ids = ids_cur(); a.kv = kv_.p;
         a.dt = sdt; a.vmax = cfg_.particle_v_max;
         a.gx = cfg_.gravity[0]; a.gy = cfg_.gravity[1]; a.gz = cfg_.gravity[2];
@@ -755,7 +756,7 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
     const int shape_grid = int(std::max<size_t>(1, std::min<size_t>((cmax_ + 255) / 256, 2048)));
     int cur = p_, nxt = o_;
     if (n > 0) {
-        run_sort(tiles);
+        run_sort(tiles, sdt);
         if (fork && iterations > 0) HIPCHECK(hipEventRecord(sorted_ev_, stream_));
         stage_begin(kRanges);
         build_ranges(keysA_.p, true);

@@ -126,7 +126,7 @@ private:
     BodyArgs body_args() const;
     void substep(float sdt, int iterations, bool collide, int slot, bool zero_next);
     BodyStepArgs body_step_args(float sdt);
-    void run_sort(int tiles);
+    void run_sort(int tiles, float sdt);   // the last pass also integrates the state over sdt
     void update_planes();
     std::vector<uint64_t> reference_sorted();
     // cell ranges of the sorted keys; cleared = the previous rows are already
@@ -191,7 +191,7 @@ private:
         DevBuf<double> relax_part;
     } tape_;
     bool use_graphs_ = true;
-    // particle state: 3 rotating float4 buffers each (state / predicted / out)
+    // particle state: 3 rotating float4 buffers each (state / two iteration buffers)
     DevBuf<float4> P_[3], V_[3];
     int s_ = 0, p_ = 1, o_ = 2;
     // The particle state lives in slot order: after every substep's sort, the

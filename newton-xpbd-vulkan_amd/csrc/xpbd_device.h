@@ -42,6 +42,24 @@ __device__ __forceinline__ float gmax(float a, float b) { return a < b ? b : a; 
 __device__ __forceinline__ float gmin(float a, float b) { return b < a ? b : a; }
 __device__ __forceinline__ float dot3(f3 a, f3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
 __device__ __forceinline__ float len3(f3 a) { return sqrtf(dot3(a, a)); }
+
+// integrate_particles.comp:44-57 for one particle with flags f: x0, v0 ->
+// x1, v1 in place (w = inverse mass in V.w; the path's forces are zero, and
+// f * w keeps the reference's arithmetic).  k_integrate derives the cell key
+// from it and the last radix pass stores its result in cell order, so both
+// evaluate this same code.
+__device__ __forceinline__ void integrate_particle(float4& P, float4& V, int f, float dt, float vmax, float gx, float gy,
+                                                   float gz) {
+    if ((f & 1) == 0) return;
+    const float w = V.w;
+    const float gs = w > 0.0f ? 1.0f : 0.0f;
+    const float fx = 0.0f * w, fy = 0.0f * w, fz = 0.0f * w;
+    f3 v1 = {V.x + (fx + gx * gs) * dt, V.y + (fy + gy * gs) * dt, V.z + (fz + gz * gs) * dt};
+    const float mag = len3(v1);
+    if (mag > vmax) v1 = v1 * (vmax / mag);
+    P = make_float4(P.x + v1.x * dt, P.y + v1.y * dt, P.z + v1.z * dt, P.w);
+    V = make_float4(v1.x, v1.y, v1.z, V.w);
+}
 __device__ __forceinline__ f3 cross3(f3 a, f3 b) {
     return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
 }

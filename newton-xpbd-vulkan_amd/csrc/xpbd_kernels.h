@@ -22,9 +22,13 @@ constexpr int kSortTile = 256 * kSortItems;
 // Last radix pass: the particle state moves into the new sorted order.  The
 // sort values are slots of the previous order; ids follow their particles and
 // rank maps each previous slot to its new one.
+// The last radix pass moves each particle to its sorted slot: the
+// substep-start state (P, V, flags, id at the previous slot) is integrated
+// there (integrate_particle with dt, vmax, g) into Xs, Vs.
 struct SortGather {
     const float4* P; const float4* V; const int* flags; const uint32_t* ids;
     float4* Xs; float4* Vs; int* Fs; uint32_t* ids_out; uint32_t* rank;
+    float dt, vmax, gx, gy, gz;
 };
 
 struct BodyArgs {
@@ -42,7 +46,6 @@ struct BodyArgs {
 struct IntegrateArgs {
     int n;
     const float4* Ps; const float4* Vs; const int* flags;   // slot order
-    float4* Pp; float4* Vp;
     const uint32_t* ids;                // particle id of each slot
     uint2* kv;                          // radix input in particle-id order: (cell key, slot)
     float dt, vmax, gx, gy, gz, inv_cell;
