@@ -613,16 +613,18 @@ __global__ __launch_bounds__(256) void k_mesh_parity(MeshArgs a, int ci) {
     }
 }
 
-void launch_mesh_contacts(const MeshArgs& a, const uint8_t* watertight, hipStream_t s) {
+void launch_mesh_contacts(const MeshArgs& a, const uint8_t* watertight, bool beside, hipStream_t s) {
     if (a.n <= 0 || a.nmesh <= 0) return;
     const int nb = (a.n + 256 * kEnqItems - 1) / (256 * kEnqItems);
     // walk grids are multiples of the XCD count, so every XCD gets the same
     // number of workgroups.  The four-lane walk holds about 90 VGPRs per lane
-    // for its whole run, and it runs beside the radix passes: 2,048 one-wave
-    // workgroups (two per SIMD) leave them registers and still finish within
-    // their time (4,096 made the frame 2% slower, 1,024 9%).
+    // for its whole run; beside the radix passes (beside = true) 2,048
+    // one-wave workgroups (two per SIMD) leave them registers and still finish
+    // within their time (4,096 made the frame 2% slower, 1,024 9%); alone it
+    // takes 4,096.
+    const int cap4 = beside ? 2048 : 4096;
     const unsigned walk_grid1 = unsigned(std::max(1, std::min(4096, (a.n + kMeshBlock - 1) / kMeshBlock) / 8)) * 8u;
-    const unsigned walk_grid4 = unsigned(std::max(1, std::min(2048, (a.n + kMeshBlock / 4 - 1) / (kMeshBlock / 4)) / 8)) * 8u;
+    const unsigned walk_grid4 = unsigned(std::max(1, std::min(cap4, (a.n + kMeshBlock / 4 - 1) / (kMeshBlock / 4)) / 8)) * 8u;
     const unsigned parity_grid = unsigned(std::min(2048, (a.n + 255 / kParityLanes) / (256 / kParityLanes)));
     for (int ci = 0; ci < a.nmesh; ci++) {
         const bool split = watertight[ci] != 0;

@@ -701,7 +701,7 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
         m.err = reinterpret_cast<uint32_t*>(cc_.p + 2);
         m.stats = mesh_stats_on_ ? mesh_stats_.p : nullptr;
         stage_begin(kMesh);
-        launch_mesh_contacts(m, mesh_split_.data(), fork ? side_ : stream_);
+        launch_mesh_contacts(m, mesh_split_.data(), fork, fork ? side_ : stream_);
         stage_end(kMesh);
     }
     if (collide && nbox > 0) {
@@ -1049,7 +1049,7 @@ void Solver::collide_only() {
         m.cpart = cpart_.p; m.cshape = cshape_.p; m.cbpos = cbpos_.p; m.cbvel = cbvel_.p; m.cnorm = cnorm_.p;
         m.err = reinterpret_cast<uint32_t*>(cc_.p + 2);
         m.stats = mesh_stats_on_ ? mesh_stats_.p : nullptr;
-        launch_mesh_contacts(m, mesh_split_.data(), stream_);
+        launch_mesh_contacts(m, mesh_split_.data(), false, stream_);
     }
     HIPCHECK(hipGetLastError());
     read_count(0);

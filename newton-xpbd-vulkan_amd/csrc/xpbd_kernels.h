@@ -269,7 +269,8 @@ void launch_restitution(const RestArgs& a, int grid, hipStream_t s);
 void launch_zero_int(int* p, int count, hipStream_t s);
 void launch_frame_begin(int* cc, uint32_t* mcnt, int nmcnt, hipStream_t s);
 // watertight[ci] != 0: four lanes per walk (split subtrees) for mesh ci
-void launch_mesh_contacts(const MeshArgs& a, const uint8_t* watertight, hipStream_t s);
+// beside: the mesh stage runs on a second stream beside the radix passes
+void launch_mesh_contacts(const MeshArgs& a, const uint8_t* watertight, bool beside, hipStream_t s);
 
 // Radix sort: one pass = upsweep (tile histograms) -> bin-row scan -> scatter.
 void launch_sort_upsweep(int bits, int n, const uint32_t* keys, const uint2* pairs, int shift, uint32_t* hist, int tiles,
