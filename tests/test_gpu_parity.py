@@ -241,6 +241,27 @@ def test_mesh_contacts_several_meshes_and_large_particles(vk):
     assert_contacts_equal(gc, oc)
 
 
+def test_profiled_frames_equal_graph_frames(vk):
+    """Stage timing runs every kernel on one stream (mesh stage included, the
+    walk on its full grid) with events between stages: the same bits as the
+    two-stream graph frames, frame after frame."""
+    rng = np.random.default_rng(3)
+    r = 0.01
+    q = scenes.lattice(24, 24, 8, 2 * r, (-0.23, -0.23, 1.03), 0.1 * r, rng)
+    verts, tris = scenes.icosphere(3)
+    sc = scene_from(q, r, meshes=[(verts, tris)])
+    a = scenes.build_solver(vk, sc)
+    b = scenes.build_solver(vk, sc, enable_profile=True)
+    for f in range(16):
+        a.step(1.0 / 60.0, 8, 1)
+        b.step(1.0 / 60.0, 8, 1)
+        np.testing.assert_array_equal(a.get_positions(), b.get_positions(), err_msg=f"frame {f}")
+        np.testing.assert_array_equal(a.get_velocities(), b.get_velocities(), err_msg=f"frame {f}")
+        assert a.last_contact_count == b.last_contact_count
+    assert sum(n for _, n in b.profile_stages().values()) > 0
+    log(f"profiled frames == graph frames over 16 frames, contacts {a.last_contact_count}")
+
+
 def test_mesh_pile_frames(vk):
     """A pile falling onto a watertight icosphere: the config-1 path at small size."""
     rng = np.random.default_rng(2)
