@@ -153,7 +153,7 @@ def bench_batched(vkxpbd, scenes, frames):
     """256 scenes (log-uniform 1,024..65,536 particles) advanced by one batched
     solver (frame graphs, autotuned block size) against 256 plain solvers
     stepped one after another, same frames; final states compared."""
-    sc = scenes.config3()
+    sc = scenes.batch_of_piles()
     counts = sc["counts"]
     total = int(counts.sum())
     b = scenes.build_batch_solver(vkxpbd, sc)
@@ -193,7 +193,7 @@ def bench_adjoint(vkxpbd, scenes, reps=3):
     tensors), and their ratio."""
     import torch
     from vkxpbd.autograd import rollout
-    sc = scenes.config4()
+    sc = scenes.rollout_pile_32768()
     p = sc["particles"]
     s = scenes.build_solver(vkxpbd, sc)
     dev = torch.device("cuda:0")
@@ -258,7 +258,7 @@ def main():
     import vkxpbd
     from vkxpbd import scenes
 
-    sc = scenes.config1()
+    sc = scenes.pile_1m_icosphere()
     N = sc["particles"]["q"].shape[0]
     verts, tris = sc["meshes"][0]
     s = scenes.build_solver(vkxpbd, sc)
@@ -289,7 +289,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded lattice + icosphere, config 1)",
+        "data": "synthetic (seeded 128 x 128 x 64 lattice over an 81,920-triangle icosphere)",
         "config": {"workload": sc["name"], "particles": N, "triangles": int(tris.shape[0]), "substeps": SUBSTEPS,
                    "iterations": 1, "frames_timed": args.steps, "dt": DT, "parallelism": "single-gpu"},
     }

@@ -72,7 +72,7 @@ def _particles(q, r, qd=None):
                 radii=np.full(n, r, np.float32), flags=np.ones(n, np.int32))
 
 
-def config0(seed=0):
+def pile_4096(seed=0):
     """4,096 particles, 16^3 lattice, r = 0.01, spacing 2r, jitter 0.1r, plane."""
     rng = np.random.default_rng(seed)
     r = 0.01
@@ -81,7 +81,7 @@ def config0(seed=0):
                 iterations=1)
 
 
-def config1(seed=1, nx=128, ny=128, nz=64, level=6):
+def pile_1m_icosphere(seed=1, nx=128, ny=128, nz=64, level=6):
     """1,048,576 particles (128 x 128 x 64, r = 0.005, spacing 2r, jitter
     0.1r, bottom layer at z = 1.05) over a radius-0.5 icosphere (81,920
     triangles) centred at (0, 0, 0.5), plus the ground plane."""
@@ -111,7 +111,7 @@ def random_quats(rng, n):
     return q.astype(np.float32)
 
 
-def config2(seed=2, nb=512, n1=64):
+def boxes_and_pile(seed=2, nb=512, n1=64):
     """nb box bodies (half-extent 0.05, density 1000: 1 kg) with uniform
     random orientations, centres uniform in [-1,1] x [-1,1] x [0.1,1.1], plus
     n1^3 particles (r = 0.005, spacing 0.01) with the bottom layer at z = 1.3."""
@@ -126,7 +126,7 @@ def config2(seed=2, nb=512, n1=64):
                 frames=60, dt=1.0 / 60.0, substeps=8, iterations=1)
 
 
-def config3(seed=3, num_scenes=256, lo=1024, hi=65536):
+def batch_of_piles(seed=3, num_scenes=256, lo=1024, hi=65536):
     """num_scenes independent piles, particle counts log-uniform on [lo, hi]
     (seeded), r = 0.005, each over its own ground plane and hash grid."""
     rng = np.random.default_rng(seed)
@@ -147,7 +147,7 @@ def scene_slice(scene, k):
 
 
 def build_batch_solver(vkxpbd_module, scene, max_contacts=0, **cfg_overrides):
-    """Initialised batched XpbdSolver for a config3-style scene dict."""
+    """Initialised batched XpbdSolver for a scene dict with per-scene counts (batch_of_piles)."""
     cfg = solver_config(vkxpbd_module, scene["r"], **cfg_overrides)
     s = vkxpbd_module.XpbdSolver()
     p = scene["particles"]
@@ -159,7 +159,7 @@ def build_batch_solver(vkxpbd_module, scene, max_contacts=0, **cfg_overrides):
     return s
 
 
-def config4(seed=4):
+def rollout_pile_32768(seed=4):
     """32,768 particles (32^3, r = 0.01, spacing 2r, jitter 0.1r) with
     N(0, 0.5) initial velocities; 64 substeps of 1/480 s; seeded targets."""
     rng = np.random.default_rng(seed)

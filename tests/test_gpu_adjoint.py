@@ -103,13 +103,13 @@ def test_checkpoint_interval_does_not_change_gradients(vk):
         assert o_[3] == out[0][3]
 
 
-def test_rollout_config4(vk):
+def test_rollout_32768_particles(vk):
     """32,768 particles, 64 substeps of 1/480 s, checkpoints every 8."""
-    sc = scenes.config4()
+    sc = scenes.rollout_pile_32768()
     p = sc["particles"]
     q, v0, tgt = p["q"], p["qd"], sc["target"]
     s, o = setup(q, v0, sc["r"], 64)
-    log("config 4 rollout: 32768 particles, 64 substeps, checkpoint every 8")
+    log("rollout: 32768 particles, 64 substeps, checkpoint every 8")
     xs, gx0, gv0, gr, loss = gpu_grad(s, q, v0, tgt, 0.9, 64, 8)
     hx0, hv0, hr = host_grad(o, xs, q, v0, tgt, 0.9, 64)
     rng = np.random.default_rng(44)

@@ -42,7 +42,7 @@ def compare_grid(g, o):
 
 
 def test_one_substep_bitexact_pile_4096(vk):
-    sc = scenes.config0()
+    sc = scenes.pile_4096()
     g, o = make_pair(vk, sc)
     log("pile_4096: step(1/60, 1, 1)")
     g.step(1.0 / 60.0, 1, 1)
@@ -55,8 +55,8 @@ def test_one_substep_bitexact_pile_4096(vk):
 
 
 def test_frames_pile_4096(vk):
-    """config 0 for 30 frames x 8 substeps, compared every frame."""
-    sc = scenes.config0()
+    """The 4,096-particle pile for 30 frames x 8 substeps, compared every frame."""
+    sc = scenes.pile_4096()
     g, o = make_pair(vk, sc)
     for f in range(30):
         g.step(1.0 / 60.0, 8, 1)
@@ -121,7 +121,7 @@ def test_inactive_and_proxy_particles(vk):
 
 
 def test_collide_only_and_injected_contacts(vk):
-    sc = scenes.config0()
+    sc = scenes.pile_4096()
     g, o = make_pair(vk, sc)
     for _ in range(20):
         g.step(1.0 / 60.0, 8, 1)
@@ -301,9 +301,9 @@ def test_update_mesh_moving_collider(vk):
 
 
 def test_large_grid_properties(vk):
-    """config 1 size (1,048,576 particles): size-independent properties of the
+    """1,048,576 particles over the icosphere: size-independent properties of the
     sort and cell tables after one substep with no iterations."""
-    sc = scenes.config1()
+    sc = scenes.pile_1m_icosphere()
     g = scenes.build_solver(vk, sc)
     g.step(1.0 / 60.0, 1, 0)   # integrate + grid + sort only
     x = g.get_positions()
@@ -438,21 +438,21 @@ def test_boxes_and_particles(vk):
     np.testing.assert_allclose(np.linalg.norm(gq[:, 3:], axis=1), 1.0, atol=1e-5)
 
 
-def test_config2_boxes_and_particles_full_size(vk):
+def test_512_boxes_and_262144_particles(vk):
     """The rigid-body configuration at full size: 512 boxes with random
     orientations falling onto the plane and 262,144 particles falling onto
     them.  After the first frame the state is within 1e-5 m of the oracle
     (body deltas are float atomics summed in arbitrary order); later frames'
     drift is reported, and the boxes stay normalised and above the plane."""
     import time
-    sc = scenes.config2()
+    sc = scenes.boxes_and_pile()
     g, o = make_pair(vk, sc)
     t0 = time.perf_counter()
     g.step(1.0 / 60.0, 8, 1)
     t1 = time.perf_counter()
     o.step(1.0 / 60.0, 8, 1)
-    log(f"config 2 frame 1: gpu {1e3 * (t1 - t0):.1f} ms (first frame, includes capture); oracle {time.perf_counter() - t1:.1f} s")
-    dx = compare_state(g, o, exact=False, what="config 2 frame 1")
+    log(f"boxes and pile frame 1: gpu {1e3 * (t1 - t0):.1f} ms (first frame, includes capture); oracle {time.perf_counter() - t1:.1f} s")
+    dx = compare_state(g, o, exact=False, what="boxes and pile frame 1")
     db = float(np.max(np.abs(g.get_body_q().astype(np.float64) - o.get_body_q())))
     log(f"  bodies frame 1: max|dq_body|={db:.3e}")
     assert db <= 1e-5 and dx <= 1e-5
@@ -475,7 +475,7 @@ def test_graph_replay_equals_eager(vk):
     """Captured frames (hipGraph replay) give the same bits as eager launches,
     across an update_mesh (which drops the captured frames) and a change of
     the step parameters."""
-    sc = scenes.config0()
+    sc = scenes.pile_4096()
     v, t = scenes.icosphere(2, 0.1, (0.0, 0.0, 0.1))
     sc["meshes"] = [(v, t)]
     a = scenes.build_solver(vk, sc)
@@ -501,7 +501,7 @@ def test_batched_scenes_match_sequential(vk, num_scenes, lo, hi, frames):
     bit for bit (each scene keeps its own grid, neighbour order and plane).
     7 scenes sort 24-bit keys in three 8-bit digits, 300 scenes 30-bit keys
     in four."""
-    sc = scenes.config3(seed=9, num_scenes=num_scenes, lo=lo, hi=hi)
+    sc = scenes.batch_of_piles(seed=9, num_scenes=num_scenes, lo=lo, hi=hi)
     sc["counts"][3] = 1   # a single-particle scene
     off = np.concatenate([[0], np.cumsum(sc["counts"])])
     sc["particles"] = {k: np.concatenate([scenes.scene_slice(sc, j)["particles"][k] for j in range(num_scenes)])
@@ -529,7 +529,7 @@ def test_batched_scenes_match_sequential(vk, num_scenes, lo, hi, frames):
 
 
 def test_autotune_keeps_state(vk):
-    sc = scenes.config0()
+    sc = scenes.pile_4096()
     a = scenes.build_solver(vk, sc)
     b = scenes.build_solver(vk, sc)
     blk = a.autotune(2, 1.0 / 60.0, 8, 1)

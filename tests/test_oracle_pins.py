@@ -302,7 +302,7 @@ def test_fp32_restatement_tracks_fp64():
     frame the absolute error is far below the 1e-5 m bar.  A granular pile is
     chaotic, so over many frames fp32 and fp64 trajectories separate; that
     drift is reported, not bounded."""
-    sc = scenes.config0()
+    sc = scenes.pile_4096()
     p = sc["particles"]
     sub = slice(0, 1024)
     outs = {}
