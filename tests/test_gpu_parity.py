@@ -262,6 +262,48 @@ def test_profiled_frames_equal_graph_frames(vk):
     log(f"profiled frames == graph frames over 16 frames, contacts {a.last_contact_count}")
 
 
+def test_diagnostic_pair_records(vk):
+    """enable_diagnostics: one record per contributing pair (scene.h:43-64).
+    A floating, overlapping lattice (no shape contacts) after one substep:
+    the records' (i, j) set is the brute-force set of pairs closer than
+    r_i + r_j at the predicted positions, their distances agree with those
+    positions, and each particle's records sum to its applied delta (the
+    oracle's, bit for bit, through get_deltas)."""
+    rng = np.random.default_rng(8)
+    r = 0.01
+    q = scenes.lattice(6, 6, 6, 1.9 * r, (-0.05, -0.05, 1.0), 0.05 * r, rng)
+    n = q.shape[0]
+    sc = scene_from(q, r)
+    g, o = make_pair(vk, sc, enable_diagnostics=True, max_diag_pairs=100000)
+    dt = 1.0 / 60.0
+    g.step(dt, 1, 1)
+    o.step(dt, 1, 1)
+    compare_state(g, o, exact=True, what="diagnostics substep")
+    np.testing.assert_array_equal(g.get_deltas().astype(np.float64), o.get_deltas())
+    recs = g.get_diag_pairs()
+    assert g.get_diag_overflow_count() == 0
+    assert g.get_diag_pair_count() == len(recs) > 0
+    # predicted positions (zero initial velocity, gravity only), in float64
+    x1 = q.astype(np.float64) + np.array([0.0, 0.0, -9.81 * dt * dt])
+    d = np.linalg.norm(x1[:, None, :] - x1[None, :, :], axis=2)
+    err = d - 2 * r
+    want = {(i, j) for i in range(n) for j in range(n) if i != j and err[i, j] < -1e-7}
+    unsure = {(i, j) for i in range(n) for j in range(n) if i != j and abs(err[i, j]) <= 1e-7}
+    got = {(rc["i"], rc["j"]) for rc in recs}
+    assert len(got) == len(recs)                      # one record per ordered pair
+    assert want <= got and got <= want | unsure
+    acc = np.zeros((n, 3))
+    for rc in recs:
+        i, j = rc["i"], rc["j"]
+        assert rc["iter_index"] == 0
+        assert abs(rc["dist"] - d[i, j]) <= 1e-6 and abs(rc["err"] - err[i, j]) <= 1e-6
+        acc[i] += rc["delta_i"]
+    np.testing.assert_allclose(acc, g.get_deltas().astype(np.float64), rtol=1e-5, atol=1e-9)
+    log(f"diagnostics: {len(recs)} pair records over {n} particles")
+    g.reset_diag_buffers()
+    assert g.get_diag_pair_count() == 0
+
+
 def test_mesh_pile_frames(vk):
     """A pile falling onto a watertight icosphere: the config-1 path at small size."""
     rng = np.random.default_rng(2)
