@@ -84,6 +84,29 @@ def test_ragged_sizes(vk, n):
     compare_state(g, o, exact=True, what=f"n={n}")
 
 
+@pytest.mark.parametrize("n", [1, 63, 1025])
+def test_ragged_sizes_with_mesh(vk, n):
+    """Particle counts that leave partial workgroups in every mesh pass (the
+    four-particle candidate threads, the walk's per-XCD queue ranges) over a
+    watertight icosphere, against the oracle bit for bit."""
+    rng = np.random.default_rng(100 + n)
+    r = 0.01
+    u = rng.normal(size=(n, 3))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    q = (np.array([0.0, 0.0, 0.5]) + u * rng.uniform(0.49, 0.53, size=(n, 1))).astype(np.float32)
+    verts, tris = scenes.icosphere(3)
+    sc = scene_from(q, r, meshes=[(verts, tris)])
+    g, o = make_pair(vk, sc)
+    log(f"ragged n={n} over an icosphere")
+    for f in range(3):
+        g.step(1.0 / 60.0, 4, 1)
+        o.step(1.0 / 60.0, 4, 1)
+        assert g.last_contact_count == o.last_contact_count
+    assert g.last_contact_count > 0
+    assert_contacts_equal(gpu_contacts(g), o.get_contacts())
+    compare_state(g, o, exact=True, what=f"n={n} with mesh")
+
+
 def test_two_iterations_cohesion_restitution(vk):
     rng = np.random.default_rng(11)
     r = 0.01
