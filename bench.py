@@ -306,7 +306,9 @@ def main():
         shapes = sp.get_contact_shapes()
         cm.append(int(np.count_nonzero(shapes > 0)))
         prof = sp.profile_stages()
-        passes = 4 if sp.wrap_planes else 3   # 27-bit keys with wrap planes, else 21
+        # radix passes per substep (three for 21-bit cell keys or one plane bit
+        # per axis, four for two plane bits), as launched
+        passes = max(1, round(prof["sort_scatter"][1] / (args.steps * SUBSTEPS)))
         del sp
         C = float(np.mean(contacts))
         Cm = float(cm[-1]) * (C / max(contacts[-1], 1))

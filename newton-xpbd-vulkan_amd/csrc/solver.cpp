@@ -142,7 +142,9 @@ void Solver::init_common(const xpbd_config& cfg, size_t n, size_t max_contacts) 
     const char* penv = std::getenv("VKXPBD_WRAP_PLANES");
     planes_allowed_ = penv ? std::strcmp(penv, "1") == 0 : n_ >= kPlanesMinParticles;
     const char* benv = std::getenv("VKXPBD_PLANE_BITS");
+    plane_bits_fixed_ = benv != nullptr;
     plane_bits_ = (benv && std::strcmp(benv, "1") == 0) ? 1 : 2;
+    table_bits_ = -1;
     planes_ = false;
     update_planes();
     grid_valid_ = false;
@@ -498,6 +500,22 @@ void Solver::set_particles(const float* q, const float* qd, const float* w, cons
     if (r) std::memcpy(radius_h_.data(), r, n * sizeof(float));
     rmax_ = 0.0f;
     for (size_t i = 0; i < n; i++) rmax_ = std::max(rmax_, std::fabs(radius_h_[i]));
+    if (!plane_bits_fixed_ && q && n) {
+        // one plane bit per axis (256 virtual cells, three 8-bit radix passes
+        // instead of four 7-bit ones) while the particles span at most 192
+        // cells on every axis; a scene that spreads further only aliases
+        // more distant cells, which the neighbour test rejects
+        float lo[3], hi[3];
+        for (int k = 0; k < 3; k++) lo[k] = hi[k] = q[k];
+        for (size_t i = 0; i < n; i++)
+            for (int k = 0; k < 3; k++) {
+                lo[k] = std::min(lo[k], q[3 * i + k]);
+                hi[k] = std::max(hi[k], q[3 * i + k]);
+            }
+        float span = 0.0f;
+        for (int k = 0; k < 3; k++) span = std::max(span, (hi[k] - lo[k]) / cfg_.search_radius);
+        plane_bits_ = span <= 192.0f ? 1 : 2;
+    }
     update_planes();
     // written into the current slot order: slot t holds particle ids[t]
     const std::vector<uint32_t> ids = host_ids();
@@ -592,11 +610,12 @@ void Solver::update_planes() {
     const float rm = std::max(rmax_, cfg_.particle_max_radius);
     const float reach = 3.0f * rm + cfg_.particle_max_radius + 2.0f * std::fabs(cfg_.particle_cohesion);
     const bool want = planes_allowed_ && num_scenes_ == 1 && reach < 126.0f * cfg_.search_radius;
-    if (want == planes_ && range_.p) return;
+    if (want == planes_ && range_.p && (!planes_ || table_bits_ == plane_bits_)) return;
     DeviceGuard g(device_);
     HIPCHECK(hipStreamSynchronize(stream_));
     invalidate_graphs();
     planes_ = want;
+    table_bits_ = plane_bits_;
     if (num_scenes_ == 1) set_sort_shape(planes_ ? kCellBits + 3 * plane_bits_ : kCellBits);
     dalloc(range_, size_t(2) * (size_t(kNumCells) << (planes_ ? 3 * plane_bits_ : 0)) * num_scenes_);
     if (planes_) dalloc(rkeys_, n_);   // zero keys: the first reset touches row 0 only

@@ -158,6 +158,8 @@ private:
     static constexpr size_t kPlanesMinParticles = size_t(1) << 18;
     bool planes_ = false, planes_allowed_ = false;
     int plane_bits_ = 2;   // wrap-plane bits per axis (1 or 2)
+    bool plane_bits_fixed_ = false;   // set by VKXPBD_PLANE_BITS; else chosen from the particles' extent
+    int table_bits_ = -1;             // plane bits the cell table and sort were shaped for
     // particle-pair block size (64, 128 or 256; autotune() picks per scene):
     // 64 measured fastest on the 1M-particle pile (2.31 against 2.38 ms per frame)
     int pp_block_ = 64;
