@@ -34,11 +34,12 @@ SUBSTEPS = 8
 DT = 1.0 / 60.0
 
 
-def stage_bytes(stage, N, C, Cm, U, tiles, mesh_bytes, passes):
+def stage_bytes(stage, N, C, Cm, U, tiles, mesh_bytes, passes, bins=128):
     """Algorithmic HBM bytes of ONE launch of a stage: every array the kernel
     must touch, counted once (neighbour re-reads are cache hits).  The state
-    lives in sorted slot order; the sort runs `passes` LSD passes."""
-    hist = tiles * 128 * 4
+    lives in sorted slot order; the sort runs `passes` LSD passes of `bins`
+    digit values over `tiles` tiles."""
+    hist = tiles * bins * 4
     if stage == "integrate":      # P,V,flags,ids in; (key, slot) pair out; plane contacts;
         # reset of the last substep's cell rows (their keys, zeroed rows)
         return N * (16 + 16 + 4 + 4 + 8) + 56 * (C - Cm) + 4 * N + 8 * U
@@ -309,17 +310,18 @@ def main():
         # radix passes per substep (three for 21-bit cell keys or one plane bit
         # per axis, four for two plane bits), as launched
         passes = max(1, round(prof["sort_scatter"][1] / (args.steps * SUBSTEPS)))
+        bins = 256 if (sp.wrap_planes and passes == 3) else 128   # 24-bit keys: 8-bit digits; 21 or 27: 7-bit
         del sp
         C = float(np.mean(contacts))
         Cm = float(cm[-1]) * (C / max(contacts[-1], 1))
-        tiles = (N + 1023) // 1024
+        tiles = (N + 2047) // 2048   # 2,048-key radix tiles
         mesh_bytes = 2 * tris.shape[0] * 32 + tris.shape[0] * 16 + verts.shape[0] * 24
         kern = {}
         for name, (ms, launches) in prof.items():
             if launches == 0 or name == "readback":
                 continue
             avg_ms = ms / launches
-            b = stage_bytes(name, N, C, Cm, U, tiles, mesh_bytes, passes)
+            b = stage_bytes(name, N, C, Cm, U, tiles, mesh_bytes, passes, bins)
             kern[name] = {"avg_us": 1e3 * avg_ms, "launches": int(launches), "alg_bytes": b,
                           "gbs": b / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0}
         # the dominant kernel: the single-kernel stage with the most GPU time
