@@ -32,7 +32,7 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
 }
 
 // keys, or the x of (key, value) pairs when pairs is non-null (the first pass)
-template <int BITS>
+template <int BITS, int ITEMS>
 __global__ __launch_bounds__(256) void k_sort_upsweep(int n, const uint32_t* __restrict__ keys,
                                                       const uint2* __restrict__ pairs, int shift,
                                                       uint32_t* __restrict__ hist, int tiles) {
@@ -42,8 +42,8 @@ __global__ __launch_bounds__(256) void k_sort_upsweep(int n, const uint32_t* __r
     for (int b = threadIdx.x; b < B; b += 256) h[b] = 0;
     __syncthreads();
 #pragma unroll
-    for (int it = 0; it < kSortItems; it++) {
-        int idx = tile * kSortTile + w * (kSortItems * kWave) + it * kWave + lane;
+    for (int it = 0; it < ITEMS; it++) {
+        int idx = tile * (256 * ITEMS) + w * (ITEMS * kWave) + it * kWave + lane;
         if (idx < n) atomicAdd(&h[((pairs ? pairs[idx].x : keys[idx]) >> shift) & (B - 1)], 1u);
     }
     __syncthreads();
@@ -82,7 +82,7 @@ __global__ __launch_bounds__(256) void k_sort_binscan(uint32_t* __restrict__ his
 // code k_integrate keyed it with) and stored with its flags and id at the
 // sorted slot (the new order of the state: the particle-particle solve reads
 // neighbours as contiguous runs), and rank records previous slot -> new slot.
-template <int BITS, bool PAIRS, bool GATHER>
+template <int BITS, bool PAIRS, bool GATHER, int ITEMS>
 __global__ __launch_bounds__(256) void k_sort_scatter(int n, const uint32_t* __restrict__ keys_in,
                                                       const uint32_t* __restrict__ vals_in,
                                                       const uint2* __restrict__ pairs_in,
@@ -90,22 +90,23 @@ __global__ __launch_bounds__(256) void k_sort_scatter(int n, const uint32_t* __r
                                                       int shift, const uint32_t* __restrict__ hist,
                                                       const uint32_t* __restrict__ bintot, int tiles, SortGather g) {
     constexpr int B = 1 << BITS;
+    constexpr int TILE = 256 * ITEMS;
     constexpr int BPT = B > 256 ? B / 256 : 1;   // bins per thread in the tile-level scans
     __shared__ uint32_t s_cnt[4][B];    // per-wave bin counters, then per-wave bin offsets
     __shared__ uint32_t s_excl[B];      // tile-local exclusive bin starts (staging order)
     __shared__ uint32_t s_gbase[B];     // global destination of the tile's first key per bin
-    __shared__ uint32_t s_keys[kSortTile];
-    __shared__ uint32_t s_vals[kSortTile];
+    __shared__ uint32_t s_keys[TILE];
+    __shared__ uint32_t s_vals[TILE];
     __shared__ uint32_t s_wsum[2][4];
     const int tile = xcd_tile(blockIdx.x, gridDim.x), w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint64_t lt = (1ull << lane) - 1ull;
     for (int k = threadIdx.x; k < 4 * B; k += 256) (&s_cnt[0][0])[k] = 0;
     __syncthreads();
 
-    uint32_t key[kSortItems], val[kSortItems], rank[kSortItems];
+    uint32_t key[ITEMS], val[ITEMS], rank[ITEMS];
 #pragma unroll
-    for (int it = 0; it < kSortItems; it++) {
-        const int idx = tile * kSortTile + w * (kSortItems * kWave) + it * kWave + lane;
+    for (int it = 0; it < ITEMS; it++) {
+        const int idx = tile * TILE + w * (ITEMS * kWave) + it * kWave + lane;
         const bool valid = idx < n;
         if (PAIRS) {
             const uint2 kv = valid ? pairs_in[idx] : make_uint2(0xFFFFFFFFu, 0u);
@@ -166,19 +167,19 @@ __global__ __launch_bounds__(256) void k_sort_scatter(int n, const uint32_t* __r
     }
     __syncthreads();
 #pragma unroll
-    for (int it = 0; it < kSortItems; it++) {
-        const int idx = tile * kSortTile + w * (kSortItems * kWave) + it * kWave + lane;
+    for (int it = 0; it < ITEMS; it++) {
+        const int idx = tile * TILE + w * (ITEMS * kWave) + it * kWave + lane;
         if (idx < n) {
             const uint32_t d = (key[it] >> shift) & (B - 1);
             const uint32_t pos = s_excl[d] + s_cnt[w][d] + rank[it];
-            if (pos < uint32_t(kSortTile)) {
+            if (pos < uint32_t(TILE)) {
                 s_keys[pos] = key[it];
                 s_vals[pos] = val[it];
             }
         }
     }
     __syncthreads();
-    const int cnt = min(kSortTile, n - tile * kSortTile);
+    const int cnt = min(TILE, n - tile * TILE);
     for (int e = threadIdx.x; e < cnt; e += 256) {
         const uint32_t k = s_keys[e];
         const uint32_t d = (k >> shift) & (B - 1);
@@ -201,48 +202,61 @@ __global__ __launch_bounds__(256) void k_sort_scatter(int n, const uint32_t* __r
     }
 }
 
-template <int BITS, bool PAIRS, bool GATHER>
+template <int BITS, bool PAIRS, bool GATHER, int ITEMS>
 static void scatter(int n, const uint32_t* keys_in, const uint32_t* vals_in, const uint2* pairs_in, uint32_t* keys_out,
                     uint32_t* vals_out, int shift, const uint32_t* hist, const uint32_t* bintot, int tiles,
                     const SortGather& g, hipStream_t s) {
-    hipLaunchKernelGGL((k_sort_scatter<BITS, PAIRS, GATHER>), dim3(tiles), dim3(256), 0, s, n, keys_in, vals_in, pairs_in,
-                       keys_out, vals_out, shift, hist, bintot, tiles, g);
+    hipLaunchKernelGGL((k_sort_scatter<BITS, PAIRS, GATHER, ITEMS>), dim3(tiles), dim3(256), 0, s, n, keys_in, vals_in,
+                       pairs_in, keys_out, vals_out, shift, hist, bintot, tiles, g);
 }
-template <int BITS>
+template <int BITS, int ITEMS>
 static void scatter_bits(int n, const uint32_t* keys_in, const uint32_t* vals_in, const uint2* pairs_in,
                          uint32_t* keys_out, uint32_t* vals_out, int shift, const uint32_t* hist, const uint32_t* bintot,
                          int tiles, const SortGather* gather, hipStream_t s) {
     const SortGather g = gather ? *gather : SortGather{};
     if (pairs_in) {
-        if (gather) scatter<BITS, true, true>(n, keys_in, vals_in, pairs_in, keys_out, vals_out, shift, hist, bintot, tiles, g, s);
-        else scatter<BITS, true, false>(n, keys_in, vals_in, pairs_in, keys_out, vals_out, shift, hist, bintot, tiles, g, s);
+        if (gather) scatter<BITS, true, true, ITEMS>(n, keys_in, vals_in, pairs_in, keys_out, vals_out, shift, hist, bintot, tiles, g, s);
+        else scatter<BITS, true, false, ITEMS>(n, keys_in, vals_in, pairs_in, keys_out, vals_out, shift, hist, bintot, tiles, g, s);
     } else {
-        if (gather) scatter<BITS, false, true>(n, keys_in, vals_in, pairs_in, keys_out, vals_out, shift, hist, bintot, tiles, g, s);
-        else scatter<BITS, false, false>(n, keys_in, vals_in, pairs_in, keys_out, vals_out, shift, hist, bintot, tiles, g, s);
+        if (gather) scatter<BITS, false, true, ITEMS>(n, keys_in, vals_in, pairs_in, keys_out, vals_out, shift, hist, bintot, tiles, g, s);
+        else scatter<BITS, false, false, ITEMS>(n, keys_in, vals_in, pairs_in, keys_out, vals_out, shift, hist, bintot, tiles, g, s);
+    }
+}
+template <int ITEMS>
+static void scatter_items(int bits, int n, const uint32_t* keys_in, const uint32_t* vals_in, const uint2* pairs_in,
+                          uint32_t* keys_out, uint32_t* vals_out, int shift, const uint32_t* hist, const uint32_t* bintot,
+                          int tiles, const SortGather* gather, hipStream_t s) {
+    switch (bits) {
+        case 7: scatter_bits<7, ITEMS>(n, keys_in, vals_in, pairs_in, keys_out, vals_out, shift, hist, bintot, tiles, gather, s); break;
+        case 8: scatter_bits<8, ITEMS>(n, keys_in, vals_in, pairs_in, keys_out, vals_out, shift, hist, bintot, tiles, gather, s); break;
+        case 9: scatter_bits<9, ITEMS>(n, keys_in, vals_in, pairs_in, keys_out, vals_out, shift, hist, bintot, tiles, gather, s); break;
+        default: scatter_bits<10, ITEMS>(n, keys_in, vals_in, pairs_in, keys_out, vals_out, shift, hist, bintot, tiles, gather, s);
+    }
+}
+template <int ITEMS>
+static void upsweep_items(int bits, int n, const uint32_t* keys, const uint2* pairs, int shift, uint32_t* hist, int tiles,
+                          hipStream_t s) {
+    switch (bits) {
+        case 7: hipLaunchKernelGGL((k_sort_upsweep<7, ITEMS>), dim3(tiles), dim3(256), 0, s, n, keys, pairs, shift, hist, tiles); break;
+        case 8: hipLaunchKernelGGL((k_sort_upsweep<8, ITEMS>), dim3(tiles), dim3(256), 0, s, n, keys, pairs, shift, hist, tiles); break;
+        case 9: hipLaunchKernelGGL((k_sort_upsweep<9, ITEMS>), dim3(tiles), dim3(256), 0, s, n, keys, pairs, shift, hist, tiles); break;
+        default: hipLaunchKernelGGL((k_sort_upsweep<10, ITEMS>), dim3(tiles), dim3(256), 0, s, n, keys, pairs, shift, hist, tiles);
     }
 }
 
-void launch_sort_upsweep(int bits, int n, const uint32_t* keys, const uint2* pairs, int shift, uint32_t* hist, int tiles,
-                         hipStream_t s) {
-    switch (bits) {
-        case 7: hipLaunchKernelGGL(k_sort_upsweep<7>, dim3(tiles), dim3(256), 0, s, n, keys, pairs, shift, hist, tiles); break;
-        case 8: hipLaunchKernelGGL(k_sort_upsweep<8>, dim3(tiles), dim3(256), 0, s, n, keys, pairs, shift, hist, tiles); break;
-        case 9: hipLaunchKernelGGL(k_sort_upsweep<9>, dim3(tiles), dim3(256), 0, s, n, keys, pairs, shift, hist, tiles); break;
-        default: hipLaunchKernelGGL(k_sort_upsweep<10>, dim3(tiles), dim3(256), 0, s, n, keys, pairs, shift, hist, tiles);
-    }
+void launch_sort_upsweep(int bits, int items, int n, const uint32_t* keys, const uint2* pairs, int shift, uint32_t* hist,
+                         int tiles, hipStream_t s) {
+    if (items == 8) upsweep_items<8>(bits, n, keys, pairs, shift, hist, tiles, s);
+    else upsweep_items<4>(bits, n, keys, pairs, shift, hist, tiles, s);
 }
 void launch_sort_binscan(int bits, uint32_t* hist, uint32_t* bintot, int tiles, hipStream_t s) {
     hipLaunchKernelGGL(k_sort_binscan, dim3(1 << bits), dim3(256), 0, s, hist, bintot, tiles);
 }
-void launch_sort_scatter(int bits, int n, const uint32_t* keys_in, const uint32_t* vals_in, const uint2* pairs_in,
+void launch_sort_scatter(int bits, int items, int n, const uint32_t* keys_in, const uint32_t* vals_in, const uint2* pairs_in,
                          uint32_t* keys_out, uint32_t* vals_out, int shift, const uint32_t* hist, const uint32_t* bintot,
                          int tiles, const SortGather* gather, hipStream_t s) {
-    switch (bits) {
-        case 7: scatter_bits<7>(n, keys_in, vals_in, pairs_in, keys_out, vals_out, shift, hist, bintot, tiles, gather, s); break;
-        case 8: scatter_bits<8>(n, keys_in, vals_in, pairs_in, keys_out, vals_out, shift, hist, bintot, tiles, gather, s); break;
-        case 9: scatter_bits<9>(n, keys_in, vals_in, pairs_in, keys_out, vals_out, shift, hist, bintot, tiles, gather, s); break;
-        default: scatter_bits<10>(n, keys_in, vals_in, pairs_in, keys_out, vals_out, shift, hist, bintot, tiles, gather, s);
-    }
+    if (items == 8) scatter_items<8>(bits, n, keys_in, vals_in, pairs_in, keys_out, vals_out, shift, hist, bintot, tiles, gather, s);
+    else scatter_items<4>(bits, n, keys_in, vals_in, pairs_in, keys_out, vals_out, shift, hist, bintot, tiles, gather, s);
 }
 
 }  // namespace xpbd

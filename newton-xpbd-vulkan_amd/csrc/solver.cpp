@@ -89,7 +89,8 @@ void Solver::alloc_all() {
     rmax_ = 0.f;
     dalloc(keysA_, n_); dalloc(valsA_, n_); dalloc(keysB_, n_); dalloc(valsB_, n_);
     dalloc(Xs_[0], n_); dalloc(Vs_[0], n_);
-    const size_t tiles = (n_ + kSortTile - 1) / kSortTile;
+    const size_t tile = size_t(256) * size_t(sort_items_for((long long)n_));
+    const size_t tiles = (n_ + tile - 1) / tile;
     dalloc(hist_, size_t(kSortMaxBins) * std::max<size_t>(tiles, 1));
     dalloc(bintot_, kSortMaxBins);
     // one 2^21-cell table per scene; the batched key scene << 21 | cell is the row
@@ -664,13 +665,13 @@ void Solver::run_sort(int tiles, float sdt) {
         const uint32_t* vin = pass == 0 ? nullptr : vb[dst ^ 1];
         const uint2* pin = pass == 0 ? kv_.p : nullptr;
         stage_begin(kSortUp);
-        launch_sort_upsweep(sort_bits_, n, kin, pin, shift, hist_.p, tiles, stream_);
+        launch_sort_upsweep(sort_bits_, sort_items_for(n), n, kin, pin, shift, hist_.p, tiles, stream_);
         stage_end(kSortUp);
         stage_begin(kSortScan);
         launch_sort_binscan(sort_bits_, hist_.p, bintot_.p, tiles, stream_);
         stage_end(kSortScan);
         stage_begin(kSortScatter);
-        launch_sort_scatter(sort_bits_, n, kin, vin, pin, kb[dst], vb[dst], shift, hist_.p, bintot_.p, tiles,
+        launch_sort_scatter(sort_bits_, sort_items_for(n), n, kin, vin, pin, kb[dst], vb[dst], shift, hist_.p, bintot_.p, tiles,
                             pass == sort_passes_ - 1 ? &g : nullptr, stream_);
         stage_end(kSortScatter);
         dst ^= 1;
@@ -692,7 +693,8 @@ BodyStepArgs Solver::body_step_args(float sdt) {
 
 void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zero_next) {
     const int n = int(n_);
-    const int tiles = int((n_ + kSortTile - 1) / kSortTile);
+    const int sort_tile = 256 * sort_items_for((long long)n_);
+    const int tiles = int((n_ + sort_tile - 1) / sort_tile);
     const int nbox = int(boxes_.size());
     const int M = int(meshes_.size());
     int* cc_gen = collide ? cc_.p + slot : nullptr;

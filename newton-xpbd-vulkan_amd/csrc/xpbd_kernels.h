@@ -14,8 +14,11 @@ constexpr int kSortMaxBits = 10;
 constexpr int kSortMaxBins = 1 << kSortMaxBits;
 constexpr int kSortUseBits = 8;
 constexpr int kCellBits = 21;       // batched solver: key = scene << 21 | cell
-constexpr int kSortItems = 8;   // 2,048-key tiles (1,024 measured 0.9% slower per frame with 8-bit digits)
-constexpr int kSortTile = 256 * kSortItems;
+// Keys per radix thread: 8 (2,048-key tiles) from kSortLargeN keys on, where
+// 1,024-key tiles measured 0.9% slower per frame (8-bit digits); 4 below it,
+// where fewer, longer tiles leave most CUs idle.
+constexpr int kSortLargeN = 1 << 18;
+__host__ __device__ constexpr int sort_items_for(long long n) { return n >= kSortLargeN ? 8 : 4; }
 
 // Sorted-order copies of the predicted particle state, written by the last
 // radix pass (positions with radius, velocities with inverse mass, flags).
@@ -273,10 +276,11 @@ void launch_frame_begin(int* cc, uint32_t* mcnt, int nmcnt, hipStream_t s);
 void launch_mesh_contacts(const MeshArgs& a, const uint8_t* watertight, bool beside, hipStream_t s);
 
 // Radix sort: one pass = upsweep (tile histograms) -> bin-row scan -> scatter.
-void launch_sort_upsweep(int bits, int n, const uint32_t* keys, const uint2* pairs, int shift, uint32_t* hist, int tiles,
-                         hipStream_t s);
+// items: keys per thread (4 or 8), a tile being 256 * items keys.
+void launch_sort_upsweep(int bits, int items, int n, const uint32_t* keys, const uint2* pairs, int shift, uint32_t* hist,
+                         int tiles, hipStream_t s);
 void launch_sort_binscan(int bits, uint32_t* hist, uint32_t* bintot, int tiles, hipStream_t s);
-void launch_sort_scatter(int bits, int n, const uint32_t* keys_in, const uint32_t* vals_in, const uint2* pairs_in,
+void launch_sort_scatter(int bits, int items, int n, const uint32_t* keys_in, const uint32_t* vals_in, const uint2* pairs_in,
                          uint32_t* keys_out, uint32_t* vals_out, int shift, const uint32_t* hist, const uint32_t* bintot,
                          int tiles, const SortGather* gather, hipStream_t s);
 void launch_fill_zero(int* p, size_t count, hipStream_t s);
