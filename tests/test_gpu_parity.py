@@ -327,6 +327,37 @@ def test_diagnostic_pair_records(vk):
     assert g.get_diag_pair_count() == 0
 
 
+def test_particles_touching_plane_and_mesh(vk):
+    """Particles in the wedge where a sphere meets the ground carry a plane
+    and a mesh contact at once: their two shape deltas are summed and applied
+    after the particle-pair solve by the owner mark (the larger contact
+    index).  States, deltas and contacts equal the oracle's bit for bit over
+    several substeps, and particles with both contacts are present."""
+    rng = np.random.default_rng(31)
+    r = 0.005
+    n = 2000
+    ang = rng.uniform(0.0, 2.0 * np.pi, n)
+    z = rng.uniform(0.004, 0.012, n)
+    rho = np.sqrt(0.5 ** 2 - (0.5 - z) ** 2) + rng.uniform(0.0, 0.012, n)   # just outside the sphere
+    q = np.stack([rho * np.cos(ang), rho * np.sin(ang), z], 1).astype(np.float32)
+    verts, tris = scenes.icosphere(4)
+    sc = scene_from(q, r, meshes=[(verts, tris)])
+    g, o = make_pair(vk, sc)
+    for f in range(3):
+        g.step(1.0 / 60.0, 4, 1)
+        o.step(1.0 / 60.0, 4, 1)
+        assert g.last_contact_count == o.last_contact_count
+        compare_state(g, o, exact=True, what=f"wedge frame {f + 1}")
+    np.testing.assert_array_equal(g.get_deltas().astype(np.float64), o.get_deltas())
+    c = gpu_contacts(g)
+    assert_contacts_equal(c, o.get_contacts())
+    plane = set(c["particles"][c["shapes"] == 0].tolist())
+    mesh = set(c["particles"][c["shapes"] == 1].tolist())
+    both = plane & mesh
+    log(f"wedge: {len(plane)} plane, {len(mesh)} mesh contacts, {len(both)} particles with both")
+    assert len(both) > 50
+
+
 def test_mesh_pile_frames(vk):
     """A pile falling onto a watertight icosphere: the config-1 path at small size."""
     rng = np.random.default_rng(2)
