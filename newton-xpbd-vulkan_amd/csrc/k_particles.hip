@@ -367,11 +367,9 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6, 8))) v
         return VALIDATE ? sbase : (a.planes ? wrap_plane(cx, cy, cz, a.planes) << kCellBits : 0u);
     };
     auto raw = [&](uint32_t base, int cell) -> int2 { return range[size_t(base) + size_t(cell)]; };
-    auto check = [&](uint32_t base, int cell, int2 rg) -> int2 {
-        if (VALIDATE && !(rg.x >= 0 && rg.x < a.n && rg.y > rg.x && a.keys[rg.x] == base + uint32_t(cell)))
-            rg = make_int2(0, 0);
-        return rg;
-    };
+    // every table (plain, wrap planes, batched) holds exactly the current
+    // substep's rows: the previous rows are reset before each build
+    auto check = [&](uint32_t, int, int2 rg) -> int2 { return rg; };
     // cheap reject before the refined length: a pair the exact test accepts
     // has |rd| <= (r_i + r_j + coh)(1 + 5 ulp) <= (r_i + r_max + |coh|)(1 + 5 ulp)
     // (r_max bounds every radius), and the plain squared length is within
