@@ -97,8 +97,7 @@ void Solver::alloc_all() {
     // row, and the keys of the rows written last (rkeys) let each build reset
     // exactly those rows first
     dalloc(range_, size_t(2) * kNumCells * num_scenes_);
-    if (num_scenes_ > 1) dalloc(rkeys_, n_);
-    else dfree(rkeys_);
+    dalloc(rkeys_, n_);
     dalloc(cpart_, cmax_); dalloc(cshape_, cmax_);
     dalloc(cbpos_, cmax_); dalloc(cbvel_, cmax_); dalloc(cnorm_, cmax_);
     dalloc(cc_, 8);   // [particle slot 0, slot 1, error flags, spare, rigid slot 0, slot 1, spare, spare]
@@ -623,24 +622,19 @@ void Solver::update_planes() {
     table_bits_ = plane_bits_;
     if (num_scenes_ == 1) set_sort_shape(planes_ ? kCellBits + 3 * plane_bits_ : kCellBits);
     dalloc(range_, size_t(2) * (size_t(kNumCells) << (planes_ ? 3 * plane_bits_ : 0)) * num_scenes_);
-    if (planes_ || num_scenes_ > 1) dalloc(rkeys_, n_);   // zero keys: the first reset touches row 0 only
-    else dfree(rkeys_);
+    dalloc(rkeys_, n_);   // zero keys: the first reset touches row 0 only
     HIPCHECK(hipDeviceSynchronize());
     grid_valid_ = false;
 }
 
-// Cell ranges of the sorted keys.  The plain 2^21-row table is zeroed whole;
-// the tables of wrap planes (2^24 or 2^27 rows) and of batched scenes (2^21
-// per scene) have the previous build's rows reset instead.
+// Cell ranges of the sorted keys.  Every table (2^21 rows, 2^24 or 2^27 with
+// wrap planes, 2^21 per batched scene) has the previous build's rows reset
+// first, from their keys (rkeys): a reset proportional to the particles, not
+// to the table.
 void Solver::build_ranges(const uint32_t* sorted_keys, bool cleared) {
     const int n = int(n_);
-    if (planes_ || num_scenes_ > 1) {
-        if (!cleared) launch_clear_ranges(n, rkeys_.p, range_.p, stream_);
-        launch_cell_ranges(n, sorted_keys, range_.p, rkeys_.p, stream_);
-    } else {
-        if (num_scenes_ == 1) launch_fill_zero(range_.p, size_t(2) * kNumCells, stream_);
-        launch_cell_ranges(n, sorted_keys, range_.p, nullptr, stream_);
-    }
+    if (!cleared) launch_clear_ranges(n, rkeys_.p, range_.p, stream_);
+    launch_cell_ranges(n, sorted_keys, range_.p, rkeys_.p, stream_);
 }
 
 // ----------------------------------------------------------------- step ----
@@ -767,7 +761,7 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
         a.scene = num_scenes_ > 1 ? scene_d_.p : nullptr;
         a.planes = planes_ ? plane_bits_ : 0;
         a.cpart = cpart_.p; a.cshape = cshape_.p; a.cbpos = cbpos_.p; a.cbvel = cbvel_.p; a.cnorm = cnorm_.p;
-        a.rkeys = (planes_ || num_scenes_ > 1) ? rkeys_.p : nullptr; a.range = range_.p;
+        a.rkeys = rkeys_.p; a.range = range_.p;
         stage_begin(kIntegrate);
         launch_integrate(a, stream_);
         stage_end(kIntegrate);
