@@ -50,14 +50,15 @@ __global__ __launch_bounds__(256) void k_sort_upsweep(int n, const uint32_t* __r
     for (int b = threadIdx.x; b < B; b += 256) hist[size_t(b) * tiles + tile] = h[b];
 }
 
-// One workgroup per bin: exclusive scan of hist[bin][0..tiles) in place.
-__global__ __launch_bounds__(256) void k_sort_binscan(uint32_t* __restrict__ hist, uint32_t* __restrict__ bintot,
+// One 512-thread workgroup per bin: exclusive scan of hist[bin][0..tiles) in
+// place (one step for up to 512 tiles, the 1,048,576-key sort).
+__global__ __launch_bounds__(512) void k_sort_binscan(uint32_t* __restrict__ hist, uint32_t* __restrict__ bintot,
                                                       int tiles) {
-    __shared__ uint32_t wsum[4];
+    __shared__ uint32_t wsum[8];
     const int b = blockIdx.x, w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     uint32_t* row = hist + size_t(b) * tiles;
     uint32_t carry = 0;
-    for (int base = 0; base < tiles; base += 256) {
+    for (int base = 0; base < tiles; base += 512) {
         int t = base + threadIdx.x;
         uint32_t x = t < tiles ? row[t] : 0u;
         uint32_t inc = wave_incl_scan(x);
@@ -65,7 +66,7 @@ __global__ __launch_bounds__(256) void k_sort_binscan(uint32_t* __restrict__ his
         __syncthreads();
         uint32_t wpre = 0, tot = 0;
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
+        for (int k = 0; k < 8; k++) {
             uint32_t s = wsum[k];
             if (k < w) wpre += s;
             tot += s;
@@ -250,7 +251,7 @@ void launch_sort_upsweep(int bits, int items, int n, const uint32_t* keys, const
     else upsweep_items<4>(bits, n, keys, pairs, shift, hist, tiles, s);
 }
 void launch_sort_binscan(int bits, uint32_t* hist, uint32_t* bintot, int tiles, hipStream_t s) {
-    hipLaunchKernelGGL(k_sort_binscan, dim3(1 << bits), dim3(256), 0, s, hist, bintot, tiles);
+    hipLaunchKernelGGL(k_sort_binscan, dim3(1 << bits), dim3(512), 0, s, hist, bintot, tiles);
 }
 void launch_sort_scatter(int bits, int items, int n, const uint32_t* keys_in, const uint32_t* vals_in, const uint2* pairs_in,
                          uint32_t* keys_out, uint32_t* vals_out, int shift, const uint32_t* hist, const uint32_t* bintot,
