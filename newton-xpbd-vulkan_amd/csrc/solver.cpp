@@ -141,8 +141,8 @@ void Solver::init_common(const xpbd_config& cfg, size_t n, size_t max_contacts) 
     mesh_stats_on_ = senv && (std::strcmp(senv, "1") == 0 || std::strcmp(senv, "true") == 0);
     const char* genv = std::getenv("VKXPBD_GRAPHS");
     use_graphs_ = !(genv && std::strcmp(genv, "0") == 0);
-    // wrap planes pay for their 1 GiB table and fourth radix pass on large
-    // scenes; the environment can force them either way
+    // wrap planes pay for their 128 MiB or 1 GiB table and wider radix keys
+    // on large scenes only; the environment can force them either way
     const char* penv = std::getenv("VKXPBD_WRAP_PLANES");
     planes_allowed_ = penv ? std::strcmp(penv, "1") == 0 : n_ >= kPlanesMinParticles;
     const char* benv = std::getenv("VKXPBD_PLANE_BITS");

@@ -396,11 +396,18 @@ def test_update_mesh_moving_collider(vk):
     compare_state(g, o, exact=True, what="moving collider")
 
 
-def test_large_grid_properties(vk):
-    """1,048,576 particles over the icosphere: size-independent properties of the
-    sort and cell tables after one substep with no iterations."""
-    sc = scenes.pile_1m_icosphere()
+@pytest.mark.parametrize("nz,planes", [(64, None), (96, None), (96, "0")])
+def test_large_grid_properties(vk, nz, planes, monkeypatch):
+    """1,048,576 and 1,572,864 particles over the icosphere (512 and 768 radix
+    tiles: the bin scan takes one and two steps), with wrap planes and with
+    plain cell keys (whose getters return the device's own order):
+    size-independent properties of the sort and cell tables after one
+    substep with no iterations."""
+    sc = scenes.pile_1m_icosphere(nz=nz)
+    if planes is not None:
+        monkeypatch.setenv("VKXPBD_WRAP_PLANES", planes)
     g = scenes.build_solver(vk, sc)
+    assert g.wrap_planes == (planes is None)
     g.step(1.0 / 60.0, 1, 0)   # integrate + grid + sort only
     x = g.get_positions()
     keys, ids = g.get_sorted_cells(), g.get_sorted_point_ids()
