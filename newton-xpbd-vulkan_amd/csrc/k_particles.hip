@@ -128,12 +128,12 @@ __global__ __launch_bounds__(256) void k_plane_contacts(PlaneArgs a) {
     }
 }
 
-// grid_offsets.comp:16-35 into one interleaved (start, end) table; the table
-// is zeroed by k_fill_zero first (the two 8 MiB fills of scene.cpp:918-923
-// become one 16 MiB vectorised fill), so empty cells read (0, 0).
-// rkeys (when non-null) records the keys whose rows were written, so that
-// the next build (k_integrate, or k_clear_ranges) can reset exactly those
-// rows.
+// grid_offsets.comp:16-35 into one interleaved (start, end) table.  Instead
+// of the reference's two 8 MiB table fills before every build
+// (scene.cpp:918-923), rkeys records the keys whose rows this build writes,
+// and the next build resets exactly those rows first (k_integrate does it
+// inside a substep, k_clear_ranges elsewhere), so every row not written by
+// the current build reads (0, 0).
 __global__ __launch_bounds__(256) void k_cell_ranges(int n, const uint32_t* __restrict__ keys, int* __restrict__ range,
                                                      uint32_t* __restrict__ rkeys) {
     int t = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;

@@ -502,8 +502,11 @@ void Solver::set_particles(const float* q, const float* qd, const float* w, cons
     HIPCHECK(hipStreamSynchronize(stream_));
     if (w) std::memcpy(invm_h_.data(), w, n * sizeof(float));
     if (r) std::memcpy(radius_h_.data(), r, n * sizeof(float));
+    const float old_rmax = rmax_;
     rmax_ = 0.0f;
     for (size_t i = 0; i < n; i++) rmax_ = std::max(rmax_, std::fabs(radius_h_[i]));
+    // captured frames bake the particle-pair solve's reject bound (r_max) in
+    if (rmax_ != old_rmax) invalidate_graphs();
     if (!plane_bits_fixed_ && q && n) {
         // one plane bit per axis (256 virtual cells, three 8-bit radix passes
         // instead of four 7-bit ones) while the particles span at most 192
@@ -941,31 +944,70 @@ int Solver::autotune(int frames, float dt, int substeps, int iterations) {
     DeviceGuard g(device_);
     if (n_ == 0 || substeps <= 0) return pp_block_;
     frames = std::max(frames, 1);
-    // snapshot of everything a frame advances (the state and its slot order)
-    DevBuf<float4> P0, V0, bq, bqd;
-    DevBuf<int32_t> F0;
-    DevBuf<uint32_t> I0;
-    dalloc(P0, n_); dalloc(V0, n_); dalloc(F0, n_); dalloc(I0, n_);
-    HIPCHECK(hipMemcpyAsync(P0.p, P_[s_].p, n_ * sizeof(float4), hipMemcpyDeviceToDevice, stream_));
-    HIPCHECK(hipMemcpyAsync(V0.p, V_[s_].p, n_ * sizeof(float4), hipMemcpyDeviceToDevice, stream_));
-    HIPCHECK(hipMemcpyAsync(F0.p, flags_cur(), n_ * sizeof(int32_t), hipMemcpyDeviceToDevice, stream_));
-    HIPCHECK(hipMemcpyAsync(I0.p, ids_cur(), n_ * sizeof(uint32_t), hipMemcpyDeviceToDevice, stream_));
-    const size_t nb = body_q_.n;
-    if (nb) {
-        dalloc(bq, nb); dalloc(bqd, nb);
-        HIPCHECK(hipMemcpyAsync(bq.p, body_q_.p, nb * sizeof(float4), hipMemcpyDeviceToDevice, stream_));
-        HIPCHECK(hipMemcpyAsync(bqd.p, body_qd_.p, nb * sizeof(float4), hipMemcpyDeviceToDevice, stream_));
+    HIPCHECK(hipStreamSynchronize(stream_));
+    // Snapshot of everything a frame advances and everything the getters or a
+    // later step_solve_only read: the state and its slot order, the bodies,
+    // the sorted keys, the deltas and the last step's contact set (the trial
+    // frames overwrite all of these).  The first four follow the rotating
+    // slot buffers: they are restored into whichever buffers are current.
+    struct Saved {
+        void* live;
+        DevBuf<char> copy;
+        size_t bytes;
+    };
+    std::vector<Saved> saved;
+    auto free_saved = [&] {
+        for (Saved& x : saved) dfree(x.copy);
+        saved.clear();
+    };
+    const size_t nc = std::min<size_t>(cmax_, size_t(std::max(last_count_, 0)));
+    auto keep = [&](void* live, size_t bytes) {
+        Saved x{live, {}, bytes};
+        if (live && bytes) {
+            dalloc(x.copy, bytes);
+            HIPCHECK(hipMemcpyAsync(x.copy.p, live, bytes, hipMemcpyDeviceToDevice, stream_));
+        }
+        saved.push_back(x);
+    };
+    try {
+        keep(P_[s_].p, n_ * sizeof(float4));
+        keep(V_[s_].p, n_ * sizeof(float4));
+        keep(flags_cur(), n_ * sizeof(int32_t));
+        keep(ids_cur(), n_ * sizeof(uint32_t));
+        keep(body_q_.p, body_q_.n * sizeof(float4));
+        keep(body_qd_.p, body_qd_.n * sizeof(float4));
+        keep(keysA_.p, n_ * sizeof(uint32_t));
+        keep(delta_.p, 3 * n_ * sizeof(float));
+        keep(cc_.p, cc_.n * sizeof(int32_t));
+        keep(cpart_.p, nc * sizeof(int32_t));
+        keep(cshape_.p, nc * sizeof(int32_t));
+        keep(cbpos_.p, nc * sizeof(float4));
+        keep(cbvel_.p, nc * sizeof(float4));
+        keep(cnorm_.p, nc * sizeof(float4));
+        keep(rc_s0_.p, rc_s0_.n * sizeof(int32_t));
+        keep(rc_s1_.p, rc_s1_.n * sizeof(int32_t));
+        keep(rc_p0_.p, rc_p0_.n * sizeof(float4));
+        keep(rc_p1_.p, rc_p1_.n * sizeof(float4));
+        keep(rc_n_.p, rc_n_.n * sizeof(float4));
+        HIPCHECK(hipStreamSynchronize(stream_));
+    } catch (...) {
+        free_saved();
+        throw;
     }
+    const int keep_count = last_count_, keep_cslot = contact_slot_, keep_rslot = rigid_slot_;
+    const bool keep_grid = grid_valid_;
     auto restore = [&] {
-        HIPCHECK(hipMemcpyAsync(P_[s_].p, P0.p, n_ * sizeof(float4), hipMemcpyDeviceToDevice, stream_));
-        HIPCHECK(hipMemcpyAsync(V_[s_].p, V0.p, n_ * sizeof(float4), hipMemcpyDeviceToDevice, stream_));
-        HIPCHECK(hipMemcpyAsync(flags_cur(), F0.p, n_ * sizeof(int32_t), hipMemcpyDeviceToDevice, stream_));
-        HIPCHECK(hipMemcpyAsync(ids_cur(), I0.p, n_ * sizeof(uint32_t), hipMemcpyDeviceToDevice, stream_));
-        if (nb) {
-            HIPCHECK(hipMemcpyAsync(body_q_.p, bq.p, nb * sizeof(float4), hipMemcpyDeviceToDevice, stream_));
-            HIPCHECK(hipMemcpyAsync(body_qd_.p, bqd.p, nb * sizeof(float4), hipMemcpyDeviceToDevice, stream_));
+        void* now[4] = {P_[s_].p, V_[s_].p, flags_cur(), ids_cur()};
+        for (size_t k = 0; k < saved.size(); k++) {
+            if (!saved[k].copy.p) continue;
+            HIPCHECK(hipMemcpyAsync(k < 4 ? now[k] : saved[k].live, saved[k].copy.p, saved[k].bytes,
+                                    hipMemcpyDeviceToDevice, stream_));
         }
         HIPCHECK(hipStreamSynchronize(stream_));
+        last_count_ = keep_count;
+        contact_slot_ = keep_cslot;
+        rigid_slot_ = keep_rslot;
+        grid_valid_ = keep_grid;
     };
     const int cands[] = {64, 128, 256};
     int best = pp_block_;
@@ -983,10 +1025,10 @@ int Solver::autotune(int frames, float dt, int substeps, int iterations) {
             if (ms < best_ms) { best_ms = ms; best = c; }
         }
     } catch (...) {
-        dfree(P0); dfree(V0); dfree(F0); dfree(I0); dfree(bq); dfree(bqd);
+        free_saved();
         throw;
     }
-    dfree(P0); dfree(V0); dfree(F0); dfree(I0); dfree(bq); dfree(bqd);
+    free_saved();
     pp_block_ = best;
     invalidate_graphs();
     return best;

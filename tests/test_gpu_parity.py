@@ -632,14 +632,52 @@ def test_batched_scenes_match_sequential(vk, num_scenes, lo, hi, frames):
 
 
 def test_autotune_keeps_state(vk):
+    """autotune runs trial frames and puts everything back: right after it,
+    the contacts, sorted cells, deltas and a step_solve_only over the last
+    contact set equal an untuned solver's, and so do later full steps."""
     sc = scenes.pile_4096()
     a = scenes.build_solver(vk, sc)
     b = scenes.build_solver(vk, sc)
+    for s in (a, b):
+        for _ in range(10):
+            s.step(1.0 / 60.0, 8, 1)
     blk = a.autotune(2, 1.0 / 60.0, 8, 1)
     log(f"autotune picked pp block {blk}")
     assert blk in (64, 128, 256)
+    assert a.last_contact_count == b.last_contact_count > 0
+    assert_contacts_equal(gpu_contacts(a), gpu_contacts(b))
+    np.testing.assert_array_equal(a.get_sorted_cells(), b.get_sorted_cells())
+    np.testing.assert_array_equal(a.get_sorted_point_ids(), b.get_sorted_point_ids())
+    np.testing.assert_array_equal(a.get_cell_starts(), b.get_cell_starts())
+    np.testing.assert_array_equal(a.get_deltas(), b.get_deltas())
+    a.step_solve_only(1.0 / 60.0, 2, 1)
+    b.step_solve_only(1.0 / 60.0, 2, 1)
+    np.testing.assert_array_equal(a.get_positions(), b.get_positions())
     for _ in range(3):
         a.step(1.0 / 60.0, 8, 1)
         b.step(1.0 / 60.0, 8, 1)
     np.testing.assert_array_equal(a.get_positions(), b.get_positions())
     np.testing.assert_array_equal(a.get_velocities(), b.get_velocities())
+
+
+def test_radius_change_after_captured_frame(vk):
+    """A captured frame bakes the particle-pair reject bound (the largest
+    radius) in: set_particles with larger radii must drop it, so the next
+    frame keeps the oracle's bits and equals eager launches."""
+    sc = scenes.pile_4096()
+    g, o = make_pair(vk, sc)
+    e = scenes.build_solver(vk, sc)
+    e.set_graph_mode(False)
+    for s in (g, o, e):
+        s.step(1.0 / 60.0, 8, 1)
+    p = sc["particles"]
+    big = (p["radii"] * np.float32(1.5)).astype(np.float32)
+    x, v = g.get_positions(), g.get_velocities()
+    for s in (g, o, e):
+        s.set_particles(x, v, p["inv_mass"], big, p["flags"])
+    for f in range(3):
+        for s in (g, o, e):
+            s.step(1.0 / 60.0, 8, 1)
+        compare_state(g, o, exact=True, what=f"radii x1.5, frame {f + 1}")
+        np.testing.assert_array_equal(g.get_positions(), e.get_positions())
+        assert g.last_contact_count == o.last_contact_count

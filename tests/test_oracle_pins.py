@@ -317,3 +317,186 @@ def test_fp32_restatement_tracks_fp64():
     err10 = float(np.max(np.abs(outs[(False, 10)] - outs[(True, 10)])))
     print(f"fp32 vs fp64: 1 frame {err1:.3e} m, 10 frames {err10:.3e} m (drift)")
     assert err1 < 1e-6, err1
+
+
+# ------------------------------------------------------------ restitution ---
+def _one_particle(z, vz, vx=0.0, e=0.5, restitution=True, **kw):
+    o = solver([[0.1, -0.2, z]], r=0.01, qd=[[vx, 0.0, vz]], fp64=True, gravity=[0, 0, 0],
+               enable_restitution=restitution, soft_contact_restitution=e, **kw)
+    o.step(1.0 / 60.0, 1, 1)
+    return o.get_positions()[0], o.get_velocities()[0]
+
+
+@pytest.mark.parametrize("e", [0.0, 0.5, 1.0])
+def test_restitution_approaching_particle(e):
+    """apply_particle_shape_restitution.comp:48-116: a particle in contact at
+    the substep start (c = z - r <= adhesion) that approaches the plane with
+    normal speed v_n leaves with exactly -e*v_n along the normal, whatever the
+    position solve left; tangential velocity and position are the
+    restitution-free step's."""
+    vz, vx = -1.0, 0.3
+    x, v = _one_particle(0.01 - 0.001, vz, vx, e)
+    x0, v0 = _one_particle(0.01 - 0.001, vz, vx, e, restitution=False)
+    np.testing.assert_allclose(v[2], -e * vz, atol=1e-12)
+    assert v0[2] != pytest.approx(-e * vz, abs=1e-3)          # the solve alone gives something else
+    np.testing.assert_array_equal(x, x0)
+    np.testing.assert_array_equal(v[:2], v0[:2])
+
+
+def test_restitution_uses_substep_start_velocity():
+    """The approach speed is the velocity before integration (v_init,
+    integrate_particles.comp:34-35), not the predicted one: with gravity g the
+    result is -e*v0_z, not -e*(v0_z + g h)."""
+    e, vz = 0.5, -1.0
+    o = solver([[0.0, 0.0, 0.009]], r=0.01, qd=[[0.0, 0.0, vz]], fp64=True, gravity=[0, 0, -9.81],
+               enable_restitution=True, soft_contact_restitution=e)
+    o.step(1.0 / 60.0, 1, 1)
+    np.testing.assert_allclose(o.get_velocities()[0, 2], -e * vz, atol=1e-12)
+
+
+@pytest.mark.parametrize("z,vz", [(0.009, 0.2), (0.012, -1.0)])
+def test_restitution_leaves_receding_and_separated_particles(z, vz):
+    """A receding particle (v_n > 0) and one whose substep-start gap
+    c = z - r is positive (above the adhesion distance 0) are untouched: the
+    state equals the restitution-free step bit for bit."""
+    x, v = _one_particle(z, vz)
+    x0, v0 = _one_particle(z, vz, restitution=False)
+    np.testing.assert_array_equal(x, x0)
+    np.testing.assert_array_equal(v, v0)
+
+
+def test_restitution_relative_to_moving_mesh():
+    """The contact's body velocity enters the relative normal velocity: a flat
+    mesh quad moving up at u leaves the particle with u + e*(u - v0) along
+    the normal (the mesh vertex velocities, interpolated at the contact)."""
+    e, vz, u = 0.5, -1.0, 0.4
+    verts = np.float32([[-1, -1, 0], [1, -1, 0], [1, 1, 0], [-1, 1, 0]])
+    tris = np.uint32([[0, 1, 2], [0, 2, 3]])
+    o = solver([[0.1, 0.2, 0.009]], r=0.01, qd=[[0.0, 0.0, vz]], fp64=True, gravity=[0, 0, 0],
+               meshes=[(verts, tris)], has_ground_plane=False, enable_restitution=True,
+               soft_contact_restitution=e)
+    o.update_mesh(0, verts, np.tile(np.float32([0.0, 0.0, u]), (4, 1)))
+    o.step(1.0 / 60.0, 1, 1)
+    assert o.last_contact_count == 1
+    np.testing.assert_allclose(o.get_velocities()[0, 2], u + e * (u - vz), atol=1e-9)
+
+
+# ---------------------------------------------------- cohesion / adhesion ---
+@pytest.mark.parametrize("w1,w2", [(10.0, 10.0), (10.0, 2.5)])
+def test_cohesion_attracts_separated_pair(w1, w2):
+    """solve_particle_particle_contacts.comp:150-177 with cohesion: a pair at
+    rest with gap 0 < d - 2r <= cohesion is a contact; each particle moves
+    toward the other by relax*(d - 2r)*w_i/(w1 + w2).  A pair whose gap
+    exceeds the cohesion distance does not move."""
+    r, coh, relax = 0.01, 0.002, 0.9
+    for gap, touch in ((0.5 * coh, True), (1.5 * coh, False)):
+        d = 2 * r + gap
+        q = np.float32([[-d / 2, 0.0, 1.0], [d / 2, 0.0, 1.0]])
+        o = solver(q, r=r, w=[w1, w2], fp64=True, gravity=[0, 0, 0], has_ground_plane=False,
+                   particle_cohesion=coh)
+        o.step(1.0 / 60.0, 1, 1)
+        x = o.get_positions()
+        err = float(q[1, 0]) - float(q[0, 0]) - 2 * float(np.float32(r))
+        if touch:
+            np.testing.assert_allclose(x[0, 0] - q[0, 0], relax * err * w1 / (w1 + w2), rtol=1e-6)
+            np.testing.assert_allclose(x[1, 0] - q[1, 0], -relax * err * w2 / (w1 + w2), rtol=1e-6)
+        else:
+            np.testing.assert_array_equal(x, q)
+        np.testing.assert_array_equal(x[:, 1:], q[:, 1:])
+
+
+def test_adhesion_pulls_particle_to_plane():
+    """solve_particle_shape_contacts.comp with particle_adhesion as the contact
+    distance ka: a particle hovering c = z - r above the plane with c <= ka
+    is pulled down by relax*c; with c > ka it stays."""
+    r, c, relax = 0.01, 0.001, 0.9
+    for ka, moves in ((0.002, True), (0.0005, False)):
+        o = solver([[0.0, 0.0, r + c]], r=r, fp64=True, gravity=[0, 0, 0], particle_adhesion=ka)
+        o.step(1.0 / 60.0, 1, 1)
+        dz = o.get_deltas()[0]
+        want = -relax * (float(np.float32(r + c)) - float(np.float32(r))) if moves else 0.0
+        np.testing.assert_allclose(dz, [0.0, 0.0, want], atol=1e-12)
+
+
+# ------------------------------------------- non-watertight mesh normals ---
+def _open_fan(ring_angles, h, rad=0.1, apex=(0.0, 0.0, 0.5)):
+    """An open cone: apex vertex 0 and a ring of vertices h below it, one
+    triangle per ring edge wound so the normals point up and out; the
+    bottom is left open (not watertight)."""
+    ring = np.stack([rad * np.cos(ring_angles), rad * np.sin(ring_angles), np.full_like(ring_angles, -h)], 1)
+    verts = (np.concatenate([np.zeros((1, 3)), ring]) + np.asarray(apex)).astype(np.float32)
+    k = len(ring_angles)
+    tris = np.array([[0, 1 + j, 1 + (j + 1) % k] for j in range(k)], np.uint32)
+    return verts, tris
+
+
+def _fan_normals(verts, tris):
+    """Angle-weighted (apex angle, in float64 from arccos) and uniform sums of
+    the unit face normals around the apex."""
+    v = verts.astype(np.float64)
+    nt, n1 = np.zeros(3), np.zeros(3)
+    for t in tris:
+        a, b, c = v[t]
+        n = np.cross(b - a, c - a)
+        n /= np.linalg.norm(n)
+        e0, e1 = (b - a) / np.linalg.norm(b - a), (c - a) / np.linalg.norm(c - a)
+        nt += np.arccos(np.clip(e0 @ e1, -1.0, 1.0)) * n
+        n1 += n
+    return nt, n1
+
+
+def test_open_mesh_face_interior_normal():
+    """generate_mesh_contacts.comp:266-313,390 on an open mesh: a point over a
+    face interior gets that face's normal, on both sides of the face; the
+    signed distance is negative behind it and body_pos is the foot point."""
+    verts, tris = _open_fan(np.array([0.0, 2.1, 4.2]), 0.02)
+    v = verts.astype(np.float64)
+    a, b, c = v[tris[0]]
+    n = np.cross(b - a, c - a)
+    n /= np.linalg.norm(n)
+    foot = (a + b + c) / 3.0
+    r, t = 0.005, 0.008
+    q = np.float32([foot + t * n, foot - t * n])
+    o = solver(q, r=r, meshes=[(verts, tris)], fp64=True, has_ground_plane=False, gravity=[0, 0, 0])
+    o.collide_only()
+    cts = o.get_contacts()
+    assert sorted(cts["particles"].tolist()) == [0, 1]
+    for k, pi in enumerate(cts["particles"]):
+        np.testing.assert_allclose(cts["normals"][k], n, atol=1e-6)
+        np.testing.assert_allclose(cts["body_positions"][k], foot, atol=1e-6)
+        sd = float(np.dot(q[pi].astype(np.float64) - cts["body_positions"][k], cts["normals"][k]))
+        assert (sd > 0) == (pi == 0)
+
+
+@pytest.mark.parametrize("side,h", [(1.0, 0.3), (-1.0, -0.3)])
+def test_open_mesh_vertex_region_uses_angle_weights(side, h):
+    """A point in the apex's region of an open, uneven cone: every face's
+    closest point is the apex, and the inside/outside sign is the sign of the
+    angle-weighted pseudo-normal (weights = apex angles, :276-281) dotted
+    with p - apex.  The direction is chosen where the uniform sum of face
+    normals gives the opposite sign, so a wrong weight fails the test.  A
+    spike (h > 0) has only outside points in its apex region, a pit (ring
+    above the apex, h < 0) only inside points."""
+    verts, tris = _open_fan(np.array([0.0, 0.25, 0.5, 2.6, 4.4]), h)
+    nt, n1 = _fan_normals(verts, tris)
+    apex = verts[0].astype(np.float64)
+    rel = verts[1:].astype(np.float64) - apex
+    rng = np.random.default_rng(12)
+    u = None
+    for _ in range(100000):
+        w = rng.normal(size=3)
+        w /= np.linalg.norm(w)
+        if (np.all(rel @ w < -1e-2) and np.sign(w @ nt) == side and np.sign(w @ n1) == -side
+                and min(abs(w @ nt), abs(w @ n1)) > 0.02):
+            u = w
+            break
+    assert u is not None
+    r, t = 0.005, 0.008
+    q = np.float32([apex + t * u])
+    o = solver(q, r=r, meshes=[(verts, tris)], fp64=True, has_ground_plane=False, gravity=[0, 0, 0])
+    o.collide_only()
+    cts = o.get_contacts()
+    assert cts["particles"].tolist() == [0]
+    np.testing.assert_allclose(cts["body_positions"][0], apex, atol=1e-9)
+    dq = q[0].astype(np.float64) - apex   # the stored (float32) point's direction
+    np.testing.assert_allclose(cts["normals"][0], side * dq / np.linalg.norm(dq), atol=1e-9)
