@@ -43,6 +43,9 @@
 //     order is whatever its global atomic counter produced.
 #include <algorithm>
 #include <array>
+#include <atomic>
+#include <cstdlib>
+#include <thread>
 #include <type_traits>
 #include <cmath>
 #include <cstdint>
@@ -57,6 +60,35 @@ namespace orc {
 static constexpr int GRID_DIM = 128;
 static constexpr int NUM_CELLS = GRID_DIM * GRID_DIM * GRID_DIM;
 static constexpr int GRID_ORIGIN = 1 << 20;
+
+// Worker threads of the restatement's per-particle loops: ORC_THREADS, else
+// the host's hardware threads (at most 16).  Every parallel loop below writes
+// only its own particle's results (or per-chunk lists joined in chunk order),
+// so the output is bit-identical to the sequential loop for any thread count.
+static int orc_threads() {
+    const char* e = std::getenv("ORC_THREADS");
+    int t = e ? std::atoi(e) : int(std::thread::hardware_concurrency());
+    return std::max(1, std::min(t, 16));
+}
+
+// f(begin, end, chunk) over [0, n) in chunks of `chunk` items, chunks handed
+// out dynamically to orc_threads() threads.
+template <class F> static void par_chunks(int64_t n, int64_t chunk, F&& f) {
+    const int64_t nchunks = (n + chunk - 1) / chunk;
+    const int nt = int(std::min<int64_t>(orc_threads(), nchunks));
+    if (nt <= 1) {
+        for (int64_t c = 0; c < nchunks; c++) f(c * chunk, std::min(n, (c + 1) * chunk), c);
+        return;
+    }
+    std::atomic<int64_t> next{0};
+    auto work = [&] {
+        for (int64_t c; (c = next.fetch_add(1)) < nchunks;) f(c * chunk, std::min(n, (c + 1) * chunk), c);
+    };
+    std::vector<std::thread> th;
+    for (int k = 1; k < nt; k++) th.emplace_back(work);
+    work();
+    for (auto& t : th) t.join();
+}
 
 template <class R> struct V3 { R x, y, z; };
 template <class R> static inline V3<R> v3(R x, R y, R z) { return V3<R>{x, y, z}; }
@@ -597,8 +629,13 @@ template <class R> struct Solver {
     V3<R> nlo(uint32_t i) const { return {R(nl[i].x), R(nl[i].y), R(nl[i].z)}; }
     V3<R> nhi(uint32_t i) const { return {R(nu[i].x), R(nu[i].y), R(nu[i].z)}; }
 
-    // generate_mesh_contacts.comp:181-414 for particle i; appends contacts.
-    void mesh_contacts_particle(int64_t i) {
+    struct MeshHit {
+        int32_t p, s;
+        V3<R> bp, bv, nn;
+    };
+    // generate_mesh_contacts.comp:181-414 for particle i; appends its contacts
+    // (one per mesh at most) to out.
+    void mesh_contacts_particle(int64_t i, std::vector<MeshHit>& out) const {
         const R margin = R(cfg.soft_contact_margin);
         const R eps = R(avg_edge) * R(1e-3f);
         const R eps2 = eps * eps;
@@ -710,20 +747,27 @@ template <class R> struct Solver {
             V3<R> nn = dlen > 0 ? exact_div(dl, dlen) * sign : V3<R>{0, 0, 1};
             R d = dlen * sign;
             V3<R> bp = {fmaR(-nn.x, d, p.x), fmaR(-nn.y, d, p.y), fmaR(-nn.z, d, p.z)};
-            emit(int32_t(i), int32_t(ci) + 1, bp, sv_, nn);
+            out.push_back(MeshHit{int32_t(i), int32_t(ci) + 1, bp, sv_, nn});
         }
     }
 
     void mesh_contacts() {
         if (meshes.empty()) return;
-        for (int64_t i = 0; i < n; i++)
-            if (flags[i] & 1) mesh_contacts_particle(i);
+        constexpr int64_t kChunk = 512;
+        std::vector<std::vector<MeshHit>> hits(size_t((n + kChunk - 1) / kChunk));
+        par_chunks(n, kChunk, [&](int64_t a, int64_t b, int64_t c) {
+            for (int64_t i = a; i < b; i++)
+                if (flags[i] & 1) mesh_contacts_particle(i, hits[size_t(c)]);
+        });
+        for (const auto& hv : hits)   // particle order, as the sequential loop emits them
+            for (const MeshHit& h : hv) emit(h.p, h.s, h.bp, h.bv, h.nn);
     }
 
     void integrate(R dt) {   // integrate_particles.comp:26-60 (force and world index are always 0)
         V3<R> g = {R(cfg.gravity[0]), R(cfg.gravity[1]), R(cfg.gravity[2])};
         R vmax = R(cfg.particle_v_max);
-        for (int64_t i = 0; i < n; i++) {
+        par_chunks(n, 16384, [&](int64_t i0, int64_t i1, int64_t) {
+        for (int64_t i = i0; i < i1; i++) {
             V3<R> x0 = q[i], v0 = qd[i];
             q_init[i] = x0; qd_init[i] = v0;
             if ((flags[i] & 1) == 0) continue;
@@ -736,11 +780,13 @@ template <class R> struct Solver {
             q[i] = x0 + v1 * dt;
             qd[i] = v1;
         }
+        });
     }
 
     void grid_and_sort() {
         R inv = R(1.0f / cfg.search_radius);   // scene.cpp:702 (fp32 reciprocal)
-        for (int64_t i = 0; i < n; i++) {      // grid_cell_indices.comp:17-32
+        par_chunks(n, 16384, [&](int64_t i0, int64_t i1, int64_t) {
+        for (int64_t i = i0; i < i1; i++) {    // grid_cell_indices.comp:17-32
             V3<R> p = q[i];
             int gx = int(std::floor(p.x * inv)) + GRID_ORIGIN;
             int gy = int(std::floor(p.y * inv)) + GRID_ORIGIN;
@@ -749,6 +795,7 @@ template <class R> struct Solver {
             cells[i] = uint32_t((gz % GRID_DIM) * GRID_DIM * GRID_DIM + (gy % GRID_DIM) * GRID_DIM + (gx % GRID_DIM));
             ids[i] = uint32_t(i);
         }
+        });
         std::vector<uint32_t> order(n);
         for (int64_t i = 0; i < n; i++) order[i] = uint32_t(i);
         std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return cells[a] < cells[b]; });
@@ -1134,7 +1181,13 @@ template <class R> struct Solver {
         R kmu = R(cfg.particle_mu), coh = R(cfg.particle_cohesion), maxr = R(cfg.particle_max_radius);
         R relax = relax_r;
         R inv = R(1.0f / cfg.search_radius);
-        for (int64_t tid = 0; tid < n; tid++) {
+        // each sorted slot adds only to its own particle's delta; the taped
+        // pair list is gathered per chunk and joined in slot order
+        constexpr int64_t kChunk = 2048;
+        std::vector<std::vector<std::pair<int32_t, int32_t>>> chunk_pairs(pair_log ? size_t((n + kChunk - 1) / kChunk) : 0);
+        par_chunks(n, kChunk, [&](int64_t t0, int64_t t1, int64_t ch) {
+        std::vector<std::pair<int32_t, int32_t>>* plog = pair_log ? &chunk_pairs[size_t(ch)] : nullptr;
+        for (int64_t tid = t0; tid < t1; tid++) {
             int i = int(svals[tid]);
             int pf = flags[i];
             if ((pf & 1) == 0) continue;
@@ -1177,7 +1230,7 @@ template <class R> struct Solver {
                             V3<R> df = {fmaR(vt_dir.x, lambda_f, -p2.x), fmaR(vt_dir.y, lambda_f, -p2.y),
                                         fmaR(vt_dir.z, lambda_f, -p2.z)};
                             acc = acc + exact_div(df, denom);
-                            if (pair_log) pair_log->push_back({int32_t(i), int32_t(j)});
+                            if (plog) plog->push_back({int32_t(i), int32_t(j)});
                         }
                     }
             V3<R> dw = {fmaR(acc.x, w1, R(0)), fmaR(acc.y, w1, R(0)), fmaR(acc.z, w1, R(0))};
@@ -1186,12 +1239,15 @@ template <class R> struct Solver {
             delta[i * 3 + 1] += tot.y;
             delta[i * 3 + 2] += tot.z;
         }
+        });
+        for (const auto& v : chunk_pairs) pair_log->insert(pair_log->end(), v.begin(), v.end());
     }
 
     // apply_particle_deltas.comp:38-58 (inactive: passthrough of x_pred / v_pred)
     void apply(const std::vector<V3<R>>& Xp, const std::vector<V3<R>>& Vp, std::vector<V3<R>>& Xo, std::vector<V3<R>>& Vo, R dt) {
         R vmax = R(cfg.particle_v_max);
-        for (int64_t i = 0; i < n; i++) {
+        par_chunks(n, 16384, [&](int64_t i0, int64_t i1, int64_t) {
+        for (int64_t i = i0; i < i1; i++) {
             if ((flags[i] & 1) == 0) { Xo[i] = Xp[i]; Vo[i] = Vp[i]; continue; }
             V3<R> x0 = q_init[i], xp = Xp[i];
             V3<R> d = {delta[i * 3], delta[i * 3 + 1], delta[i * 3 + 2]};
@@ -1204,6 +1260,7 @@ template <class R> struct Solver {
             }
             Xo[i] = xn; Vo[i] = vn;
         }
+        });
     }
 
     // apply_particle_shape_restitution.comp:48-116.  The reference reads and
