@@ -284,6 +284,20 @@ int xpbd_parity_grid_candidates(const float* verts, size_t num_verts, const uint
 int xpbd_dist_grid_bounds(const float* verts, size_t num_verts, const uint32_t* tris, size_t num_tris, float cap,
                           const float* points, size_t num_points, float* bounds);
 
+/* Stable LSD radix sort of n (key, value) pairs on caller-owned device
+ * buffers of the current HIP device: (keys_out, vals_out) receive the pairs
+ * ordered by key, equal keys in input order.  Keys must be below
+ * 2^key_bits (key_bits in [1, 32]; the solver's cell keys take 21, or 24
+ * with one wrap-plane bit per axis).  The inputs are not written and the
+ * outputs must not overlap them.  Allocates its scratch (temporary pairs,
+ * tile histograms, bin totals) for the call and returns once the sort has
+ * completed on `stream` (a hipStream_t; NULL is the default stream).  Errors
+ * are reported through xpbd_last_error(NULL).
+ * Replaces GpuSort::create/record (src/gpu_sort.h:18-129), the reusable sort
+ * the reference runs over (cell, particle id) pairs every substep. */
+int xpbd_sort_pairs(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out, uint32_t* vals_out,
+                    size_t n, int key_bits, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -10,6 +10,7 @@
 
 #include "../../include/xpbd_capi.h"
 #include "solver.h"
+#include "sort_entry.h"
 
 struct xpbd_solver {
     xpbd::Solver s;
@@ -260,6 +261,18 @@ int xpbd_get_bodies(xpbd_solver* h, float* q7, float* qd6, size_t cap) {
 }
 int xpbd_set_body_velocities(xpbd_solver* h, const float* qd6, size_t n) {
     return guarded(h, [&] { h->s.set_body_velocities(qd6, n); });
+}
+
+int xpbd_sort_pairs(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out, uint32_t* vals_out,
+                    size_t n, int key_bits, void* stream) {
+    try {
+        xpbd::sort_pairs_alloc(keys_in, vals_in, keys_out, vals_out, n, key_bits, static_cast<hipStream_t>(stream));
+        g_err.clear();
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return 1;
+    }
 }
 
 int xpbd_lbvh_build(const float* verts, size_t nv, const uint32_t* tris, size_t nt, float* lowers, float* uppers,

@@ -1,0 +1,31 @@
+// sort_entry.h -- stand-alone stable radix sort of (key, value) pairs on
+// caller-owned device buffers (see sort_entry.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace xpbd {
+
+struct SortShape {
+    int passes = 0, bits = 0, items = 0;
+    size_t tiles = 0;
+};
+
+// Digit passes for keys below 2^key_bits: the fewest of at most 8 bits, the
+// width spread evenly over them (never below 7 bits); tiles of 256 x items keys.
+SortShape sort_shape(size_t n, int key_bits);
+// Scratch bytes one sort of n pairs uses (temporary keys and values, tile
+// histograms, bin totals).
+size_t sort_scratch_bytes(size_t n, int key_bits);
+void sort_validate(const void* keys_in, const void* vals_in, const void* keys_out, const void* vals_out, size_t n,
+                   int key_bits);
+// Launches the passes on stream with the given scratch block.
+void sort_run(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out, uint32_t* vals_out, size_t n,
+              int key_bits, void* scratch, hipStream_t stream);
+// Allocates the scratch, sorts, waits for the stream and frees the scratch.
+void sort_pairs_alloc(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out, uint32_t* vals_out,
+                      size_t n, int key_bits, hipStream_t stream);
+
+}  // namespace xpbd

@@ -98,3 +98,16 @@ def test_add_mesh_watertight_detection_host_only():
     s.add_mesh(v.reshape(-1), t.reshape(-1))
     s.add_mesh(v.reshape(-1), t[:-1].reshape(-1))
     assert s.mesh_count == 2
+
+
+def test_sort_pairs_rejects_bad_arguments_without_a_gpu():
+    """xpbd_sort_pairs validates its arguments before any device work and
+    reports through xpbd_last_error(NULL)."""
+    lib = C.CDLL(LIB)
+    lib.xpbd_last_error.restype = C.c_char_p
+    lib.xpbd_sort_pairs.argtypes = [C.c_void_p] * 4 + [C.c_size_t, C.c_int, C.c_void_p]
+    assert lib.xpbd_sort_pairs(None, None, None, None, 16, 0, None) == 1
+    assert b"key_bits" in lib.xpbd_last_error(None)
+    assert lib.xpbd_sort_pairs(None, None, None, None, 16, 24, None) == 1
+    assert b"null buffer" in lib.xpbd_last_error(None)
+    assert lib.xpbd_sort_pairs(None, None, None, None, 0, 24, None) == 0   # nothing to sort
