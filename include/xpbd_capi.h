@@ -298,6 +298,20 @@ int xpbd_dist_grid_bounds(const float* verts, size_t num_verts, const uint32_t* 
 int xpbd_sort_pairs(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out, uint32_t* vals_out,
                     size_t n, int key_bits, void* stream);
 
+/* xpbd_sort_pairs without any allocation: no counterpart in the reference.
+ * For callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and results as xpbd_sort_pairs, for at most 1,048,576 pairs
+ * (larger n is rejected with an error); the temporaries live in a 9 MiB
+ * static device array of the library, one per device it runs on, held until
+ * the process ends.  Launches on `stream` and returns without waiting.  The
+ * first call on a device looks up that array's address (a call with n = 0
+ * does only this, e.g. before a capture).  Every call on a device shares
+ * the array: calls that overlap on one device (other streams, other
+ * threads) race. */
+int xpbd_sort_pairs_noalloc(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out,
+                            uint32_t* vals_out, size_t n, int key_bits, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

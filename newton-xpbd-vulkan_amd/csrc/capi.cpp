@@ -275,6 +275,18 @@ int xpbd_sort_pairs(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* 
     }
 }
 
+int xpbd_sort_pairs_noalloc(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out,
+                            uint32_t* vals_out, size_t n, int key_bits, void* stream) {
+    try {
+        xpbd::sort_pairs_noalloc(keys_in, vals_in, keys_out, vals_out, n, key_bits, static_cast<hipStream_t>(stream));
+        g_err.clear();
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return 1;
+    }
+}
+
 int xpbd_lbvh_build(const float* verts, size_t nv, const uint32_t* tris, size_t nt, float* lowers, float* uppers,
                     uint32_t* prim, uint32_t* root) {
     (void)nv;

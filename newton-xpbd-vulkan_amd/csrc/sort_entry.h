@@ -28,4 +28,11 @@ void sort_run(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_o
 void sort_pairs_alloc(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out, uint32_t* vals_out,
                       size_t n, int key_bits, hipStream_t stream);
 
+// As sort_pairs_alloc, with the library's static device scratch instead of an
+// allocation (n <= kNoAllocMaxPairs); launches on stream and returns without
+// waiting.  Calls that overlap on one device share the scratch and race.
+constexpr size_t kNoAllocMaxPairs = size_t(1) << 20;
+void sort_pairs_noalloc(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out, uint32_t* vals_out,
+                        size_t n, int key_bits, hipStream_t stream);
+
 }  // namespace xpbd

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 
@@ -20,6 +21,35 @@ namespace {
 
 void check(hipError_t e, const char* what) {
     if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// Scratch of the allocation-free variant: the temporaries of one sort of up
+// to kNoAllocMaxPairs pairs (sort_scratch_bytes is at most 8.5 MiB there for
+// every key width, digits being at most 8 bits wide).  Every device the
+// library runs on holds its own copy until the process ends.  In host code
+// `scratch` names a host-side stand-in: its device address comes from
+// hipGetSymbolAddress, per device.
+constexpr size_t kNoAllocScratchBytes = size_t(9) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up on the first
+// call there (the one set-up step; it allocates nothing).
+void* scratch_on_current_device() {
+    int dev = -1;
+    check(hipGetDevice(&dev), "sort_pairs_noalloc: current device");
+    if (dev < 0 || dev >= kMaxDevices) throw std::runtime_error("sort_pairs_noalloc: device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "sort_pairs_noalloc: scratch address");
+        if (!p) throw std::runtime_error("sort_pairs_noalloc: scratch address is null");
+        g_scratch_ptr[dev] = p;
+    }
+    return g_scratch_ptr[dev];
 }
 
 }  // namespace
@@ -100,6 +130,22 @@ void sort_pairs_alloc(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t
         throw;
     }
     check(hipFree(scratch), "sort_pairs: scratch release");
+}
+
+void sort_pairs_noalloc(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out, uint32_t* vals_out,
+                        size_t n, int key_bits, hipStream_t stream) {
+    sort_validate(keys_in, vals_in, keys_out, vals_out, n, key_bits);
+    if (n > kNoAllocMaxPairs)
+        throw std::runtime_error("sort_pairs_noalloc: n = " + std::to_string(n) + " exceeds the maximum of " +
+                                 std::to_string(kNoAllocMaxPairs) + " pairs of its static scratch");
+    if (sort_scratch_bytes(n, key_bits) > kNoAllocScratchBytes)
+        throw std::runtime_error("sort_pairs_noalloc: scratch too small for this sort");
+    // the set-up runs on every call (cheap after the first on a device), so a
+    // call with n = 0 prepares a device before, say, a graph capture
+    void* s = scratch_on_current_device();
+    // nothing in the scratch has to start at zero: every pass writes its
+    // histograms, bin totals and temporaries in full before reading them
+    sort_run(keys_in, vals_in, keys_out, vals_out, n, key_bits, s, stream);
 }
 
 }  // namespace xpbd
