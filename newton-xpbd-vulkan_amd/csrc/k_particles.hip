@@ -299,10 +299,10 @@ __device__ __forceinline__ void pp_pair(const PPArgs& a, int t, int k, float4 Pj
 }
 
 // apply_particle_deltas.comp:44-57 for sorted slot t: position x (radius
-// rad, inverse mass w1) moved by its accumulated delta dl.
-__device__ __forceinline__ void apply_delta(const float4* Ps, const uint32_t* vals, int t, f3 x, float rad, float w1,
-                                            f3 dl, float dt, float vmax, float4* Po, float4* Vo) {
-    const f3 x0 = xyz(Ps[vals[t]]);
+// rad, inverse mass w1) moved by its accumulated delta dl from the
+// substep-start position x0.
+__device__ __forceinline__ void apply_delta(f3 x0, int t, f3 x, float rad, float w1, f3 dl, float dt, float vmax,
+                                            float4* Po, float4* Vo) {
     f3 xn = x + dl;
     f3 vnew = markstein_div(xn - x0, dt);
     const float mag = len3(vnew);
@@ -333,10 +333,18 @@ __device__ __forceinline__ void apply_delta(const float4* Ps, const uint32_t* va
 // reference maps every such cell to x = 0) is visited cell by cell.
 constexpr int kSurv = 16;   // recorded survivors per lane before a drain
 
+// Measured on the 1,048,576-particle pile (rocprofv3 SQ counters): the
+// vector ALUs are busy about two thirds of the kernel's time and the waves
+// wait on their chain of dependent loads (own state, cell rows, candidate
+// batches, survivors) most of the rest.  Candidates are read four at a time,
+// the last batch of a run masked, so a run of up to four costs one round
+// trip.  Six waves per SIMD (80 VGPRs) measured faster than five with wider
+// batches or paired survivor drains.
+constexpr int kScanBatch = 4;
+
 template <bool DIAG, bool VALIDATE, int BLOCK>
-// Six waves per SIMD measured faster than the five that 88 VGPRs would give,
-// despite a few spilled registers.
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_pp_apply(PPArgs a) {
+    constexpr int SB = kScanBatch;
     __shared__ uint32_t s_surv[kSurv][BLOCK];
     const int t = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (t >= a.n) return;
@@ -412,16 +420,18 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6, 8))) v
         if (__any(cnt > kSurv - need)) drain();
     };
     auto scan = [&](int s0, int e0) {
-        int k = s0;
-        for (; k + 3 < e0; k += 4) {   // four loads issue before the tests
-            const f3 P0 = pos(k), P1 = pos(k + 1), P2 = pos(k + 2), P3 = pos(k + 3);
-            room(4);
-            consider(k, P0);
-            consider(k + 1, P1);
-            consider(k + 2, P2);
-            consider(k + 3, P3);
+        // SB loads issue before the tests; the last batch of a run reads its
+        // last slot in place of the slots past the run and tests only those
+        // inside it (one round trip instead of one per remaining candidate)
+        for (int k = s0; k < e0; k += SB) {
+            f3 P[SB];
+#pragma unroll
+            for (int j = 0; j < SB; j++) P[j] = pos(min(k + j, e0 - 1));
+            room(SB);
+#pragma unroll
+            for (int j = 0; j < SB; j++)
+                if (k + j < e0) consider(k + j, P[j]);
         }
-        for (; k < e0; k++) { room(1); consider(k, pos(k)); }
     };
     // One x-run of a row: cells [cx, cx + len) inside one wrap, hence
     // consecutive cell ids and one contiguous run of sorted slots.  The first
@@ -482,7 +492,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6, 8))) v
     const f3 tot = {fma_(dw.x, a.relax, 0.f), fma_(dw.y, a.relax, 0.f), fma_(dw.z, a.relax, 0.f)};
     const f3 dl = f3{0.f, 0.f, 0.f} + tot;
     a.delta[3 * t] = dl.x; a.delta[3 * t + 1] = dl.y; a.delta[3 * t + 2] = dl.z;
-    apply_delta(a.Ps, a.vals, t, x, rad, w1, dl, a.dt, a.vmax, a.Po, a.Vo);
+    apply_delta(xyz(a.Ps[a.vals[t]]), t, x, rad, w1, dl, a.dt, a.vmax, a.Po, a.Vo);
 }
 
 // The shape half of the delta apply: runs after both k_shape_solve and
@@ -504,7 +514,7 @@ __global__ __launch_bounds__(256) void k_shape_apply(ShapeApplyArgs a) {
         ds[0] = 0.0f; ds[1] = 0.0f; ds[2] = 0.0f;
         dp[0] = dl.x; dp[1] = dl.y; dp[2] = dl.z;
         const float4 Pi = a.Xs[pi], Vi = a.Vs[pi];
-        apply_delta(a.Ps, a.vals, pi, xyz(Pi), Pi.w, Vi.w, dl, a.dt, a.vmax, a.Po, a.Vo);
+        apply_delta(xyz(a.Ps[a.vals[pi]]), pi, xyz(Pi), Pi.w, Vi.w, dl, a.dt, a.vmax, a.Po, a.Vo);
     }
 }
 
