@@ -253,6 +253,17 @@ int xpbd_wrap_planes(const xpbd_solver* h, int* out);
  * Results are identical either way.  No reference counterpart (the reference
  * re-records its command buffer every frame, src/scene.cpp:805-992). */
 int xpbd_set_graph_mode(xpbd_solver* h, int on);
+/* Deterministic mode (no counterpart in the reference, whose float atomics
+ * make such sums depend on scheduling): every sum the reference takes with
+ * float atomics -- a particle's shape deltas and restitution updates, a
+ * body's deltas -- is taken in a canonical order instead (a particle's
+ * contacts by shape index; a box's particle contacts by particle id, then
+ * its corner contacts by corner), the order of a sequential pass over the
+ * contacts as the generators emit them.  Results then do not depend on
+ * scheduling.  Off by default (VKXPBD_DETERMINISTIC=1 at init turns it on);
+ * costs two radix sorts of the contact buffer per substep. */
+int xpbd_set_deterministic(xpbd_solver* h, int on);
+int xpbd_deterministic(const xpbd_solver* h, int* out);
 /* Times the candidate particle-particle block sizes over `frames` frames of
  * the current state, restores the state, keeps the fastest and returns it in
  * *block (may be NULL).  Results do not depend on the choice. */

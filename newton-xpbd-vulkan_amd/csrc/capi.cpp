@@ -245,6 +245,12 @@ int xpbd_wrap_planes(const xpbd_solver* h, int* out) {
     return 0;
 }
 int xpbd_set_graph_mode(xpbd_solver* h, int on) { return guarded(h, [&] { h->s.set_graph_mode(on != 0); }); }
+int xpbd_set_deterministic(xpbd_solver* h, int on) { return guarded(h, [&] { h->s.set_deterministic(on != 0); }); }
+int xpbd_deterministic(const xpbd_solver* h, int* out) {
+    if (!h || !out) { g_err = "null argument"; return 1; }
+    *out = h->s.deterministic() ? 1 : 0;
+    return 0;
+}
 int xpbd_autotune(xpbd_solver* h, int frames, float dt, int substeps, int iterations, int* block) {
     return guarded(h, [&] {
         const int b = h->s.autotune(frames, dt, substeps, iterations);

@@ -94,6 +94,17 @@ __global__ __launch_bounds__(64) void k_body_plane_contacts(BodyContactGenArgs a
         wc = tf_point(xyz(a.q[2 * b]), a.q[2 * b + 1], lc);
         want = wc.z - a.altitude < a.margin;
     }
+    if (a.fixed) {   // deterministic mode: slot 8k + c, in the order the corners are enumerated
+        if (t == 0) *a.rcc = a.rcmax;
+        if (k < a.nbox && t < a.rcmax) {
+            a.s0[t] = want ? a.shape0 + k : 0;
+            a.s1[t] = 0;
+            a.p0[t] = make_float4(lc.x, lc.y, lc.z, 0.f);
+            a.p1[t] = make_float4(wc.x, wc.y, a.altitude, 0.f);
+            a.n[t] = make_float4(0.f, 0.f, -1.f, 0.f);
+        }
+        return;
+    }
     const int slot = wave_reserve(a.rcc, want);
     if (want && slot < a.rcmax) {
         a.s0[slot] = a.shape0 + k;
@@ -181,6 +192,7 @@ __global__ __launch_bounds__(64) void k_body_contact_solve(BodySolveArgs a) {
     const int count = min(a.rcmax, *a.rcc);
     for (int ci = blockIdx.x * blockDim.x + threadIdx.x; ci < count; ci += gridDim.x * blockDim.x) {
         const int sa = a.s0[ci], sb = a.s1[ci];
+        if (a.rcd) a.rcd[2 * size_t(ci)] = make_float4(0.f, 0.f, 0.f, 0.f);   // no contribution unless stored
         if (sa == sb) continue;
         const int ba = sa >= 0 ? a.shape_body[sa] : -1, bb = sb >= 0 ? a.shape_body[sb] : -1;
         if (ba == bb) continue;
@@ -255,7 +267,12 @@ __global__ __launch_bounds__(64) void k_body_contact_solve(BodySolveArgs a) {
                 ada = ada - rn * lr; adb = adb + rn * lr;
             }
         }
-        if (ba >= 0) {
+        if (a.rcd) {   // deterministic mode: the corner contacts only move their box (body b is the plane)
+            if (ba >= 0) {
+                a.rcd[2 * size_t(ci)] = make_float4(lda.x, lda.y, lda.z, 1.f);
+                a.rcd[2 * size_t(ci) + 1] = make_float4(ada.x, ada.y, ada.z, 0.f);
+            }
+        } else if (ba >= 0) {
             float* d8 = a.delta + 8 * ba;
             atomicAdd(d8 + 0, lda.x); atomicAdd(d8 + 1, lda.y); atomicAdd(d8 + 2, lda.z);
             atomicAdd(d8 + 4, ada.x); atomicAdd(d8 + 5, ada.y); atomicAdd(d8 + 6, ada.z);

@@ -183,6 +183,10 @@ __global__ __launch_bounds__(256) void k_shape_solve(ShapeArgs a) {
         int s = a.cshape[t];
         int body = a.body.shape_body[s];
         int pf = a.flags[pi];
+        if (a.cfd) {   // deterministic mode: no contribution unless stored below
+            a.cfd[t] = make_float4(0.f, 0.f, 0.f, 0.f);
+            a.cbd[2 * size_t(t)] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
         if ((pf & 1) == 0) continue;
         if (pf & 2) {
             if (body < 0) continue;
@@ -230,6 +234,16 @@ __global__ __launch_bounds__(256) void k_shape_solve(ShapeArgs a) {
                    fma_(vt_dir.z, lambda_f, -delta_n.z)};
         f3 dtot = exact_div(diff, denom) * a.relax;
         f3 fd = w1 * dtot;
+        if (a.cfd) {
+            a.cfd[t] = make_float4(fd.x, fd.y, fd.z, 1.f);
+            if (body >= 0 && a.body.invm[body] > 0.0f) {
+                f3 dv = {dtot.x / a.dt, dtot.y / a.dt, dtot.z / a.dt};
+                f3 dw = cross3(r, dv);
+                a.cbd[2 * size_t(t)] = make_float4(-dv.x, -dv.y, -dv.z, 1.f);
+                a.cbd[2 * size_t(t) + 1] = make_float4(-dw.x, -dw.y, -dw.z, 0.f);
+            }
+            continue;
+        }
         atomicMax(&a.owner[pi], t);
         atomicAdd(&a.dshape[3 * pi + 0], fd.x);
         atomicAdd(&a.dshape[3 * pi + 1], fd.y);

@@ -26,6 +26,7 @@
 #include <stdexcept>
 
 #include "devbuf.h"
+#include "sort_entry.h"
 
 namespace xpbd {
 
@@ -69,6 +70,8 @@ void Solver::free_all() {
     invalidate_graphs();
     free_tape();
     dfree(scene_d_);
+    dfree(dk1_); dfree(dv1_); dfree(dk1s_); dfree(dv1s_); dfree(dk2_); dfree(dv2_); dfree(dk2s_); dfree(dv2s_);
+    dfree(dscr_); dfree(cfd_); dfree(cbd_); dfree(rcd_); dfree(dseg_);
     if (count_host_) { (void)hipHostFree(count_host_); count_host_ = nullptr; }
     if (fork_ev_) { (void)hipEventDestroy(fork_ev_); fork_ev_ = nullptr; }
     if (join_ev_) { (void)hipEventDestroy(join_ev_); join_ev_ = nullptr; }
@@ -139,6 +142,8 @@ void Solver::init_common(const xpbd_config& cfg, size_t n, size_t max_contacts) 
     if (profiling_) std::printf("[vkxpbd] GPU profiler enabled (HIP events)\n");
     const char* senv = std::getenv("VKXPBD_MESH_STATS");
     mesh_stats_on_ = senv && (std::strcmp(senv, "1") == 0 || std::strcmp(senv, "true") == 0);
+    const char* denv = std::getenv("VKXPBD_DETERMINISTIC");
+    det_ = denv && (std::strcmp(denv, "1") == 0 || std::strcmp(denv, "true") == 0);
     const char* genv = std::getenv("VKXPBD_GRAPHS");
     use_graphs_ = !(genv && std::strcmp(genv, "0") == 0);
     // wrap planes pay for their 128 MiB or 1 GiB table and wider radix keys
@@ -680,6 +685,19 @@ void Solver::run_sort(int tiles, float sdt) {
     lay_ ^= 1;   // flags and ids of the sorted order are current from here on
 }
 
+DetApplyArgs Solver::det_apply_args(const float4* Xin, const float4* Vin, int out, float sdt) {
+    DetApplyArgs d;
+    d.cmax = int(cmax_);
+    d.key1s = dk1s_.p; d.val1s = dv1s_.p;
+    d.pad1 = det_kb1_ >= 32 ? 0xFFFFFFFFu : (1u << det_kb1_) - 1u;
+    d.sbits = det_sbits_;
+    d.cfd = cfd_.p; d.delta = delta_.p;
+    d.Ps = P_[s_].p; d.vals = valsA_.p; d.Xs = Xin; d.Vs = Vin;
+    d.Po = P_[out].p; d.Vo = V_[out].p;
+    d.dt = sdt; d.vmax = cfg_.particle_v_max;
+    return d;
+}
+
 BodyStepArgs Solver::body_step_args(float sdt) {
     BodyStepArgs b;
     b.b0 = int(meshes_.size());
@@ -742,6 +760,7 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
             bg.margin = cfg_.soft_contact_margin; bg.altitude = cfg_.ground_plane_altitude;
             bg.rcmax = 8 * nbox; bg.rcc = rcc_solve;
             bg.s0 = rc_s0_.p; bg.s1 = rc_s1_.p; bg.p0 = rc_p0_.p; bg.p1 = rc_p1_.p; bg.n = rc_n_.p;
+            bg.fixed = det_ ? 1 : 0;
             launch_body_plane_contacts(bg, stream_);
         }
         launch_fill_zero(reinterpret_cast<int*>(rc_impulse_.p), rc_impulse_.n & ~size_t(3), stream_);
@@ -784,8 +803,32 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
         build_ranges(keysA_.p, true);
         stage_end(kRanges);
         grid_valid_ = true;
+        if (det_ && iterations > 0) {
+            // deterministic mode: every contact is in (mesh stage joined), then
+            // the canonical orders: by particle slot and shape, by box and id
+            if (fork) {
+                HIPCHECK(hipEventRecord(join_ev_, side_));
+                HIPCHECK(hipStreamWaitEvent(stream_, join_ev_, 0));
+            }
+            const uint32_t pad1 = det_kb1_ >= 32 ? 0xFFFFFFFFu : (1u << det_kb1_) - 1u;
+            const uint32_t pad2 = det_kb2_ >= 32 ? 0xFFFFFFFFu : (1u << det_kb2_) - 1u;
+            DetKeyArgs ka;
+            ka.cmax = int(cmax_); ka.cc = cc_solve;
+            ka.cpart = cpart_.p; ka.cshape = cshape_.p; ka.remap = rank_.p; ka.ids = ids_cur();
+            ka.sbits = det_sbits_; ka.idbits = det_idbits_; ka.box_shape0 = M + 1; ka.nbox = nbox;
+            ka.pad1 = pad1; ka.pad2 = pad2;
+            ka.key1 = dk1_.p; ka.val1 = dv1_.p; ka.key2 = nbox ? dk2_.p : nullptr; ka.val2 = nbox ? dv2_.p : nullptr;
+            launch_det_keys(ka, stream_);
+            sort_run(dk1_.p, dv1_.p, dk1s_.p, dv1s_.p, cmax_, det_kb1_, dscr_.p, stream_);
+            if (nbox > 0) {
+                sort_run(dk2_.p, dv2_.p, dk2s_.p, dv2s_.p, cmax_, det_kb2_, dscr_.p, stream_);
+                launch_fill_zero(dseg_.p, dseg_.n, stream_);
+                launch_det_box_ranges(int(cmax_), dk2s_.p, pad2, det_idbits_, dseg_.p, stream_);
+            }
+        }
         // contacts were generated in the previous slot order: the first shape
-        // solve maps them through rank; with no iteration a kernel does
+        // solve maps them through rank (the key kernel in deterministic mode);
+        // with no iteration a kernel does
         if (iterations == 0) {   // no solve: the predicted state still moves to the sorted order
             if (fork) {
                 HIPCHECK(hipEventRecord(join_ev_, side_));
@@ -803,7 +846,9 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
         const float4* Vin = it == 0 ? Vs_[0].p : V_[cur].p;
         ShapeArgs sa;
         sa.cmax = int(cmax_); sa.cc = cc_solve;
-        sa.remap = (it == 0 && n > 0) ? rank_.p : nullptr;
+        sa.remap = (it == 0 && n > 0 && !det_) ? rank_.p : nullptr;
+        sa.cfd = det_ ? cfd_.p : nullptr;
+        sa.cbd = det_ ? cbd_.p : nullptr;
         sa.cpart = cpart_.p; sa.cshape = cshape_.p; sa.cbpos = cbpos_.p; sa.cbvel = cbvel_.p; sa.cnorm = cnorm_.p;
         sa.P = Xin; sa.V = Vin; sa.flags = flags_cur();
         sa.body = body_args();
@@ -812,7 +857,7 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
         // the first shape solve waits for the mesh contacts, on the side
         // stream, while the particle-particle solve (which no longer reads
         // the shape deltas) runs; k_shape_apply joins the two
-        const bool side = fork && it == 0;
+        const bool side = fork && it == 0 && !det_;
         if (side) HIPCHECK(hipStreamWaitEvent(side_, sorted_ev_, 0));
         stage_begin(kShape);
         launch_shape_solve(sa, shape_grid, side ? side_ : stream_);
@@ -832,14 +877,19 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
         launch_pp_apply(pa, cfg_.enable_diagnostics != 0, num_scenes_ > 1, pp_block_, stream_);
         stage_end(kPP);
         if (side) HIPCHECK(hipStreamWaitEvent(stream_, join_ev_, 0));
-        ShapeApplyArgs aa;
-        aa.cmax = int(cmax_); aa.cc = cc_solve; aa.cpart = cpart_.p;
-        aa.owner = owner_.p; aa.dshape = dshape_.p; aa.delta = delta_.p;
-        aa.Ps = P_[s_].p; aa.vals = valsA_.p; aa.Xs = Xin; aa.Vs = Vin;
-        aa.Po = P_[nxt].p; aa.Vo = V_[nxt].p;
-        aa.dt = sdt; aa.vmax = cfg_.particle_v_max;
         stage_begin(kShapeApply);
-        launch_shape_apply(aa, shape_grid, stream_);
+        if (det_) {
+            DetApplyArgs da = det_apply_args(Xin, Vin, nxt, sdt);
+            launch_det_shape_apply(da, stream_);
+        } else {
+            ShapeApplyArgs aa;
+            aa.cmax = int(cmax_); aa.cc = cc_solve; aa.cpart = cpart_.p;
+            aa.owner = owner_.p; aa.dshape = dshape_.p; aa.delta = delta_.p;
+            aa.Ps = P_[s_].p; aa.vals = valsA_.p; aa.Xs = Xin; aa.Vs = Vin;
+            aa.Po = P_[nxt].p; aa.Vo = V_[nxt].p;
+            aa.dt = sdt; aa.vmax = cfg_.particle_v_max;
+            launch_shape_apply(aa, shape_grid, stream_);
+        }
         stage_end(kShapeApply);
         std::swap(cur, nxt);
         if (nbox > 0) {
@@ -851,8 +901,17 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
             bv.invm = body_invm_.p; bv.flags = body_flags_.p;
             bv.delta = body_delta_.p; bv.invw = body_invw_.p; bv.impulse = rc_impulse_.p;
             bv.relax = rigid_relax_; bv.dt = sdt;
+            bv.rcd = det_ ? rcd_.p : nullptr;
             stage_begin(kBodySolve);
             launch_body_contact_solve(bv, stream_);
+            if (det_) {
+                DetBodyArgs db;
+                db.nbox = nbox; db.b0 = M;
+                db.seg = dseg_.p; db.val2s = dv2s_.p; db.cbd = cbd_.p;
+                db.rcd = rcd_.p;
+                db.delta = body_delta_.p;
+                launch_det_body_reduce(db, stream_);
+            }
             stage_end(kBodySolve);
             stage_begin(kBodyApply);
             launch_apply_body_deltas(bs, stream_);
@@ -867,7 +926,8 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
         ra.body = body_args();
         ra.ka = cfg_.particle_adhesion; ra.restitution = cfg_.soft_contact_restitution;
         stage_begin(kRest);
-        launch_restitution(ra, shape_grid, stream_);
+        if (det_ && n > 0 && iterations > 0) launch_det_restitution(ra, det_apply_args(nullptr, nullptr, cur, sdt), stream_);
+        else launch_restitution(ra, shape_grid, stream_);
         stage_end(kRest);
     }
     const int old_s = s_;
@@ -1034,9 +1094,47 @@ int Solver::autotune(int frames, float dt, int substeps, int iterations) {
     return best;
 }
 
+// Bits of a count: the smallest b with v < 2^b (at least 1).
+static int bits_for(size_t v) {
+    int b = 1;
+    while (b < 63 && (size_t(1) << b) <= v) b++;
+    return b;
+}
+
+void Solver::ensure_det() {
+    if (!det_) return;
+    const size_t nshape = meshes_.size() + 1 + boxes_.size(), nbox = boxes_.size();
+    // keys stay below the padding key (all ones): shape < 2^sbits - 1 and
+    // box < 2^bbits - 1 leave the top value free
+    const int sbits = bits_for(nshape), idbits = bits_for(n_ > 0 ? n_ - 1 : 0), bbits = bits_for(nbox);
+    const int kb1 = bits_for(n_ > 0 ? n_ - 1 : 0) + sbits, kb2 = nbox ? bbits + idbits : 0;
+    if (kb1 > 32 || kb2 > 32)
+        throw std::runtime_error("deterministic mode: particles x shapes exceed the 32-bit contact keys");
+    const bool same = det_sbits_ == sbits && det_idbits_ == idbits && det_bbits_ == bbits && dk1_.n == cmax_ &&
+                      (nbox ? rcd_.n == 2 * 8 * nbox : rcd_.n == 0) && dscr_.p;
+    if (same) return;
+    DeviceGuard g(device_);
+    HIPCHECK(hipStreamSynchronize(stream_));
+    invalidate_graphs();
+    det_sbits_ = sbits; det_idbits_ = idbits; det_bbits_ = bbits; det_kb1_ = kb1; det_kb2_ = kb2;
+    dalloc(dk1_, cmax_); dalloc(dv1_, cmax_); dalloc(dk1s_, cmax_); dalloc(dv1s_, cmax_);
+    dalloc(cfd_, cmax_); dalloc(cbd_, 2 * cmax_);
+    if (nbox) {
+        dalloc(dk2_, cmax_); dalloc(dv2_, cmax_); dalloc(dk2s_, cmax_); dalloc(dv2s_, cmax_);
+        dalloc(rcd_, 2 * 8 * nbox);
+        dalloc(dseg_, (2 * nbox + 3) & ~size_t(3));   // cleared in 16-byte steps
+    } else {
+        dfree(dk2_); dfree(dv2_); dfree(dk2s_); dfree(dv2s_); dfree(rcd_); dfree(dseg_);
+    }
+    // the layout depends on the digit width, which is not monotonic in the
+    // key width: room for the larger of the two sorts
+    dalloc(dscr_, std::max(sort_scratch_bytes(cmax_, kb1), nbox ? sort_scratch_bytes(cmax_, kb2) : size_t(0)));
+}
+
 void Solver::step(float dt, int substeps, int iterations) {
     require_init();
     if (!finalized_) finalize_meshes();
+    ensure_det();
     DeviceGuard g(device_);
     const auto t0 = std::chrono::high_resolution_clock::now();
     const float sdt = dt / float(substeps);
@@ -1069,6 +1167,7 @@ void Solver::step(float dt, int substeps, int iterations) {
 void Solver::step_solve_only(float dt, int substeps, int iterations) {
     require_init();
     if (!finalized_) finalize_meshes();
+    ensure_det();
     DeviceGuard g(device_);
     const float sdt = dt / float(substeps);
     for (int sub = 0; sub < substeps; sub++) substep(sdt, iterations, false, contact_slot_, false);

@@ -35,6 +35,11 @@ public:
     // hipGraph capture of whole frames in step() (default on; VKXPBD_GRAPHS=0 disables)
     void set_graph_mode(bool on) { use_graphs_ = on; invalidate_graphs(); }
     void set_wrap_planes(bool on) { planes_allowed_ = on; update_planes(); }
+    // Deterministic mode: every float sum the reference takes with atomics
+    // (shape deltas, body deltas, restitution) in a canonical order instead
+    // (VKXPBD_DETERMINISTIC=1 at init also turns it on).
+    void set_deterministic(bool on) { det_ = on; invalidate_graphs(); }
+    bool deterministic() const { return det_; }
     // Times candidate launch shapes on the current state (then restores it) and
     // keeps the fastest; returns the chosen particle-particle block size.
     int autotune(int frames, float dt, int substeps, int iterations);
@@ -126,6 +131,7 @@ private:
     BodyArgs body_args() const;
     void substep(float sdt, int iterations, bool collide, int slot, bool zero_next);
     BodyStepArgs body_step_args(float sdt);
+    DetApplyArgs det_apply_args(const float4* Xin, const float4* Vin, int out, float sdt);
     void run_sort(int tiles, float sdt);   // the last pass also integrates the state over sdt
     void update_planes();
     std::vector<uint64_t> reference_sorted();
@@ -133,6 +139,7 @@ private:
     // zero (k_integrate clears them when wrap planes are on)
     void build_ranges(const uint32_t* sorted_keys, bool cleared = false);
     void read_count(int slot);
+    void ensure_det();   // deterministic mode's buffers (outside any graph capture)
     void stage_begin(Stage s);
     void stage_end(Stage s);
     void sync_profile();
@@ -195,6 +202,15 @@ private:
         DevBuf<double> relax_part;
     } tape_;
     bool use_graphs_ = true;
+    // deterministic mode: canonical contact keys (by particle slot and shape;
+    // by box and particle id), their sorted copies, the sort's scratch,
+    // per-contact deltas and each box's range of the box-key order
+    bool det_ = false;
+    int det_sbits_ = 0, det_idbits_ = 0, det_bbits_ = 0, det_kb1_ = 0, det_kb2_ = 0;
+    DevBuf<uint32_t> dk1_, dv1_, dk1s_, dv1s_, dk2_, dv2_, dk2s_, dv2s_;
+    DevBuf<char> dscr_;
+    DevBuf<float4> cfd_, cbd_, rcd_;
+    DevBuf<int32_t> dseg_;
     // particle state: 3 rotating float4 buffers each (state / two iteration buffers)
     DevBuf<float4> P_[3], V_[3];
     int s_ = 0, p_ = 1, o_ = 2;

@@ -471,6 +471,17 @@ PYBIND11_MODULE(_vkxpbd, m) {
             "set_graph_mode", [](PySolver& s, bool on) { s.check(xpbd_set_graph_mode(s.h, on ? 1 : 0)); },
             py::arg("enabled"), "Capture and replay whole frames as hipGraphs (default on).")
         .def(
+            "set_deterministic", [](PySolver& s, bool on) { s.check(xpbd_set_deterministic(s.h, on ? 1 : 0)); },
+            py::arg("enabled"),
+            "Take every sum the reference takes with float atomics (shape deltas, body deltas, restitution) in a "
+            "canonical order, so results do not depend on scheduling (default off).")
+        .def_property_readonly("deterministic",
+                               [](const PySolver& s) {
+                                   int v = 0;
+                                   xpbd_deterministic(s.h, &v);
+                                   return v != 0;
+                               })
+        .def(
             "autotune",
             [](PySolver& s, int frames, float dt, int substeps, int iterations) {
                 int b = 0, rc;

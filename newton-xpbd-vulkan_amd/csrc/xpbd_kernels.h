@@ -94,6 +94,10 @@ struct ShapeArgs {
     float pmu, ka, dt, relax;
     float* dshape;
     int* owner;                 // per sorted slot: largest contact index that added to dshape, else -1
+    // deterministic mode (non-null): each contact's particle delta (w = 1
+    // when it contributes) and body delta (2 float4: -dv with w flag, -dw)
+    // are stored per contact instead of added with atomics
+    float4* cfd; float4* cbd;
 };
 
 // Adds the shape deltas to the particles that received any, after k_pp_apply
@@ -196,6 +200,33 @@ struct BodyContactGenArgs {
     float margin, altitude;
     int rcmax; int* rcc;
     int* s0; int* s1; float4* p0; float4* p1; float4* n;
+    int fixed;   // deterministic mode: corner c of box k at slot 8k + c (inactive slots: s0 = s1 = 0)
+};
+
+// Deterministic mode (k_det.hip): canonical contact keys and the sums over them.
+struct DetKeyArgs {
+    int cmax; const int* cc;
+    int* cpart; const int* cshape; const uint32_t* remap; const uint32_t* ids;
+    int sbits, idbits, box_shape0, nbox;
+    uint32_t pad1, pad2;
+    uint32_t* key1; uint32_t* val1; uint32_t* key2; uint32_t* val2;   // key2 null without boxes
+};
+struct DetApplyArgs {
+    int cmax;
+    const uint32_t* key1s; const uint32_t* val1s;   // key1 order
+    uint32_t pad1; int sbits;
+    const float4* cfd;
+    float* delta;
+    const float4* Ps; const uint32_t* vals;
+    const float4* Xs; const float4* Vs;
+    float4* Po; float4* Vo;
+    float dt, vmax;
+};
+struct DetBodyArgs {
+    int nbox, b0;
+    const int* seg; const uint32_t* val2s; const float4* cbd;   // seg null: no particle contacts
+    const float4* rcd;                                           // null: no corner contacts
+    float* delta;
 };
 
 struct ParticleBoxArgs {
@@ -214,6 +245,7 @@ struct BodySolveArgs {
     const float4* q; const float4* qd; const float* com; const float* invI; const float* invm; const int* flags;
     float* delta; float* invw; float* impulse;
     float relax, dt;
+    float4* rcd;   // deterministic mode: body a's (linear with w flag, angular) delta per contact, no atomics
 };
 
 // Reverse pass of one substep (differentiable rollout).
@@ -269,6 +301,11 @@ void launch_shape_apply(const ShapeApplyArgs& a, int grid, hipStream_t s);
 // only if keys[start] equals the looked-up key.  block: 64, 128 or 256.
 void launch_pp_apply(const PPArgs& a, bool diag, bool validate, int block, hipStream_t s);
 void launch_restitution(const RestArgs& a, int grid, hipStream_t s);
+void launch_det_keys(const DetKeyArgs& a, hipStream_t s);
+void launch_det_box_ranges(int cmax, const uint32_t* key2s, uint32_t pad, int idbits, int* seg, hipStream_t s);
+void launch_det_shape_apply(const DetApplyArgs& a, hipStream_t s);
+void launch_det_body_reduce(const DetBodyArgs& a, hipStream_t s);
+void launch_det_restitution(const RestArgs& r, const DetApplyArgs& a, hipStream_t s);
 void launch_zero_int(int* p, int count, hipStream_t s);
 void launch_frame_begin(int* cc, uint32_t* mcnt, int nmcnt, hipStream_t s);
 // watertight[ci] != 0: four lanes per walk (split subtrees) for mesh ci
