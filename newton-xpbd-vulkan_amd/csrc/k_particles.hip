@@ -623,6 +623,33 @@ __global__ __launch_bounds__(256) void k_scatter_f3(float* __restrict__ dst, con
     const float4 v = src[t];
     dst[3 * i] = v.x; dst[3 * i + 1] = v.y; dst[3 * i + 2] = v.z;
 }
+// dst[3 ids[t] + c] = src[3 t + c]: slot-order float triples -> particle order.
+__global__ __launch_bounds__(256) void k_scatter3(float* __restrict__ dst, const float* __restrict__ src,
+                                                  const uint32_t* __restrict__ ids, int n) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const uint32_t i = ids[t];
+    dst[3 * size_t(i)] = src[3 * size_t(t)];
+    dst[3 * size_t(i) + 1] = src[3 * size_t(t) + 1];
+    dst[3 * size_t(i) + 2] = src[3 * size_t(t) + 2];
+}
+// dst[k] = ids[src[k]] for the contact list's slots (slots outside [0, n) kept).
+__global__ __launch_bounds__(256) void k_slots_to_ids(int* __restrict__ dst, const int* __restrict__ src,
+                                                      const uint32_t* __restrict__ ids, int count, int n) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    const int s = src[k];
+    dst[k] = (s >= 0 && s < n) ? int(ids[s]) : s;
+}
+// dst[3 k + c] = src[k].xyz: float4 rows -> packed float triples.
+__global__ __launch_bounds__(256) void k_pack3(float* __restrict__ dst, const float4* __restrict__ src, int count) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    const float4 v = src[k];
+    dst[3 * size_t(k)] = v.x;
+    dst[3 * size_t(k) + 1] = v.y;
+    dst[3 * size_t(k) + 2] = v.z;
+}
 __global__ __launch_bounds__(256) void k_iota(uint32_t* __restrict__ p, int n) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < n) p[t] = uint32_t(t);
@@ -641,6 +668,15 @@ void launch_remap_contacts(int* cpart, const int* cc, int cmax, const uint32_t* 
 }
 void launch_scatter_f3(float* dst, const float4* src, const uint32_t* ids, int n, hipStream_t s) {
     if (n > 0) hipLaunchKernelGGL(k_scatter_f3, dim3(blocks_for(n, 256)), dim3(256), 0, s, dst, src, ids, n);
+}
+void launch_scatter3(float* dst, const float* src, const uint32_t* ids, int n, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_scatter3, dim3(blocks_for(n, 256)), dim3(256), 0, s, dst, src, ids, n);
+}
+void launch_slots_to_ids(int* dst, const int* src, const uint32_t* ids, int count, int n, hipStream_t s) {
+    if (count > 0) hipLaunchKernelGGL(k_slots_to_ids, dim3(blocks_for(count, 256)), dim3(256), 0, s, dst, src, ids, count, n);
+}
+void launch_pack3(float* dst, const float4* src, int count, hipStream_t s) {
+    if (count > 0) hipLaunchKernelGGL(k_pack3, dim3(blocks_for(count, 256)), dim3(256), 0, s, dst, src, count);
 }
 void launch_iota(uint32_t* p, int n, hipStream_t s) {
     if (n > 0) hipLaunchKernelGGL(k_iota, dim3(blocks_for(n, 256)), dim3(256), 0, s, p, n);

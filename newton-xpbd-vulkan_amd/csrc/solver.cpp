@@ -73,6 +73,9 @@ void Solver::free_all() {
     dfree(dk1_); dfree(dv1_); dfree(dk1s_); dfree(dv1s_); dfree(dk2_); dfree(dv2_); dfree(dk2s_); dfree(dv2s_);
     dfree(dscr_); dfree(cfd_); dfree(cbd_); dfree(rcd_); dfree(dseg_);
     if (count_host_) { (void)hipHostFree(count_host_); count_host_ = nullptr; }
+    dfree(io_dev_);
+    if (io_host_) { (void)hipHostFree(io_host_); io_host_ = nullptr; }
+    io_cap_ = 0;
     if (fork_ev_) { (void)hipEventDestroy(fork_ev_); fork_ev_ = nullptr; }
     if (join_ev_) { (void)hipEventDestroy(join_ev_); join_ev_ = nullptr; }
     if (sorted_ev_) { (void)hipEventDestroy(sorted_ev_); sorted_ev_ = nullptr; }
@@ -1224,51 +1227,67 @@ std::vector<uint32_t> Solver::host_ids() {
     HIPCHECK(hipStreamSynchronize(stream_));
     return ids;
 }
-// Particle rows of a slot-order float4 buffer (xyz), in particle-id order.
-static void rows_by_id(const std::vector<float4>& t, const std::vector<uint32_t>& ids, float* out, size_t n) {
-    for (size_t s = 0; s < t.size(); s++) {
-        const size_t i = ids[s];
-        if (i >= n) continue;
-        out[3 * i] = t[s].x; out[3 * i + 1] = t[s].y; out[3 * i + 2] = t[s].z;
+void* Solver::stage_out(size_t words) {
+    if (words > io_cap_) {
+        dfree(io_dev_);
+        if (io_host_) { HIPCHECK(hipHostFree(io_host_)); io_host_ = nullptr; }
+        io_cap_ = 0;
+        dalloc(io_dev_, words);
+        HIPCHECK(hipHostMalloc(reinterpret_cast<void**>(&io_host_), words * sizeof(float), hipHostMallocDefault));
+        io_cap_ = words;
     }
+    return io_dev_.p;
 }
+void Solver::copy_out(void* out, size_t bytes) {
+    if (bytes) HIPCHECK(hipMemcpyAsync(io_host_, io_dev_.p, bytes, hipMemcpyDeviceToHost, stream_));
+    HIPCHECK(hipStreamSynchronize(stream_));
+    if (bytes) std::memcpy(out, io_host_, bytes);
+}
+// The state lives in slot order: the getters scatter it into particle order
+// on the device (scene.cpp:1247-1267 reads it in particle order), then one
+// DMA brings it to pinned host memory (the reference keeps host-cached
+// staging for the same reason, buffer.h:164-219).
 void Solver::get_positions(float* out, size_t n) {
     require_init();
     DeviceGuard g(device_);
     n = std::min(n, n_);
-    std::vector<float4> t(n_);
-    if (n_) HIPCHECK(hipMemcpyAsync(t.data(), P_[s_].p, n_ * sizeof(float4), hipMemcpyDeviceToHost, stream_));
-    rows_by_id(t, host_ids(), out, n);
+    if (!n_) return;
+    float* d = static_cast<float*>(stage_out(3 * n_));
+    launch_scatter_f3(d, P_[s_].p, ids_cur(), int(n_), stream_);
+    HIPCHECK(hipGetLastError());
+    copy_out(out, 3 * n * sizeof(float));
 }
 void Solver::get_velocities(float* out, size_t n) {
     require_init();
     DeviceGuard g(device_);
     n = std::min(n, n_);
-    std::vector<float4> t(n_);
-    if (n_) HIPCHECK(hipMemcpyAsync(t.data(), V_[s_].p, n_ * sizeof(float4), hipMemcpyDeviceToHost, stream_));
-    rows_by_id(t, host_ids(), out, n);
+    if (!n_) return;
+    float* d = static_cast<float*>(stage_out(3 * n_));
+    launch_scatter_f3(d, V_[s_].p, ids_cur(), int(n_), stream_);
+    HIPCHECK(hipGetLastError());
+    copy_out(out, 3 * n * sizeof(float));
 }
 void Solver::get_contact_ints(bool shapes, int32_t* out, size_t cap) {
     require_init();
     DeviceGuard g(device_);
     const size_t c = std::min<size_t>(size_t(std::max(last_count_, 0)), cap);
-    if (c) HIPCHECK(hipMemcpyAsync(out, shapes ? cshape_.p : cpart_.p, c * sizeof(int32_t), hipMemcpyDeviceToHost, stream_));
-    HIPCHECK(hipStreamSynchronize(stream_));
-    if (!shapes && c) {   // contacts name slots of the current order
-        const std::vector<uint32_t> ids = host_ids();
-        for (size_t k = 0; k < c; k++)
-            if (out[k] >= 0 && size_t(out[k]) < n_) out[k] = int32_t(ids[size_t(out[k])]);
-    }
+    if (!c) return;
+    int* d = static_cast<int*>(stage_out(c));
+    if (shapes) HIPCHECK(hipMemcpyAsync(d, cshape_.p, c * sizeof(int32_t), hipMemcpyDeviceToDevice, stream_));
+    else launch_slots_to_ids(d, cpart_.p, ids_cur(), int(c), int(n_), stream_);   // contacts name slots
+    HIPCHECK(hipGetLastError());
+    copy_out(out, c * sizeof(int32_t));
 }
 void Solver::get_contact_vec(int which, float* out, size_t cap) {
     require_init();
     DeviceGuard g(device_);
     const size_t c = std::min<size_t>(size_t(std::max(last_count_, 0)), cap);
-    std::vector<float4> t(c);
+    if (!c) return;
     const float4* src = which == 0 ? cnorm_.p : (which == 1 ? cbpos_.p : cbvel_.p);
-    if (c) HIPCHECK(hipMemcpyAsync(t.data(), src, c * sizeof(float4), hipMemcpyDeviceToHost, stream_));
-    HIPCHECK(hipStreamSynchronize(stream_));
-    for (size_t i = 0; i < c; i++) { out[3 * i] = t[i].x; out[3 * i + 1] = t[i].y; out[3 * i + 2] = t[i].z; }
+    float* d = static_cast<float*>(stage_out(3 * c));
+    launch_pack3(d, src, int(c), stream_);
+    HIPCHECK(hipGetLastError());
+    copy_out(out, 3 * c * sizeof(float));
 }
 void Solver::set_contact_data(const int32_t* p, const int32_t* s, const float* bp, const float* nrm, int count) {
     require_init();
@@ -1360,13 +1379,11 @@ void Solver::get_deltas(float* out, size_t n) {
     require_init();
     DeviceGuard g(device_);
     n = std::min(n, n_);
-    std::vector<float> d(3 * n_);
-    if (n_) HIPCHECK(hipMemcpyAsync(d.data(), delta_.p, 3 * n_ * sizeof(float), hipMemcpyDeviceToHost, stream_));
-    const std::vector<uint32_t> ids = host_ids();   // deltas are in sorted (current) order
-    for (size_t t = 0; t < n_; t++) {
-        const size_t i = ids[t];
-        if (i < n) { out[3 * i] = d[3 * t]; out[3 * i + 1] = d[3 * t + 1]; out[3 * i + 2] = d[3 * t + 2]; }
-    }
+    if (!n_) return;
+    float* d = static_cast<float*>(stage_out(3 * n_));
+    launch_scatter3(d, delta_.p, ids_cur(), int(n_), stream_);   // deltas are in sorted (current) order
+    HIPCHECK(hipGetLastError());
+    copy_out(out, 3 * n * sizeof(float));
 }
 uint32_t Solver::diag_word(int k) {
     if (!initialized_) return 0;

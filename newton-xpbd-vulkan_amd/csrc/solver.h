@@ -226,6 +226,13 @@ private:
     int32_t* flags_cur() { return flags_[lay_].p; }
     uint32_t* ids_cur() { return ids_[lay_].p; }
     std::vector<uint32_t> host_ids();
+    // host I/O staging (persistent): a device buffer the getters fill in
+    // particle order and a pinned host buffer it is copied to in one DMA
+    DevBuf<float> io_dev_;
+    float* io_host_ = nullptr;
+    size_t io_cap_ = 0;
+    void* stage_out(size_t words);   // ensures both hold `words` 4-byte words; returns io_dev_
+    void copy_out(void* out, size_t bytes);   // io_dev_ -> pinned -> out
     std::vector<float> radius_h_, invm_h_;
     float rmax_ = 0.f;   // largest particle radius (host copy of the radii)
     // grid and sort

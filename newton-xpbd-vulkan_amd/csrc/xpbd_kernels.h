@@ -292,6 +292,9 @@ void launch_cell_ranges(int n, const uint32_t* keys, int* range, uint32_t* rkeys
 void launch_clear_ranges(int n, const uint32_t* rkeys, int* range, hipStream_t s);
 void launch_remap_contacts(int* cpart, const int* cc, int cmax, const uint32_t* map, int grid, hipStream_t s);
 void launch_scatter_f3(float* dst, const float4* src, const uint32_t* ids, int n, hipStream_t s);
+void launch_scatter3(float* dst, const float* src, const uint32_t* ids, int n, hipStream_t s);
+void launch_slots_to_ids(int* dst, const int* src, const uint32_t* ids, int count, int n, hipStream_t s);
+void launch_pack3(float* dst, const float4* src, int count, hipStream_t s);
 void launch_iota(uint32_t* p, int n, hipStream_t s);
 void launch_gather_f3(float4* dst, const float* src, const uint32_t* ids, int n, hipStream_t s);
 void launch_cell_tables(int n, const uint32_t* keys, int* starts, int* ends, hipStream_t s);
