@@ -264,10 +264,27 @@ int xpbd_set_graph_mode(xpbd_solver* h, int on);
  * costs two radix sorts of the contact buffer per substep. */
 int xpbd_set_deterministic(xpbd_solver* h, int on);
 int xpbd_deterministic(const xpbd_solver* h, int* out);
-/* Times the candidate particle-particle block sizes over `frames` frames of
- * the current state, restores the state, keeps the fastest and returns it in
- * *block (may be NULL).  Results do not depend on the choice. */
+/* Times the candidate launch shapes -- particle-particle block size (64,
+ * 128, 256) x radix tile (1,024 or 2,048 keys per workgroup of the sort that
+ * builds the cell tables) -- over `frames` frames of the current state,
+ * restores the state, keeps the fastest and returns its block size in *block
+ * (may be NULL).  Results do not depend on the choice. */
 int xpbd_autotune(xpbd_solver* h, int frames, float dt, int substeps, int iterations, int* block);
+/* Keys per workgroup of the radix sort currently in use (1,024 or 2,048). */
+int xpbd_radix_tile(const xpbd_solver* h, int* out);
+/* Particle-particle block size currently in use (64, 128 or 256). */
+int xpbd_pp_block(const xpbd_solver* h, int* out);
+/* Sets the launch shapes xpbd_autotune picks -- pp_block 64, 128 or 256,
+ * radix_tile 1,024 or 2,048 -- e.g. to reuse one tuning for solvers of the
+ * same workload (no counterpart in the reference).  Results do not depend
+ * on the choice. */
+int xpbd_set_tuning(xpbd_solver* h, int pp_block, int radix_tile);
+/* Device bytes held by the cell-range tables: the dense 2^21-row table (2^24
+ * or 2^27 rows with wrap planes) of a plain solver, or the row table of a
+ * batched solver (8 bytes per grid row of each scene, 2^14 rows per scene,
+ * plus 8 lower bounds per occupied group of 8 cells, at most 36 bytes per
+ * particle). */
+int xpbd_cell_table_bytes(const xpbd_solver* h, size_t* out);
 
 /* Host-only LBVH build of one mesh, the builder finalize_meshes uses
  * (MeshBvh::build, src/mesh_bvh.h:40-69 -> src/mesh_lbvh.h:49-223).  Needs no

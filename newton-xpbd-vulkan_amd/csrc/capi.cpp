@@ -245,6 +245,24 @@ int xpbd_wrap_planes(const xpbd_solver* h, int* out) {
     return 0;
 }
 int xpbd_set_graph_mode(xpbd_solver* h, int on) { return guarded(h, [&] { h->s.set_graph_mode(on != 0); }); }
+int xpbd_radix_tile(const xpbd_solver* h, int* out) {
+    if (!h || !out) { g_err = "null argument"; return 1; }
+    *out = h->s.radix_tile();
+    return 0;
+}
+int xpbd_pp_block(const xpbd_solver* h, int* out) {
+    if (!h || !out) { g_err = "null argument"; return 1; }
+    *out = h->s.pp_block();
+    return 0;
+}
+int xpbd_set_tuning(xpbd_solver* h, int pp_block, int radix_tile) {
+    return guarded(h, [&] { h->s.set_tuning(pp_block, radix_tile); });
+}
+int xpbd_cell_table_bytes(const xpbd_solver* h, size_t* out) {
+    if (!h || !out) { g_err = "null argument"; return 1; }
+    *out = h->s.cell_table_bytes();
+    return 0;
+}
 int xpbd_set_deterministic(xpbd_solver* h, int on) { return guarded(h, [&] { h->s.set_deterministic(on != 0); }); }
 int xpbd_deterministic(const xpbd_solver* h, int* out) {
     if (!h || !out) { g_err = "null argument"; return 1; }

@@ -17,13 +17,21 @@
 
 namespace xpbd {
 
+// Fills every byte of b's allocation and waits for the fill: hipMemset runs on
+// the null stream, which the solvers' non-blocking streams do not wait for,
+// so work queued on them next could otherwise run before the fill lands (or
+// be overwritten by it).
+template <class T> inline void dfill(DevBuf<T>& b, int byte) {
+    HIPCHECK(hipMemset(b.p, byte, (b.n ? b.n : 1) * sizeof(T)));
+    HIPCHECK(hipDeviceSynchronize());
+}
 // (Re)allocates b to n elements, zero filled (one element for n = 0).
 template <class T> inline void dalloc(DevBuf<T>& b, size_t n) {
     if (b.p) { (void)hipFree(b.p); b.p = nullptr; b.n = 0; }
     size_t m = n ? n : 1;
     HIPCHECK(hipMalloc(reinterpret_cast<void**>(&b.p), m * sizeof(T)));
-    HIPCHECK(hipMemset(b.p, 0, m * sizeof(T)));
     b.n = n;
+    dfill(b, 0);
 }
 template <class T> inline void dfree(DevBuf<T>& b) {
     if (b.p) (void)hipFree(b.p);

@@ -166,7 +166,7 @@ __global__ __launch_bounds__(256) void k_adj_shape(AdjShapeArgs a) {
 // (integrate_particles.comp:44-57); writes the substep-start cotangents.
 constexpr int kSurvAdj = 16;   // recorded candidates per lane before a drain
 
-template <bool VALIDATE>
+template <bool BATCHED>
 __global__ __launch_bounds__(256) void k_adj_pp(AdjArgs a) {
     __shared__ uint32_t s_surv[kSurvAdj][256];
     const int t = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
@@ -189,7 +189,7 @@ __global__ __launch_bounds__(256) void k_adj_pp(AdjArgs a) {
             const int ye = min((int)floorf((x.y + R) * inv), ys + kGridDim - 1);
             const int ze = min((int)floorf((x.z + R) * inv), zs + kGridDim - 1);
             const int2* range = reinterpret_cast<const int2*>(a.range);
-            const uint32_t base = VALIDATE ? (a.keys[t] & ~((1u << kCellBits) - 1u)) : 0u;
+            const uint32_t base = BATCHED ? (a.keys[t] & ~((1u << kCellBits) - 1u)) : 0u;
             f3 gx = {0.f, 0.f, 0.f}, gv = {0.f, 0.f, 0.f};
             float rsum = 0.0f;
             // the forward's cell skipping (every active pair, from either side,
@@ -245,10 +245,9 @@ __global__ __launch_bounds__(256) void k_adj_pp(AdjArgs a) {
                 for (int cy = ys; cy <= ye; cy++)
                     for (int cx = xs; cx <= xe; cx++) {
                         if (gap2(ux, cx) + gap2(uy, cy) + gap2(uz, cz) > lu2) continue;
-                        const uint32_t tb = VALIDATE ? base : (a.planes ? wrap_plane(cx, cy, cz, a.planes) << kCellBits : 0u);
+                        const uint32_t tb = BATCHED ? base : (a.planes ? wrap_plane(cx, cy, cz, a.planes) << kCellBits : 0u);
                         const uint32_t cell = tb + uint32_t(cell_index(cx, cy, cz));
-                        int2 rg = range[size_t(cell)];
-                        if (VALIDATE && !(rg.x >= 0 && rg.x < a.n && rg.y > rg.x && a.keys[rg.x] == cell)) continue;
+                        const int2 rg = BATCHED ? row_cell(a.rowtab, cell) : range[size_t(cell)];
                         for (int m = rg.x; m < rg.y; m++) {
                             const float4 Pj = a.Xs[m];
                             const float dx = x.x - Pj.x, dy = x.y - Pj.y, dz = x.z - Pj.z;
@@ -314,9 +313,9 @@ void launch_adj_prep(const AdjArgs& a, hipStream_t s) {
 void launch_adj_shape(const AdjShapeArgs& a, int grid, hipStream_t s) {
     hipLaunchKernelGGL(k_adj_shape, dim3(grid), dim3(256), 0, s, a);
 }
-void launch_adj_pp(const AdjArgs& a, bool validate, hipStream_t s) {
+void launch_adj_pp(const AdjArgs& a, bool batched, hipStream_t s) {
     if (a.n <= 0) return;
-    if (validate) hipLaunchKernelGGL(k_adj_pp<true>, dim3(blocks256(a.n)), dim3(256), 0, s, a);
+    if (batched) hipLaunchKernelGGL(k_adj_pp<true>, dim3(blocks256(a.n)), dim3(256), 0, s, a);
     else hipLaunchKernelGGL(k_adj_pp<false>, dim3(blocks256(a.n)), dim3(256), 0, s, a);
 }
 void launch_xyz_in(float4* dst, const float* src, int n, hipStream_t s) {

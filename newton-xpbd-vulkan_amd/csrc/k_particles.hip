@@ -102,6 +102,10 @@ __global__ __launch_bounds__(256) void k_integrate(IntegrateArgs a) {
             const uint32_t c = a.rkeys[idx];
             if (idx == 0 || a.rkeys[idx - 1] != c) *reinterpret_cast<int2*>(a.range + size_t(2) * c) = make_int2(0, 0);
         }
+        if (a.rslot != nullptr) {   // k_clear_rows, folded in
+            const int r = a.rslot[idx];
+            if (r >= 0) a.rows[r] = make_uint2(0u, 0u);
+        }
         uint32_t key = cell_of(Pn.x, Pn.y, Pn.z, a.inv_cell);
         if (a.scene != nullptr) key |= uint32_t(a.scene[idx]) << kCellBits;   // scenes keep their slot ranges
         else if (a.planes) key |= plane_of(Pn.x, Pn.y, Pn.z, a.inv_cell, a.planes) << kCellBits;
@@ -152,6 +156,148 @@ __global__ __launch_bounds__(256) void k_clear_ranges(int n, const uint32_t* __r
     if (t >= n) return;
     const uint32_t c = rkeys[t];
     if (t == 0 || rkeys[t - 1] != c) *reinterpret_cast<int2*>(range + size_t(2) * c) = make_int2(0, 0);
+}
+
+// grid_offsets.comp:16-35 for a batched solver, as the row table: the
+// occupied groups of 8 cells are numbered in sorted order (a scan of the
+// groups' first positions over tiles of kGroupTile positions), then each
+// position writes what it decides.
+constexpr int kGroupTile = 1024;   // 256 threads x 4 positions
+
+__device__ __forceinline__ bool group_first(const uint32_t* __restrict__ keys, int t) {
+    return t == 0 || (keys[t - 1] >> 3) != (keys[t] >> 3);
+}
+
+// Group starts per tile.
+__global__ __launch_bounds__(256) void k_group_counts(int n, const uint32_t* __restrict__ keys,
+                                                      uint32_t* __restrict__ tilecnt) {
+    __shared__ uint32_t s_w[4];
+    const int tile = blockIdx.x, w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    uint32_t c = 0;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const int t = tile * kGroupTile + i * 256 + threadIdx.x;
+        c += uint32_t(__popcll(__ballot(t < n && group_first(keys, t))));
+    }
+    if (lane == 0) s_w[w] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) tilecnt[tile] = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+}
+
+// Exclusive scan of the tile counts (one workgroup).
+__global__ __launch_bounds__(1024) void k_group_scan(int tiles, const uint32_t* __restrict__ tilecnt,
+                                                     uint32_t* __restrict__ tileoff) {
+    __shared__ uint32_t s_w[16];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    uint32_t carry = 0;
+    for (int b = 0; b < tiles; b += 1024) {
+        const int t = b + threadIdx.x;
+        const uint32_t x = t < tiles ? tilecnt[t] : 0u;
+        const uint32_t inc = uint32_t(wave_incl_scan_i(int(x)));
+        if (lane == 63) s_w[w] = inc;
+        __syncthreads();
+        uint32_t pre = 0, tot = 0;
+        for (int k = 0; k < 16; k++) {
+            const uint32_t v = s_w[k];
+            if (k < w) pre += v;
+            tot += v;
+        }
+        if (t < tiles) tileoff[t] = carry + pre + inc - x;
+        carry += tot;
+        __syncthreads();
+    }
+}
+
+// Every position: its group's number gi (the group starts at or before it,
+// minus one); a cell starting at t is the first with key >= group * 8 + j for
+// each j after the group's previous cell, up to its own; the group's last
+// position ends every j after its cell; the last position writes the
+// sentinel.  A row's mask gathers its groups' bits in LDS at the row's first
+// position, which then stores the row's entry (first group, mask) and
+// records the row so the next build empties it.  Only rows that cross a tile
+// boundary meet in global memory, with atomics.
+__global__ __launch_bounds__(256) void k_group_build(int n, const uint32_t* __restrict__ keys,
+                                                     const uint32_t* __restrict__ tileoff, uint2* __restrict__ rows,
+                                                     uint32_t* __restrict__ glb, int* __restrict__ rslot) {
+    __shared__ uint32_t s_c[16];       // group starts per chunk (chunk = 4 i + w, in position order)
+    __shared__ int s_rlast[16];        // per chunk: local index of its last row start, or -1
+    __shared__ uint32_t s_mask[kGroupTile];
+    const int tile = blockIdx.x, w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int t0 = tile * kGroupTile;
+    const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+    uint32_t k[4], kp[4], kn[4], gi[4];
+    uint64_t fm[4], rm[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const int t = t0 + i * 256 + threadIdx.x;
+        k[i] = kp[i] = kn[i] = 0u;
+        if (t < n) {
+            k[i] = keys[t];
+            if (t > 0) kp[i] = keys[t - 1];
+            if (t + 1 < n) kn[i] = keys[t + 1];
+        }
+        fm[i] = __ballot(t < n && (t == 0 || (kp[i] >> 3) != (k[i] >> 3)));
+        rm[i] = __ballot(t < n && (t == 0 || (kp[i] >> 7) != (k[i] >> 7)));
+        if (lane == 0) {
+            s_c[4 * i + w] = uint32_t(__popcll(fm[i]));
+            s_rlast[4 * i + w] = rm[i] ? i * 256 + w * 64 + 63 - int(__clzll((long long)rm[i])) : -1;
+        }
+        s_mask[i * 256 + threadIdx.x] = 0u;
+    }
+    __syncthreads();
+    const uint32_t pre = tileoff[tile];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const int c = 4 * i + w;
+        uint32_t before = pre;   // group starts in the tile's earlier chunks
+        for (int q = 0; q < c; q++) before += s_c[q];
+        gi[i] = before + uint32_t(__popcll(fm[i] & upto)) - 1u;
+        const int t = t0 + i * 256 + threadIdx.x;
+        if (t >= n) continue;
+        const uint32_t key = k[i], grp = key >> 3, row = key >> 7;
+        const bool first = (fm[i] >> lane) & 1ull;
+        const bool starts = first || kp[i] != key;
+        const bool last = t + 1 == n || (kn[i] >> 3) != grp;
+        uint32_t* lb = glb + 8 * size_t(gi[i]);
+        const int j = int(key & 7u);
+        if (starts)
+            for (int q = first ? 0 : int(kp[i] & 7u) + 1; q <= j; q++) lb[q] = uint32_t(t);
+        if (last)
+            for (int q = j + 1; q < 8; q++) lb[q] = uint32_t(t + 1);
+        if (first) {
+            // the row's first position in this tile: in this chunk, else the
+            // last of an earlier chunk; none when the row began in an earlier tile
+            const uint64_t m = rm[i] & upto;
+            int lead = m ? i * 256 + w * 64 + 63 - int(__clzll((long long)m)) : -1;
+            for (int q = c - 1; lead < 0 && q >= 0; q--) lead = s_rlast[q];
+            if (lead >= 0) atomicOr(&s_mask[lead], 1u << (grp & 15u));
+            else atomicOr(&rows[row].y, 1u << (grp & 15u));
+        }
+        rslot[t] = ((rm[i] >> lane) & 1ull) ? int(row) : -1;
+        if (t + 1 == n) glb[8 * size_t(gi[i] + 1)] = uint32_t(n);   // the sentinel group
+    }
+    __syncthreads();
+    const bool spill_next = t0 + kGroupTile < n;   // the next tile may continue this tile's last row
+    const uint32_t next_row = spill_next ? keys[t0 + kGroupTile] >> 7 : 0xFFFFFFFFu;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        if (!((rm[i] >> lane) & 1ull)) continue;
+        const uint32_t row = k[i] >> 7, mask = s_mask[i * 256 + threadIdx.x];
+        if (spill_next && row == next_row) {   // later tiles add their groups' bits
+            rows[row].x = gi[i];
+            atomicOr(&rows[row].y, mask);
+        } else {
+            rows[row] = make_uint2(gi[i], mask);
+        }
+    }
+}
+
+// Empties the rows the last build filled (outside a substep).
+__global__ __launch_bounds__(256) void k_clear_rows(int n, const int* __restrict__ rslot, uint2* __restrict__ rows) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int r = rslot[t];
+    if (r >= 0) rows[r] = make_uint2(0u, 0u);
 }
 
 // Dense tables exactly as the reference leaves them after grid_offsets
@@ -356,7 +502,7 @@ constexpr int kSurv = 16;   // recorded survivors per lane before a drain
 // batches or paired survivor drains.
 constexpr int kScanBatch = 4;
 
-template <bool DIAG, bool VALIDATE, int BLOCK>
+template <bool DIAG, bool BATCHED, int BLOCK>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_pp_apply(PPArgs a) {
     constexpr int SB = kScanBatch;
     __shared__ uint32_t s_surv[kSurv][BLOCK];
@@ -381,17 +527,15 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6, 8))) v
     const int ye = min((int)floorf((x.y + R) * inv), ys + kGridDim - 1);
     const int ze = min((int)floorf((x.z + R) * inv), zs + kGridDim - 1);
     const int2* range = reinterpret_cast<const int2*>(a.range);
-    // batched mode: this particle's scene selects its own 2^21-cell table
-    // (keys reach 2^32 - 1 with 2048 scenes: row arithmetic stays unsigned);
-    // with wrap planes each visited virtual cell selects its plane's table
-    const uint32_t sbase = VALIDATE ? (a.keys[t] & ~((1u << kCellBits) - 1u)) : 0u;
+    // batched mode: the particle's scene bits complete the key of each
+    // visited cell; with wrap planes each visited virtual cell selects its
+    // plane's table.  Every table holds exactly the current substep's cells:
+    // the previous ones are reset before each build.
+    const uint32_t sbase = BATCHED ? (a.keys[t] & ~((1u << kCellBits) - 1u)) : 0u;
     auto table = [&](int cx, int cy, int cz) -> uint32_t {
-        return VALIDATE ? sbase : (a.planes ? wrap_plane(cx, cy, cz, a.planes) << kCellBits : 0u);
+        return BATCHED ? sbase : (a.planes ? wrap_plane(cx, cy, cz, a.planes) << kCellBits : 0u);
     };
     auto raw = [&](uint32_t base, int cell) -> int2 { return range[size_t(base) + size_t(cell)]; };
-    // every table (plain, wrap planes, batched) holds exactly the current
-    // substep's rows: the previous rows are reset before each build
-    auto check = [&](uint32_t, int, int2 rg) -> int2 { return rg; };
     // cheap reject before the refined length: a pair the exact test accepts
     // has |rd| <= (r_i + r_j + coh)(1 + 5 ulp) <= (r_i + r_max + |coh|)(1 + 5 ulp)
     // (r_max bounds every radius), and the plain squared length is within
@@ -453,13 +597,17 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6, 8))) v
     auto run = [&](int cx, int cy, int cz, int len, int2 p0, int2 p1, int2 p2, bool pre) {
         const int c0 = cell_index(cx, cy, cz);
         const uint32_t base = table(cx, cy, cz);
+        if constexpr (BATCHED) {
+            const int2 sp = row_span(a.rowtab, base + uint32_t(c0), len);
+            scan(sp.x, sp.y);
+            return;
+        }
         int s0 = 0, e0 = 0;
         bool any = false;
         for (int m = 0; m < len; m++) {
             int2 rg;
             if (pre && m < 3) rg = m == 0 ? p0 : (m == 1 ? p1 : p2);
             else rg = raw(base, c0 + m);
-            rg = check(base, c0 + m, rg);
             if (rg.y > rg.x) {
                 if (!any) s0 = rg.x;
                 e0 = rg.y;
@@ -468,37 +616,78 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6, 8))) v
         }
         scan(s0, e0);
     };
-    // Rows (cz, cy) in the reference's order.  Every row's first x-run starts
-    // at xs, so it has the same length in every row; its first three table
-    // rows are requested one row ahead, while the current row is scanned.
     const int vx0 = xs + kGridOrigin;
     const bool clamp0 = vx0 < 0;   // clamped virtual cells: one physical cell per run
     const int len0 = clamp0 ? 1 : min(xe - xs + 1, kGridDim - vx0 % kGridDim);
-    const int npre = min(len0, 3);
-    auto prefetch = [&](int cy, int cz, int2& q0, int2& q1, int2& q2) {
-        const int c0 = cell_index(xs, cy, cz);
-        const uint32_t base = table(xs, cy, cz);
-        q0 = raw(base, c0);
-        q1 = npre > 1 ? raw(base, c0 + 1) : make_int2(0, 0);
-        q2 = npre > 2 ? raw(base, c0 + 2) : make_int2(0, 0);
+    // the rest of a row after its first x-run: past an x wrap, or clamped cells
+    auto rest = [&](int cy, int cz) {
+        int cx = xs + len0;
+        while (cx <= xe) {
+            const int vx = cx + kGridOrigin;
+            const int len = vx < 0 ? 1 : min(xe - cx + 1, kGridDim - vx % kGridDim);
+            run(cx, cy, cz, len, make_int2(0, 0), make_int2(0, 0), make_int2(0, 0), false);
+            cx += len;
+        }
     };
-    int2 r0, r1, r2;
-    prefetch(ys, zs, r0, r1, r2);
-    for (int cz = zs; cz <= ze; cz++) {
-        for (int cy = ys; cy <= ye; cy++) {
-            int ny = cy + 1, nz = cz;
-            if (ny > ye) { ny = ys; nz++; }
-            int2 n0 = make_int2(0, 0), n1 = n0, n2 = n0;
-            if (nz <= ze) prefetch(ny, nz, n0, n1, n2);
-            run(xs, cy, cz, len0, r0, r1, r2, true);
-            int cx = xs + len0;
-            while (cx <= xe) {   // the rest of the row: past an x wrap, or clamped cells
-                const int vx = cx + kGridOrigin;
-                const int len = vx < 0 ? 1 : min(xe - cx + 1, kGridDim - vx % kGridDim);
-                run(cx, cy, cz, len, r0, r1, r2, false);
-                cx += len;
+    if constexpr (BATCHED) {
+        // Rows (cz, cy) in the reference's order.  Every row's first x-run
+        // starts at xs, at the same offset xa in every row; its slots are two
+        // bounds behind the row's table entry.  Entries are requested two
+        // rows ahead and bounds one row ahead, so neither wait falls on a
+        // scan.
+        const RowTable& T = a.rowtab;
+        auto rowkey = [&](int cy, int cz) -> uint32_t { return sbase + uint32_t(cell_index(xs, cy, cz)); };
+        auto advance = [&](int& cy, int& cz) {
+            if (++cy > ye) { cy = ys; cz++; }
+        };
+        int cy = ys, cz = zs;
+        int cy1 = cy, cz1 = cz;
+        advance(cy1, cz1);
+        int cy2 = cy1, cz2 = cz1;
+        advance(cy2, cz2);
+        const uint32_t k0 = rowkey(cy, cz);
+        const int xa = int(k0 & 127u);
+        const uint2 e0 = T.rows[k0 >> 7];
+        int2 sp = make_int2(int(row_lb(T, e0, xa)), int(row_lb(T, e0, xa + len0)));
+        uint2 e1 = cz1 <= ze ? T.rows[rowkey(cy1, cz1) >> 7] : make_uint2(0u, 0u);
+        while (cz <= ze) {
+            int2 sp1 = make_int2(0, 0);
+            if (cz1 <= ze) sp1 = make_int2(int(row_lb(T, e1, xa)), int(row_lb(T, e1, xa + len0)));
+            uint2 e2 = make_uint2(0u, 0u);
+            if (cz2 <= ze) e2 = T.rows[rowkey(cy2, cz2) >> 7];
+            scan(sp.x, sp.y);
+            rest(cy, cz);
+            sp = sp1;
+            e1 = e2;
+            cy = cy1; cz = cz1;
+            cy1 = cy2; cz1 = cz2;
+            advance(cy2, cz2);
+        }
+    } else {
+        // Rows (cz, cy) in the reference's order.  Every row's first x-run
+        // starts at xs, so it has the same length in every row; its first
+        // three table rows are requested one row ahead, while the current
+        // row is scanned.
+        const int npre = min(len0, 3);
+        auto prefetch = [&](int cy, int cz, int2& q0, int2& q1, int2& q2) {
+            const int c0 = cell_index(xs, cy, cz);
+            const uint32_t base = table(xs, cy, cz);
+            q0 = raw(base, c0);
+            q1 = npre > 1 ? raw(base, c0 + 1) : make_int2(0, 0);
+            q2 = npre > 2 ? raw(base, c0 + 2) : make_int2(0, 0);
+        };
+        int2 r0, r1, r2;
+        prefetch(ys, zs, r0, r1, r2);
+        for (int cz = zs; cz <= ze; cz++) {
+            for (int cy = ys; cy <= ye; cy++) {
+                int ny = cy + 1, nz = cz;
+                if (ny > ye) { ny = ys; nz++; }
+                int2 n0 = make_int2(0, 0), n1 = n0, n2 = n0;
+                if (nz <= ze) prefetch(ny, nz, n0, n1, n2);
+                run(xs, cy, cz, len0, r0, r1, r2, true);
+                rest(cy, cz);
+                r0 = n0; r1 = n1; r2 = n2;
             }
-            r0 = n0; r1 = n1; r2 = n2;
         }
     }
     drain();
@@ -684,6 +873,18 @@ void launch_iota(uint32_t* p, int n, hipStream_t s) {
 void launch_gather_f3(float4* dst, const float* src, const uint32_t* ids, int n, hipStream_t s) {
     if (n > 0) hipLaunchKernelGGL(k_gather_f3, dim3(blocks_for(n, 256)), dim3(256), 0, s, dst, src, ids, n);
 }
+int group_tiles(int n) { return (n + kGroupTile - 1) / kGroupTile; }
+void launch_row_table(int n, const uint32_t* keys, uint32_t* tilecnt, uint32_t* tileoff, uint2* rows, uint32_t* glb,
+                      int* rslot, hipStream_t s) {
+    if (n <= 0) return;
+    const int tiles = group_tiles(n);
+    hipLaunchKernelGGL(k_group_counts, dim3(tiles), dim3(256), 0, s, n, keys, tilecnt);
+    hipLaunchKernelGGL(k_group_scan, dim3(1), dim3(1024), 0, s, tiles, tilecnt, tileoff);
+    hipLaunchKernelGGL(k_group_build, dim3(tiles), dim3(256), 0, s, n, keys, tileoff, rows, glb, rslot);
+}
+void launch_clear_rows(int n, const int* rslot, uint2* rows, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_clear_rows, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, rslot, rows);
+}
 void launch_clear_ranges(int n, const uint32_t* rkeys, int* range, hipStream_t s) {
     if (n > 0) hipLaunchKernelGGL(k_clear_ranges, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, rkeys, range);
 }
@@ -696,20 +897,20 @@ void launch_shape_solve(const ShapeArgs& a, int grid, hipStream_t s) {
 void launch_shape_apply(const ShapeApplyArgs& a, int grid, hipStream_t s) {
     hipLaunchKernelGGL(k_shape_apply, dim3(grid), dim3(256), 0, s, a);
 }
-void launch_pp_apply(const PPArgs& a, bool diag, bool validate, int block, hipStream_t s) {
+void launch_pp_apply(const PPArgs& a, bool diag, bool batched, int block, hipStream_t s) {
     if (a.n <= 0) return;
     const int bs = (block == 64 || block == 128) ? block : 256;
     const dim3 g(blocks_for(a.n, bs)), b(bs);
     auto go = [&](auto kernel) { hipLaunchKernelGGL(kernel, g, b, 0, s, a); };
     if (bs == 64) {
-        if (diag) { if (validate) go(k_pp_apply<true, true, 64>); else go(k_pp_apply<true, false, 64>); }
-        else { if (validate) go(k_pp_apply<false, true, 64>); else go(k_pp_apply<false, false, 64>); }
+        if (diag) { if (batched) go(k_pp_apply<true, true, 64>); else go(k_pp_apply<true, false, 64>); }
+        else { if (batched) go(k_pp_apply<false, true, 64>); else go(k_pp_apply<false, false, 64>); }
     } else if (bs == 128) {
-        if (diag) { if (validate) go(k_pp_apply<true, true, 128>); else go(k_pp_apply<true, false, 128>); }
-        else { if (validate) go(k_pp_apply<false, true, 128>); else go(k_pp_apply<false, false, 128>); }
+        if (diag) { if (batched) go(k_pp_apply<true, true, 128>); else go(k_pp_apply<true, false, 128>); }
+        else { if (batched) go(k_pp_apply<false, true, 128>); else go(k_pp_apply<false, false, 128>); }
     } else {
-        if (diag) { if (validate) go(k_pp_apply<true, true, 256>); else go(k_pp_apply<true, false, 256>); }
-        else { if (validate) go(k_pp_apply<false, true, 256>); else go(k_pp_apply<false, false, 256>); }
+        if (diag) { if (batched) go(k_pp_apply<true, true, 256>); else go(k_pp_apply<true, false, 256>); }
+        else { if (batched) go(k_pp_apply<false, true, 256>); else go(k_pp_apply<false, false, 256>); }
     }
 }
 void launch_restitution(const RestArgs& a, int grid, hipStream_t s) {

@@ -177,6 +177,7 @@ void Solver::rollout_backward(const float* gx, const float* gv, float* gx0, floa
                 a.gx = cfg_.gravity[0]; a.gy = cfg_.gravity[1]; a.gz = cfg_.gravity[2];
                 a.rmax = rmax_;
                 a.planes = planes_ ? plane_bits_ : 0;
+                a.rowtab = rowtab();
                 launch_adj_prep(a, stream_);
                 AdjShapeArgs sa;
                 sa.cmax = int(cmax_); sa.cc = t.counts.p + j;

@@ -60,7 +60,7 @@ void Solver::free_all() {
     dfree(Xs_[0]); dfree(Vs_[0]);
     dfree(body_half_); dfree(body_I_); dfree(body_invw_); dfree(mat_mu_t_); dfree(mat_mu_r_);
     dfree(rc_s0_); dfree(rc_s1_); dfree(rc_p0_); dfree(rc_p1_); dfree(rc_n_); dfree(rc_impulse_);
-    dfree(range_); dfree(rkeys_); dfree(tbl_); dfree(cpart_); dfree(cshape_); dfree(cc_); dfree(cbpos_); dfree(cbvel_); dfree(cnorm_);
+    dfree(range_); dfree(rkeys_); dfree(rows_); dfree(glb_); dfree(rslot_); dfree(gtcnt_); dfree(gtoff_); dfree(tbl_); dfree(cpart_); dfree(cshape_); dfree(cc_); dfree(cbpos_); dfree(cbvel_); dfree(cnorm_);
     dfree(dshape_); dfree(delta_); dfree(owner_); dfree(body_q_); dfree(body_qd_); dfree(body_com_); dfree(body_invI_);
     dfree(body_invm_); dfree(body_delta_); dfree(mat_mu_); dfree(body_flags_); dfree(shape_body_);
     dfree(verts_d_); dfree(vvel_d_); dfree(tris_d_); dfree(prim_d_); dfree(err_d_); dfree(nodes_d_); dfree(cnodes_d_); dfree(lbox_d_); dfree(pg_start_); dfree(lb_); dfree(lb_node_); dfree(tsoup_); dfree(pg_soup_); dfree(mcnt_); dfree(mq_); dfree(pq_);
@@ -95,21 +95,33 @@ void Solver::alloc_all() {
     rmax_ = 0.f;
     dalloc(keysA_, n_); dalloc(valsA_, n_); dalloc(keysB_, n_); dalloc(valsB_, n_);
     dalloc(Xs_[0], n_); dalloc(Vs_[0], n_);
-    const size_t tile = size_t(256) * size_t(sort_items_for((long long)n_));
+    // radix tiles: 2,048 keys from 262,144 particles on, else 1,024 (autotune
+    // may switch); the histograms are sized for the smaller tiles
+    sort_items_ = sort_items_for((long long)n_);
+    const size_t tile = size_t(256) * 4;
     const size_t tiles = (n_ + tile - 1) / tile;
     dalloc(hist_, size_t(kSortMaxBins) * std::max<size_t>(tiles, 1));
     dalloc(bintot_, kSortMaxBins);
-    // one 2^21-cell table per scene; the batched key scene << 21 | cell is the
-    // row, and the keys of the rows written last (rkeys) let each build reset
-    // exactly those rows first
-    dalloc(range_, size_t(2) * kNumCells * num_scenes_);
+    // the plain solver's dense 2^21-row cell table (the key is the row; the
+    // keys of the rows written last, rkeys, let each build reset exactly those
+    // rows first); a batched solver's row table instead: 8 bytes per row
+    // (2^14 per scene) and 32 bytes per particle
+    dalloc(range_, num_scenes_ > 1 ? size_t(4) : size_t(2) * kNumCells);
     dalloc(rkeys_, n_);
+    if (num_scenes_ > 1) {
+        dalloc(rows_, num_scenes_ << 14);           // every row empty
+        dalloc(glb_, 8 * (n_ + 1));                 // at most one group per particle, and the sentinel
+        dalloc(rslot_, n_);
+        dfill(rslot_, 0xff);                        // no row filled
+        dalloc(gtcnt_, size_t(group_tiles(int(n_))));
+        dalloc(gtoff_, size_t(group_tiles(int(n_))));
+    }
     dalloc(cpart_, cmax_); dalloc(cshape_, cmax_);
     dalloc(cbpos_, cmax_); dalloc(cbvel_, cmax_); dalloc(cnorm_, cmax_);
     dalloc(cc_, 8);   // [particle slot 0, slot 1, error flags, spare, rigid slot 0, slot 1, spare, spare]
     dalloc(dshape_, 3 * n_); dalloc(delta_, 3 * n_);
     dalloc(owner_, n_);
-    HIPCHECK(hipMemset(owner_.p, 0xff, owner_.n * sizeof(int)));   // -1: no shape delta pending
+    dfill(owner_, 0xff);   // -1: no shape delta pending
     dalloc(diag_header_, 2);
     dalloc(mesh_stats_, 8);
     dalloc(mq_, n_);   // mesh candidate queue (its counters are sized by finalize_meshes)
@@ -176,6 +188,7 @@ void Solver::init(const xpbd_config& cfg, size_t n, size_t max_contacts) {
 
 void Solver::init_batch(const xpbd_config& cfg, const size_t* counts, size_t num_scenes, size_t max_contacts) {
     if (num_scenes == 0) throw std::runtime_error("init_batch: num_scenes must be positive");
+    // 11 key bits above the cell
     if (num_scenes > 2048) throw std::runtime_error("init_batch: at most 2048 scenes (11 key bits above the cell)");
     size_t total = 0;
     scene_off_.assign(1, 0);
@@ -632,7 +645,7 @@ void Solver::update_planes() {
     planes_ = want;
     table_bits_ = plane_bits_;
     if (num_scenes_ == 1) set_sort_shape(planes_ ? kCellBits + 3 * plane_bits_ : kCellBits);
-    dalloc(range_, size_t(2) * (size_t(kNumCells) << (planes_ ? 3 * plane_bits_ : 0)) * num_scenes_);
+    if (num_scenes_ == 1) dalloc(range_, size_t(2) * (size_t(kNumCells) << (planes_ ? 3 * plane_bits_ : 0)));
     dalloc(rkeys_, n_);   // zero keys: the first reset touches row 0 only
     HIPCHECK(hipDeviceSynchronize());
     grid_valid_ = false;
@@ -644,6 +657,11 @@ void Solver::update_planes() {
 // to the table.
 void Solver::build_ranges(const uint32_t* sorted_keys, bool cleared) {
     const int n = int(n_);
+    if (num_scenes_ > 1) {
+        if (!cleared) launch_clear_rows(n, rslot_.p, rows_.p, stream_);
+        launch_row_table(n, sorted_keys, gtcnt_.p, gtoff_.p, rows_.p, glb_.p, rslot_.p, stream_);
+        return;
+    }
     if (!cleared) launch_clear_ranges(n, rkeys_.p, range_.p, stream_);
     launch_cell_ranges(n, sorted_keys, range_.p, rkeys_.p, stream_);
 }
@@ -674,13 +692,13 @@ void Solver::run_sort(int tiles, float sdt) {
         const uint32_t* vin = pass == 0 ? nullptr : vb[dst ^ 1];
         const uint2* pin = pass == 0 ? kv_.p : nullptr;
         stage_begin(kSortUp);
-        launch_sort_upsweep(sort_bits_, sort_items_for(n), n, kin, pin, shift, hist_.p, tiles, stream_);
+        launch_sort_upsweep(sort_bits_, sort_items_, n, kin, pin, shift, hist_.p, tiles, stream_);
         stage_end(kSortUp);
         stage_begin(kSortScan);
         launch_sort_binscan(sort_bits_, hist_.p, bintot_.p, tiles, stream_);
         stage_end(kSortScan);
         stage_begin(kSortScatter);
-        launch_sort_scatter(sort_bits_, sort_items_for(n), n, kin, vin, pin, kb[dst], vb[dst], shift, hist_.p, bintot_.p, tiles,
+        launch_sort_scatter(sort_bits_, sort_items_, n, kin, vin, pin, kb[dst], vb[dst], shift, hist_.p, bintot_.p, tiles,
                             pass == sort_passes_ - 1 ? &g : nullptr, stream_);
         stage_end(kSortScatter);
         dst ^= 1;
@@ -715,7 +733,7 @@ BodyStepArgs Solver::body_step_args(float sdt) {
 
 void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zero_next) {
     const int n = int(n_);
-    const int sort_tile = 256 * sort_items_for((long long)n_);
+    const int sort_tile = 256 * sort_items_;
     const int tiles = int((n_ + sort_tile - 1) / sort_tile);
     const int nbox = int(boxes_.size());
     const int M = int(meshes_.size());
@@ -786,7 +804,8 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
         a.scene = num_scenes_ > 1 ? scene_d_.p : nullptr;
         a.planes = planes_ ? plane_bits_ : 0;
         a.cpart = cpart_.p; a.cshape = cshape_.p; a.cbpos = cbpos_.p; a.cbvel = cbvel_.p; a.cnorm = cnorm_.p;
-        a.rkeys = rkeys_.p; a.range = range_.p;
+        a.rkeys = num_scenes_ > 1 ? nullptr : rkeys_.p; a.range = range_.p;
+        a.rslot = num_scenes_ > 1 ? rslot_.p : nullptr; a.rows = rows_.p;
         stage_begin(kIntegrate);
         launch_integrate(a, stream_);
         stage_end(kIntegrate);
@@ -874,6 +893,7 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
         pa.dt = sdt; pa.relax = relax_; pa.inv_cell = inv_cell; pa.vmax = cfg_.particle_v_max;
         pa.rmax = rmax_;
         pa.planes = planes_ ? plane_bits_ : 0;
+        pa.rowtab = rowtab();
         pa.iter_index = uint32_t(it); pa.max_diag_pairs = max_pairs_;
         pa.diag_header = diag_header_.p; pa.diag_pairs = diag_pairs_.p;
         stage_begin(kPP);
@@ -1072,20 +1092,27 @@ int Solver::autotune(int frames, float dt, int substeps, int iterations) {
         rigid_slot_ = keep_rslot;
         grid_valid_ = keep_grid;
     };
-    const int cands[] = {64, 128, 256};
-    int best = pp_block_;
+    // candidates: the particle-pair block size and the radix tile (keys per
+    // workgroup of the sort that builds the cell tables)
+    const int blocks[] = {64, 128, 256};
+    const int items[] = {4, 8};
+    int best = pp_block_, best_items = sort_items_;
     double best_ms = 1e300;
     try {
-        for (int c : cands) {
-            pp_block_ = c;
-            invalidate_graphs();
-            step(dt, substeps, iterations);   // warm-up (and capture)
-            restore();
-            const auto t0 = std::chrono::high_resolution_clock::now();
-            for (int f = 0; f < frames; f++) step(dt, substeps, iterations);
-            const double ms = std::chrono::duration<double, std::milli>(std::chrono::high_resolution_clock::now() - t0).count();
-            restore();
-            if (ms < best_ms) { best_ms = ms; best = c; }
+        for (int it : items) {
+            for (int c : blocks) {
+                pp_block_ = c;
+                sort_items_ = it;
+                invalidate_graphs();
+                step(dt, substeps, iterations);   // warm-up (and capture)
+                restore();
+                const auto t0 = std::chrono::high_resolution_clock::now();
+                for (int f = 0; f < frames; f++) step(dt, substeps, iterations);
+                const double ms =
+                    std::chrono::duration<double, std::milli>(std::chrono::high_resolution_clock::now() - t0).count();
+                restore();
+                if (ms < best_ms) { best_ms = ms; best = c; best_items = it; }
+            }
         }
     } catch (...) {
         free_saved();
@@ -1093,8 +1120,19 @@ int Solver::autotune(int frames, float dt, int substeps, int iterations) {
     }
     free_saved();
     pp_block_ = best;
+    sort_items_ = best_items;
     invalidate_graphs();
     return best;
+}
+
+void Solver::set_tuning(int pp_block, int radix_tile) {
+    if (pp_block != 64 && pp_block != 128 && pp_block != 256)
+        throw std::runtime_error("set_tuning: pp_block must be 64, 128 or 256");
+    if (radix_tile != 1024 && radix_tile != 2048)
+        throw std::runtime_error("set_tuning: radix_tile must be 1024 or 2048");
+    pp_block_ = pp_block;
+    sort_items_ = radix_tile / 256;
+    invalidate_graphs();
 }
 
 // Bits of a count: the smallest b with v < 2^b (at least 1).
@@ -1490,7 +1528,7 @@ void Solver::profile_reset() {
     if (initialized_) {
         DeviceGuard g(device_);
         HIPCHECK(hipStreamSynchronize(stream_));
-        HIPCHECK(hipMemset(mesh_stats_.p, 0, 8 * sizeof(uint64_t)));
+        dfill(mesh_stats_, 0);
     }
 }
 

@@ -40,9 +40,20 @@ public:
     // (VKXPBD_DETERMINISTIC=1 at init also turns it on).
     void set_deterministic(bool on) { det_ = on; invalidate_graphs(); }
     bool deterministic() const { return det_; }
-    // Times candidate launch shapes on the current state (then restores it) and
-    // keeps the fastest; returns the chosen particle-particle block size.
+    // Times candidate launch shapes (particle-pair block size x radix tile)
+    // on the current state (then restores it) and keeps the fastest; returns
+    // the chosen particle-particle block size.
     int autotune(int frames, float dt, int substeps, int iterations);
+    int radix_tile() const { return 256 * sort_items_; }
+    int pp_block() const { return pp_block_; }
+    // sets what autotune() picks: pp_block 64, 128 or 256; radix_tile 1,024 or 2,048
+    void set_tuning(int pp_block, int radix_tile);
+    // device bytes of the cell-range tables (dense rows, or the batched row
+    // table, group bounds and tile scratch)
+    size_t cell_table_bytes() const {
+        return range_.n * sizeof(int32_t) + rkeys_.n * sizeof(uint32_t) + rows_.n * sizeof(uint2) +
+               glb_.n * sizeof(uint32_t) + rslot_.n * sizeof(int32_t) + (gtcnt_.n + gtoff_.n) * sizeof(uint32_t);
+    }
     void add_mesh(const float* v, size_t nv, const uint32_t* t, size_t nt, float margin, const float* rot, int wt);
     void finalize_meshes();
     void set_particles(const float* q, const float* qd, const float* w, const float* r, const int32_t* f, size_t n);
@@ -170,6 +181,7 @@ private:
     // particle-pair block size (64, 128 or 256; autotune() picks per scene):
     // 64 measured fastest on the 1M-particle pile (2.31 against 2.38 ms per frame)
     int pp_block_ = 64;
+    int sort_items_ = 8;   // keys per radix thread: 4 (1,024-key tiles) or 8 (2,048); autotune() may pick
     // captured frames: rotation state and parameters -> executable graph
     struct FrameGraph {
         int s, p, o;
@@ -240,6 +252,15 @@ private:
     DevBuf<float4> Xs_[1], Vs_[1];   // predicted state in sorted order (the first iteration's input)
     DevBuf<int32_t> range_;   // int2 per key (start, end); rows of the last build are reset before the next
     DevBuf<uint32_t> rkeys_;  // with wrap planes: the sorted keys whose rows range_ holds
+    // batched solvers: the cell ranges as a row table (8 bytes per row of
+    // every scene) over 8 lower bounds per occupied group of 8 cells, the
+    // row each sorted position of the last build filled (-1: none), and the
+    // per-tile scratch of the group numbering
+    DevBuf<uint2> rows_;
+    DevBuf<uint32_t> glb_;
+    DevBuf<int32_t> rslot_;
+    DevBuf<uint32_t> gtcnt_, gtoff_;
+    RowTable rowtab() const { return RowTable{rows_.p, glb_.p}; }
     DevBuf<int32_t> tbl_;     // 2 x cells scratch for the dense getters
     // contacts: SoA of float4 payloads, counter slots
     DevBuf<int32_t> cpart_, cshape_, cc_;

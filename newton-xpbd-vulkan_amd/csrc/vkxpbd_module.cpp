@@ -475,6 +475,29 @@ PYBIND11_MODULE(_vkxpbd, m) {
             py::arg("enabled"),
             "Take every sum the reference takes with float atomics (shape deltas, body deltas, restitution) in a "
             "canonical order, so results do not depend on scheduling (default off).")
+        .def_property_readonly("cell_table_bytes",
+                               [](const PySolver& s) {
+                                   size_t v = 0;
+                                   xpbd_cell_table_bytes(s.h, &v);
+                                   return v;
+                               })
+        .def_property_readonly("radix_tile",
+                               [](const PySolver& s) {
+                                   int v = 0;
+                                   xpbd_radix_tile(s.h, &v);
+                                   return v;
+                               })
+        .def_property_readonly("pp_block",
+                               [](const PySolver& s) {
+                                   int v = 0;
+                                   xpbd_pp_block(s.h, &v);
+                                   return v;
+                               })
+        .def(
+            "set_tuning", [](PySolver& s, int pp_block, int radix_tile) { s.check(xpbd_set_tuning(s.h, pp_block, radix_tile)); },
+            py::arg("pp_block"), py::arg("radix_tile"),
+            "Set the launch shapes autotune picks (pp_block 64/128/256, radix_tile 1024/2048); results do not depend "
+            "on them.")
         .def_property_readonly("deterministic",
                                [](const PySolver& s) {
                                    int v = 0;
@@ -493,7 +516,8 @@ PYBIND11_MODULE(_vkxpbd, m) {
                 return b;
             },
             py::arg("frames") = 3, py::arg("dt") = 1.0f / 60.0f, py::arg("substeps") = 8, py::arg("iterations") = 1,
-            "Pick the fastest particle-particle block size on the current state (state is restored).")
+            "Time the particle-particle block sizes x radix tiles on the current state (state is restored), keep the "
+            "fastest and return its block size.")
         .def(
             "add_box_body",
             [](PySolver& s, farr pos, farr quat, farr half_extents, float density, int flags) {

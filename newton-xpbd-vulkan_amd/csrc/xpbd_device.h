@@ -189,6 +189,40 @@ __device__ __forceinline__ uint32_t plane_of(float px, float py, float pz, float
     return wrap_plane((int)floorf(px * inv), (int)floorf(py * inv), (int)floorf(pz * inv), pb);
 }
 
+// Cell ranges of a batched solver, without a table over every cell.  Keys
+// are scene << 21 | cell, cell = (z * 128 + y) * 128 + x, so a row (scene, z,
+// y) is key >> 7 and the sorted slots of any x-run of cells in one row are
+// contiguous.  A group is 8 consecutive cells of a row (key >> 3; 16 per
+// row).  The occupied groups are numbered in sorted order; glb holds 8 lower
+// bounds per group, glb[8g + j] = the first sorted position whose key is at
+// least group * 8 + j, plus one sentinel group whose first word is the
+// particle count.  A dense row table (8 bytes per row of every scene) gives
+// each row's first occupied group and a 16-bit mask of its occupied groups.
+// The first position at or past cell x of a row (x = 0..128) is then the
+// group's bound when x's group is occupied, else the first bound of the
+// row's next occupied group -- which, groups being numbered in sorted order,
+// is also right past the row's last group (the next row's first group, or
+// the sentinel).  No probing, no branches.
+struct RowTable {
+    const uint2* rows;   // per row: (first group, occupied-group mask); (0, 0) when empty
+    const uint32_t* glb;
+};
+__device__ __forceinline__ uint32_t row_lb(const RowTable& T, uint2 e, int x) {
+    const int gx = x >> 3;
+    const uint32_t below = e.y & ((1u << gx) - 1u);
+    const uint32_t g = e.x + uint32_t(__popc(below));
+    const int j = ((e.y >> gx) & 1u) ? (x & 7) : 0;
+    return T.glb[8 * size_t(g) + size_t(j)];
+}
+// The slots of the cells [key0, key0 + len) of one row.
+__device__ __forceinline__ int2 row_span(const RowTable& T, uint32_t key0, int len) {
+    const uint2 e = T.rows[key0 >> 7];
+    const int x = int(key0 & 127u);
+    return make_int2(int(row_lb(T, e, x)), int(row_lb(T, e, x + len)));
+}
+// The slots of one cell, (s, s) when it is empty.
+__device__ __forceinline__ int2 row_cell(const RowTable& T, uint32_t key) { return row_span(T, key, 1); }
+
 // Wave-aggregated slot reservation on a global counter: one atomic per wave
 // for all active lanes with want == true.  Returns the lane's slot (or -1).
 __device__ __forceinline__ int wave_reserve(int* counter, bool want) {

@@ -60,6 +60,9 @@ struct IntegrateArgs {
     // non-null: zero the cell-table rows the last k_cell_ranges wrote (their
     // keys, in slot order), before this substep's ranges are built
     const uint32_t* rkeys; int* range;
+    // batched solvers: the row-table rows the last build filled (per sorted
+    // position, -1 for none) are emptied instead
+    const int* rslot; uint2* rows;
 };
 
 struct PlaneArgs {
@@ -125,6 +128,7 @@ struct PPArgs {
     float mu, coh, max_r, dt, relax, inv_cell, vmax;
     float rmax;                                     // largest particle radius (bounds every accepted pair)
     int planes;                                     // keys carry the wrap plane above the cell
+    RowTable rowtab;                                // batched solvers: the cell ranges
     uint32_t iter_index, max_diag_pairs;
     uint32_t* diag_header; float* diag_pairs;
 };
@@ -263,6 +267,7 @@ struct AdjArgs {
     float mu, coh, max_r, dt, relax, inv_cell, vmax, gx, gy, gz;
     float rmax;
     int planes;
+    RowTable rowtab;   // batched solvers: the cell ranges
 };
 struct AdjShapeArgs {
     int cmax; const int* cc;
@@ -275,7 +280,7 @@ struct AdjShapeArgs {
 
 void launch_adj_prep(const AdjArgs& a, hipStream_t s);
 void launch_adj_shape(const AdjShapeArgs& a, int grid, hipStream_t s);
-void launch_adj_pp(const AdjArgs& a, bool validate, hipStream_t s);
+void launch_adj_pp(const AdjArgs& a, bool batched, hipStream_t s);
 void launch_xyz_in(float4* dst, const float* src, int n, hipStream_t s);
 void launch_xyz_out(float* dst, const float4* src, int n, hipStream_t s);
 void launch_fill_f4(float4* dst, int n, hipStream_t s);
@@ -289,6 +294,13 @@ void launch_body_contact_solve(const BodySolveArgs& a, hipStream_t s);
 void launch_integrate(const IntegrateArgs& a, hipStream_t s);
 void launch_plane_contacts(const PlaneArgs& a, hipStream_t s);
 void launch_cell_ranges(int n, const uint32_t* keys, int* range, uint32_t* rkeys, hipStream_t s);
+// batched solvers: the row table of the sorted keys (tile scratch of
+// group_tiles(n) words each), recording per sorted position the row it
+// filled; empty the last build's rows
+int group_tiles(int n);
+void launch_row_table(int n, const uint32_t* keys, uint32_t* tilecnt, uint32_t* tileoff, uint2* rows, uint32_t* glb,
+                      int* rslot, hipStream_t s);
+void launch_clear_rows(int n, const int* rslot, uint2* rows, hipStream_t s);
 void launch_clear_ranges(int n, const uint32_t* rkeys, int* range, hipStream_t s);
 void launch_remap_contacts(int* cpart, const int* cc, int cmax, const uint32_t* map, int grid, hipStream_t s);
 void launch_scatter_f3(float* dst, const float4* src, const uint32_t* ids, int n, hipStream_t s);
@@ -300,9 +312,8 @@ void launch_gather_f3(float4* dst, const float* src, const uint32_t* ids, int n,
 void launch_cell_tables(int n, const uint32_t* keys, int* starts, int* ends, hipStream_t s);
 void launch_shape_solve(const ShapeArgs& a, int grid, hipStream_t s);
 void launch_shape_apply(const ShapeApplyArgs& a, int grid, hipStream_t s);
-// validate: the cell table is not cleared (batched mode); an entry is used
-// only if keys[start] equals the looked-up key.  block: 64, 128 or 256.
-void launch_pp_apply(const PPArgs& a, bool diag, bool validate, int block, hipStream_t s);
+// batched: cell ranges come from the row table.  block: 64, 128 or 256.
+void launch_pp_apply(const PPArgs& a, bool diag, bool batched, int block, hipStream_t s);
 void launch_restitution(const RestArgs& a, int grid, hipStream_t s);
 void launch_det_keys(const DetKeyArgs& a, hipStream_t s);
 void launch_det_box_ranges(int cmax, const uint32_t* key2s, uint32_t pad, int idbits, int* seg, hipStream_t s);

@@ -111,3 +111,13 @@ def test_sort_pairs_rejects_bad_arguments_without_a_gpu():
     assert lib.xpbd_sort_pairs(None, None, None, None, 16, 24, None) == 1
     assert b"null buffer" in lib.xpbd_last_error(None)
     assert lib.xpbd_sort_pairs(None, None, None, None, 0, 24, None) == 0   # nothing to sort
+
+
+def test_init_batch_rejects_2049_scenes_without_a_gpu():
+    """A batched solver takes at most 2,048 scenes: the batched key holds the
+    scene in the 11 bits above the 21-bit cell.  The check comes before any
+    device work."""
+    import vkxpbd
+    s = vkxpbd.XpbdSolver()
+    with pytest.raises(RuntimeError, match="at most 2048 scenes"):
+        s.init_batch(vkxpbd.SolverConfig(), np.ones(2049, np.int64))

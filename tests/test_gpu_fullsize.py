@@ -88,15 +88,22 @@ def test_pile_1m_icosphere_first_ten_frames(vk):
 
 def test_batch_256_piles_full_size(vk):
     """The 256-scene batch (counts log-uniform on [1024, 65536], 3,813,381
-    particles): two frames of the batched solver equal 256 plain solvers bit
-    for bit, and the smallest, the largest and two seeded scenes equal the
-    oracle bit for bit after the second frame (plane and particle contacts)."""
+    particles): the batched solver's cell ranges take under 0.5 GiB (a row
+    table of 8 bytes per grid row and group bounds of at most 36 bytes per
+    particle, not a cell table per scene); after autotuning, two frames of it equal
+    256 plain solvers bit for bit, and the smallest, the largest and two
+    seeded scenes equal the oracle bit for bit after the second frame (plane
+    and particle contacts)."""
     sc = scenes.batch_of_piles()
     counts = np.asarray(sc["counts"])
     off = np.concatenate([[0], np.cumsum(counts)])
     log(f"batch: {counts.size} scenes, {int(off[-1])} particles")
     assert counts.size == 256 and off[-1] > 3_000_000
     bs = scenes.build_batch_solver(vk, sc)
+    log(f"batched cell-range tables: {bs.cell_table_bytes / 2**20:.1f} MiB")
+    assert bs.cell_table_bytes < 512 * 1024 * 1024
+    block = bs.autotune(1, DT, 8, 1)
+    log(f"autotune: particle-pair block {block}, radix tile {bs.radix_tile} keys")
     singles = [scenes.build_solver(vk, scenes.scene_slice(sc, k)) for k in range(counts.size)]
     picks = sorted({int(np.argmin(counts)), int(np.argmax(counts))} |
                    set(int(k) for k in np.random.default_rng(3).choice(counts.size, 2, replace=False)))

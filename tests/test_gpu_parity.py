@@ -625,10 +625,64 @@ def test_batched_scenes_match_sequential(vk, num_scenes, lo, hi, frames):
             np.testing.assert_array_equal(gp[off[k]:off[k + 1]], s.get_positions(), err_msg=f"scene {k} frame {f}")
             np.testing.assert_array_equal(gv[off[k]:off[k + 1]], s.get_velocities(), err_msg=f"scene {k} frame {f}")
         assert bs.last_contact_count == tot
+    # the cell ranges: a row table and group bounds, not a cell table per scene
+    assert bs.cell_table_bytes < 64 * 1024 * 1024
     # scenes overlap in space but never interact: keys carry the scene index
     keys = bs.get_sorted_cells()
     assert np.all(np.diff(keys.astype(np.int64)) >= 0)
     assert int(keys.max()) >> 21 == num_scenes - 1
+
+
+def test_2048_scenes_use_every_scene_bit(vk):
+    """The largest batch (2,048 scenes, keys up to 2^32 - 1): the first, a
+    middle and the last scenes equal plain solvers bit for bit, and the top
+    scene's cells sort last."""
+    num_scenes = 2048
+    sc = scenes.batch_of_piles(seed=17, num_scenes=num_scenes, lo=8, hi=24)
+    off = np.concatenate([[0], np.cumsum(sc["counts"])])
+    bs = scenes.build_batch_solver(vk, sc)
+    assert bs.scene_count == num_scenes
+    picks = [0, 1, 1023, 2046, 2047]
+    singles = {k: scenes.build_solver(vk, scenes.scene_slice(sc, k)) for k in picks}
+    for f in range(3):
+        bs.step(1.0 / 60.0, 8, 1)
+        for s in singles.values():
+            s.step(1.0 / 60.0, 8, 1)
+        gp, gv = bs.get_positions(), bs.get_velocities()
+        for k, s in singles.items():
+            np.testing.assert_array_equal(gp[off[k]:off[k + 1]], s.get_positions(), err_msg=f"scene {k} frame {f}")
+            np.testing.assert_array_equal(gv[off[k]:off[k + 1]], s.get_velocities(), err_msg=f"scene {k} frame {f}")
+    keys = bs.get_sorted_cells()
+    assert np.all(np.diff(keys.astype(np.int64)) >= 0)
+    assert int(keys.max()) >> 21 == num_scenes - 1
+    log(f"2048 scenes, {int(off[-1])} particles, cell tables {bs.cell_table_bytes / 2**20:.1f} MiB")
+
+
+def test_every_tuning_gives_the_same_bits(vk):
+    """Each launch shape autotune may pick (pp block 64/128/256 x radix tile
+    1,024/2,048 keys) gives the default's bits: full frames (captured), then
+    a step_solve_only over the last contact set."""
+    sc = scenes.pile_4096()
+    ref = scenes.build_solver(vk, sc)
+    for _ in range(6):
+        ref.step(1.0 / 60.0, 8, 1)
+    ref.step_solve_only(1.0 / 60.0, 2, 1)
+    want_x, want_v = ref.get_positions(), ref.get_velocities()
+    for block in (64, 128, 256):
+        for tile in (1024, 2048):
+            s = scenes.build_solver(vk, sc)
+            s.set_tuning(block, tile)
+            assert (s.pp_block, s.radix_tile) == (block, tile)
+            for _ in range(6):
+                s.step(1.0 / 60.0, 8, 1)
+            s.step_solve_only(1.0 / 60.0, 2, 1)
+            log(f"  block {block} tile {tile}: max|dx| {float(np.max(np.abs(s.get_positions() - want_x))):.3e}")
+            np.testing.assert_array_equal(s.get_positions(), want_x, err_msg=f"block {block} tile {tile}")
+            np.testing.assert_array_equal(s.get_velocities(), want_v, err_msg=f"block {block} tile {tile}")
+    with pytest.raises(Exception, match="pp_block"):
+        ref.set_tuning(32, 1024)
+    with pytest.raises(Exception, match="radix_tile"):
+        ref.set_tuning(64, 4096)
 
 
 def test_autotune_keeps_state(vk):
