@@ -20,7 +20,9 @@ objects cover the other measured rows: "batched" (256 scenes of 1,024-65,536
 particles in one batched solver vs 256 plain solvers stepped in sequence,
 same frames, final states compared), "adjoint" (differentiable rollout of
 32,768 particles over 64 substeps with checkpoints every 8: forward and
-reverse times and their ratio) and "host_io" (get_positions at 1M).
+reverse times and their ratio), "host_io" (get_positions at 1M) and
+"animated_collider" (update_mesh of the 81,920-triangle icosphere, then a
+frame).
 """
 import argparse
 import json
@@ -346,6 +348,18 @@ def main():
         a0 = time.perf_counter()
         s.get_positions()
         io.append(time.perf_counter() - a0)
+    # animated collider: update_mesh of the icosphere, moved a little each
+    # call (vertex upload, device refit, grid rebuilds), median of ten; then
+    # one frame, which replays the captured frame graph
+    um = []
+    for k in range(10):
+        v = (verts + np.float32([0.0, 0.0, 1e-4 * (k + 1)])).astype(np.float32)
+        a0 = time.perf_counter()
+        s.update_mesh(0, v.reshape(-1))
+        um.append(time.perf_counter() - a0)
+    a0 = time.perf_counter()
+    s.step(DT, SUBSTEPS, 1)
+    frame_after_update = time.perf_counter() - a0
     del s
 
     # per-frame times over the whole 60-frame fall (graph replays)
@@ -380,6 +394,9 @@ def main():
                       "41-60": round(float(ft[40:60].mean()), 4), "1-60": round(float(ft.mean()), 4),
                       "particle_substeps_per_s_1_60": N * SUBSTEPS / (float(ft.mean()) * 1e-3)},
         "host_io": {"get_positions_ms": round(1e3 * float(np.median(io)), 3), "particles": N},
+        "animated_collider": {"update_mesh_ms": round(1e3 * float(np.median(um)), 3),
+                              "triangles": int(tris.shape[0]),
+                              "frame_after_update_ms": round(1e3 * frame_after_update, 3)},
     }
 
     if not args.no_profile:

@@ -182,8 +182,6 @@ void lbvh_refit(MeshBvhHost& b, const float* pts, const uint32_t* tris) {
     if (!b.lo.empty()) refit_node(b, pts, tris, b.root);
 }
 
-namespace {
-struct RayAxes { int kx, ky, kz; float Sx, Sy; };
 RayAxes parity_axes() {
     const float dir[3] = {1.029700994f, 1.086472273f, 1.034694910f};
     const float ad[3] = {std::fabs(dir[0]), std::fabs(dir[1]), std::fabs(dir[2])};
@@ -195,7 +193,6 @@ RayAxes parity_axes() {
     if (dir[kz] < 0.0f) std::swap(kx, ky);
     return RayAxes{kx, ky, kz, dir[kx] / dir[kz], dir[ky] / dir[kz]};
 }
-}  // namespace
 
 void dist_grid_build(const float* pts, const uint32_t* tris, uint32_t nt, float cap, DistGrid& g, const MeshBvhHost* bvh) {
     g = DistGrid{};
@@ -223,6 +220,7 @@ void dist_grid_build(const float* pts, const uint32_t* tris, uint32_t nt, float 
                            uint32_t(std::ceil((hi[2] - lo[2]) / h)) + 1};
     g.ox = float(lo[0]); g.oy = float(lo[1]); g.oz = float(lo[2]);
     g.inv = float(1.0 / h);
+    g.h = h;
     g.cap = cap;
     g.nx = n[0]; g.ny = n[1]; g.nz = n[2];
     // cell (i, j, k) covers [o + c*h', o + (c+1)*h'] with h' = 1/inv as the

@@ -43,6 +43,11 @@ struct ParityGrid {
     std::vector<uint32_t> start, tri;
 };
 
+// Projection of the parity ray: the axes (kz the dominant direction) and the
+// shears of generate_mesh_contacts.comp's watertight ray test, in float.
+struct RayAxes { int kx, ky, kz; float Sx, Sy; };
+RayAxes parity_axes();
+
 // pts/tris: vertex xyz and global vertex indices; tri_base: global id of the
 // mesh's first triangle (stored in the grid).
 void parity_grid_build(const float* pts, const uint32_t* tris, uint32_t num_tris, uint32_t tri_base, ParityGrid& g);
@@ -60,6 +65,7 @@ void parity_grid_build(const float* pts, const uint32_t* tris, uint32_t num_tris
 // search from the root, whose other branches hold no triangle within reach.
 struct DistGrid {
     float ox = 0.f, oy = 0.f, oz = 0.f, inv = 0.f, cap = 0.f;
+    double h = 0.0;                // cell edge the dimensions were derived with
     uint32_t nx = 0, ny = 0, nz = 0;
     std::vector<float> lb;
     std::vector<uint32_t> start;   // local node index per cell (with a BVH)

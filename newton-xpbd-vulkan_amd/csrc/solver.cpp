@@ -340,28 +340,38 @@ void Solver::build_query_grids(size_t c) {
     dist_grid_build(all_verts_.data(), tris, nt, cap, dgrid_[c], &bvh_[c]);
 }
 
-// Concatenates the per-mesh grids (parity: one cell-start table with each
-// mesh's cells at pg_cell_off and one triangle list; bounds: one array with
-// each mesh's cells at lb_off) and refreshes the mesh descriptors.
+// Concatenates the per-mesh grids and refreshes the mesh descriptors.  Each
+// mesh gets a block of room: parity cell starts (dim^2 + 1, global entry
+// positions) and an entry segment of twice its entries plus 1,024; bounds and
+// start nodes for twice its cells plus 64.  update_mesh rebuilds a mesh's
+// grids inside its blocks on the device.
 void Solver::upload_query_grids() {
-    std::vector<uint32_t> start, tri, lbn;
+    const size_t M = pgrid_.size();
+    std::vector<uint32_t> start, lbn;
     std::vector<float> lb;
     std::vector<float4> psoup;
-    for (size_t c = 0; c < pgrid_.size(); c++) {
+    lb_room_.assign(M, 0);
+    pg_base_.assign(M, 0);
+    pg_room_.assign(M, 0);
+    for (size_t c = 0; c < M; c++) {
         const ParityGrid& g = pgrid_[c];
         info_h_[c].pg_u0 = g.u0;
         info_h_[c].pg_w0 = g.w0;
         info_h_[c].pg_inv = g.inv;
         info_h_[c].pg_dim = g.dim;
         info_h_[c].pg_cell_off = uint32_t(start.size());
-        const uint32_t base = uint32_t(tri.size());
-        for (size_t k = 0; k + 1 < g.start.size(); k++) start.push_back(base + g.start[k]);
-        tri.insert(tri.end(), g.tri.begin(), g.tri.end());
+        const uint32_t base = uint32_t(psoup.size() / 3);
+        pg_base_[c] = base;
+        for (const uint32_t v : g.start) start.push_back(base + v);
         for (uint32_t t : g.tri)   // the registered triangles' corners, in cell order
             for (int q = 0; q < 3; q++) {
                 const uint32_t v = all_tris_[3 * size_t(t) + q];
                 psoup.push_back(make_float4(all_verts_[3 * v], all_verts_[3 * v + 1], all_verts_[3 * v + 2], 0.f));
             }
+        if (g.dim > 0) {
+            pg_room_[c] = 2 * g.tri.size() + 1024;
+            psoup.resize(3 * (base + pg_room_[c]), make_float4(0.f, 0.f, 0.f, 0.f));
+        }
         const DistGrid& d = dgrid_[c];
         info_h_[c].lb_ox = d.ox; info_h_[c].lb_oy = d.oy; info_h_[c].lb_oz = d.oz; info_h_[c].lb_inv = d.inv;
         info_h_[c].lb_nx = d.nx; info_h_[c].lb_ny = d.ny; info_h_[c].lb_nz = d.nz;
@@ -370,8 +380,13 @@ void Solver::upload_query_grids() {
         lb.insert(lb.end(), d.lb.begin(), d.lb.end());
         for (size_t k = 0; k < d.lb.size(); k++)   // start nodes as stack words
             lbn.push_back(node_word(c, k < d.start.size() ? d.start[k] : bvh_[c].root));
+        if (!d.lb.empty()) {
+            lb_room_[c] = 2 * d.lb.size() + 64;
+            lb.resize(info_h_[c].lb_off + lb_room_[c], 0.f);
+            lbn.resize(info_h_[c].lb_off + lb_room_[c], 0u);
+        }
     }
-    start.push_back(uint32_t(tri.size()));
+    start.push_back(uint32_t(psoup.size() / 3));
     upload(pg_start_, start.data(), start.size());
     upload(pg_soup_, psoup.data(), psoup.size());
     upload(lb_, lb.data(), lb.size());
@@ -507,6 +522,7 @@ void Solver::finalize_meshes() {
     dgrid_.assign(M, DistGrid{});
     for (size_t c = 0; c < M; c++) build_query_grids(c);
     upload_query_grids();
+    prepare_mesh_updates();
     dalloc(mcnt_, 4 * std::max<size_t>(M, 1));
     // watertight meshes walk with four lanes per particle (VKXPBD_MESH_LANES=1: one)
     const char* lenv = std::getenv("VKXPBD_MESH_LANES");
@@ -562,26 +578,6 @@ void Solver::set_particles(const float* q, const float* qd, const float* w, cons
     }
 }
 
-void Solver::update_mesh(size_t mi, const float* v, size_t nv, const float* vel) {
-    if (!finalized_) throw std::runtime_error("update_mesh: call finalize_meshes() first");
-    if (mi >= meshes_.size()) throw std::runtime_error("update_mesh: mesh_index out of range");
-    if (nv != meshes_[mi].verts.size() / 3) throw std::runtime_error("update_mesh: vertex count mismatch");
-    DeviceGuard g(device_);
-    HIPCHECK(hipStreamSynchronize(stream_));
-    invalidate_graphs();
-    const size_t vs = vert_off_[mi];
-    std::memcpy(all_verts_.data() + 3 * vs, v, nv * 3 * sizeof(float));
-    if (vel) std::memcpy(all_vels_.data() + 3 * vs, vel, nv * 3 * sizeof(float));
-    else std::fill(all_vels_.begin() + 3 * vs, all_vels_.begin() + 3 * (vs + nv), 0.f);
-    std::memcpy(meshes_[mi].verts.data(), v, nv * 3 * sizeof(float));
-    lbvh_refit(bvh_[mi], all_verts_.data(), all_tris_.data() + 3 * tri_off_[mi]);
-    build_query_grids(mi);
-    upload_query_grids();
-    HIPCHECK(hipMemcpy(verts_d_.p + 3 * vs, v, nv * 3 * sizeof(float), hipMemcpyHostToDevice));
-    HIPCHECK(hipMemcpy(vvel_d_.p + 3 * vs, all_vels_.data() + 3 * vs, nv * 3 * sizeof(float), hipMemcpyHostToDevice));
-    upload_nodes();
-    upload_tri_soup();
-}
 
 BodyArgs Solver::body_args() const {
     BodyArgs b;
@@ -1451,10 +1447,22 @@ size_t Solver::shape_contact_diag_len() const {
     if (!initialized_) return 0;
     return cfg_.enable_diagnostics ? size_t(4) * cmax_ * 32 : 32;
 }
+// Boxes from the device (update_mesh refits there), packed words from the
+// host topology (mesh-local, as the reference returns them).
 void Solver::bvh_half(size_t mi, bool upper, float* out, size_t cap) const {
     if (mi >= bvh_.size()) return;
     const std::vector<NodeHalf>& v = upper ? bvh_[mi].hi : bvh_[mi].lo;
-    std::memcpy(out, v.data(), std::min(cap, v.size()) * sizeof(NodeHalf));
+    const size_t n = std::min(cap, v.size());
+    if (n == 0) return;
+    std::vector<float4> nodes(2 * n);
+    HIPCHECK(hipStreamSynchronize(stream_));
+    HIPCHECK(hipMemcpy(nodes.data(), nodes_d_.p + 2 * node_off_[mi], 2 * n * sizeof(float4), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n; i++) {
+        NodeHalf h = v[i];
+        const float4 f = nodes[2 * i + (upper ? 1 : 0)];
+        h.x = f.x; h.y = f.y; h.z = f.z;
+        std::memcpy(out + 4 * i, &h, sizeof(NodeHalf));
+    }
 }
 
 void Solver::add_box_body(const float* pos, const float* quat, const float* half, float density, int flags) {

@@ -139,6 +139,13 @@ private:
     void upload_tri_soup();
     uint32_t node_word(size_t mesh, uint32_t node) const;
     void upload_query_grids();
+    // animated colliders (mesh_update.cpp): static tables for refitting and
+    // rebuilding a mesh's derived data on the device; the in-place update
+    // (false when a grid outgrew its room); the host rebuild of every mesh's
+    // derived data (finalize's tail, after a refit of every BVH)
+    void prepare_mesh_updates();
+    bool update_mesh_on_device(size_t mesh);
+    void rebuild_meshes_on_host();
     BodyArgs body_args() const;
     void substep(float sdt, int iterations, bool collide, int slot, bool zero_next);
     BodyStepArgs body_step_args(float sdt);
@@ -307,6 +314,17 @@ private:
     DevBuf<MeshInfo> info_d_;
     std::vector<MeshInfo> info_h_;
     float avg_edge_ = 0.f;
+    // animated colliders: room per mesh in the bound grid (cells) and the
+    // parity entries (first entry, capacity); refit order (global nodes by
+    // height, per mesh the offsets of its heights); per global node its
+    // leaf-order range and stack word; per global triangle its leaf position;
+    // per mesh the vertices its triangles use; scratch for the rebuilds
+    std::vector<size_t> lb_room_, pg_base_, pg_room_;
+    DevBuf<uint32_t> refit_list_d_;
+    std::vector<std::vector<uint32_t>> refit_lvl_;
+    DevBuf<uint32_t> nfirst_d_, nlast_d_, nword_d_, tpos_d_;
+    std::vector<std::vector<uint32_t>> mesh_vref_;
+    DevBuf<uint32_t> span_lo_d_, span_hi_d_, pcnt_d_, ptotal_d_;
     // diagnostics
     DevBuf<uint32_t> diag_header_;
     DevBuf<uint64_t> mesh_stats_;

@@ -292,6 +292,45 @@ void launch_particle_box_contacts(const ParticleBoxArgs& a, hipStream_t s);
 void launch_body_contact_solve(const BodySolveArgs& a, hipStream_t s);
 
 void launch_integrate(const IntegrateArgs& a, hipStream_t s);
+// Animated colliders (k_meshupd.hip): refit of one height's nodes (with the
+// walk records and leaf boxes), triangle corners in leaf order, and the
+// distance-bound and parity grids of one mesh rebuilt in place.
+struct MeshRefitArgs {
+    float4* nodes; float4* rec; float4* lbox; float4* tsoup;
+    const uint32_t* prim; const uint32_t* tris; const float* verts;
+    uint32_t nprim;
+};
+struct DistGridArgs {
+    int tri0, ntri;                       // the mesh's global triangles
+    const uint32_t* tris; const float* verts;
+    const uint32_t* tpos;                 // global triangle -> global leaf position
+    double ox, oy, oz, hc;                // origin and cell edge (as the bound lookups compute cells)
+    uint32_t nx, ny, nz;
+    float cap;
+    float* lb; uint32_t* slo; uint32_t* shi; uint32_t* lb_node;   // the mesh's cells
+    const float4* nodes; const uint32_t* nfirst; const uint32_t* nlast; const uint32_t* nword;
+    uint32_t root;                        // global root node
+};
+struct ParGridArgs {
+    int tri0, ntri;
+    const uint32_t* tris; const float* verts;
+    int kx, ky, kz;
+    double Sx, Sy, pad;
+    float u0, w0, inv;
+    uint32_t dim;
+    uint32_t* cnt;                        // dim^2 counters (zeroed before the count)
+    uint32_t* start;                      // the mesh's dim^2 + 1 cell starts
+    uint32_t base;                        // the mesh's first entry
+    uint32_t* total;
+    float4* soup;
+};
+void launch_refit_level(const MeshRefitArgs& a, const uint32_t* list, int count, hipStream_t s);
+void launch_tri_soup(const MeshRefitArgs& a, uint32_t p0, int count, hipStream_t s);
+void launch_dist_grid(const DistGridArgs& a, hipStream_t s);
+void launch_par_count_scan(const ParGridArgs& a, hipStream_t s);
+void launch_par_fill(const ParGridArgs& a, hipStream_t s);
+void launch_zero_u32(uint32_t* p, size_t n, hipStream_t s);
+
 void launch_plane_contacts(const PlaneArgs& a, hipStream_t s);
 void launch_cell_ranges(int n, const uint32_t* keys, int* range, uint32_t* rkeys, hipStream_t s);
 // batched solvers: the row table of the sorted keys (tile scratch of
