@@ -547,9 +547,10 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6, 8))) v
         return !((dx * dx + dy * dy) + dz * dz > lim2);
     };
     // the scan needs positions only (12-byte loads; the radius enters
-    // through r_max)
+    // through r_max), addressed by a 32-bit byte offset from the array base
+    // (16 bytes per slot, at most 2^28 slots)
     auto pos = [&](int k) -> f3 {
-        const float3 p = *reinterpret_cast<const float3*>(a.Xs + k);
+        const float3 p = *reinterpret_cast<const float3*>(reinterpret_cast<const char*>(a.Xs) + (uint32_t(k) << 4));
         return f3{p.x, p.y, p.z};
     };
     f3 acc = {0.f, 0.f, 0.f};
@@ -571,24 +572,29 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6, 8))) v
     };
     // a lane about to overflow drains together with every lane of the wave
     // at the same loop point, so the projections stay in lockstep
-    auto consider = [&](int k, f3 Pj) {
-        if (k != t && near(Pj)) s_surv[cnt++][threadIdx.x] = uint32_t(k);
-    };
     auto room = [&](int need) {
         if (__any(cnt > kSurv - need)) drain();
     };
     auto scan = [&](int s0, int e0) {
         // SB loads issue before the tests; the last batch of a run reads its
         // last slot in place of the slots past the run and tests only those
-        // inside it (one round trip instead of one per remaining candidate)
+        // inside it (one round trip instead of one per remaining candidate).
+        // Each candidate is written to the next column slot unconditionally
+        // and kept (cnt advanced) if it is inside the run, not the particle
+        // itself and near: no branch per candidate.  room() leaves SB free
+        // slots, so the write stays inside the column.
         for (int k = s0; k < e0; k += SB) {
             f3 P[SB];
 #pragma unroll
             for (int j = 0; j < SB; j++) P[j] = pos(min(k + j, e0 - 1));
             room(SB);
 #pragma unroll
-            for (int j = 0; j < SB; j++)
-                if (k + j < e0) consider(k + j, P[j]);
+            for (int j = 0; j < SB; j++) {
+                const int kk = k + j;
+                const bool keep = (kk < e0) & (kk != t) & near(P[j]);
+                s_surv[cnt][threadIdx.x] = uint32_t(kk);
+                cnt += keep ? 1 : 0;
+            }
         }
     };
     // One x-run of a row: cells [cx, cx + len) inside one wrap, hence
