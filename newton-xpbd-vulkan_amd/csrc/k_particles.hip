@@ -538,13 +538,14 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6, 8))) v
     auto raw = [&](uint32_t base, int cell) -> int2 { return range[size_t(base) + size_t(cell)]; };
     // cheap reject before the refined length: a pair the exact test accepts
     // has |rd| <= (r_i + r_j + coh)(1 + 5 ulp) <= (r_i + r_max + |coh|)(1 + 5 ulp)
-    // (r_max bounds every radius), and the plain squared length is within
-    // 3 ulp of |rd|^2, far inside the 1e-4 relative slack
+    // (r_max bounds every radius), and this squared length (the refined
+    // length's own fma sum) is within 2 ulp of |rd|^2, far inside the 1e-4
+    // relative slack
     const float lim = rad + a.rmax + fabsf(a.coh);
     const float lim2 = lim * lim * 1.0001f + 1e-30f;
     auto near = [&](f3 Pj) -> bool {
         const float dx = x.x - Pj.x, dy = x.y - Pj.y, dz = x.z - Pj.z;
-        return !((dx * dx + dy * dy) + dz * dz > lim2);
+        return !(fma_(dz, dz, fma_(dx, dx, dy * dy)) > lim2);
     };
     // the scan needs positions only (12-byte loads; the radius enters
     // through r_max), addressed by a 32-bit byte offset from the array base
