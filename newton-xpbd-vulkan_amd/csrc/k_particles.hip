@@ -412,12 +412,13 @@ __global__ __launch_bounds__(256) void k_shape_solve(ShapeArgs a) {
 template <bool DIAG>
 __device__ __forceinline__ void pp_pair(const PPArgs& a, int t, int k, float4 Pj, float4 Vj, int nf, int i, int proxy, f3 x,
                                         f3 v, float rad, float w1, float R, f3& acc) {
-    if (k == t) return;   // j == i
-    if ((nf & 1) == 0) return;
-    if (proxy != 0 && (((nf & 2) != 0) || (Vj.w <= 0.0f))) return;
+    // the first four tests in one branch, so the flags and both state loads
+    // are in flight together
     const f3 rd = x - xyz(Pj);
-    if (fabsf(rd.x) > R || fabsf(rd.y) > R || fabsf(rd.z) > R) return;
-    const RootInv dl = exact_length_inv(rd);
+    const int skip = int(k == t) | int((nf & 1) == 0) | (int(proxy != 0) & (int((nf & 2) != 0) | int(Vj.w <= 0.0f))) |
+                     int(fabsf(rd.x) > R) | int(fabsf(rd.y) > R) | int(fabsf(rd.z) > R);
+    if (skip) return;
+    const RootInv dl = exact_length_inv_fast(rd);
     const float d = dl.d;
     const float err = d - rad - Pj.w;
     if (err > a.coh || d <= 0.0f) return;
@@ -429,13 +430,13 @@ __device__ __forceinline__ void pp_pair(const PPArgs& a, int t, int k, float4 Pj
     const float lambda_n = err;
     const float vn = exact_dot(nn, vrel);
     const f3 vt = {fma_(vn, -nn.x, vrel.x), fma_(vn, -nn.y, vrel.y), fma_(vn, -nn.z, vrel.z)};
-    const RootInv vl = exact_length_inv(vt);
+    const RootInv vl = exact_length_inv_fast(vt);
     const float vt_len = vl.d;
     const float lambda_f = gmax(a.mu * lambda_n, -vt_len * a.dt);
     const f3 vt_dir = vt_len > 0.0f ? exact_div_by(vt, vt_len, vl.inv) : f3{0.f, 0.f, 0.f};
     const f3 p2 = nn * err;
     const f3 df = {fma_(vt_dir.x, lambda_f, -p2.x), fma_(vt_dir.y, lambda_f, -p2.y), fma_(vt_dir.z, lambda_f, -p2.z)};
-    const f3 pair_d = exact_div(df, denom);
+    const f3 pair_d = exact_div_by(df, denom, recip_fast(denom));
     acc = acc + pair_d;
     if (DIAG) {
         const uint32_t slot = atomicAdd(a.diag_header, 1u);
@@ -669,6 +670,67 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6, 8))) v
             cy = cy1; cz = cz1;
             cy1 = cy2; cz1 = cz2;
             advance(cy2, cz2);
+        }
+    } else if ((xe - xs <= 2) & (ye - ys <= 2) & (ze - zs <= 2) & !clamp0) {
+        // A query box of at most 3 x 3 x 3 cells (every query when R is at
+        // most one cell edge).  A row's cells [xs, xe] are one run of sorted
+        // slots, or two when the row crosses an x wrap (piece a: the first
+        // len0 cells, piece b: the rest, in the next wrap's table); all rows
+        // split at the same cell.  A piece's run is [last end - total length,
+        // last end) of its cells (unoccupied cells read (0, 0)).  Each z slab
+        // is scanned slab by slab as two streams of at most three runs --
+        // (row 0, row 1, row 2) and nothing, or (a0, b0, a1) and (b1, a2, b2)
+        // when the rows split -- in batches of SB: flat index f maps to its
+        // run's slot by two compares against the runs' cumulative lengths.
+        // The visiting order, hence every sum, is the general path's.
+        const int nxc = xe - xs + 1;
+        const bool split = len0 < nxc;
+        auto stream = [&](int2 u0, int2 u1, int2 u2) {   // runs as (first slot, length)
+            const int c1 = u0.y, c2 = c1 + u1.y, c3 = c2 + u2.y;
+            const int b0 = u0.x, b1 = u1.x - c1, b2 = u2.x - c2;
+            for (int f = 0; f < c3; f += SB) {
+                int kk[SB];
+                f3 P[SB];
+#pragma unroll
+                for (int j = 0; j < SB; j++) {
+                    const int fj = f + j;
+                    const int k = (fj < c1 ? b0 : (fj < c2 ? b1 : b2)) + fj;
+                    kk[j] = fj < c3 ? k : t;
+                    P[j] = pos(kk[j]);
+                }
+                room(SB);
+#pragma unroll
+                for (int j = 0; j < SB; j++) {
+                    const bool keep = (kk[j] != t) & near(P[j]);
+                    s_surv[cnt][threadIdx.x] = uint32_t(kk[j]);
+                    cnt += keep ? 1 : 0;
+                }
+            }
+        };
+        for (int dz = 0; dz < 3; dz++) {
+            if (zs + dz > ze) break;
+            const int cz = zs + dz;
+            int2 pa[3], pb[3];   // per row: pieces a and b as (first slot, length)
+#pragma unroll
+            for (int dy = 0; dy < 3; dy++) {
+                const bool in = ys + dy <= ye;
+                const int cy = in ? ys + dy : ys;
+                const int ca = cell_index(xs, cy, cz), cb = cell_index(xs + len0, cy, cz);
+                const uint32_t ta = table(xs, cy, cz), tb = table(xs + len0, cy, cz);
+                int la = 0, ea = 0, lb = 0, eb = 0;
+#pragma unroll
+                for (int m = 0; m < 3; m++) {
+                    const bool inb = m >= len0;
+                    const int2 rg = m < nxc ? raw(inb ? tb : ta, inb ? cb + (m - len0) : ca + m) : make_int2(0, 0);
+                    const int l = rg.y - rg.x;
+                    if (inb) { lb += l; eb = max(eb, rg.y); } else { la += l; ea = max(ea, rg.y); }
+                }
+                pa[dy] = in ? make_int2(ea - la, la) : make_int2(0, 0);
+                pb[dy] = in ? make_int2(eb - lb, lb) : make_int2(0, 0);
+            }
+            const int2 z0 = make_int2(0, 0);
+            stream(pa[0], split ? pb[0] : pa[1], split ? pa[1] : pa[2]);
+            stream(split ? pb[1] : z0, split ? pa[2] : z0, split ? pb[2] : z0);
         }
     } else {
         // Each lane walks its x-runs -- rows (cz, cy) in the reference's

@@ -120,6 +120,64 @@ __device__ __forceinline__ RootInv exact_length_inv(f3 a) {
     float sq = fma_(a.z, a.z, xx_yy);
     return exact_sqrt_inv(sq);
 }
+// Shorter sequences for the correctly rounded reciprocal and square root on
+// gfx950, bit-identical to the compiler's 1.0f / d and sqrtf(a) (built with
+// -fhip-fp32-correctly-rounded-divide-sqrt) inside the ranges named below:
+// both were compared with them on every float of those ranges on an MI355X.
+// The compiler's sequences add operand scaling for subnormal and extreme
+// exponents and special-value fix-ups that are identities inside them.
+//  * rcp_cr: v_rcp_f32 and one fma Newton step, for |d| in [2^-126, 2^126).
+//  * sqrt_cr: v_sqrt_f32, then the neighbour whose fma residual brackets a
+//    (the compiler's own correction), for a = +0 and a in [2^-96, +inf).
+__device__ __forceinline__ float rcp_cr(float d) {
+    const float r = __builtin_amdgcn_rcpf(d);
+    return fma_(fma_(-d, r, 1.0f), r, r);
+}
+__device__ __forceinline__ bool rcp_cr_ok(float d) {
+    const float m = fabsf(d);
+    return m >= 0x1p-126f && m < 0x1p126f;
+}
+__device__ __forceinline__ float sqrt_cr(float a) {
+    const float s = __builtin_amdgcn_sqrtf(a);
+    const float sdn = __uint_as_float(__float_as_uint(s) - 1u);
+    const float sup = __uint_as_float(__float_as_uint(s) + 1u);
+    const float t = fma_(-sdn, s, a) <= 0.0f ? sdn : s;
+    return fma_(-sup, s, a) > 0.0f ? sup : t;
+}
+__device__ __forceinline__ bool sqrt_cr_ok(float a) { return a == 0.0f || (a >= 0x1p-96f && a < __builtin_inff()); }
+// exact_sqrt_inv with the sequences above where they apply (every a the
+// contact solve meets but subnormal, infinite or NaN squared lengths); the
+// same values bit for bit: for a = 0 the reciprocal of the root is +inf, as
+// the division gives, and the correction then runs the same operations.
+__device__ __forceinline__ RootInv exact_sqrt_inv_fast(float a) {
+    if (__builtin_expect(!sqrt_cr_ok(a), 0)) return exact_sqrt_inv(a);
+    const float d0 = sqrt_cr(a);
+    const float i0 = d0 == 0.0f ? __builtin_inff() : rcp_cr(d0);
+    const float r = fma_(-d0, d0, a);
+    const float d = fma_(r, 0.5f * i0, d0);
+    float inv = i0;
+    if (d != d0) {
+        float dd = d;
+        asm volatile("" : "+v"(dd));
+        inv = 1.0f / dd;
+    }
+    return {d, inv};
+}
+__device__ __forceinline__ RootInv exact_length_inv_fast(f3 a) {
+    float yy = a.y * a.y;
+    float xx_yy = fma_(a.x, a.x, yy);
+    float sq = fma_(a.z, a.z, xx_yy);
+    return exact_sqrt_inv_fast(sq);
+}
+// 1.0f / d, correctly rounded
+__device__ __forceinline__ float recip_fast(float d) {
+    if (__builtin_expect(!rcp_cr_ok(d), 0)) {
+        float dd = d;
+        asm volatile("" : "+v"(dd));
+        return 1.0f / dd;
+    }
+    return rcp_cr(d);
+}
 __device__ __forceinline__ float exact_div1(float v, float d) {
     float inv = 1.0f / d;
     float q = v * inv;
