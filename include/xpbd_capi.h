@@ -340,6 +340,17 @@ int xpbd_sort_pairs(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* 
 int xpbd_sort_pairs_noalloc(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out,
                             uint32_t* vals_out, size_t n, int key_bits, void* stream);
 
+/* Self-check of the device arithmetic the contact projection relies on; no
+ * counterpart in the reference.  The solver takes correctly rounded
+ * reciprocals and square roots through shorter gfx950 sequences (v_rcp_f32
+ * with one Newton step; v_sqrt_f32 with a neighbour correction) inside the
+ * ranges where they equal the compiler's correctly rounded 1.0f / d and
+ * sqrtf(a).  This compares them there on every float, on the current HIP
+ * device: out3[0] = reciprocal mismatches over |d| in [2^-126, 2^126),
+ * out3[1] = square-root mismatches over a in [2^-96, +inf), out3[2] = at
+ * a = +0.  All three are zero on a device where the solver's results hold. */
+int xpbd_check_arith(uint64_t* out3);
+
 #ifdef __cplusplus
 }
 #endif

@@ -299,6 +299,18 @@ int xpbd_sort_pairs(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* 
     }
 }
 
+int xpbd_check_arith(uint64_t* out3) {
+    try {
+        if (!out3) throw std::runtime_error("xpbd_check_arith: null output");
+        xpbd::check_arith(out3, nullptr);
+        g_err.clear();
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return 1;
+    }
+}
+
 int xpbd_sort_pairs_noalloc(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out,
                             uint32_t* vals_out, size_t n, int key_bits, void* stream) {
     try {

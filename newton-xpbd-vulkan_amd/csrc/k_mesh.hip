@@ -1,23 +1,23 @@
 // k_mesh.hip -- particle-vs-triangle-mesh narrowphase over the packed LBVH,
 // the replacement of shaders/generate_mesh_contacts.comp:181-414.
 //
-// Three launches per mesh, one block-level slot reservation per block for the
+// Two launches per mesh, one block-level slot reservation per block for the
 // contacts emitted (same-address global atomics serialise, so per-wave
 // reservations throttle the kernels):
 //   * early outs: the root box and a grid of distance lower bounds; the
 //     particles left are queued for the tree walk;
 //   * closest triangle: depth-first search with the DFS stack in LDS (one
 //     32-deep column per thread, bank = thread), the reference's pruning, push
-//     order and strict-less tie rule, so the chosen face is the same;
-//   * watertight sign (a separate pass, eight lanes per contact): parity of
-//     ray/triangle hits along the reference's fixed ray.  The reference tests
-//     every triangle of the mesh (:372-388); here the same watertight ray test
-//     runs only on the triangles registered in the origin's cell of a grid of
-//     triangle projections along that ray direction.  Every ray has the
-//     reference's direction, so a triangle can only be hit by a ray whose
-//     origin projects into the triangle's projection.  Registration is padded far beyond the test's rounding error,
-//     so no triangle the test would report is skipped and the hit count, hence
-//     the sign, is unchanged;
+//     order and strict-less tie rule, so the chosen face is the same; then,
+//     in the same wave, the watertight sign: parity of ray/triangle hits
+//     along the reference's fixed ray.  The reference tests every triangle of
+//     the mesh (:372-388); here the same watertight ray test runs only on the
+//     triangles registered in the origin's cell of a grid of triangle
+//     projections along that ray direction.  Every ray has the reference's
+//     direction, so a triangle can only be hit by a ray whose origin projects
+//     into the triangle's projection.  Registration is padded far beyond the
+//     test's rounding error, so no triangle the test would report is skipped
+//     and the hit count, hence the sign, is unchanged;
 //   * non-watertight sign: the angle-weighted pseudo-normal (:266-313, :390).
 // Node boxes are interleaved (lower, upper) float4 pairs: one 32-byte read per
 // node.
@@ -351,8 +351,7 @@ __device__ __forceinline__ uint32_t mesh_candidate(const MeshArgs& a, const Mesh
     return search_start(a, mi, p, thr);
 }
 
-// Queue counters of mesh ci for this substep: (queue length, parity-queue
-// length).  Substeps alternate between two pairs; each substep's enqueue
+// Queue counters of mesh ci for this substep: (queue length, unused).  Substeps alternate between two pairs; each substep's enqueue
 // resets the other pair for the next one (the frame start resets both).
 __device__ __forceinline__ uint32_t* mesh_counters(const MeshArgs& a, int ci, int slot) {
     return a.mcnt + 2 * (size_t(slot) * size_t(a.nmesh) + size_t(ci));
@@ -363,9 +362,10 @@ __device__ __forceinline__ uint32_t* mesh_counters(const MeshArgs& a, int ci, in
 // thread tests kEnqItems particles of its block's range (wave-striped), their
 // loads issued together.
 constexpr int kEnqItems = 4;
+constexpr int kEnqBlock = 1024;   // fewer blocks, fewer same-address queue reservations
 template <bool STATS>
-__global__ __launch_bounds__(256) void k_mesh_enqueue(MeshArgs a, int ci) {
-    __shared__ int s_tmp[256 / 64 + 1];
+__global__ __launch_bounds__(kEnqBlock) void k_mesh_enqueue(MeshArgs a, int ci) {
+    __shared__ int s_tmp[kEnqBlock / 64 + 1];
     const MeshInfo mi = a.info[ci];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         uint32_t* o = mesh_counters(a, ci, a.slot ^ 1);
@@ -373,7 +373,7 @@ __global__ __launch_bounds__(256) void k_mesh_enqueue(MeshArgs a, int ci) {
         o[1] = 0u;
     }
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int i0 = blockIdx.x * (256 * kEnqItems) + w * (64 * kEnqItems) + lane;
+    const int i0 = blockIdx.x * (kEnqBlock * kEnqItems) + w * (64 * kEnqItems) + lane;
     float4 P[kEnqItems];
     int f[kEnqItems];
 #pragma unroll
@@ -460,6 +460,47 @@ __device__ int split_roots(const MeshArgs& a, uint32_t start, f3 p, float th, ui
     return n;
 }
 
+// Hits of the reference's fixed ray from p (generate_mesh_contacts.comp:
+// 372-388) on the triangles registered in p's cell of mesh mi's projected
+// parity grid, counted by the G consecutive lanes of a group (lane j of the
+// group takes every G-th triangle; G is 1, 2 or 4) and summed over the group.
+// The watertight test against every triangle of the mesh hits exactly these:
+// a triangle the ray can hit has its padded projection registered in the
+// cell of the ray's origin.  Every lane of the wave calls this; lanes with
+// need == false test nothing and add 0.
+template <int G>
+__device__ __forceinline__ uint32_t ray_hits(const MeshArgs& a, const MeshInfo& mi, f3 p, bool need, int j,
+                                             uint32_t& tested) {
+    uint32_t b = 0, e = 0;
+    const Ray ray = make_ray(p);
+    if (need) {
+        const float u = comp3(p, ray.kx) - ray.Sx * comp3(p, ray.kz);
+        const float w = comp3(p, ray.ky) - ray.Sy * comp3(p, ray.kz);
+        const float fu = floorf((u - mi.pg_u0) * mi.pg_inv), fw = floorf((w - mi.pg_w0) * mi.pg_inv);
+        if (fu >= 0.0f && fw >= 0.0f && fu < float(mi.pg_dim) && fw < float(mi.pg_dim)) {
+            const uint32_t cell = mi.pg_cell_off + uint32_t(fw) * mi.pg_dim + uint32_t(fu);
+            b = a.pg_start[cell];
+            e = a.pg_start[cell + 1];
+        }
+    }
+    uint32_t hits = 0;
+    uint32_t t = b + uint32_t(j);
+    for (; t + G < e; t += 2 * G) {   // two triangles' loads in flight
+        const float4 A0 = a.pg_soup[3 * t], B0 = a.pg_soup[3 * t + 1], C0 = a.pg_soup[3 * t + 2];
+        const float4 A1 = a.pg_soup[3 * (t + G)], B1 = a.pg_soup[3 * (t + G) + 1], C1 = a.pg_soup[3 * (t + G) + 2];
+        hits += ray_tri(ray, xyz(A0), xyz(B0), xyz(C0)) ? 1u : 0u;
+        hits += ray_tri(ray, xyz(A1), xyz(B1), xyz(C1)) ? 1u : 0u;
+    }
+    if (t < e) {
+        const float4 A = a.pg_soup[3 * t], B = a.pg_soup[3 * t + 1], C = a.pg_soup[3 * t + 2];
+        hits += ray_tri(ray, xyz(A), xyz(B), xyz(C)) ? 1u : 0u;
+    }
+    if (j == 0) tested += e - b;
+#pragma unroll
+    for (int o = 1; o < G; o <<= 1) hits += __shfl_xor(hits, o);
+    return hits;
+}
+
 // Pass 2: the tree walk, one wave per workgroup so the walks spread over
 // every SIMD; contact slots reserved once per block, then the payloads
 // (generate_mesh_contacts.comp:333-411).  Grid-stride over the queue with a
@@ -470,10 +511,9 @@ __device__ int split_roots(const MeshArgs& a, uint32_t start, f3 p, float th, ui
 // what the single search would prune (its threshold is never below the
 // single search's at the same node), and the single search keeps the first
 // face in its visiting order among those at the minimum, which is the
-// earliest root's: the face is the same.  A watertight mesh's contacts are
-// written with the outside sign; those with a nonzero offset are queued for
-// k_mesh_parity, which flips the normal of the inside ones (the contact point
-// does not depend on the sign: (-n)(-d) = n d exactly).
+// earliest root's: the face is the same.  A watertight mesh's sign is then
+// the parity of ray hits (ray_hits, the group's lanes sharing the
+// triangles), decided before the contact is written.
 template <bool STATS, int G>
 __global__ __launch_bounds__(kMeshBlock) void k_mesh_walk(MeshArgs a, int ci) {
     __shared__ uint32_t stk[kStack][kMeshBlock];
@@ -526,26 +566,38 @@ __global__ __launch_bounds__(kMeshBlock) void k_mesh_walk(MeshArgs a, int ci) {
                     hit = true; dsq = d2; rank = r2; face = f2; sv = sv2; sw = sw2;
                 }
             }
-            hit = hit && r == 0;   // the group's first lane writes the contact
         }
-        if (STATS) st[5] += hit ? 1u : 0u;
-        const int slot = block_reserve(a.cc, hit ? 1 : 0, s_tmp);
-        bool parity = false;
-        if (hit && slot < a.cmax) {
-            const f3 p = xyz(P);
-            const float u_m = (1.0f - sv) - sw, v_m = sv;
-            const uint32_t i0 = a.tris[3 * face], i1 = a.tris[3 * face + 1], i2 = a.tris[3 * face + 2];
+        // the contact's surface point and offset, in every lane of a hit group
+        const f3 p = xyz(P);
+        f3 shp = {0.f, 0.f, 0.f}, dl = {0.f, 0.f, 0.f};
+        float dlen = 0.0f;
+        uint32_t i0 = 0, i1 = 0, i2 = 0;
+        const float u_m = (1.0f - sv) - sw, v_m = sv;
+        const float w_e = (1.0f - u_m) - v_m;
+        if (hit) {
+            i0 = a.tris[3 * face]; i1 = a.tris[3 * face + 1]; i2 = a.tris[3 * face + 2];
             const f3 v0 = ldv(a.verts, i0), v1 = ldv(a.verts, i1), v2 = ldv(a.verts, i2);
-            const float w_e = (1.0f - u_m) - v_m;
             const f3 tb = v1 * v_m;
             const f3 tab = {fma_(v0.x, u_m, tb.x), fma_(v0.y, u_m, tb.y), fma_(v0.z, u_m, tb.z)};
-            const f3 shp = {fma_(v2.x, w_e, tab.x), fma_(v2.y, w_e, tab.y), fma_(v2.z, w_e, tab.z)};
+            shp = {fma_(v2.x, w_e, tab.x), fma_(v2.y, w_e, tab.y), fma_(v2.z, w_e, tab.z)};
+            dl = p - shp;
+            dlen = exact_length(dl);
+        }
+        // watertight: the outside sign, flipped when the ray's hit count is
+        // odd (the reference tests the sign only for a nonzero offset)
+        if (mi.watertight) {
+            const bool need = hit && dlen > 0.0f;
+            const uint32_t h = ray_hits<G>(a, mi, p, need, r, st[4]);
+            if (need && (h & 1u)) sg = -sg;
+        }
+        if (G > 1) hit = hit && r == 0;   // the group's first lane writes the contact
+        if (STATS) st[5] += hit ? 1u : 0u;
+        const int slot = block_reserve(a.cc, hit ? 1 : 0, s_tmp);
+        if (hit && slot < a.cmax) {
             const f3 w0 = ldv(a.vvel, i0), w1 = ldv(a.vvel, i1), w2 = ldv(a.vvel, i2);
             const f3 vb_ = w1 * v_m;
             const f3 vab = {fma_(w0.x, u_m, vb_.x), fma_(w0.y, u_m, vb_.y), fma_(w0.z, u_m, vb_.z)};
             const f3 svel = {fma_(w2.x, w_e, vab.x), fma_(w2.y, w_e, vab.y), fma_(w2.z, w_e, vab.z)};
-            const f3 dl = p - shp;
-            const float dlen = exact_length(dl);
             const f3 nn = dlen > 0.0f ? exact_div(dl, dlen) * sg : f3{0.f, 0.f, 1.f};
             const float d = dlen * sg;
             const f3 bp = {fma_(-nn.x, d, p.x), fma_(-nn.y, d, p.y), fma_(-nn.z, d, p.z)};
@@ -554,10 +606,7 @@ __global__ __launch_bounds__(kMeshBlock) void k_mesh_walk(MeshArgs a, int ci) {
             a.cbpos[slot] = make_float4(bp.x, bp.y, bp.z, 0.f);
             a.cbvel[slot] = make_float4(svel.x, svel.y, svel.z, 0.f);
             a.cnorm[slot] = make_float4(nn.x, nn.y, nn.z, 0.f);
-            parity = mi.watertight && dlen > 0.0f;
         }
-        const int pslot = block_reserve(reinterpret_cast<int*>(cnt + 1), parity ? 1 : 0, s_tmp);
-        if (parity) a.pq[pslot] = uint32_t(slot);
     }
     if (STATS) {
 #pragma unroll
@@ -568,54 +617,9 @@ __global__ __launch_bounds__(kMeshBlock) void k_mesh_walk(MeshArgs a, int ci) {
     }
 }
 
-// Pass 3: the watertight sign of each queued contact, eight lanes per
-// contact over the triangles registered in the ray origin's projected cell
-// (the hit count, hence its parity, is the all-triangle loop's); an odd count
-// flips the stored normal (generate_mesh_contacts.comp:372-392).
-constexpr int kParityLanes = 8;
-template <bool STATS>
-__global__ __launch_bounds__(256) void k_mesh_parity(MeshArgs a, int ci) {
-    const MeshInfo mi = a.info[ci];
-    const uint32_t total = mesh_counters(a, ci, a.slot)[1];
-    const int j = threadIdx.x % kParityLanes;
-    const uint32_t per_block = 256 / kParityLanes;
-    uint64_t tested = 0;
-    for (uint32_t k = blockIdx.x * per_block + threadIdx.x / kParityLanes; k < total; k += gridDim.x * per_block) {
-        const uint32_t slot = a.pq[k];
-        const f3 p = xyz(a.P[a.cpart[slot]]);
-        const Ray ray = make_ray(p);
-        const float u = comp3(p, ray.kx) - ray.Sx * comp3(p, ray.kz);
-        const float w = comp3(p, ray.ky) - ray.Sy * comp3(p, ray.kz);
-        const float fu = floorf((u - mi.pg_u0) * mi.pg_inv), fw = floorf((w - mi.pg_w0) * mi.pg_inv);
-        uint32_t b = 0, e = 0;
-        if (fu >= 0.0f && fw >= 0.0f && fu < float(mi.pg_dim) && fw < float(mi.pg_dim)) {
-            const uint32_t cell = mi.pg_cell_off + uint32_t(fw) * mi.pg_dim + uint32_t(fu);
-            b = a.pg_start[cell];
-            e = a.pg_start[cell + 1];
-        }
-        uint32_t hits = 0;
-        for (uint32_t t = b + uint32_t(j); t < e; t += kParityLanes) {
-            const float4 A = a.pg_soup[3 * t], B = a.pg_soup[3 * t + 1], C = a.pg_soup[3 * t + 2];
-            if (ray_tri(ray, xyz(A), xyz(B), xyz(C))) hits++;
-        }
-        hits += __shfl_xor(hits, 1);
-        hits += __shfl_xor(hits, 2);
-        hits += __shfl_xor(hits, 4);
-        if (STATS && j == 0) tested += e - b;
-        if (j == 0 && (hits & 1u)) {
-            const float4 n = a.cnorm[slot];
-            a.cnorm[slot] = make_float4(-n.x, -n.y, -n.z, n.w);
-        }
-    }
-    if (STATS) {
-        const uint64_t v = wave_sum_u64(tested);
-        if (__lane_id() == 0 && v) atomicAdd(reinterpret_cast<unsigned long long*>(a.stats + 4), (unsigned long long)v);
-    }
-}
-
 void launch_mesh_contacts(const MeshArgs& a, const uint8_t* watertight, bool beside, hipStream_t s) {
     if (a.n <= 0 || a.nmesh <= 0) return;
-    const int nb = (a.n + 256 * kEnqItems - 1) / (256 * kEnqItems);
+    const int nb = (a.n + kEnqBlock * kEnqItems - 1) / (kEnqBlock * kEnqItems);
     // walk grids are multiples of the XCD count, so every XCD gets the same
     // number of workgroups.  The four-lane walk holds about 90 VGPRs per lane
     // for its whole run; beside the radix passes (beside = true) 2,048
@@ -625,19 +629,16 @@ void launch_mesh_contacts(const MeshArgs& a, const uint8_t* watertight, bool bes
     const int cap4 = beside ? 2048 : 4096;
     const unsigned walk_grid1 = unsigned(std::max(1, std::min(4096, (a.n + kMeshBlock - 1) / kMeshBlock) / 8)) * 8u;
     const unsigned walk_grid4 = unsigned(std::max(1, std::min(cap4, (a.n + kMeshBlock / 4 - 1) / (kMeshBlock / 4)) / 8)) * 8u;
-    const unsigned parity_grid = unsigned(std::min(2048, (a.n + 255 / kParityLanes) / (256 / kParityLanes)));
     for (int ci = 0; ci < a.nmesh; ci++) {
         const bool split = watertight[ci] != 0;
         if (a.stats) {
-            hipLaunchKernelGGL(k_mesh_enqueue<true>, dim3(nb), dim3(256), 0, s, a, ci);
+            hipLaunchKernelGGL(k_mesh_enqueue<true>, dim3(nb), dim3(kEnqBlock), 0, s, a, ci);
             if (split) hipLaunchKernelGGL((k_mesh_walk<true, 4>), dim3(walk_grid4), dim3(kMeshBlock), 0, s, a, ci);
             else hipLaunchKernelGGL((k_mesh_walk<true, 1>), dim3(walk_grid1), dim3(kMeshBlock), 0, s, a, ci);
-            hipLaunchKernelGGL(k_mesh_parity<true>, dim3(parity_grid), dim3(256), 0, s, a, ci);
         } else {
-            hipLaunchKernelGGL(k_mesh_enqueue<false>, dim3(nb), dim3(256), 0, s, a, ci);
+            hipLaunchKernelGGL(k_mesh_enqueue<false>, dim3(nb), dim3(kEnqBlock), 0, s, a, ci);
             if (split) hipLaunchKernelGGL((k_mesh_walk<false, 4>), dim3(walk_grid4), dim3(kMeshBlock), 0, s, a, ci);
             else hipLaunchKernelGGL((k_mesh_walk<false, 1>), dim3(walk_grid1), dim3(kMeshBlock), 0, s, a, ci);
-            hipLaunchKernelGGL(k_mesh_parity<false>, dim3(parity_grid), dim3(256), 0, s, a, ci);
         }
     }
 }

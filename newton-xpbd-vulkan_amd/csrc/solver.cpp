@@ -63,7 +63,7 @@ void Solver::free_all() {
     dfree(range_); dfree(rkeys_); dfree(rows_); dfree(glb_); dfree(rslot_); dfree(gtcnt_); dfree(gtoff_); dfree(tbl_); dfree(cpart_); dfree(cshape_); dfree(cc_); dfree(cbpos_); dfree(cbvel_); dfree(cnorm_);
     dfree(dshape_); dfree(delta_); dfree(owner_); dfree(body_q_); dfree(body_qd_); dfree(body_com_); dfree(body_invI_);
     dfree(body_invm_); dfree(body_delta_); dfree(mat_mu_); dfree(body_flags_); dfree(shape_body_);
-    dfree(verts_d_); dfree(vvel_d_); dfree(tris_d_); dfree(prim_d_); dfree(err_d_); dfree(nodes_d_); dfree(cnodes_d_); dfree(lbox_d_); dfree(pg_start_); dfree(lb_); dfree(lb_node_); dfree(tsoup_); dfree(pg_soup_); dfree(mcnt_); dfree(mq_); dfree(pq_);
+    dfree(verts_d_); dfree(vvel_d_); dfree(tris_d_); dfree(prim_d_); dfree(err_d_); dfree(nodes_d_); dfree(cnodes_d_); dfree(lbox_d_); dfree(pg_start_); dfree(lb_); dfree(lb_node_); dfree(tsoup_); dfree(pg_soup_); dfree(mcnt_); dfree(mq_);
     dfree(info_d_); dfree(diag_header_); dfree(diag_pairs_); dfree(mesh_stats_);
     for (hipEvent_t e : ev_pool_) (void)hipEventDestroy(e);
     ev_pool_.clear();
@@ -125,7 +125,6 @@ void Solver::alloc_all() {
     dalloc(diag_header_, 2);
     dalloc(mesh_stats_, 8);
     dalloc(mq_, n_);   // mesh candidate queue (its counters are sized by finalize_meshes)
-    dalloc(pq_, n_);
     max_pairs_ = cfg_.max_diag_pairs ? cfg_.max_diag_pairs : 100000u;
     dalloc(diag_pairs_, cfg_.enable_diagnostics ? size_t(20) * max_pairs_ : 20);
     HIPCHECK(hipHostMalloc(reinterpret_cast<void**>(&count_host_), 4 * sizeof(int32_t), hipHostMallocDefault));
@@ -749,7 +748,7 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
         m.n = n; m.P = P_[s_].p; m.flags = flags_cur();
         m.verts = verts_d_.p; m.tris = tris_d_.p; m.nodes = nodes_d_.p; m.cnodes = cnodes_d_.p; m.lbox = lbox_d_.p; m.prim = prim_d_.p;
         m.pg_start = pg_start_.p; m.pg_soup = pg_soup_.p; m.tsoup = tsoup_.p; m.lb = lb_.p; m.lb_node = lb_node_.p;
-        m.mcnt = mcnt_.p; m.mq = mq_.p; m.pq = pq_.p;
+        m.mcnt = mcnt_.p; m.mq = mq_.p;
         m.slot = slot;
         m.vvel = vvel_d_.p; m.info = info_d_.p; m.nmesh = M;
         m.margin = cfg_.soft_contact_margin; m.eps = avg_edge_ * 1e-3f;
@@ -1238,7 +1237,7 @@ void Solver::collide_only() {
         m.n = int(n_); m.P = P_[s_].p; m.flags = flags_cur();
         m.verts = verts_d_.p; m.tris = tris_d_.p; m.nodes = nodes_d_.p; m.cnodes = cnodes_d_.p; m.lbox = lbox_d_.p; m.prim = prim_d_.p;
         m.pg_start = pg_start_.p; m.pg_soup = pg_soup_.p; m.tsoup = tsoup_.p; m.lb = lb_.p; m.lb_node = lb_node_.p;
-        m.mcnt = mcnt_.p; m.mq = mq_.p; m.pq = pq_.p;
+        m.mcnt = mcnt_.p; m.mq = mq_.p;
         m.slot = 0;
         m.vvel = vvel_d_.p; m.info = info_d_.p; m.nmesh = int(meshes_.size());
         m.margin = cfg_.soft_contact_margin; m.eps = avg_edge_ * 1e-3f;

@@ -309,7 +309,6 @@ private:
     DevBuf<float4> pg_soup_;               // triangle corners of every parity-grid entry
     DevBuf<uint32_t> mcnt_;                // mesh queue counters: 2 substep parities x meshes x 2
     DevBuf<int2> mq_;                      // mesh candidate queue (particle, start node)
-    DevBuf<uint32_t> pq_;                  // watertight mesh contacts awaiting their sign
     std::vector<uint8_t> mesh_split_;      // per mesh: walk with four lanes per particle
     DevBuf<MeshInfo> info_d_;
     std::vector<MeshInfo> info_h_;

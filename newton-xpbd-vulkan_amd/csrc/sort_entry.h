@@ -25,6 +25,9 @@ void sort_validate(const void* keys_in, const void* vals_in, const void* keys_ou
 void sort_run(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out, uint32_t* vals_out, size_t n,
               int key_bits, void* scratch, hipStream_t stream);
 // Allocates the scratch, sorts, waits for the stream and frees the scratch.
+// Exhaustive check of rcp_cr / sqrt_cr against the correctly rounded
+// operations (k_selfcheck.hip): mismatch counts per range.
+void check_arith(uint64_t out[3], hipStream_t s);
 void sort_pairs_alloc(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out, uint32_t* vals_out,
                       size_t n, int key_bits, hipStream_t stream);
 

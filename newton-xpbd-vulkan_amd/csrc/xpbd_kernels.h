@@ -185,7 +185,6 @@ struct MeshArgs {
     uint32_t* mcnt;             // per (substep parity, mesh): queue length and parity-queue length
     int slot;                   // substep parity: which counter pair this substep uses
     int2* mq;                   // candidate queue: (particle, search start node word)
-    uint32_t* pq;               // watertight contacts whose normal sign the parity pass decides
 };
 
 // Rigid bodies (dynamic bodies occupy indices [b0, b1) of the body arrays).
