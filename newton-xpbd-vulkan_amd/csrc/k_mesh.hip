@@ -165,81 +165,19 @@ __device__ bool closest_face(const MeshArgs& a, const MeshInfo& mi, uint32_t sta
     float sv = 0.f, sw = 0.f;
     bool found = false;
     float min_dist_sq = max_dist * max_dist;
-    while (sp > 0) {
-        const uint32_t word = stk[--sp][lt];
-        if (STATS) pops++;
-        if (word >> 31) {   // leaf: the reference's pop-time box test, then its triangles
-            // short leaves carry their primitive range in the word, so the
-            // box and the first triangle load together; others go through
-            // their node
-            const uint32_t cnt = (word >> 27) & 15u;
-            uint32_t li, ri;
-            float4 L, U;
-            float4 nA = make_float4(0.f, 0.f, 0.f, 0.f), nB = nA, nC = nA;
-            if (cnt) {
-                li = word & 0x07FFFFFFu;
-                ri = li + cnt;
-                nA = a.tsoup[3 * li]; nB = a.tsoup[3 * li + 1]; nC = a.tsoup[3 * li + 2];
-                L = a.lbox[2 * size_t(li)]; U = a.lbox[2 * size_t(li) + 1];
-            } else {
-                const uint32_t ni = word & 0x07FFFFFFu;
-                L = node_lo(a.nodes, ni); U = node_hi(a.nodes, ni);
-                li = __float_as_uint(L.w) & 0x3FFFFFFFu; ri = __float_as_uint(U.w) & 0x3FFFFFFFu;
-                if (li < ri) { nA = a.tsoup[3 * li]; nB = a.tsoup[3 * li + 1]; nC = a.tsoup[3 * li + 2]; }
-            }
-            const float nds = dist_sq_aabb(p, xyz(L), xyz(U));
-            if (wt) { if (nds > min_dist_sq) continue; }
-            else if (nds > (min_dist + eps) * (min_dist + eps)) continue;
-            // the next triangle's corners load while this one is tested
-            for (uint32_t pc = li; pc < ri; pc++) {
-                const float4 A = nA, B = nB, C = nC;
-                if (pc + 1 < ri) { nA = a.tsoup[3 * pc + 3]; nB = a.tsoup[3 * pc + 4]; nC = a.tsoup[3 * pc + 5]; }
-                const uint32_t t = __float_as_uint(A.w);
-                if (STATS) tests++;
-                const f3 va = xyz(A), vb = xyz(B), vc = xyz(C);
-                const f3 e0 = vb - va, e1 = vc - va, e2 = vc - vb;
-                const float e0n = dot3(e0, e0), e1n = dot3(e1, e1), e2n = dot3(e2, e2);
-                f3 cp, tn, bc;
-                if (wt) {
-                    // the degenerate-triangle skip (:250-251) only matters for a
-                    // triangle that would become the closest: test it there
-                    closest_point_triangle<false>(p, va, vb, vc, cp, tn, bc);
-                    const f3 df = cp - p;
-                    const float dsq = dot3(df, df);
-                    if (dsq < min_dist_sq) {
-                        if (len3(cross3(e0, e1)) / ((e0n + e1n) + e2n) < 1.0e-6f) continue;
-                        min_dist_sq = dsq; min_face = t; sv = bc.y; sw = bc.z; found = true;
-                    }
-                } else {
-                    if (len3(cross3(e0, e1)) / ((e0n + e1n) + e2n) < 1.0e-6f) continue;
-                    closest_point_triangle<true>(p, va, vb, vc, cp, tn, bc);
-                    const float dist = exact_length(cp - p);
-                    if (dist < min_dist + eps) {
-                        float weight = 0.0f;
-                        const f3 cpa = cp - va, cqb = cp - vb, crc = cp - vc;
-                        const float lp2 = dot3(cpa, cpa), lq2 = dot3(cqb, cqb), lr2 = dot3(crc, crc);
-                        if (lp2 < eps2) weight = acosf(gmin(gmax(dot3(normalize3(e0), normalize3(e1)), -1.0f), 1.0f));
-                        else if (lq2 < eps2) weight = acosf(gmin(gmax(dot3(normalize3(e2), normalize3(-e0)), -1.0f), 1.0f));
-                        else if (lr2 < eps2) weight = acosf(gmin(gmax(dot3(normalize3(-e1), normalize3(-e2)), -1.0f), 1.0f));
-                        else {
-                            const float e0cp = dot3(e0, cpa), e2cq = dot3(e2, cqb), e1cp = dot3(e1, cpa);
-                            if ((lp2 * e0n - e0cp * e0cp < eps2 * e0n) || (lq2 * e2n - e2cq * e2cq < eps2 * e2n) ||
-                                (lp2 * e1n - e1cp * e1cp < eps2 * e1n))
-                                weight = PI;
-                            else weight = 2.0f * PI;
-                        }
-                        if (dist > min_dist - eps) {
-                            acc_n = acc_n + weight * tn;
-                            if (dist < min_dist) { min_dist = dist; min_face = t; sv = bc.y; sw = bc.z; }
-                        } else {
-                            min_dist = dist; min_face = t; sv = bc.y; sw = bc.z;
-                            acc_n = weight * tn;
-                        }
-                        found = true;
-                    }
-                }
-            }
-        } else {
+    // Two nested loops so the lanes of a wave test triangles together: each
+    // lane pops and expands internal nodes until it pops a leaf (or its stack
+    // empties), then every lane holding a leaf tests it.  Each lane's own
+    // sequence of pops, tests and threshold updates is the single loop's, so
+    // the chosen face is too; lanes no longer run the triangle tests of one
+    // iteration beside the node expansion of another.
+    while (true) {
+        uint32_t word = 0;
+        bool leaf = false;
+        while (sp > 0) {
+            word = stk[--sp][lt];
+            if (STATS) pops++;
+            if (word >> 31) { leaf = true; break; }
             // internal node: one read of a record holding both children's
             // boxes and, for internal children, their children's boxes.  The
             // node's own pop-time box test is not repeated: a node it would
@@ -286,6 +224,78 @@ __device__ bool closest_face(const MeshArgs& a, const MeshInfo& mi, uint32_t sta
             } else {
                 if (ql) stk[sp++][lt] = clw;
                 if (qr) stk[sp++][lt] = crw;
+            }
+        }
+        if (!leaf) break;
+        // leaf: the reference's pop-time box test, then its triangles
+        // short leaves carry their primitive range in the word, so the
+        // box and the first triangle load together; others go through
+        // their node
+        const uint32_t cnt = (word >> 27) & 15u;
+        uint32_t li, ri;
+        float4 L, U;
+        float4 nA = make_float4(0.f, 0.f, 0.f, 0.f), nB = nA, nC = nA;
+        if (cnt) {
+            li = word & 0x07FFFFFFu;
+            ri = li + cnt;
+            nA = a.tsoup[3 * li]; nB = a.tsoup[3 * li + 1]; nC = a.tsoup[3 * li + 2];
+            L = a.lbox[2 * size_t(li)]; U = a.lbox[2 * size_t(li) + 1];
+        } else {
+            const uint32_t ni = word & 0x07FFFFFFu;
+            L = node_lo(a.nodes, ni); U = node_hi(a.nodes, ni);
+            li = __float_as_uint(L.w) & 0x3FFFFFFFu; ri = __float_as_uint(U.w) & 0x3FFFFFFFu;
+            if (li < ri) { nA = a.tsoup[3 * li]; nB = a.tsoup[3 * li + 1]; nC = a.tsoup[3 * li + 2]; }
+        }
+        const float nds = dist_sq_aabb(p, xyz(L), xyz(U));
+        if (wt) { if (nds > min_dist_sq) continue; }
+        else if (nds > (min_dist + eps) * (min_dist + eps)) continue;
+        // the next triangle's corners load while this one is tested
+        for (uint32_t pc = li; pc < ri; pc++) {
+            const float4 A = nA, B = nB, C = nC;
+            if (pc + 1 < ri) { nA = a.tsoup[3 * pc + 3]; nB = a.tsoup[3 * pc + 4]; nC = a.tsoup[3 * pc + 5]; }
+            const uint32_t t = __float_as_uint(A.w);
+            if (STATS) tests++;
+            const f3 va = xyz(A), vb = xyz(B), vc = xyz(C);
+            const f3 e0 = vb - va, e1 = vc - va, e2 = vc - vb;
+            const float e0n = dot3(e0, e0), e1n = dot3(e1, e1), e2n = dot3(e2, e2);
+            f3 cp, tn, bc;
+            if (wt) {
+                // the degenerate-triangle skip (:250-251) only matters for a
+                // triangle that would become the closest: test it there
+                closest_point_triangle<false>(p, va, vb, vc, cp, tn, bc);
+                const f3 df = cp - p;
+                const float dsq = dot3(df, df);
+                if (dsq < min_dist_sq) {
+                    if (len3(cross3(e0, e1)) / ((e0n + e1n) + e2n) < 1.0e-6f) continue;
+                    min_dist_sq = dsq; min_face = t; sv = bc.y; sw = bc.z; found = true;
+                }
+            } else {
+                if (len3(cross3(e0, e1)) / ((e0n + e1n) + e2n) < 1.0e-6f) continue;
+                closest_point_triangle<true>(p, va, vb, vc, cp, tn, bc);
+                const float dist = exact_length(cp - p);
+                if (dist < min_dist + eps) {
+                    float weight = 0.0f;
+                    const f3 cpa = cp - va, cqb = cp - vb, crc = cp - vc;
+                    const float lp2 = dot3(cpa, cpa), lq2 = dot3(cqb, cqb), lr2 = dot3(crc, crc);
+                    if (lp2 < eps2) weight = acosf(gmin(gmax(dot3(normalize3(e0), normalize3(e1)), -1.0f), 1.0f));
+                    else if (lq2 < eps2) weight = acosf(gmin(gmax(dot3(normalize3(e2), normalize3(-e0)), -1.0f), 1.0f));
+                    else if (lr2 < eps2) weight = acosf(gmin(gmax(dot3(normalize3(-e1), normalize3(-e2)), -1.0f), 1.0f));
+                    else {
+                        const float e0cp = dot3(e0, cpa), e2cq = dot3(e2, cqb), e1cp = dot3(e1, cpa);
+                        if ((lp2 * e0n - e0cp * e0cp < eps2 * e0n) || (lq2 * e2n - e2cq * e2cq < eps2 * e2n) ||
+                            (lp2 * e1n - e1cp * e1cp < eps2 * e1n))
+                            weight = PI;
+                        else weight = 2.0f * PI;
+                    }
+                    if (dist > min_dist - eps) {
+                        acc_n = acc_n + weight * tn;
+                        if (dist < min_dist) { min_dist = dist; min_face = t; sv = bc.y; sw = bc.z; }
+                    } else {
+                        min_dist = dist; min_face = t; sv = bc.y; sw = bc.z;
+                        acc_n = weight * tn;
+                    }
+                    found = true;
+                }
             }
         }
     }
