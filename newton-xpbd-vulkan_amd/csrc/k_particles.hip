@@ -671,7 +671,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6, 8))) v
             cy1 = cy2; cz1 = cz2;
             advance(cy2, cz2);
         }
-    } else if ((xe - xs <= 2) & (ye - ys <= 2) & (ze - zs <= 2) & !clamp0) {
+    } else if ((xe - xs <= 2) & (ye - ys <= 2) & (ze - zs <= 2) & !clamp0 & (ys + kGridOrigin >= 0) &
+               (zs + kGridOrigin >= 0)) {
         // A query box of at most 3 x 3 x 3 cells (every query when R is at
         // most one cell edge).  A row's cells [xs, xe] are one run of sorted
         // slots, or two when the row crosses an x wrap (piece a: the first
@@ -682,7 +683,10 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6, 8))) v
         // (row 0, row 1, row 2) and nothing, or (a0, b0, a1) and (b1, a2, b2)
         // when the rows split -- in batches of SB: flat index f maps to its
         // run's slot by two compares against the runs' cumulative lengths.
-        // The visiting order, hence every sum, is the general path's.
+        // The visiting order, hence every sum, is the general path's.  No
+        // virtual coordinate of the box is clamped, so a cell's key is its
+        // wrap plane's bits over the low seven bits of each coordinate: a
+        // slab's z part, a row's y part and a piece's x part, combined.
         const int nxc = xe - xs + 1;
         const bool split = len0 < nxc;
         auto stream = [&](int2 u0, int2 u1, int2 u2) {   // runs as (first slot, length)
@@ -707,21 +711,27 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6, 8))) v
                 }
             }
         };
+        const int pbits = a.planes;
+        const uint32_t pm = pbits ? (1u << pbits) - 1u : 0u;
+        auto part = [&](uint32_t v, int shift, int pshift) -> uint32_t {   // one coordinate's key bits
+            return (((v >> 7) & pm) << (kCellBits + pshift)) | ((v & 127u) << shift);
+        };
+        const uint32_t xka = part(uint32_t(vx0), 0, 0), xkb = part(uint32_t(vx0 + len0), 0, 0);
+        const uint32_t vy0 = uint32_t(ys + kGridOrigin), vz0 = uint32_t(zs + kGridOrigin);
         for (int dz = 0; dz < 3; dz++) {
             if (zs + dz > ze) break;
-            const int cz = zs + dz;
+            const uint32_t zk = part(vz0 + uint32_t(dz), 14, 2 * pbits);
             int2 pa[3], pb[3];   // per row: pieces a and b as (first slot, length)
 #pragma unroll
             for (int dy = 0; dy < 3; dy++) {
                 const bool in = ys + dy <= ye;
-                const int cy = in ? ys + dy : ys;
-                const int ca = cell_index(xs, cy, cz), cb = cell_index(xs + len0, cy, cz);
-                const uint32_t ta = table(xs, cy, cz), tb = table(xs + len0, cy, cz);
+                const uint32_t yk = zk | part(vy0 + uint32_t(in ? dy : 0), 7, pbits);
+                const uint32_t ka = yk | xka, kb = yk | xkb;
                 int la = 0, ea = 0, lb = 0, eb = 0;
 #pragma unroll
                 for (int m = 0; m < 3; m++) {
                     const bool inb = m >= len0;
-                    const int2 rg = m < nxc ? raw(inb ? tb : ta, inb ? cb + (m - len0) : ca + m) : make_int2(0, 0);
+                    const int2 rg = m < nxc ? range[size_t(inb ? kb + uint32_t(m - len0) : ka + uint32_t(m))] : make_int2(0, 0);
                     const int l = rg.y - rg.x;
                     if (inb) { lb += l; eb = max(eb, rg.y); } else { la += l; ea = max(ea, rg.y); }
                 }
