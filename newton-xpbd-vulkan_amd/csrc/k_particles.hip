@@ -149,6 +149,33 @@ __global__ __launch_bounds__(256) void k_cell_ranges(int n, const uint32_t* __re
     if (rkeys) rkeys[t] = c;
 }
 
+// k_cell_ranges plus the move of the state into the sorted order, one sorted
+// slot t per thread: the particle at previous slot vals[t] is integrated
+// (integrate_particle, the code k_integrate keyed it with) and stored at t
+// with its flags and id, and rank records previous slot -> t.  The last radix
+// pass then only moves keys and slots.
+__global__ __launch_bounds__(256) void k_cell_ranges_gather(int n, const uint32_t* __restrict__ keys,
+                                                            const uint32_t* __restrict__ vals, int* __restrict__ range,
+                                                            uint32_t* __restrict__ rkeys, SortGather g) {
+    int t = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const uint32_t v = vals[t];
+    float4 P = g.P[v], V = g.V[v];
+    const int f = g.flags[v];
+    const uint32_t id = g.ids[v];
+    const uint32_t c = keys[t];
+    const size_t row = size_t(2) * c;
+    if (t == 0 || keys[t - 1] != c) range[row] = t;
+    if (t == n - 1 || keys[t + 1] != c) range[row + 1] = t + 1;
+    rkeys[t] = c;
+    integrate_particle(P, V, f, g.dt, g.vmax, g.gx, g.gy, g.gz);
+    g.Xs[t] = P;
+    g.Vs[t] = V;
+    g.Fs[t] = f;
+    g.ids_out[t] = id;
+    g.rank[v] = uint32_t(t);
+}
+
 // Zeroes the rows k_cell_ranges wrote from the sorted keys rkeys: with wrap
 // planes the table has 2^27 rows, too many to clear whole every substep.
 __global__ __launch_bounds__(256) void k_clear_ranges(int n, const uint32_t* __restrict__ rkeys, int* __restrict__ range) {
@@ -927,6 +954,11 @@ void launch_plane_contacts(const PlaneArgs& a, hipStream_t s) {
 }
 void launch_cell_ranges(int n, const uint32_t* keys, int* range, uint32_t* rkeys, hipStream_t s) {
     if (n > 0) hipLaunchKernelGGL(k_cell_ranges, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, keys, range, rkeys);
+}
+void launch_cell_ranges_gather(int n, const uint32_t* keys, const uint32_t* vals, int* range, uint32_t* rkeys,
+                               const SortGather& g, hipStream_t s) {
+    if (n > 0)
+        hipLaunchKernelGGL(k_cell_ranges_gather, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, keys, vals, range, rkeys, g);
 }
 // Contact particle indices through a slot map (previous slot -> new slot
 // after a sort, or slot -> particle id when the layout is reset).

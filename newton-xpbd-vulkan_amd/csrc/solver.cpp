@@ -670,7 +670,13 @@ void Solver::set_sort_shape(int key_bits) {
 }
 
 
-void Solver::run_sort(int tiles, float sdt) {
+SortGather Solver::sort_gather(float sdt) {
+    return SortGather{P_[s_].p, V_[s_].p, flags_cur(), ids_cur(), Xs_[0].p, Vs_[0].p,
+                      flags_[lay_ ^ 1].p, ids_[lay_ ^ 1].p, rank_.p,
+                      sdt, cfg_.particle_v_max, cfg_.gravity[0], cfg_.gravity[1], cfg_.gravity[2]};
+}
+
+void Solver::run_sort(int tiles, const SortGather* last) {
     const int n = int(n_);
     uint32_t* kb[2] = {keysA_.p, keysB_.p};
     uint32_t* vb[2] = {valsA_.p, valsB_.p};
@@ -678,9 +684,6 @@ void Solver::run_sort(int tiles, float sdt) {
     // between the A and B buffers and the last one must land in A (the
     // buffers the getters read)
     int dst = (sort_passes_ & 1) ? 0 : 1;
-    const SortGather g{P_[s_].p, V_[s_].p, flags_cur(), ids_cur(), Xs_[0].p, Vs_[0].p,
-                       flags_[lay_ ^ 1].p, ids_[lay_ ^ 1].p, rank_.p,
-                       sdt, cfg_.particle_v_max, cfg_.gravity[0], cfg_.gravity[1], cfg_.gravity[2]};
     for (int pass = 0; pass < sort_passes_; pass++) {
         const int shift = pass * sort_bits_;
         const uint32_t* kin = pass == 0 ? nullptr : kb[dst ^ 1];
@@ -694,7 +697,7 @@ void Solver::run_sort(int tiles, float sdt) {
         stage_end(kSortScan);
         stage_begin(kSortScatter);
         launch_sort_scatter(sort_bits_, sort_items_, n, kin, vin, pin, kb[dst], vb[dst], shift, hist_.p, bintot_.p, tiles,
-                            pass == sort_passes_ - 1 ? &g : nullptr, stream_);
+                            pass == sort_passes_ - 1 ? last : nullptr, stream_);
         stage_end(kSortScatter);
         dst ^= 1;
     }
@@ -814,11 +817,16 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
     const int shape_grid = int(std::max<size_t>(1, std::min<size_t>((cmax_ + 255) / 256, 2048)));
     int cur = p_, nxt = o_;
     if (n > 0) {
-        run_sort(tiles, sdt);
-        if (fork && iterations > 0) HIPCHECK(hipEventRecord(sorted_ev_, stream_));
+        // plain solvers move the state into the sorted order in the cell-range
+        // pass (one thread per sorted slot); batched ones in the last radix pass
+        const SortGather g = sort_gather(sdt);
+        const bool gather_in_ranges = num_scenes_ == 1;
+        run_sort(tiles, gather_in_ranges ? nullptr : &g);
         stage_begin(kRanges);
-        build_ranges(keysA_.p, true);
+        if (gather_in_ranges) launch_cell_ranges_gather(n, keysA_.p, valsA_.p, range_.p, rkeys_.p, g, stream_);
+        else build_ranges(keysA_.p, true);
         stage_end(kRanges);
+        if (fork && iterations > 0) HIPCHECK(hipEventRecord(sorted_ev_, stream_));
         grid_valid_ = true;
         if (det_ && iterations > 0) {
             // deterministic mode: every contact is in (mesh stage joined), then

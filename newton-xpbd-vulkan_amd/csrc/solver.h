@@ -150,7 +150,10 @@ private:
     void substep(float sdt, int iterations, bool collide, int slot, bool zero_next);
     BodyStepArgs body_step_args(float sdt);
     DetApplyArgs det_apply_args(const float4* Xin, const float4* Vin, int out, float sdt);
-    void run_sort(int tiles, float sdt);   // the last pass also integrates the state over sdt
+    // the state's move into the sorted order (integrated over sdt), taken
+    // before the sort flips the layout
+    SortGather sort_gather(float sdt);
+    void run_sort(int tiles, const SortGather* last);   // last: the last pass also moves the state
     void update_planes();
     std::vector<uint64_t> reference_sorted();
     // cell ranges of the sorted keys; cleared = the previous rows are already

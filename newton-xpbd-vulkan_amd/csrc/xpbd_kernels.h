@@ -332,6 +332,8 @@ void launch_zero_u32(uint32_t* p, size_t n, hipStream_t s);
 
 void launch_plane_contacts(const PlaneArgs& a, hipStream_t s);
 void launch_cell_ranges(int n, const uint32_t* keys, int* range, uint32_t* rkeys, hipStream_t s);
+void launch_cell_ranges_gather(int n, const uint32_t* keys, const uint32_t* vals, int* range, uint32_t* rkeys,
+                               const SortGather& g, hipStream_t s);
 // batched solvers: the row table of the sorted keys (tile scratch of
 // group_tiles(n) words each), recording per sorted position the row it
 // filled; empty the last build's rows
