@@ -21,15 +21,7 @@
 
 namespace xpbd {
 
-__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        uint32_t y = __shfl_up(x, o);
-        if (lane >= o) x += y;
-    }
-    return x;
-}
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) { return uint32_t(wave_incl_scan_i(int(x))); }
 
 // keys, or the x of (key, value) pairs when pairs is non-null (the first pass)
 template <int BITS, int ITEMS>

@@ -296,14 +296,18 @@ __device__ __forceinline__ int wave_reserve(int* counter, bool want) {
     return base + __popcll(below);
 }
 
-// Inclusive prefix sum over the 64 lanes of a wave (all lanes active).
+// Inclusive prefix sum over the 64 lanes of a wave (all lanes active), on
+// DPP lane moves: row_shr 1, 2, 4 and 8 scan each row of 16 lanes (a lane
+// whose source lies before its row start takes the 0 passed as old), then
+// row_bcast:15 adds row 0's total to row 1 and row 2's to row 3, and
+// row_bcast:31 adds the total of rows 0-1 to rows 2 and 3.
 __device__ __forceinline__ int wave_incl_scan_i(int x) {
-    const int lane = __lane_id();
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const int y = __shfl_up(x, o);
-        if (lane >= o) x += y;
-    }
+    x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xF, 0xF, false);   // row_shr:1
+    x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xF, 0xF, false);   // row_shr:2
+    x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xF, 0xF, false);   // row_shr:4
+    x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xF, 0xF, false);   // row_shr:8
+    x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xA, 0xF, false);   // row_bcast:15 into rows 1, 3
+    x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xC, 0xF, false);   // row_bcast:31 into rows 2, 3
     return x;
 }
 
