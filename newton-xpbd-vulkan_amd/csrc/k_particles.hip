@@ -534,15 +534,20 @@ template <bool DIAG, bool BATCHED, int BLOCK>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_pp_apply(PPArgs a) {
     constexpr int SB = kScanBatch;
     __shared__ uint32_t s_surv[kSurv][BLOCK];
-    const int t = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (t >= a.n) return;
+    // Every lane stays to the end (the final drain shares the wave's
+    // survivors over all of its lanes): a lane past the particle count
+    // shadows the last slot, and it and an inactive particle's lane scan an
+    // empty box and write nothing more.
+    const int tb = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x;
+    const bool in_range = tb + int(threadIdx.x) < a.n;
+    const int t = in_range ? tb + int(threadIdx.x) : a.n - 1;
     const int pf = a.Fs[t];
     const float4 Pi = a.Xs[t], Vi = a.Vs[t];
-    if ((pf & 1) == 0) {   // inactive: passthrough (the reference leaves x_out unwritten)
+    const bool live = in_range && (pf & 1) != 0;
+    if (in_range && !live) {   // inactive: passthrough (the reference leaves x_out unwritten)
         a.delta[3 * t] = 0.0f; a.delta[3 * t + 1] = 0.0f; a.delta[3 * t + 2] = 0.0f;
         a.Po[t] = Pi;
         a.Vo[t] = Vi;
-        return;
     }
     const int i = DIAG ? int(a.ids[t]) : 0;
     const int proxy = pf & 2;
@@ -553,7 +558,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6, 8))) v
     const int xs = (int)floorf((x.x - R) * inv), ys = (int)floorf((x.y - R) * inv), zs = (int)floorf((x.z - R) * inv);
     const int xe = min((int)floorf((x.x + R) * inv), xs + kGridDim - 1);
     const int ye = min((int)floorf((x.y + R) * inv), ys + kGridDim - 1);
-    const int ze = min((int)floorf((x.z + R) * inv), zs + kGridDim - 1);
+    const int ze = live ? min((int)floorf((x.z + R) * inv), zs + kGridDim - 1) : zs - 1;
     const int2* range = reinterpret_cast<const int2*>(a.range);
     // batched mode: the particle's scene bits complete the key of each
     // visited cell; with wrap planes each visited virtual cell selects its
@@ -850,7 +855,73 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6, 8))) v
             }
         }
     }
-    drain();
+    // (slots and lanes share one 32-bit word: up to 2^26 particles)
+    if (BLOCK == 64 && a.n <= (1 << 26)) {
+        // Final drain over the whole wave.  The lanes' survivors, in the flat
+        // order (m-th survivor of every lane, lanes ascending, m ascending),
+        // are dealt to the 64 lanes a round at a time; each projection runs
+        // with its owner's state (from LDS) and its 0 + pair delta returns to
+        // the owner, which adds its own in visiting order (the owner's state
+        // comes over by lane shuffles).  acc is never -0
+        // (it starts at +0 and round-to-nearest sums are -0 only for two -0
+        // terms), so adding +0 for a rejected candidate or 0 + d in place of
+        // d leaves every sum as the lane's own loop would.
+        __shared__ uint4 s_lvl[kSurv];   // per level m: ballot of cnt > m, first flat position
+        const int lane = int(threadIdx.x);
+        const uint64_t below = (1ull << lane) - 1ull;
+        uint32_t* flat = &s_surv[0][0];
+        // compaction in place: position(lane, m) <= 64 m + lane, the entry's
+        // own index, and level m is read before it is written
+        uint32_t total = 0;
+        for (int m = 0;; m++) {
+            const bool has = cnt > m;
+            const uint64_t mk = __ballot(has);
+            if (mk == 0) break;
+            const uint32_t e = has ? s_surv[m][lane] : 0u;
+            if (lane == 0) s_lvl[m] = make_uint4(uint32_t(mk), uint32_t(mk >> 32), total, 0u);
+            if (has) flat[total + uint32_t(__popcll(mk & below))] = e | (uint32_t(lane) << 26);
+            total += uint32_t(__popcll(mk));
+        }
+        __syncthreads();
+        int mn = 0;   // this lane's next survivor to collect
+        for (uint32_t base = 0; base < total; base += 64) {
+            f3 pd = {0.f, 0.f, 0.f};
+            const uint32_t q = base + uint32_t(lane);
+            const uint32_t e = q < total ? flat[q] : 0u;
+            const int ol = int(e >> 26), k = int(e & 0x3FFFFFFu);
+            const f3 ox = {__shfl(x.x, ol), __shfl(x.y, ol), __shfl(x.z, ol)};
+            const f3 ov = {__shfl(v.x, ol), __shfl(v.y, ol), __shfl(v.z, ol)};
+            const float orad = __shfl(rad, ol), ow1 = __shfl(w1, ol);
+            const int oprox = __shfl(proxy, ol);
+            if (q < total) {
+                const int ot = tb + ol;
+                const float4 Pj = a.Xs[k], Vj = a.Vs[k];
+                const int nf = a.Fs[k];
+                const int oi = DIAG ? int(a.ids[ot]) : 0;
+                pp_pair<DIAG>(a, ot, k, Pj, Vj, nf, oi, oprox, ox, ov, orad, ow1, orad + a.max_r + a.coh, pd);
+            }
+            while (true) {
+                bool take = false;
+                int src = lane;
+                if (mn < cnt) {
+                    const uint4 L = s_lvl[mn];
+                    const uint64_t mk = uint64_t(L.x) | (uint64_t(L.y) << 32);
+                    const uint32_t pos = L.z + uint32_t(__popcll(mk & below));
+                    if (pos < base + 64u) { take = true; src = int(pos - base); }
+                }
+                if (!__any(take)) break;
+                const float rx = __shfl(pd.x, src), ry = __shfl(pd.y, src), rz = __shfl(pd.z, src);
+                if (take) {
+                    acc = acc + f3{rx, ry, rz};
+                    mn++;
+                }
+            }
+        }
+        cnt = 0;
+    } else {
+        drain();
+    }
+    if (!live) return;
     const f3 dw = {fma_(acc.x, w1, 0.f), fma_(acc.y, w1, 0.f), fma_(acc.z, w1, 0.f)};
     const f3 tot = {fma_(dw.x, a.relax, 0.f), fma_(dw.y, a.relax, 0.f), fma_(dw.z, a.relax, 0.f)};
     const f3 dl = f3{0.f, 0.f, 0.f} + tot;
