@@ -404,6 +404,30 @@ __global__ __launch_bounds__(kEnqBlock) void k_mesh_enqueue(MeshArgs a, int ci) 
 #pragma unroll
     for (int it = 0; it < kEnqItems; it++)
         if (start[it] != kNoNode) a.mq[pos++] = make_int2(i0 + 64 * it, int(start[it]));
+    if (a.plane && ci == 0) {
+        // generate_plane_contacts.comp:27-54: active particles within margin
+        // + radius of the plane, one contact slot reservation per block
+        bool want[kEnqItems];
+        int np = 0;
+#pragma unroll
+        for (int it = 0; it < kEnqItems; it++) {
+            want[it] = i0 + 64 * it < a.n && (f[it] & 1) && P[it].z - a.altitude < a.margin + P[it].w;
+            np += want[it] ? 1 : 0;
+        }
+        int slot = block_reserve(a.cc, np, s_tmp);
+#pragma unroll
+        for (int it = 0; it < kEnqItems; it++) {
+            if (!want[it]) continue;
+            if (slot < a.cmax) {
+                a.cpart[slot] = i0 + 64 * it;
+                a.cshape[slot] = 0;
+                a.cbpos[slot] = make_float4(P[it].x, P[it].y, a.altitude, 0.f);
+                a.cbvel[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
+                a.cnorm[slot] = make_float4(0.f, 0.f, 1.f, 0.f);
+            }
+            slot++;
+        }
+    }
     if (STATS) {
         uint32_t act = 0;
 #pragma unroll

@@ -348,11 +348,7 @@ __global__ __launch_bounds__(256) void k_cell_tables(int n, const uint32_t* __re
 __global__ __launch_bounds__(256) void k_shape_solve(ShapeArgs a) {
     const int count = min(a.cmax, *a.cc);
     for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < count; t += gridDim.x * blockDim.x) {
-        int pi = a.cpart[t];
-        if (a.remap) {   // contacts were generated in the previous slot order
-            pi = int(a.remap[pi]);
-            a.cpart[t] = pi;
-        }
+        const int pi = a.cpart[t];
         int s = a.cshape[t];
         int body = a.body.shape_body[s];
         int pf = a.flags[pi];
@@ -367,6 +363,7 @@ __global__ __launch_bounds__(256) void k_shape_solve(ShapeArgs a) {
             if (a.body.invm[body] == 0.0f) continue;
         }
         float4 P = a.P[pi], Vv = a.V[pi];
+        if (a.from_old) integrate_particle(P, Vv, pf, a.g_dt, a.g_vmax, a.gx, a.gy, a.gz);
         f3 px = xyz(P), pv = xyz(Vv);
         BodyFrame F = load_body(a.body, body);
         f3 bx = F.pos + quat_rotate_sc(F.rot, xyz(a.cbpos[t]));
@@ -935,20 +932,26 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6, 8))) v
 // (shape sum) + (the particle-particle delta k_pp_apply stored, 0 + pp, which
 // adds to any shape sum exactly as pp does, the sum never being -0), applied
 // as k_pp_apply applies it.  The owner clears the particle's shape sum and
-// mark for the next solve.
+// mark for the next solve.  After a solve in the previous slot order (remap
+// non-null) every contact is mapped to its particle's sorted slot here.
 __global__ __launch_bounds__(256) void k_shape_apply(ShapeApplyArgs a) {
     const int count = min(a.cmax, *a.cc);
     for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < count; t += gridDim.x * blockDim.x) {
-        const int pi = a.cpart[t];
+        const int pi = a.cpart[t];   // the slot dshape and owner use
+        int ps = pi;                 // the particle's slot in the sorted order
+        if (a.remap) {
+            ps = int(a.remap[pi]);
+            a.cpart[t] = ps;
+        }
         if (a.owner[pi] != t) continue;
         a.owner[pi] = -1;
         float* ds = a.dshape + 3 * pi;
-        float* dp = a.delta + 3 * pi;
+        float* dp = a.delta + 3 * ps;
         const f3 dl = f3{ds[0], ds[1], ds[2]} + f3{dp[0], dp[1], dp[2]};
         ds[0] = 0.0f; ds[1] = 0.0f; ds[2] = 0.0f;
         dp[0] = dl.x; dp[1] = dl.y; dp[2] = dl.z;
-        const float4 Pi = a.Xs[pi], Vi = a.Vs[pi];
-        apply_delta(xyz(a.Ps[a.vals[pi]]), pi, xyz(Pi), Pi.w, Vi.w, dl, a.dt, a.vmax, a.Po, a.Vo);
+        const float4 Pi = a.Xs[ps], Vi = a.Vs[ps];
+        apply_delta(xyz(a.Ps[a.vals[ps]]), ps, xyz(Pi), Pi.w, Vi.w, dl, a.dt, a.vmax, a.Po, a.Vo);
     }
 }
 

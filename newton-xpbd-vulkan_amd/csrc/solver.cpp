@@ -742,6 +742,10 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
     // the mesh walk is latency bound and fills few SIMDs: it overlaps the
     // bandwidth-bound integrate and sort (unless stage times are measured)
     const bool fork = collide && !meshes_.empty() && n > 0 && !profiling_;
+    // with a mesh, its candidate pass emits the plane contacts too (on the
+    // side stream), so the first shape solve needs nothing from the main one
+    const bool plane_in_mesh = collide && !meshes_.empty() && cfg_.has_ground_plane;
+    const int* flags_prev = flags_cur();   // the previous slot order's flags (the sort flips them)
     if (fork) {
         HIPCHECK(hipEventRecord(fork_ev_, stream_));
         HIPCHECK(hipStreamWaitEvent(side_, fork_ev_, 0));
@@ -759,6 +763,7 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
         m.cpart = cpart_.p; m.cshape = cshape_.p; m.cbpos = cbpos_.p; m.cbvel = cbvel_.p; m.cnorm = cnorm_.p;
         m.err = reinterpret_cast<uint32_t*>(cc_.p + 2);
         m.stats = mesh_stats_on_ ? mesh_stats_.p : nullptr;
+        m.plane = plane_in_mesh ? 1 : 0; m.altitude = cfg_.ground_plane_altitude;
         stage_begin(kMesh);
         launch_mesh_contacts(m, mesh_split_.data(), fork, fork ? side_ : stream_);
         stage_end(kMesh);
@@ -793,9 +798,9 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
         a.dt = sdt; a.vmax = cfg_.particle_v_max;
         a.gx = cfg_.gravity[0]; a.gy = cfg_.gravity[1]; a.gz = cfg_.gravity[2];
         a.inv_cell = inv_cell;
-        a.has_plane = (collide && cfg_.has_ground_plane) ? 1 : 0;
+        a.has_plane = (collide && cfg_.has_ground_plane && !plane_in_mesh) ? 1 : 0;
         a.margin = cfg_.soft_contact_margin; a.altitude = cfg_.ground_plane_altitude;
-        a.cmax = int(cmax_); a.cc = cc_gen;
+        a.cmax = int(cmax_); a.cc = plane_in_mesh ? nullptr : cc_gen;
         a.cc_next = zero_next ? cc_.p + (slot ^ 1) : nullptr;
         a.rcc_next = zero_next ? cc_.p + 4 + (slot ^ 1) : nullptr;
         a.mesh_next = zero_next ? cc_.p + 6 : nullptr;
@@ -813,6 +818,8 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
         stage_begin(kBodyIntegrate);
         launch_integrate_bodies(bs, stream_);
         stage_end(kBodyIntegrate);
+        // the first shape solve (side stream) reads the box contacts and poses
+        if (fork && iterations > 0 && !det_) HIPCHECK(hipEventRecord(sorted_ev_, stream_));
     }
     const int shape_grid = int(std::max<size_t>(1, std::min<size_t>((cmax_ + 255) / 256, 2048)));
     int cur = p_, nxt = o_;
@@ -826,7 +833,6 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
         if (gather_in_ranges) launch_cell_ranges_gather(n, keysA_.p, valsA_.p, range_.p, rkeys_.p, g, stream_);
         else build_ranges(keysA_.p, true);
         stage_end(kRanges);
-        if (fork && iterations > 0) HIPCHECK(hipEventRecord(sorted_ev_, stream_));
         grid_valid_ = true;
         if (det_ && iterations > 0) {
             // deterministic mode: every contact is in (mesh stage joined), then
@@ -852,8 +858,8 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
             }
         }
         // contacts were generated in the previous slot order: the first shape
-        // solve maps them through rank (the key kernel in deterministic mode);
-        // with no iteration a kernel does
+        // solve works in it and k_shape_apply maps them through rank (the key
+        // kernel in deterministic mode); with no iteration a kernel does
         if (iterations == 0) {   // no solve: the predicted state still moves to the sorted order
             if (fork) {
                 HIPCHECK(hipEventRecord(join_ev_, side_));
@@ -871,19 +877,24 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
         const float4* Vin = it == 0 ? Vs_[0].p : V_[cur].p;
         ShapeArgs sa;
         sa.cmax = int(cmax_); sa.cc = cc_solve;
-        sa.remap = (it == 0 && n > 0 && !det_) ? rank_.p : nullptr;
+        const bool prev_order = it == 0 && n > 0 && !det_;
         sa.cfd = det_ ? cfd_.p : nullptr;
         sa.cbd = det_ ? cbd_.p : nullptr;
         sa.cpart = cpart_.p; sa.cshape = cshape_.p; sa.cbpos = cbpos_.p; sa.cbvel = cbvel_.p; sa.cnorm = cnorm_.p;
-        sa.P = Xin; sa.V = Vin; sa.flags = flags_cur();
+        sa.P = prev_order ? P_[s_].p : Xin; sa.V = prev_order ? V_[s_].p : Vin;
+        sa.flags = prev_order ? flags_prev : flags_cur();
+        sa.from_old = prev_order ? 1 : 0;
+        sa.g_dt = sdt; sa.g_vmax = cfg_.particle_v_max;
+        sa.gx = cfg_.gravity[0]; sa.gy = cfg_.gravity[1]; sa.gz = cfg_.gravity[2];
         sa.body = body_args();
         sa.pmu = cfg_.soft_contact_mu; sa.ka = cfg_.particle_adhesion; sa.dt = sdt; sa.relax = relax_;
         sa.dshape = dshape_.p; sa.owner = owner_.p;
-        // the first shape solve waits for the mesh contacts, on the side
-        // stream, while the particle-particle solve (which no longer reads
-        // the shape deltas) runs; k_shape_apply joins the two
+        // the first shape solve follows the mesh contacts on the side stream,
+        // from the previous slot order, so it waits for nothing on the main
+        // stream but the boxes; the particle-particle solve (which does not
+        // read the shape deltas) runs meanwhile and k_shape_apply joins the two
         const bool side = fork && it == 0 && !det_;
-        if (side) HIPCHECK(hipStreamWaitEvent(side_, sorted_ev_, 0));
+        if (side && nbox > 0) HIPCHECK(hipStreamWaitEvent(side_, sorted_ev_, 0));
         stage_begin(kShape);
         launch_shape_solve(sa, shape_grid, side ? side_ : stream_);
         stage_end(kShape);
@@ -910,6 +921,7 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
         } else {
             ShapeApplyArgs aa;
             aa.cmax = int(cmax_); aa.cc = cc_solve; aa.cpart = cpart_.p;
+            aa.remap = prev_order ? rank_.p : nullptr;
             aa.owner = owner_.p; aa.dshape = dshape_.p; aa.delta = delta_.p;
             aa.Ps = P_[s_].p; aa.vals = valsA_.p; aa.Xs = Xin; aa.Vs = Vin;
             aa.Po = P_[nxt].p; aa.Vo = V_[nxt].p;

@@ -90,9 +90,13 @@ __device__ __forceinline__ BodyFrame load_body(const BodyArgs& b, int body) {
 
 struct ShapeArgs {
     int cmax; const int* cc;
-    const uint32_t* remap;      // non-null: map each contact's slot through it first (and store it)
     int* cpart; const int* cshape; const float4* cbpos; const float4* cbvel; const float4* cnorm;
+    // the particle state at the contacts' slots; from_old: the substep-start
+    // state of the previous slot order (where the contacts were generated),
+    // integrated here over g_dt exactly as the sorted copy is (so the solve
+    // need not wait for the sort), and dshape/owner indexed by those slots
     const float4* P; const float4* V; const int* flags;
+    int from_old; float g_dt, g_vmax, gx, gy, gz;
     BodyArgs body;
     float pmu, ka, dt, relax;
     float* dshape;
@@ -107,7 +111,8 @@ struct ShapeArgs {
 // wrote every particle with its particle-particle delta alone.
 struct ShapeApplyArgs {
     int cmax; const int* cc;
-    const int* cpart;
+    int* cpart;
+    const uint32_t* remap;   // non-null: contacts, dshape and owner name previous slots; cpart is mapped through it
     int* owner; float* dshape; float* delta;        // sorted order; owner and dshape left cleared
     const float4* Ps; const uint32_t* vals;         // substep-start positions, sorted slot -> their slot
     const float4* Xs; const float4* Vs;             // iteration input state, sorted order
@@ -185,6 +190,9 @@ struct MeshArgs {
     uint32_t* mcnt;             // per (substep parity, mesh): queue length and parity-queue length
     int slot;                   // substep parity: which counter pair this substep uses
     int2* mq;                   // candidate queue: (particle, search start node word)
+    // plane > 0: the first mesh's candidate pass also emits the ground-plane
+    // contacts (generate_plane_contacts on the same substep-start positions)
+    int plane = 0; float altitude = 0.0f;
 };
 
 // Rigid bodies (dynamic bodies occupy indices [b0, b1) of the body arrays).
