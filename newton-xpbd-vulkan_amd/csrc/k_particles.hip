@@ -342,14 +342,82 @@ __global__ __launch_bounds__(256) void k_cell_tables(int n, const uint32_t* __re
     if (t == n - 1) ends[c] = t + 1;
 }
 
+// One particle-shape contact of solve_particle_shape_contacts.comp:121-188:
+// the particle's state (P, Vv), the shape's body (-1: none) and friction mu,
+// the contact's body-local point and velocity and its normal.  Returns false
+// where the shader adds nothing (farther than the adhesion distance, or no
+// inverse mass on either side); else the particle's delta fd, the unweighted
+// delta dtot and the lever arm r.  The shape solve runs every contact
+// through it, and the particle-pair solve and the shape-delta apply run the
+// ground plane's through it (body -1, the contact a pure function of the
+// particle's substep-start position), so all three compute the same floats.
+__device__ __forceinline__ bool shape_contact(float4 P, float4 Vv, int body, const BodyArgs& B, float mu, f3 cbpos,
+                                              f3 cbvel, f3 n, float ka, float dt, float relax, f3& fd, f3& dtot, f3& r) {
+    f3 px = xyz(P), pv = xyz(Vv);
+    BodyFrame F = load_body(B, body);
+    f3 bx = F.pos + quat_rotate_sc(F.rot, cbpos);
+    r = bx - (F.pos + quat_rotate_sc(F.rot, F.com));
+    f3 d = px - bx;
+    float c = fma_(n.z, d.z, fma_(n.x, d.x, n.y * d.y)) - P.w;
+    if (c > ka) return false;
+    f3 bv = F.v + cross3(F.w, r) + quat_rotate_sc(F.rot, cbvel);
+    f3 v = pv - bv;
+    float lambda_n = c;
+    f3 delta_n = n * lambda_n;
+    float vn = n.y * v.y;
+    vn = fma_(n.x, v.x, vn);
+    vn = fma_(n.z, v.z, vn);
+    f3 vt = {fma_(-n.x, vn, v.x), fma_(-n.y, vn, v.y), fma_(-n.z, vn, v.z)};
+    float w1 = Vv.w;
+    float w2 = 0.0f;
+    if (body >= 0) {
+        f3 ang = cross3(r, n);
+        f3 ra = quat_rotate_inv_sc(F.rot, ang);
+        const float* I = B.invI + 9 * body;   // row-major, as this shader reads it
+        f3 Ira = {I[0] * ra.x + I[1] * ra.y + I[2] * ra.z, I[3] * ra.x + I[4] * ra.y + I[5] * ra.z,
+                  I[6] * ra.x + I[7] * ra.y + I[8] * ra.z};
+        w2 = B.invm[body] + dot3(ra, Ira);
+    }
+    float denom = w1 + w2;
+    if (denom == 0.0f) return false;
+    float l2 = vt.y * vt.y;
+    l2 = fma_(vt.x, vt.x, l2);
+    l2 = fma_(vt.z, vt.z, l2);
+    const RootInv vl = exact_sqrt_inv(l2);
+    const float vt_len = vl.d;
+    float lambda_f = gmax(mu * lambda_n, -vt_len * dt);
+    f3 vt_dir = vt_len > 0.0f ? exact_div_by(vt, vt_len, vl.inv) : f3{0.f, 0.f, 0.f};
+    f3 diff = {fma_(vt_dir.x, lambda_f, -delta_n.x), fma_(vt_dir.y, lambda_f, -delta_n.y),
+               fma_(vt_dir.z, lambda_f, -delta_n.z)};
+    dtot = exact_div(diff, denom) * relax;
+    fd = w1 * dtot;
+    return true;
+}
+
+// The ground plane's contact of a particle: generate_plane_contacts.comp:
+// 27-54 emits it from the substep-start position x0 (active particles within
+// margin + radius of the plane; a proxy's plane contact adds nothing,
+// solve_particle_shape_contacts.comp:115-116),
+// and shape_contact solves it on the iteration's state (P, Vv).
+__device__ __forceinline__ bool plane_contact(float4 x0, float4 P, float4 Vv, int pf, float altitude, float margin,
+                                              float mu, float ka, float dt, float relax, f3& fd) {
+    if ((pf & 3) != 1 || !(x0.z - altitude < margin + x0.w)) return false;
+    const BodyArgs none{};
+    f3 dtot, r;
+    return shape_contact(P, Vv, -1, none, mu, f3{x0.x, x0.y, altitude}, f3{0.f, 0.f, 0.f}, f3{0.f, 0.f, 1.f}, ka, dt,
+                         relax, fd, dtot, r);
+}
+
 // solve_particle_shape_contacts.comp:104-200.  Accumulates into dshape and
 // marks each receiving particle with its largest contact index (owner), which
-// k_shape_apply consumes and clears.
+// k_shape_apply consumes and clears.  skip_plane: the ground plane's contacts
+// are taken by the particle-pair solve and the shape-delta apply instead.
 __global__ __launch_bounds__(256) void k_shape_solve(ShapeArgs a) {
     const int count = min(a.cmax, *a.cc);
     for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < count; t += gridDim.x * blockDim.x) {
         const int pi = a.cpart[t];
         int s = a.cshape[t];
+        if (a.skip_plane && s == 0) continue;
         int body = a.body.shape_body[s];
         int pf = a.flags[pi];
         if (a.cfd) {   // deterministic mode: no contribution unless stored below
@@ -364,46 +432,11 @@ __global__ __launch_bounds__(256) void k_shape_solve(ShapeArgs a) {
         }
         float4 P = a.P[pi], Vv = a.V[pi];
         if (a.from_old) integrate_particle(P, Vv, pf, a.g_dt, a.g_vmax, a.gx, a.gy, a.gz);
-        f3 px = xyz(P), pv = xyz(Vv);
-        BodyFrame F = load_body(a.body, body);
-        f3 bx = F.pos + quat_rotate_sc(F.rot, xyz(a.cbpos[t]));
-        f3 r = bx - (F.pos + quat_rotate_sc(F.rot, F.com));
-        f3 n = xyz(a.cnorm[t]);
-        f3 d = px - bx;
-        float c = fma_(n.z, d.z, fma_(n.x, d.x, n.y * d.y)) - P.w;
-        if (c > a.ka) continue;
-        float mu = 0.5f * (a.pmu + a.body.mat_mu[s]);
-        f3 bv = F.v + cross3(F.w, r) + quat_rotate_sc(F.rot, xyz(a.cbvel[t]));
-        f3 v = pv - bv;
-        float lambda_n = c;
-        f3 delta_n = n * lambda_n;
-        float vn = n.y * v.y;
-        vn = fma_(n.x, v.x, vn);
-        vn = fma_(n.z, v.z, vn);
-        f3 vt = {fma_(-n.x, vn, v.x), fma_(-n.y, vn, v.y), fma_(-n.z, vn, v.z)};
-        float w1 = Vv.w;
-        float w2 = 0.0f;
-        if (body >= 0) {
-            f3 ang = cross3(r, n);
-            f3 ra = quat_rotate_inv_sc(F.rot, ang);
-            const float* I = a.body.invI + 9 * body;   // row-major, as this shader reads it
-            f3 Ira = {I[0] * ra.x + I[1] * ra.y + I[2] * ra.z, I[3] * ra.x + I[4] * ra.y + I[5] * ra.z,
-                      I[6] * ra.x + I[7] * ra.y + I[8] * ra.z};
-            w2 = a.body.invm[body] + dot3(ra, Ira);
-        }
-        float denom = w1 + w2;
-        if (denom == 0.0f) continue;
-        float l2 = vt.y * vt.y;
-        l2 = fma_(vt.x, vt.x, l2);
-        l2 = fma_(vt.z, vt.z, l2);
-        const RootInv vl = exact_sqrt_inv(l2);
-        const float vt_len = vl.d;
-        float lambda_f = gmax(mu * lambda_n, -vt_len * a.dt);
-        f3 vt_dir = vt_len > 0.0f ? exact_div_by(vt, vt_len, vl.inv) : f3{0.f, 0.f, 0.f};
-        f3 diff = {fma_(vt_dir.x, lambda_f, -delta_n.x), fma_(vt_dir.y, lambda_f, -delta_n.y),
-                   fma_(vt_dir.z, lambda_f, -delta_n.z)};
-        f3 dtot = exact_div(diff, denom) * a.relax;
-        f3 fd = w1 * dtot;
+        const float mu = 0.5f * (a.pmu + a.body.mat_mu[s]);
+        f3 fd, dtot, r;
+        if (!shape_contact(P, Vv, body, a.body, mu, xyz(a.cbpos[t]), xyz(a.cbvel[t]), xyz(a.cnorm[t]), a.ka, a.dt,
+                           a.relax, fd, dtot, r))
+            continue;
         if (a.cfd) {
             a.cfd[t] = make_float4(fd.x, fd.y, fd.z, 1.f);
             if (body >= 0 && a.body.invm[body] > 0.0f) {
@@ -921,9 +954,19 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6, 8))) v
     if (!live) return;
     const f3 dw = {fma_(acc.x, w1, 0.f), fma_(acc.y, w1, 0.f), fma_(acc.z, w1, 0.f)};
     const f3 tot = {fma_(dw.x, a.relax, 0.f), fma_(dw.y, a.relax, 0.f), fma_(dw.z, a.relax, 0.f)};
-    const f3 dl = f3{0.f, 0.f, 0.f} + tot;
+    const float4 x0 = a.Ps[a.vals[t]];
+    f3 dl = f3{0.f, 0.f, 0.f} + tot;
+    if (a.plane_fold) {
+        // the reference's delta buffer: 0 + the plane's delta (the shape
+        // dispatch), then + this particle's pair sum
+        f3 fd;
+        if (plane_contact(x0, Pi, Vi, pf, a.altitude, a.margin, 0.5f * (a.pmu + a.mat_mu0), a.ka, a.dt, a.relax, fd)) {
+            a.ppd[3 * t] = dl.x; a.ppd[3 * t + 1] = dl.y; a.ppd[3 * t + 2] = dl.z;
+            dl = (f3{0.f, 0.f, 0.f} + fd) + tot;
+        }
+    }
     a.delta[3 * t] = dl.x; a.delta[3 * t + 1] = dl.y; a.delta[3 * t + 2] = dl.z;
-    apply_delta(xyz(a.Ps[a.vals[t]]), t, x, rad, w1, dl, a.dt, a.vmax, a.Po, a.Vo);
+    apply_delta(xyz(x0), t, x, rad, w1, dl, a.dt, a.vmax, a.Po, a.Vo);
 }
 
 // The shape half of the delta apply: runs after both k_shape_solve and
@@ -947,11 +990,22 @@ __global__ __launch_bounds__(256) void k_shape_apply(ShapeApplyArgs a) {
         a.owner[pi] = -1;
         float* ds = a.dshape + 3 * pi;
         float* dp = a.delta + 3 * ps;
-        const f3 dl = f3{ds[0], ds[1], ds[2]} + f3{dp[0], dp[1], dp[2]};
+        const float4 Pi = a.Xs[ps], Vi = a.Vs[ps];
+        const float4 x0 = a.Ps[a.vals[ps]];
+        f3 sh = f3{ds[0], ds[1], ds[2]};
+        f3 pp = f3{dp[0], dp[1], dp[2]};
+        if (a.plane_fold) {   // its plane contact was added by the particle-pair solve
+            f3 fd;
+            if (plane_contact(x0, Pi, Vi, a.flags[ps], a.altitude, a.margin, 0.5f * (a.pmu + a.mat_mu0), a.ka, a.dt,
+                              a.relax, fd)) {
+                sh = sh + fd;
+                pp = f3{a.ppd[3 * ps], a.ppd[3 * ps + 1], a.ppd[3 * ps + 2]};
+            }
+        }
+        const f3 dl = sh + pp;
         ds[0] = 0.0f; ds[1] = 0.0f; ds[2] = 0.0f;
         dp[0] = dl.x; dp[1] = dl.y; dp[2] = dl.z;
-        const float4 Pi = a.Xs[ps], Vi = a.Vs[ps];
-        apply_delta(xyz(a.Ps[a.vals[ps]]), ps, xyz(Pi), Pi.w, Vi.w, dl, a.dt, a.vmax, a.Po, a.Vo);
+        apply_delta(xyz(x0), ps, xyz(Pi), Pi.w, Vi.w, dl, a.dt, a.vmax, a.Po, a.Vo);
     }
 }
 

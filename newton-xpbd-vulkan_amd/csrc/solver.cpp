@@ -61,7 +61,7 @@ void Solver::free_all() {
     dfree(body_half_); dfree(body_I_); dfree(body_invw_); dfree(mat_mu_t_); dfree(mat_mu_r_);
     dfree(rc_s0_); dfree(rc_s1_); dfree(rc_p0_); dfree(rc_p1_); dfree(rc_n_); dfree(rc_impulse_);
     dfree(range_); dfree(rkeys_); dfree(rows_); dfree(glb_); dfree(rslot_); dfree(gtcnt_); dfree(gtoff_); dfree(tbl_); dfree(cpart_); dfree(cshape_); dfree(cc_); dfree(cbpos_); dfree(cbvel_); dfree(cnorm_);
-    dfree(dshape_); dfree(delta_); dfree(owner_); dfree(body_q_); dfree(body_qd_); dfree(body_com_); dfree(body_invI_);
+    dfree(dshape_); dfree(delta_); dfree(ppd_); dfree(owner_); dfree(body_q_); dfree(body_qd_); dfree(body_com_); dfree(body_invI_);
     dfree(body_invm_); dfree(body_delta_); dfree(mat_mu_); dfree(body_flags_); dfree(shape_body_);
     dfree(verts_d_); dfree(vvel_d_); dfree(tris_d_); dfree(prim_d_); dfree(err_d_); dfree(nodes_d_); dfree(cnodes_d_); dfree(lbox_d_); dfree(pg_start_); dfree(lb_); dfree(lb_node_); dfree(tsoup_); dfree(pg_soup_); dfree(mcnt_); dfree(mq_);
     dfree(info_d_); dfree(diag_header_); dfree(diag_pairs_); dfree(mesh_stats_);
@@ -119,7 +119,7 @@ void Solver::alloc_all() {
     dalloc(cpart_, cmax_); dalloc(cshape_, cmax_);
     dalloc(cbpos_, cmax_); dalloc(cbvel_, cmax_); dalloc(cnorm_, cmax_);
     dalloc(cc_, 8);   // [particle slot 0, slot 1, error flags, spare, rigid slot 0, slot 1, spare, spare]
-    dalloc(dshape_, 3 * n_); dalloc(delta_, 3 * n_);
+    dalloc(dshape_, 3 * n_); dalloc(delta_, 3 * n_); dalloc(ppd_, 3 * n_);
     dalloc(owner_, n_);
     dfill(owner_, 0xff);   // -1: no shape delta pending
     dalloc(diag_header_, 2);
@@ -746,6 +746,11 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
     // side stream), so the first shape solve needs nothing from the main one
     const bool plane_in_mesh = collide && !meshes_.empty() && cfg_.has_ground_plane;
     const int* flags_prev = flags_cur();   // the previous slot order's flags (the sort flips them)
+    // the ground plane's contacts solved inside the particle-pair solve (the
+    // shape solve then has only the mesh contacts): every plane contact the
+    // candidate pass emits must be stored (no overflow possible) and no box
+    // shares the shape dispatch
+    const bool plane_fold = plane_in_mesh && !det_ && nbox == 0 && cmax_ >= n_ * (1 + size_t(M));
     if (fork) {
         HIPCHECK(hipEventRecord(fork_ev_, stream_));
         HIPCHECK(hipStreamWaitEvent(side_, fork_ev_, 0));
@@ -889,6 +894,7 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
         sa.body = body_args();
         sa.pmu = cfg_.soft_contact_mu; sa.ka = cfg_.particle_adhesion; sa.dt = sdt; sa.relax = relax_;
         sa.dshape = dshape_.p; sa.owner = owner_.p;
+        sa.skip_plane = plane_fold ? 1 : 0;
         // the first shape solve follows the mesh contacts on the side stream,
         // from the previous slot order, so it waits for nothing on the main
         // stream but the boxes; the particle-particle solve (which does not
@@ -908,6 +914,9 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
         pa.rmax = rmax_;
         pa.planes = planes_ ? plane_bits_ : 0;
         pa.rowtab = rowtab();
+        pa.plane_fold = plane_fold ? 1 : 0;
+        pa.altitude = cfg_.ground_plane_altitude; pa.margin = cfg_.soft_contact_margin; pa.ka = cfg_.particle_adhesion;
+        pa.pmu = cfg_.soft_contact_mu; pa.mat_mu0 = cfg_.shape_material_mu; pa.ppd = ppd_.p;
         pa.iter_index = uint32_t(it); pa.max_diag_pairs = max_pairs_;
         pa.diag_header = diag_header_.p; pa.diag_pairs = diag_pairs_.p;
         stage_begin(kPP);
@@ -926,6 +935,9 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
             aa.Ps = P_[s_].p; aa.vals = valsA_.p; aa.Xs = Xin; aa.Vs = Vin;
             aa.Po = P_[nxt].p; aa.Vo = V_[nxt].p;
             aa.dt = sdt; aa.vmax = cfg_.particle_v_max;
+            aa.plane_fold = plane_fold ? 1 : 0; aa.flags = flags_cur(); aa.ppd = ppd_.p;
+            aa.altitude = cfg_.ground_plane_altitude; aa.margin = cfg_.soft_contact_margin; aa.ka = cfg_.particle_adhesion;
+            aa.pmu = cfg_.soft_contact_mu; aa.mat_mu0 = cfg_.shape_material_mu; aa.relax = relax_;
             launch_shape_apply(aa, shape_grid, stream_);
         }
         stage_end(kShapeApply);

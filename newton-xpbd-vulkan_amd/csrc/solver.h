@@ -278,6 +278,7 @@ private:
     int contact_slot_ = 0, last_count_ = 0;
     int32_t* count_host_ = nullptr;   // pinned
     DevBuf<float> dshape_, delta_;
+    DevBuf<float> ppd_;   // 0 + pair sum of particles whose plane contact the pair solve added
     DevBuf<int> owner_;   // per sorted slot: contact that finishes its shape delta, else -1
     // bodies (one static body per mesh, scene.cpp:220-238) and shapes
     DevBuf<float4> body_q_, body_qd_, body_half_;

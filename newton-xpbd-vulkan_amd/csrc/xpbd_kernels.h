@@ -97,6 +97,7 @@ struct ShapeArgs {
     // need not wait for the sort), and dshape/owner indexed by those slots
     const float4* P; const float4* V; const int* flags;
     int from_old; float g_dt, g_vmax, gx, gy, gz;
+    int skip_plane;             // the ground plane's contacts are left to the particle-pair solve
     BodyArgs body;
     float pmu, ka, dt, relax;
     float* dshape;
@@ -114,6 +115,11 @@ struct ShapeApplyArgs {
     int* cpart;
     const uint32_t* remap;   // non-null: contacts, dshape and owner name previous slots; cpart is mapped through it
     int* owner; float* dshape; float* delta;        // sorted order; owner and dshape left cleared
+    // plane_fold: the particle-pair solve added each particle's ground-plane
+    // contact (and kept its own 0 + pair sum in ppd); a particle with shape
+    // contacts as well gets (its shape sum + the plane's) + that sum here
+    int plane_fold; const int* flags; const float* ppd;
+    float altitude, margin, ka, pmu, mat_mu0, relax;
     const float4* Ps; const uint32_t* vals;         // substep-start positions, sorted slot -> their slot
     const float4* Xs; const float4* Vs;             // iteration input state, sorted order
     float4* Po; float4* Vo;                         // output state, sorted order
@@ -134,6 +140,10 @@ struct PPArgs {
     float rmax;                                     // largest particle radius (bounds every accepted pair)
     int planes;                                     // keys carry the wrap plane above the cell
     RowTable rowtab;                                // batched solvers: the cell ranges
+    // plane_fold: each particle's ground-plane contact (generated from its
+    // substep-start position, solved as the shape solve would) is added here
+    // ahead of the pair sum; ppd receives 0 + the pair sum of such particles
+    int plane_fold; float altitude, margin, ka, pmu, mat_mu0; float* ppd;
     uint32_t iter_index, max_diag_pairs;
     uint32_t* diag_header; float* diag_pairs;
 };
