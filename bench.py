@@ -47,59 +47,64 @@ def survey_bytes(stage, N, C, Cp, Cm, U, U_prev, mesh_bytes):
     as cache hits), one iteration per substep.  Stages of this build that fuse
     several reference stages count the sum of theirs; the radix sort is the
     model's fixed 36 bytes per particle for the whole sort of a substep."""
-    if stage == "integrate":        # plane contacts + integrate + cell index + clear of the last rows
-        return (24 + 124 + 24) * N + 44 * Cp + 8 * U_prev
+    if stage == "integrate":        # integrate + cell index + clear of the last rows
+        return (124 + 24) * N + 8 * U_prev
     if stage == "sort":             # every pass of a substep together
         return 36 * N
     if stage == "cell_ranges":      # grid offsets
         return 4 * N + 8 * U
-    if stage == "shape_solve":
-        return 112 * C
-    if stage == "pp_solve_apply":   # particle-particle solve + apply
-        return (72 + 80) * N + 8 * U
-    if stage == "mesh_contacts":    # the mesh tables once per frame, i.e. an eighth per substep
-        return 24 * N + 44 * Cm + mesh_bytes / SUBSTEPS
+    if stage == "shape_solve":      # the mesh contacts' shape solve
+        return 112 * Cm
+    if stage == "pp_solve_apply":   # particle-particle solve + apply, and the plane contacts' shape solve
+        return (72 + 80) * N + 8 * U + 112 * Cp
+    if stage == "mesh_contacts":    # mesh contacts (the mesh tables once per frame, i.e. an eighth per
+        # substep) + plane contacts
+        return 24 * N + 44 * Cm + mesh_bytes / SUBSTEPS + 24 * N + 44 * Cp
     return None
 
 
 def stage_bytes(stage, N, C, Cm, U, tiles, mesh_bytes, passes, bins=128):
+    Cp = C - Cm
     """This build's own count of the HBM bytes of ONE launch of a stage:
     every array the kernel must touch in its actual layout, counted once
     (neighbour re-reads are cache hits).  The state lives in sorted slot
     order; the sort runs `passes` LSD passes of `bins` digit values over
     `tiles` tiles."""
     hist = tiles * bins * 4
-    if stage == "integrate":      # P,V,flags,ids in; (key, slot) pair out; plane contacts;
+    if stage == "integrate":      # P,V,flags,ids in; (key, slot) pair out;
         # reset of the last substep's cell rows (their keys, zeroed rows)
-        return N * (16 + 16 + 4 + 4 + 8) + 56 * (C - Cm) + 4 * N + 8 * U
+        return N * (16 + 16 + 4 + 4 + 8) + 4 * N + 8 * U
     if stage == "sort_upsweep":   # pass 0 reads the pairs, the others the keys
         return (8 * N + 4 * N * (passes - 1)) / passes + hist
     if stage == "sort_binscan":
         return 2 * hist
-    if stage == "sort_scatter":   # mean over the passes; the last also integrates and stores P,V, moves flags,id, writes rank
-        return (16 * N * passes + (16 + 16 + 4 + 4) * 2 * N + 4 * N) / passes + hist
-    if stage == "cell_ranges":    # sorted keys, new rows, key record
-        return 4 * N + 8 * U + 4 * N
-    if stage == "shape_solve":    # contact record, slot remap, particle P,V gather, delta RMW, owner
-        return C * (56 + 8 + 32 + 24 + 4)
-    if stage == "pp_solve_apply":  # vals, flags, Xs/Vs, x0, delta, P/V out; cell ranges
-        return N * (4 + 4 + 32 + 16 + 12 + 32) + 8 * U
-    if stage == "shape_apply":     # per contact: particle slot, owner; per owner (taken as every
-        # contact): owner clear, shape delta rw, delta rw, Xs/Vs, slot + x0, P/V out
-        return C * (4 + 4 + 4 + 24 + 24 + 32 + 4 + 16 + 32)
-    if stage == "mesh_contacts":
-        return N * (16 + 4) + 56 * Cm + mesh_bytes
+    if stage == "sort_scatter":   # keys and slots in and out, per pass
+        return 16 * N + hist
+    if stage == "cell_ranges":    # sorted keys, new rows, key record; the state's move into the
+        # sorted order: slot, P,V,flags,id gathered, integrated P,V, flags, id and rank stored
+        return 4 * N + 8 * U + 4 * N + N * (4 + 32 + 4 + 4 + 32 + 4 + 4 + 4)
+    if stage == "shape_solve":    # mesh contacts: record, substep-start P,V and flags, delta RMW, owner;
+        # plane contacts: shape and slot read, skipped
+        return Cm * (56 + 4 + 32 + 4 + 24 + 4) + Cp * 8
+    if stage == "pp_solve_apply":  # vals, flags, Xs/Vs, x0, delta, P/V out; cell ranges; pair sums
+        # kept for particles with a plane contact
+        return N * (4 + 4 + 32 + 16 + 12 + 32) + 8 * U + 12 * Cp
+    if stage == "shape_apply":     # per contact: slot, new slot, slot store, owner; per owner (the
+        # mesh contacts): owner clear, shape delta rw, delta rw, Xs/Vs, slot + x0, flags, pair sum, P/V out
+        return C * 16 + Cm * (4 + 24 + 24 + 32 + 4 + 16 + 4 + 12 + 32)
+    if stage == "mesh_contacts":   # particles in, mesh and plane contacts out, mesh tables
+        return N * (16 + 4) + 56 * C + mesh_bytes
     return 0
 
 
 # profiler stage -> kernels it brackets (for the measured-traffic lookup)
 STAGE_KERNELS = {
-    "mesh_contacts": ["k_mesh_enqueue", "k_mesh_walk", "k_mesh_parity"],
+    "mesh_contacts": ["k_mesh_enqueue", "k_mesh_walk"],
     "integrate": ["k_integrate"],
     "sort_upsweep": ["k_sort_upsweep"],
     "sort_binscan": ["k_sort_binscan"],
     "sort_scatter": ["k_sort_scatter"],
-    "cell_ranges": ["k_fill_zero", "k_clear_ranges", "k_cell_ranges", "k_remap_contacts"],
+    "cell_ranges": ["k_fill_zero", "k_clear_ranges", "k_cell_ranges", "k_cell_ranges_gather", "k_remap_contacts"],
     "shape_solve": ["k_shape_solve"],
     "pp_solve_apply": ["k_pp_apply"],
     "shape_apply": ["k_shape_apply"],
@@ -441,7 +446,8 @@ def main():
         ach = d["survey_bytes"] / (d["avg_us"] * 1e-6) / 1e9
         out["roofline"] = {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                            "frac": ach / HBM_PEAK_GBS, "traffic": measured_traffic(dom),
-                           "alg_bytes_per_launch": d["survey_bytes"], "bytes_model": "reference layout, 152N + 8U",
+                           "alg_bytes_per_launch": d["survey_bytes"],
+                           "bytes_model": "reference layout, 152N + 8U + 112Cp (the plane contacts' shape solve runs inside)",
                            "avg_us": d["avg_us"]}
         out["kernels"] = {}
         for k, v in kern.items():
