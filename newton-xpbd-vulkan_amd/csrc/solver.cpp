@@ -747,10 +747,12 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
     const bool plane_in_mesh = collide && !meshes_.empty() && cfg_.has_ground_plane;
     const int* flags_prev = flags_cur();   // the previous slot order's flags (the sort flips them)
     // the ground plane's contacts solved inside the particle-pair solve (the
-    // shape solve then has only the mesh contacts): every plane contact the
-    // candidate pass emits must be stored (no overflow possible) and no box
-    // shares the shape dispatch
-    const bool plane_fold = plane_in_mesh && !det_ && nbox == 0 && cmax_ >= n_ * (1 + size_t(M));
+    // shape solve then has only the mesh contacts): every plane contact
+    // emitted must be stored (no overflow possible) and no box shares the
+    // shape dispatch
+    const bool plane_fold = collide && cfg_.has_ground_plane && !det_ && nbox == 0 && cmax_ >= n_ * (1 + size_t(M));
+    // without meshes and boxes every contact is the plane's: no shape solve
+    const bool shape_work = !(plane_fold && M == 0);
     if (fork) {
         HIPCHECK(hipEventRecord(fork_ev_, stream_));
         HIPCHECK(hipStreamWaitEvent(side_, fork_ev_, 0));
@@ -902,7 +904,7 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
         const bool side = fork && it == 0 && !det_;
         if (side && nbox > 0) HIPCHECK(hipStreamWaitEvent(side_, sorted_ev_, 0));
         stage_begin(kShape);
-        launch_shape_solve(sa, shape_grid, side ? side_ : stream_);
+        if (shape_work) launch_shape_solve(sa, shape_grid, side ? side_ : stream_);
         stage_end(kShape);
         if (side) HIPCHECK(hipEventRecord(join_ev_, side_));
         PPArgs pa;
@@ -927,6 +929,8 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
         if (det_) {
             DetApplyArgs da = det_apply_args(Xin, Vin, nxt, sdt);
             launch_det_shape_apply(da, stream_);
+        } else if (!shape_work) {   // nothing to apply; the contacts still move to the sorted slots
+            if (prev_order) launch_remap_contacts(cpart_.p, cc_solve, int(cmax_), rank_.p, shape_grid, stream_);
         } else {
             ShapeApplyArgs aa;
             aa.cmax = int(cmax_); aa.cc = cc_solve; aa.cpart = cpart_.p;
