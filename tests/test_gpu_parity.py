@@ -327,12 +327,18 @@ def test_diagnostic_pair_records(vk):
     assert g.get_diag_pair_count() == 0
 
 
-def test_particles_touching_plane_and_mesh(vk):
+@pytest.mark.parametrize("max_contacts", [0, 5999], ids=["plane_in_pair_solve", "plane_in_shape_solve"])
+def test_particles_touching_plane_and_mesh(vk, max_contacts):
     """Particles in the wedge where a sphere meets the ground carry a plane
     and a mesh contact at once: their two shape deltas are summed and applied
     after the particle-pair solve by the owner mark (the larger contact
     index).  States, deltas and contacts equal the oracle's bit for bit over
-    several substeps, and particles with both contacts are present."""
+    several substeps, and particles with both contacts are present.  With
+    the default contact capacity (2N) the plane contacts are solved inside
+    the particle-pair solve; with room for fewer than every possible contact
+    (2N - 1) they go through the shape solve.  1,000 particles high above
+    the scene keep the contacts emitted (at most two per wedge particle)
+    below either capacity, so both runs must give the oracle's bits."""
     rng = np.random.default_rng(31)
     r = 0.005
     n = 2000
@@ -340,13 +346,16 @@ def test_particles_touching_plane_and_mesh(vk):
     z = rng.uniform(0.004, 0.012, n)
     rho = np.sqrt(0.5 ** 2 - (0.5 - z) ** 2) + rng.uniform(0.0, 0.012, n)   # just outside the sphere
     q = np.stack([rho * np.cos(ang), rho * np.sin(ang), z], 1).astype(np.float32)
+    far = scenes.lattice(10, 10, 10, 3 * r, (-0.05, -0.05, 3.0), 0.0, rng)   # nothing within reach
+    q = np.concatenate([q, far]).astype(np.float32)
     verts, tris = scenes.icosphere(4)
     sc = scene_from(q, r, meshes=[(verts, tris)])
-    g, o = make_pair(vk, sc)
+    g, o = make_pair(vk, sc, max_contacts=max_contacts)
     for f in range(3):
         g.step(1.0 / 60.0, 4, 1)
         o.step(1.0 / 60.0, 4, 1)
         assert g.last_contact_count == o.last_contact_count
+        assert g.last_contact_count <= 2 * n   # every emitted contact was kept
         compare_state(g, o, exact=True, what=f"wedge frame {f + 1}")
     np.testing.assert_array_equal(g.get_deltas().astype(np.float64), o.get_deltas())
     c = gpu_contacts(g)
