@@ -699,42 +699,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6, 8))) v
             cx += len;
         }
     };
-    if constexpr (BATCHED) {
-        // Rows (cz, cy) in the reference's order.  Every row's first x-run
-        // starts at xs, at the same offset xa in every row; its slots are two
-        // bounds behind the row's table entry.  Entries are requested two
-        // rows ahead and bounds one row ahead, so neither wait falls on a
-        // scan.
-        const RowTable& T = a.rowtab;
-        auto rowkey = [&](int cy, int cz) -> uint32_t { return sbase + uint32_t(cell_index(xs, cy, cz)); };
-        auto advance = [&](int& cy, int& cz) {
-            if (++cy > ye) { cy = ys; cz++; }
-        };
-        int cy = ys, cz = zs;
-        int cy1 = cy, cz1 = cz;
-        advance(cy1, cz1);
-        int cy2 = cy1, cz2 = cz1;
-        advance(cy2, cz2);
-        const uint32_t k0 = rowkey(cy, cz);
-        const int xa = int(k0 & 127u);
-        const uint2 e0 = T.rows[k0 >> 7];
-        int2 sp = make_int2(int(row_lb(T, e0, xa)), int(row_lb(T, e0, xa + len0)));
-        uint2 e1 = cz1 <= ze ? T.rows[rowkey(cy1, cz1) >> 7] : make_uint2(0u, 0u);
-        while (cz <= ze) {
-            int2 sp1 = make_int2(0, 0);
-            if (cz1 <= ze) sp1 = make_int2(int(row_lb(T, e1, xa)), int(row_lb(T, e1, xa + len0)));
-            uint2 e2 = make_uint2(0u, 0u);
-            if (cz2 <= ze) e2 = T.rows[rowkey(cy2, cz2) >> 7];
-            scan(sp.x, sp.y);
-            rest(cy, cz);
-            sp = sp1;
-            e1 = e2;
-            cy = cy1; cz = cz1;
-            cy1 = cy2; cz1 = cz2;
-            advance(cy2, cz2);
-        }
-    } else if ((xe - xs <= 2) & (ye - ys <= 2) & (ze - zs <= 2) & !clamp0 & (ys + kGridOrigin >= 0) &
-               (zs + kGridOrigin >= 0)) {
+    if ((xe - xs <= 2) & (ye - ys <= 2) & (ze - zs <= 2) & !clamp0 & (ys + kGridOrigin >= 0) &
+        (zs + kGridOrigin >= 0)) {
         // A query box of at most 3 x 3 x 3 cells (every query when R is at
         // most one cell edge).  A row's cells [xs, xe] are one run of sorted
         // slots, or two when the row crosses an x wrap (piece a: the first
@@ -782,8 +748,31 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6, 8))) v
         const uint32_t vy0 = uint32_t(ys + kGridOrigin), vz0 = uint32_t(zs + kGridOrigin);
         for (int dz = 0; dz < 3; dz++) {
             if (zs + dz > ze) break;
-            const uint32_t zk = part(vz0 + uint32_t(dz), 14, 2 * pbits);
             int2 pa[3], pb[3];   // per row: pieces a and b as (first slot, length)
+            if constexpr (BATCHED) {
+                // a batched solver's row (scene, z, y) is one row-table entry;
+                // piece a is its cells [xa, xa + len0), piece b (past the x
+                // wrap) its cells [0, nxc - len0); a row outside the box
+                // reads the empty entry
+                const RowTable& T = a.rowtab;
+                const int xa = vx0 & (kGridDim - 1);
+                uint2 e[3];
+#pragma unroll
+                for (int dy = 0; dy < 3; dy++)
+                    e[dy] = ys + dy <= ye ? T.rows[(sbase + uint32_t(cell_index(xs, ys + dy, zs + dz))) >> 7]
+                                          : make_uint2(0u, 0u);
+#pragma unroll
+                for (int dy = 0; dy < 3; dy++) {
+                    const int a0 = int(row_lb(T, e[dy], xa)), a1 = int(row_lb(T, e[dy], xa + len0));
+                    pa[dy] = make_int2(a0, a1 - a0);
+                    pb[dy] = make_int2(0, 0);
+                    if (split) {
+                        const int b0 = int(row_lb(T, e[dy], 0)), b1 = int(row_lb(T, e[dy], nxc - len0));
+                        pb[dy] = make_int2(b0, b1 - b0);
+                    }
+                }
+            } else {
+            const uint32_t zk = part(vz0 + uint32_t(dz), 14, 2 * pbits);
 #pragma unroll
             for (int dy = 0; dy < 3; dy++) {
                 const bool in = ys + dy <= ye;
@@ -800,9 +789,44 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6, 8))) v
                 pa[dy] = in ? make_int2(ea - la, la) : make_int2(0, 0);
                 pb[dy] = in ? make_int2(eb - lb, lb) : make_int2(0, 0);
             }
+            }
             const int2 z0 = make_int2(0, 0);
             stream(pa[0], split ? pb[0] : pa[1], split ? pa[1] : pa[2]);
             stream(split ? pb[1] : z0, split ? pa[2] : z0, split ? pb[2] : z0);
+        }
+    } else if constexpr (BATCHED) {
+        // Rows (cz, cy) in the reference's order.  Every row's first x-run
+        // starts at xs, at the same offset xa in every row; its slots are two
+        // bounds behind the row's table entry.  Entries are requested two
+        // rows ahead and bounds one row ahead, so neither wait falls on a
+        // scan.
+        const RowTable& T = a.rowtab;
+        auto rowkey = [&](int cy, int cz) -> uint32_t { return sbase + uint32_t(cell_index(xs, cy, cz)); };
+        auto advance = [&](int& cy, int& cz) {
+            if (++cy > ye) { cy = ys; cz++; }
+        };
+        int cy = ys, cz = zs;
+        int cy1 = cy, cz1 = cz;
+        advance(cy1, cz1);
+        int cy2 = cy1, cz2 = cz1;
+        advance(cy2, cz2);
+        const uint32_t k0 = rowkey(cy, cz);
+        const int xa = int(k0 & 127u);
+        const uint2 e0 = T.rows[k0 >> 7];
+        int2 sp = make_int2(int(row_lb(T, e0, xa)), int(row_lb(T, e0, xa + len0)));
+        uint2 e1 = cz1 <= ze ? T.rows[rowkey(cy1, cz1) >> 7] : make_uint2(0u, 0u);
+        while (cz <= ze) {
+            int2 sp1 = make_int2(0, 0);
+            if (cz1 <= ze) sp1 = make_int2(int(row_lb(T, e1, xa)), int(row_lb(T, e1, xa + len0)));
+            uint2 e2 = make_uint2(0u, 0u);
+            if (cz2 <= ze) e2 = T.rows[rowkey(cy2, cz2) >> 7];
+            scan(sp.x, sp.y);
+            rest(cy, cz);
+            sp = sp1;
+            e1 = e2;
+            cy = cy1; cz = cz1;
+            cy1 = cy2; cz1 = cz2;
+            advance(cy2, cz2);
         }
     } else {
         // Each lane walks its x-runs -- rows (cz, cy) in the reference's
