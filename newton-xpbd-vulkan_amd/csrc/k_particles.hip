@@ -516,11 +516,24 @@ __device__ __forceinline__ void pp_pair(const PPArgs& a, int t, int k, float4 Pj
     }
 }
 
+// The next substep's radix input for the particle finished at sorted slot t
+// with state (P, V) and flags f: what k_integrate computes at the start of
+// the next substep (integrate_particle, the cell key, the wrap plane above
+// it), stored as (key, t) at the particle's id.
+__device__ __forceinline__ void write_next_key(const NextKey& nk, int t, float4 P, float4 V, int f, float dt,
+                                               float vmax) {
+    integrate_particle(P, V, f, dt, vmax, nk.gx, nk.gy, nk.gz);
+    uint32_t key = cell_of(P.x, P.y, P.z, nk.inv_cell);
+    if (nk.planes) key |= plane_of(P.x, P.y, P.z, nk.inv_cell, nk.planes) << kCellBits;
+    nk.kv[nk.ids[t]] = make_uint2(key, uint32_t(t));
+}
+
 // apply_particle_deltas.comp:44-57 for sorted slot t: position x (radius
 // rad, inverse mass w1) moved by its accumulated delta dl from the
-// substep-start position x0.
+// substep-start position x0; with nk.kv set, the next substep's key of the
+// result too (flags f).
 __device__ __forceinline__ void apply_delta(f3 x0, int t, f3 x, float rad, float w1, f3 dl, float dt, float vmax,
-                                            float4* Po, float4* Vo) {
+                                            float4* Po, float4* Vo, const NextKey& nk, int f) {
     f3 xn = x + dl;
     f3 vnew = markstein_div(xn - x0, dt);
     const float mag = len3(vnew);
@@ -528,8 +541,10 @@ __device__ __forceinline__ void apply_delta(f3 x0, int t, f3 x, float rad, float
         vnew = vnew * (vmax / mag);
         xn = x0 + vnew * dt;
     }
-    Po[t] = make_float4(xn.x, xn.y, xn.z, rad);
-    Vo[t] = make_float4(vnew.x, vnew.y, vnew.z, w1);
+    const float4 P = make_float4(xn.x, xn.y, xn.z, rad), V = make_float4(vnew.x, vnew.y, vnew.z, w1);
+    Po[t] = P;
+    Vo[t] = V;
+    if (nk.kv != nullptr) write_next_key(nk, t, P, V, f, dt, vmax);
 }
 
 // solve_particle_particle_contacts.comp:107-218 fused with
@@ -578,6 +593,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6, 8))) v
         a.delta[3 * t] = 0.0f; a.delta[3 * t + 1] = 0.0f; a.delta[3 * t + 2] = 0.0f;
         a.Po[t] = Pi;
         a.Vo[t] = Vi;
+        if (a.nk.kv != nullptr) write_next_key(a.nk, t, Pi, Vi, pf, a.dt, a.vmax);
     }
     const int i = DIAG ? int(a.ids[t]) : 0;
     const int proxy = pf & 2;
@@ -990,7 +1006,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6, 8))) v
         }
     }
     a.delta[3 * t] = dl.x; a.delta[3 * t + 1] = dl.y; a.delta[3 * t + 2] = dl.z;
-    apply_delta(xyz(x0), t, x, rad, w1, dl, a.dt, a.vmax, a.Po, a.Vo);
+    apply_delta(xyz(x0), t, x, rad, w1, dl, a.dt, a.vmax, a.Po, a.Vo, a.nk, pf);
 }
 
 // The shape half of the delta apply: runs after both k_shape_solve and
@@ -1029,7 +1045,7 @@ __global__ __launch_bounds__(256) void k_shape_apply(ShapeApplyArgs a) {
         const f3 dl = sh + pp;
         ds[0] = 0.0f; ds[1] = 0.0f; ds[2] = 0.0f;
         dp[0] = dl.x; dp[1] = dl.y; dp[2] = dl.z;
-        apply_delta(xyz(x0), ps, xyz(Pi), Pi.w, Vi.w, dl, a.dt, a.vmax, a.Po, a.Vo);
+        apply_delta(xyz(x0), ps, xyz(Pi), Pi.w, Vi.w, dl, a.dt, a.vmax, a.Po, a.Vo, a.nk, a.flags[ps]);
     }
 }
 

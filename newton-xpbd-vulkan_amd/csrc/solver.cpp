@@ -56,7 +56,7 @@ void Solver::require_init() const {
 void Solver::free_all() {
     for (int k = 0; k < 3; k++) { dfree(P_[k]); dfree(V_[k]); }
     for (int k = 0; k < 2; k++) { dfree(flags_[k]); dfree(ids_[k]); }
-    dfree(rank_); dfree(kv_); dfree(keysA_); dfree(valsA_); dfree(keysB_); dfree(valsB_); dfree(hist_); dfree(bintot_);
+    dfree(rank_); dfree(kv_); dfree(keysA_); dfree(valsA_); dfree(keysB_); dfree(valsB_); dfree(hist_); dfree(bintot_); dfree(chunks_);
     dfree(Xs_[0]); dfree(Vs_[0]);
     dfree(body_half_); dfree(body_I_); dfree(body_invw_); dfree(mat_mu_t_); dfree(mat_mu_r_);
     dfree(rc_s0_); dfree(rc_s1_); dfree(rc_p0_); dfree(rc_p1_); dfree(rc_n_); dfree(rc_impulse_);
@@ -100,8 +100,14 @@ void Solver::alloc_all() {
     sort_items_ = sort_items_for((long long)n_);
     const size_t tile = size_t(256) * 4;
     const size_t tiles = (n_ + tile - 1) / tile;
-    dalloc(hist_, size_t(kSortMaxBins) * std::max<size_t>(tiles, 1));
+    // (rows padded to whole chunks for the chunked passes)
+    const size_t ctiles = (std::max<size_t>(tiles, 1) + kSortChunkTiles - 1) / kSortChunkTiles;
+    dalloc(hist_, size_t(kSortMaxBins) * ctiles * kSortChunkTiles);
     dalloc(bintot_, kSortMaxBins);
+    // chunk totals of the chunked passes: one array per pass index, each
+    // 256 bins x the chunks of the smaller tiles; zero from here on between sorts
+    sort_chunk_words_ = size_t(256) * ctiles;
+    dalloc(chunks_, kSortMaxPasses * sort_chunk_words_);
     // the plain solver's dense 2^21-row cell table (the key is the row; the
     // keys of the rows written last, rkeys, let each build reset exactly those
     // rows first); a batched solver's row table instead: 8 bytes per row
@@ -160,6 +166,10 @@ void Solver::init_common(const xpbd_config& cfg, size_t n, size_t max_contacts) 
     det_ = denv && (std::strcmp(denv, "1") == 0 || std::strcmp(denv, "true") == 0);
     const char* genv = std::getenv("VKXPBD_GRAPHS");
     use_graphs_ = !(genv && std::strcmp(genv, "0") == 0);
+    const char* cenv = std::getenv("VKXPBD_SORT_CHUNKED");
+    sort_chunked_ = cenv && std::strcmp(cenv, "1") == 0;
+    const char* fenv = std::getenv("VKXPBD_FUSE_KEYS");
+    fuse_keys_ = !(fenv && std::strcmp(fenv, "0") == 0);
     // wrap planes pay for their 128 MiB or 1 GiB table and wider radix keys
     // on large scenes only; the environment can force them either way
     const char* penv = std::getenv("VKXPBD_WRAP_PLANES");
@@ -676,7 +686,7 @@ SortGather Solver::sort_gather(float sdt) {
                       sdt, cfg_.particle_v_max, cfg_.gravity[0], cfg_.gravity[1], cfg_.gravity[2]};
 }
 
-void Solver::run_sort(int tiles, const SortGather* last) {
+void Solver::run_sort(int tiles, const SortGather* last, const SortReset& rs) {
     const int n = int(n_);
     uint32_t* kb[2] = {keysA_.p, keysB_.p};
     uint32_t* vb[2] = {valsA_.p, valsB_.p};
@@ -684,20 +694,36 @@ void Solver::run_sort(int tiles, const SortGather* last) {
     // between the A and B buffers and the last one must land in A (the
     // buffers the getters read)
     int dst = (sort_passes_ & 1) ? 0 : 1;
+    // chunked passes (no bin-scan launch) for digits of at most 8 bits; pass
+    // p's scatter zeroes the totals pass (p + 1) mod P adds into
+    const int chunks = (tiles + kSortChunkTiles - 1) / kSortChunkTiles;
+    const bool chunked = sort_chunked_ && sort_bits_ <= 8 && sort_passes_ >= 2 && sort_passes_ <= kSortMaxPasses &&
+                         size_t(256) * size_t(chunks) <= sort_chunk_words_;
     for (int pass = 0; pass < sort_passes_; pass++) {
         const int shift = pass * sort_bits_;
         const uint32_t* kin = pass == 0 ? nullptr : kb[dst ^ 1];
         const uint32_t* vin = pass == 0 ? nullptr : vb[dst ^ 1];
         const uint2* pin = pass == 0 ? kv_.p : nullptr;
+        SortChunks ch;
+        if (chunked) {
+            ch.chunk = chunks_.p + size_t(pass) * sort_chunk_words_;
+            ch.chunks = chunks;
+            ch.tstride = chunks * kSortChunkTiles;
+            ch.zero_next = chunks_.p + size_t((pass + 1) % sort_passes_) * sort_chunk_words_;
+            ch.zero_count = chunks << sort_bits_;
+        }
         stage_begin(kSortUp);
-        launch_sort_upsweep(sort_bits_, sort_items_, n, kin, pin, shift, hist_.p, tiles, stream_);
+        launch_sort_upsweep(sort_bits_, sort_items_, n, kin, pin, shift, hist_.p, tiles, ch, pass == 0 ? rs : SortReset{},
+                            stream_);
         stage_end(kSortUp);
-        stage_begin(kSortScan);
-        launch_sort_binscan(sort_bits_, hist_.p, bintot_.p, tiles, stream_);
-        stage_end(kSortScan);
+        if (!chunked) {
+            stage_begin(kSortScan);
+            launch_sort_binscan(sort_bits_, hist_.p, bintot_.p, tiles, stream_);
+            stage_end(kSortScan);
+        }
         stage_begin(kSortScatter);
         launch_sort_scatter(sort_bits_, sort_items_, n, kin, vin, pin, kb[dst], vb[dst], shift, hist_.p, bintot_.p, tiles,
-                            pass == sort_passes_ - 1 ? last : nullptr, stream_);
+                            pass == sort_passes_ - 1 ? last : nullptr, ch, stream_);
         stage_end(kSortScatter);
         dst ^= 1;
     }
@@ -729,7 +755,24 @@ BodyStepArgs Solver::body_step_args(float sdt) {
     return b;
 }
 
-void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zero_next) {
+// Substeps inside a frame may key each other (the last iteration's apply
+// writes the next substep's radix input, which k_integrate would compute from
+// the same state by the same code) where k_integrate has no other work: a
+// plain solver without boxes, restitution or deterministic mode, and no plane
+// contacts of its own to emit (a mesh's candidate pass emits them, or there is
+// no ground plane).  A frame's first substep is always keyed by k_integrate,
+// the host may have changed the state before it.
+bool Solver::fuse_keys_ok(int iterations) const {
+    return fuse_keys_ && num_scenes_ == 1 && !det_ && boxes_.empty() && !cfg_.enable_restitution && iterations >= 1 &&
+           n_ > 0 && (!meshes_.empty() || !cfg_.has_ground_plane);
+}
+
+int Solver::frame_fuse(int sub, int substeps, int iterations) const {
+    if (!fuse_keys_ok(iterations)) return 0;
+    return (sub > 0 ? kKeyed : 0) | (sub + 1 < substeps ? kKeyNext : 0);
+}
+
+void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zero_next, int fuse) {
     const int n = int(n_);
     const int sort_tile = 256 * sort_items_;
     const int tiles = int((n_ + sort_tile - 1) / sort_tile);
@@ -797,7 +840,8 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
         launch_fill_zero(reinterpret_cast<int*>(rc_impulse_.p), rc_impulse_.n & ~size_t(3), stream_);
         stage_end(kBodyContacts);
     }
-    if (n > 0) {
+    const bool keyed = (fuse & kKeyed) != 0;
+    if (n > 0 && !keyed) {
         IntegrateArgs a;
         a.n = n;
         a.Ps = P_[s_].p; a.Vs = V_[s_].p; a.flags = flags_cur();
@@ -830,12 +874,25 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
     }
     const int shape_grid = int(std::max<size_t>(1, std::min<size_t>((cmax_ + 255) / 256, 2048)));
     int cur = p_, nxt = o_;
+    NextKey nk{};
+    if (fuse & kKeyNext) {
+        nk.kv = kv_.p;
+        nk.gx = cfg_.gravity[0]; nk.gy = cfg_.gravity[1]; nk.gz = cfg_.gravity[2];
+        nk.inv_cell = inv_cell;
+        nk.planes = planes_ ? plane_bits_ : 0;
+    }
     if (n > 0) {
         // plain solvers move the state into the sorted order in the cell-range
         // pass (one thread per sorted slot); batched ones in the last radix pass
         const SortGather g = sort_gather(sdt);
         const bool gather_in_ranges = num_scenes_ == 1;
-        run_sort(tiles, gather_in_ranges ? nullptr : &g);
+        SortReset rs;
+        if (keyed) {   // k_integrate's resets
+            rs.rkeys = rkeys_.p;
+            rs.range = range_.p;
+            rs.cc_next = zero_next ? cc_.p + (slot ^ 1) : nullptr;
+        }
+        run_sort(tiles, gather_in_ranges ? nullptr : &g, rs);
         stage_begin(kRanges);
         if (gather_in_ranges) launch_cell_ranges_gather(n, keysA_.p, valsA_.p, range_.p, rkeys_.p, g, stream_);
         else build_ranges(keysA_.p, true);
@@ -921,6 +978,9 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
         pa.pmu = cfg_.soft_contact_mu; pa.mat_mu0 = cfg_.shape_material_mu; pa.ppd = ppd_.p;
         pa.iter_index = uint32_t(it); pa.max_diag_pairs = max_pairs_;
         pa.diag_header = diag_header_.p; pa.diag_pairs = diag_pairs_.p;
+        const bool last_it = it + 1 == iterations;
+        if (last_it && nk.kv != nullptr) nk.ids = ids_cur();
+        pa.nk = last_it ? nk : NextKey{};
         stage_begin(kPP);
         launch_pp_apply(pa, cfg_.enable_diagnostics != 0, num_scenes_ > 1, pp_block_, stream_);
         stage_end(kPP);
@@ -942,6 +1002,7 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
             aa.plane_fold = plane_fold ? 1 : 0; aa.flags = flags_cur(); aa.ppd = ppd_.p;
             aa.altitude = cfg_.ground_plane_altitude; aa.margin = cfg_.soft_contact_margin; aa.ka = cfg_.particle_adhesion;
             aa.pmu = cfg_.soft_contact_mu; aa.mat_mu0 = cfg_.shape_material_mu; aa.relax = relax_;
+            aa.nk = last_it ? nk : NextKey{};
             launch_shape_apply(aa, shape_grid, stream_);
         }
         stage_end(kShapeApply);
@@ -1036,7 +1097,8 @@ void Solver::record_frame(float dt, int substeps, int iterations) {
     hipGraph_t graph = nullptr;
     try {
         launch_frame_begin(cc_.p, mcnt_.p, int(mcnt_.n), stream_);
-        for (int sub = 0; sub < substeps; sub++) substep(sdt, iterations, true, sub & 1, true);
+        for (int sub = 0; sub < substeps; sub++)
+            substep(sdt, iterations, true, sub & 1, true, frame_fuse(sub, substeps, iterations));
     } catch (...) {
         (void)hipStreamEndCapture(stream_, &graph);
         if (graph) (void)hipGraphDestroy(graph);
@@ -1214,7 +1276,8 @@ void Solver::step(float dt, int substeps, int iterations) {
         record_frame(dt, substeps, iterations);
     } else {
         launch_frame_begin(cc_.p, mcnt_.p, int(mcnt_.n), stream_);
-        for (int sub = 0; sub < substeps; sub++) substep(sdt, iterations, true, sub & 1, true);
+        for (int sub = 0; sub < substeps; sub++)
+            substep(sdt, iterations, true, sub & 1, true, frame_fuse(sub, substeps, iterations));
     }
     if (substeps > 0) contact_slot_ = rigid_slot_ = (substeps - 1) & 1;
     read_count(substeps > 0 ? contact_slot_ : 0);

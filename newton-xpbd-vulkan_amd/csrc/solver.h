@@ -147,13 +147,20 @@ private:
     bool update_mesh_on_device(size_t mesh);
     void rebuild_meshes_on_host();
     BodyArgs body_args() const;
-    void substep(float sdt, int iterations, bool collide, int slot, bool zero_next);
+    // fuse: kKeyed -- this substep's radix input was written by the previous
+    // one (k_integrate skipped, its resets done by the first upsweep);
+    // kKeyNext -- the last iteration writes the next substep's radix input
+    static constexpr int kKeyed = 1, kKeyNext = 2;
+    void substep(float sdt, int iterations, bool collide, int slot, bool zero_next, int fuse = 0);
+    bool fuse_keys_ok(int iterations) const;
+    int frame_fuse(int sub, int substeps, int iterations) const;
     BodyStepArgs body_step_args(float sdt);
     DetApplyArgs det_apply_args(const float4* Xin, const float4* Vin, int out, float sdt);
     // the state's move into the sorted order (integrated over sdt), taken
     // before the sort flips the layout
     SortGather sort_gather(float sdt);
-    void run_sort(int tiles, const SortGather* last);   // last: the last pass also moves the state
+    // last: the last pass also moves the state; rs: resets done by the first upsweep
+    void run_sort(int tiles, const SortGather* last, const SortReset& rs = SortReset{});
     void update_planes();
     std::vector<uint64_t> reference_sorted();
     // cell ranges of the sorted keys; cleared = the previous rows are already
@@ -259,6 +266,10 @@ private:
     float rmax_ = 0.f;   // largest particle radius (host copy of the radii)
     // grid and sort
     DevBuf<uint32_t> keysA_, valsA_, keysB_, valsB_, hist_, bintot_;
+    DevBuf<uint32_t> chunks_;          // chunk totals of the chunked radix passes, kSortMaxPasses arrays
+    size_t sort_chunk_words_ = 0;      // words per pass array
+    bool sort_chunked_ = false;        // chunked radix passes (VKXPBD_SORT_CHUNKED=1; else upsweep, binscan, scatter)
+    bool fuse_keys_ = true;            // substeps keyed by the previous one's apply (VKXPBD_FUSE_KEYS=0: off)
     DevBuf<float4> Xs_[1], Vs_[1];   // predicted state in sorted order (the first iteration's input)
     DevBuf<int32_t> range_;   // int2 per key (start, end); rows of the last build are reset before the next
     DevBuf<uint32_t> rkeys_;  // with wrap planes: the sorted keys whose rows range_ holds

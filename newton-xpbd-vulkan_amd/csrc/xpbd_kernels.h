@@ -20,6 +20,32 @@ constexpr int kCellBits = 21;       // batched solver: key = scene << 21 | cell
 constexpr int kSortLargeN = 1 << 18;
 __host__ __device__ constexpr int sort_items_for(long long n) { return n >= kSortLargeN ? 8 : 4; }
 
+// Chunked radix passes (no bin-scan launch): tiles are grouped in chunks of
+// kSortChunkTiles; chunk holds each chunk's per-bin totals, chunk[c][bin]
+// (chunks x 2^bits words), summed by the upsweep with device atomics; the tile
+// counts are hist[bin][tile] with rows of tstride words (a multiple of
+// kSortChunkTiles).  zero_next / zero_count: the totals of the next pass,
+// zeroed by this pass's scatter.
+// chunk == nullptr: the three-launch pass (upsweep, binscan, scatter).
+// Work the first radix upsweep does for a substep keyed by the previous one
+// (k_integrate skipped): the rows of the last cell-range build are reset from
+// their keys (rkeys, as k_integrate resets them) and the next substep's
+// contact counter is zeroed.  rkeys == nullptr: nothing.
+struct SortReset {
+    const uint32_t* rkeys = nullptr;
+    int* range = nullptr;
+    int* cc_next = nullptr;
+};
+constexpr int kSortChunkTiles = 32;
+constexpr int kSortMaxPasses = 4;   // 32-bit keys in 8-bit digits
+struct SortChunks {
+    uint32_t* chunk = nullptr;
+    int chunks = 0;
+    int tstride = 0;
+    uint32_t* zero_next = nullptr;
+    int zero_count = 0;
+};
+
 // Sorted-order copies of the predicted particle state, written by the last
 // radix pass (positions with radius, velocities with inverse mass, flags).
 // Last radix pass: the particle state moves into the new sorted order.  The
@@ -44,6 +70,16 @@ struct BodyArgs {
     const int* shape_body;
     const float* mat_mu;
     float* delta;         // 8 per body: lin at +0..2, ang at +4..6
+};
+
+// The next substep's radix input written where a substep finishes its
+// particles (k_pp_apply, k_shape_apply): integrate_particle on the final
+// state, its cell key (with the wrap plane above it), (key, slot) at the
+// particle's id.  kv == nullptr: not written (k_integrate keys the substep).
+struct NextKey {
+    uint2* kv; const uint32_t* ids;   // sorted slot -> particle id
+    float gx, gy, gz, inv_cell;
+    int planes;
 };
 
 struct IntegrateArgs {
@@ -124,6 +160,7 @@ struct ShapeApplyArgs {
     const float4* Xs; const float4* Vs;             // iteration input state, sorted order
     float4* Po; float4* Vo;                         // output state, sorted order
     float dt, vmax;
+    NextKey nk;
 };
 
 struct PPArgs {
@@ -146,6 +183,7 @@ struct PPArgs {
     int plane_fold; float altitude, margin, ka, pmu, mat_mu0; float* ppd;
     uint32_t iter_index, max_diag_pairs;
     uint32_t* diag_header; float* diag_pairs;
+    NextKey nk;
 };
 
 struct RestArgs {
@@ -386,12 +424,14 @@ void launch_mesh_contacts(const MeshArgs& a, const uint8_t* watertight, bool bes
 
 // Radix sort: one pass = upsweep (tile histograms) -> bin-row scan -> scatter.
 // items: keys per thread (4 or 8), a tile being 256 * items keys.
+// Chunked passes (ch.chunk set, bits <= 8): upsweep -> scatter, hist
+// tile-major and no bin totals.
 void launch_sort_upsweep(int bits, int items, int n, const uint32_t* keys, const uint2* pairs, int shift, uint32_t* hist,
-                         int tiles, hipStream_t s);
+                         int tiles, const SortChunks& ch, const SortReset& rs, hipStream_t s);
 void launch_sort_binscan(int bits, uint32_t* hist, uint32_t* bintot, int tiles, hipStream_t s);
 void launch_sort_scatter(int bits, int items, int n, const uint32_t* keys_in, const uint32_t* vals_in, const uint2* pairs_in,
                          uint32_t* keys_out, uint32_t* vals_out, int shift, const uint32_t* hist, const uint32_t* bintot,
-                         int tiles, const SortGather* gather, hipStream_t s);
+                         int tiles, const SortGather* gather, const SortChunks& ch, hipStream_t s);
 void launch_fill_zero(int* p, size_t count, hipStream_t s);
 
 }  // namespace xpbd
