@@ -1018,6 +1018,11 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6, 8))) v
 // mark for the next solve.  After a solve in the previous slot order (remap
 // non-null) every contact is mapped to its particle's sorted slot here.
 __global__ __launch_bounds__(256) void k_shape_apply(ShapeApplyArgs a) {
+    if (a.rk != nullptr)   // the particle-pair solve, the last reader of these rows, has finished
+        for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < a.n; t += gridDim.x * blockDim.x) {
+            const uint32_t c = a.rk[t];
+            if (t == 0 || a.rk[t - 1] != c) *reinterpret_cast<int2*>(a.range + size_t(2) * c) = make_int2(0, 0);
+        }
     const int count = min(a.cmax, *a.cc);
     for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < count; t += gridDim.x * blockDim.x) {
         const int pi = a.cpart[t];   // the slot dshape and owner use

@@ -887,9 +887,11 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
         const SortGather g = sort_gather(sdt);
         const bool gather_in_ranges = num_scenes_ == 1;
         SortReset rs;
-        if (keyed) {   // k_integrate's resets
-            rs.rkeys = rkeys_.p;
-            rs.range = range_.p;
+        if (keyed) {   // k_integrate's resets (the rows: the last shape-delta apply's, if it ran)
+            if (!shape_work) {
+                rs.rkeys = rkeys_.p;
+                rs.range = range_.p;
+            }
             rs.cc_next = zero_next ? cc_.p + (slot ^ 1) : nullptr;
         }
         run_sort(tiles, gather_in_ranges ? nullptr : &g, rs);
@@ -1003,6 +1005,10 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
             aa.altitude = cfg_.ground_plane_altitude; aa.margin = cfg_.soft_contact_margin; aa.ka = cfg_.particle_adhesion;
             aa.pmu = cfg_.soft_contact_mu; aa.mat_mu0 = cfg_.shape_material_mu; aa.relax = relax_;
             aa.nk = last_it ? nk : NextKey{};
+            // keying the next substep: its rows' reset, here where nothing else runs
+            aa.rk = last_it && nk.kv != nullptr ? rkeys_.p : nullptr;
+            aa.range = range_.p;
+            aa.n = n;
             launch_shape_apply(aa, shape_grid, stream_);
         }
         stage_end(kShapeApply);

@@ -161,6 +161,9 @@ struct ShapeApplyArgs {
     float4* Po; float4* Vo;                         // output state, sorted order
     float dt, vmax;
     NextKey nk;
+    // non-null: reset the rows of this substep's cell-range build (their keys
+    // rk, n of them) for the next substep's build, which k_integrate would do
+    const uint32_t* rk; int* range; int n;
 };
 
 struct PPArgs {
