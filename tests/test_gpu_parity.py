@@ -385,6 +385,39 @@ def test_mesh_pile_frames(vk):
             compare_state(g, o, exact=True, what=f"mesh pile frame {f + 1}")
 
 
+@pytest.mark.parametrize("env", [{"VKXPBD_SORT_CHUNKED": "1"},
+                                 {"VKXPBD_SORT_CHUNKED": "1", "VKXPBD_WRAP_PLANES": "1"},
+                                 {"VKXPBD_FUSE_KEYS": "0"}],
+                         ids=["chunked_radix", "chunked_radix_wrap_planes", "integrate_every_substep"])
+def test_sort_and_keying_variants(vk, env, monkeypatch):
+    """The radix passes without a bin-scan launch (per-chunk totals, 65,536
+    keys: two chunks of 32 tiles), with plain and wrap-plane keys, and
+    substeps all keyed by k_integrate give the oracle's bits, as the default
+    path (later substeps keyed by the previous substep's apply) does: a pile
+    landing on an icosphere, compared every frame."""
+    rng = np.random.default_rng(21)
+    r = 0.005
+    q = scenes.lattice(64, 64, 16, 2 * r, (-0.315, -0.315, 0.505), 0.1 * r, rng)
+    verts, tris = scenes.icosphere(3)
+    sc = scene_from(q, r, meshes=[(verts, tris)])
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    g, o = make_pair(vk, sc)
+    for k in env:
+        monkeypatch.delenv(k)
+    d = scenes.build_solver(vk, sc)   # the defaults
+    log(f"variant {env}: wrap planes {g.wrap_planes}")
+    for f in range(4):
+        for s in (g, o, d):
+            s.step(1.0 / 60.0, 8, 1)
+        assert g.last_contact_count == o.last_contact_count == d.last_contact_count
+        compare_state(g, o, exact=True, what=f"variant frame {f + 1}")
+        np.testing.assert_array_equal(g.get_positions(), d.get_positions())
+        np.testing.assert_array_equal(g.get_velocities(), d.get_velocities())
+    compare_grid(g, o)
+    assert_contacts_equal(gpu_contacts(g), o.get_contacts())
+
+
 def test_update_mesh_moving_collider(vk):
     rng = np.random.default_rng(9)
     r = 0.01
