@@ -640,6 +640,7 @@ __global__ __launch_bounds__(kMeshBlock) void k_mesh_walk(MeshArgs a, int ci) {
             a.cbpos[slot] = make_float4(bp.x, bp.y, bp.z, 0.f);
             a.cbvel[slot] = make_float4(svel.x, svel.y, svel.z, 0.f);
             a.cnorm[slot] = make_float4(nn.x, nn.y, nn.z, 0.f);
+            if (a.solve_inline) shape_solve_one(a.solve, slot, pi, ci + 1, bp, svel, nn);
         }
     }
     if (STATS) {

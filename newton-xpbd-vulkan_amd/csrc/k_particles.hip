@@ -342,58 +342,6 @@ __global__ __launch_bounds__(256) void k_cell_tables(int n, const uint32_t* __re
     if (t == n - 1) ends[c] = t + 1;
 }
 
-// One particle-shape contact of solve_particle_shape_contacts.comp:121-188:
-// the particle's state (P, Vv), the shape's body (-1: none) and friction mu,
-// the contact's body-local point and velocity and its normal.  Returns false
-// where the shader adds nothing (farther than the adhesion distance, or no
-// inverse mass on either side); else the particle's delta fd, the unweighted
-// delta dtot and the lever arm r.  The shape solve runs every contact
-// through it, and the particle-pair solve and the shape-delta apply run the
-// ground plane's through it (body -1, the contact a pure function of the
-// particle's substep-start position), so all three compute the same floats.
-__device__ __forceinline__ bool shape_contact(float4 P, float4 Vv, int body, const BodyArgs& B, float mu, f3 cbpos,
-                                              f3 cbvel, f3 n, float ka, float dt, float relax, f3& fd, f3& dtot, f3& r) {
-    f3 px = xyz(P), pv = xyz(Vv);
-    BodyFrame F = load_body(B, body);
-    f3 bx = F.pos + quat_rotate_sc(F.rot, cbpos);
-    r = bx - (F.pos + quat_rotate_sc(F.rot, F.com));
-    f3 d = px - bx;
-    float c = fma_(n.z, d.z, fma_(n.x, d.x, n.y * d.y)) - P.w;
-    if (c > ka) return false;
-    f3 bv = F.v + cross3(F.w, r) + quat_rotate_sc(F.rot, cbvel);
-    f3 v = pv - bv;
-    float lambda_n = c;
-    f3 delta_n = n * lambda_n;
-    float vn = n.y * v.y;
-    vn = fma_(n.x, v.x, vn);
-    vn = fma_(n.z, v.z, vn);
-    f3 vt = {fma_(-n.x, vn, v.x), fma_(-n.y, vn, v.y), fma_(-n.z, vn, v.z)};
-    float w1 = Vv.w;
-    float w2 = 0.0f;
-    if (body >= 0) {
-        f3 ang = cross3(r, n);
-        f3 ra = quat_rotate_inv_sc(F.rot, ang);
-        const float* I = B.invI + 9 * body;   // row-major, as this shader reads it
-        f3 Ira = {I[0] * ra.x + I[1] * ra.y + I[2] * ra.z, I[3] * ra.x + I[4] * ra.y + I[5] * ra.z,
-                  I[6] * ra.x + I[7] * ra.y + I[8] * ra.z};
-        w2 = B.invm[body] + dot3(ra, Ira);
-    }
-    float denom = w1 + w2;
-    if (denom == 0.0f) return false;
-    float l2 = vt.y * vt.y;
-    l2 = fma_(vt.x, vt.x, l2);
-    l2 = fma_(vt.z, vt.z, l2);
-    const RootInv vl = exact_sqrt_inv(l2);
-    const float vt_len = vl.d;
-    float lambda_f = gmax(mu * lambda_n, -vt_len * dt);
-    f3 vt_dir = vt_len > 0.0f ? exact_div_by(vt, vt_len, vl.inv) : f3{0.f, 0.f, 0.f};
-    f3 diff = {fma_(vt_dir.x, lambda_f, -delta_n.x), fma_(vt_dir.y, lambda_f, -delta_n.y),
-               fma_(vt_dir.z, lambda_f, -delta_n.z)};
-    dtot = exact_div(diff, denom) * relax;
-    fd = w1 * dtot;
-    return true;
-}
-
 // The ground plane's contact of a particle: generate_plane_contacts.comp:
 // 27-54 emits it from the substep-start position x0 (active particles within
 // margin + radius of the plane; a proxy's plane contact adds nothing,
@@ -408,22 +356,25 @@ __device__ __forceinline__ bool plane_contact(float4 x0, float4 P, float4 Vv, in
                          relax, fd, dtot, r);
 }
 
-// solve_particle_shape_contacts.comp:104-200.  Accumulates into dshape and
-// marks each receiving particle with its largest contact index (owner), which
-// k_shape_apply consumes and clears.  skip_plane: the ground plane's contacts
-// are taken by the particle-pair solve and the shape-delta apply instead.
+// solve_particle_shape_contacts.comp:104-200 over the contact buffer
+// (shape_solve_one per contact; deterministic mode stores each contact's
+// deltas instead).  skip_plane: the ground plane's contacts are taken by the
+// particle-pair solve and the shape-delta apply instead.
 __global__ __launch_bounds__(256) void k_shape_solve(ShapeArgs a) {
     const int count = min(a.cmax, *a.cc);
     for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < count; t += gridDim.x * blockDim.x) {
         const int pi = a.cpart[t];
         int s = a.cshape[t];
         if (a.skip_plane && s == 0) continue;
+        if (a.cfd == nullptr) {
+            shape_solve_one(a, t, pi, s, xyz(a.cbpos[t]), xyz(a.cbvel[t]), xyz(a.cnorm[t]));
+            continue;
+        }
         int body = a.body.shape_body[s];
         int pf = a.flags[pi];
-        if (a.cfd) {   // deterministic mode: no contribution unless stored below
-            a.cfd[t] = make_float4(0.f, 0.f, 0.f, 0.f);
-            a.cbd[2 * size_t(t)] = make_float4(0.f, 0.f, 0.f, 0.f);
-        }
+        // deterministic mode: no contribution unless stored below
+        a.cfd[t] = make_float4(0.f, 0.f, 0.f, 0.f);
+        a.cbd[2 * size_t(t)] = make_float4(0.f, 0.f, 0.f, 0.f);
         if ((pf & 1) == 0) continue;
         if (pf & 2) {
             if (body < 0) continue;
@@ -437,26 +388,12 @@ __global__ __launch_bounds__(256) void k_shape_solve(ShapeArgs a) {
         if (!shape_contact(P, Vv, body, a.body, mu, xyz(a.cbpos[t]), xyz(a.cbvel[t]), xyz(a.cnorm[t]), a.ka, a.dt,
                            a.relax, fd, dtot, r))
             continue;
-        if (a.cfd) {
-            a.cfd[t] = make_float4(fd.x, fd.y, fd.z, 1.f);
-            if (body >= 0 && a.body.invm[body] > 0.0f) {
-                f3 dv = {dtot.x / a.dt, dtot.y / a.dt, dtot.z / a.dt};
-                f3 dw = cross3(r, dv);
-                a.cbd[2 * size_t(t)] = make_float4(-dv.x, -dv.y, -dv.z, 1.f);
-                a.cbd[2 * size_t(t) + 1] = make_float4(-dw.x, -dw.y, -dw.z, 0.f);
-            }
-            continue;
-        }
-        atomicMax(&a.owner[pi], t);
-        atomicAdd(&a.dshape[3 * pi + 0], fd.x);
-        atomicAdd(&a.dshape[3 * pi + 1], fd.y);
-        atomicAdd(&a.dshape[3 * pi + 2], fd.z);
+        a.cfd[t] = make_float4(fd.x, fd.y, fd.z, 1.f);
         if (body >= 0 && a.body.invm[body] > 0.0f) {
             f3 dv = {dtot.x / a.dt, dtot.y / a.dt, dtot.z / a.dt};
             f3 dw = cross3(r, dv);
-            float* bd = a.body.delta + 8 * body;
-            atomicAdd(bd + 0, -dv.x); atomicAdd(bd + 1, -dv.y); atomicAdd(bd + 2, -dv.z);
-            atomicAdd(bd + 4, -dw.x); atomicAdd(bd + 5, -dw.y); atomicAdd(bd + 6, -dw.z);
+            a.cbd[2 * size_t(t)] = make_float4(-dv.x, -dv.y, -dv.z, 1.f);
+            a.cbd[2 * size_t(t) + 1] = make_float4(-dw.x, -dw.y, -dw.z, 0.f);
         }
     }
 }

@@ -168,6 +168,8 @@ void Solver::init_common(const xpbd_config& cfg, size_t n, size_t max_contacts) 
     use_graphs_ = !(genv && std::strcmp(genv, "0") == 0);
     const char* cenv = std::getenv("VKXPBD_SORT_CHUNKED");
     sort_chunked_ = cenv && std::strcmp(cenv, "1") == 0;
+    const char* wenv = std::getenv("VKXPBD_SOLVE_IN_WALK");
+    solve_in_walk_ = !(wenv && std::strcmp(wenv, "0") == 0);
     const char* fenv = std::getenv("VKXPBD_FUSE_KEYS");
     fuse_keys_ = !(fenv && std::strcmp(fenv, "0") == 0);
     // wrap planes pay for their 128 MiB or 1 GiB table and wider radix keys
@@ -788,6 +790,10 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
     // with a mesh, its candidate pass emits the plane contacts too (on the
     // side stream), so the first shape solve needs nothing from the main one
     const bool plane_in_mesh = collide && !meshes_.empty() && cfg_.has_ground_plane;
+    if (fork) {
+        HIPCHECK(hipEventRecord(fork_ev_, stream_));
+        HIPCHECK(hipStreamWaitEvent(side_, fork_ev_, 0));
+    }
     const int* flags_prev = flags_cur();   // the previous slot order's flags (the sort flips them)
     // the ground plane's contacts solved inside the particle-pair solve (the
     // shape solve then has only the mesh contacts): every plane contact
@@ -796,12 +802,32 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
     const bool plane_fold = collide && cfg_.has_ground_plane && !det_ && nbox == 0 && cmax_ >= n_ * (1 + size_t(M));
     // without meshes and boxes every contact is the plane's: no shape solve
     const bool shape_work = !(plane_fold && M == 0);
-    if (fork) {
-        HIPCHECK(hipEventRecord(fork_ev_, stream_));
-        HIPCHECK(hipStreamWaitEvent(side_, fork_ev_, 0));
-    }
+    // the shape solve of iteration it: the first one in the previous slot
+    // order (the contacts' order), integrating the state itself
+    auto shape_args = [&](int it, const float4* Xin, const float4* Vin) {
+        ShapeArgs sa{};
+        sa.cmax = int(cmax_); sa.cc = cc_solve;
+        const bool prev_order = it == 0 && n > 0 && !det_;
+        sa.cfd = det_ ? cfd_.p : nullptr;
+        sa.cbd = det_ ? cbd_.p : nullptr;
+        sa.cpart = cpart_.p; sa.cshape = cshape_.p; sa.cbpos = cbpos_.p; sa.cbvel = cbvel_.p; sa.cnorm = cnorm_.p;
+        sa.P = prev_order ? P_[s_].p : Xin; sa.V = prev_order ? V_[s_].p : Vin;
+        sa.flags = prev_order ? flags_prev : flags_cur();
+        sa.from_old = prev_order ? 1 : 0;
+        sa.g_dt = sdt; sa.g_vmax = cfg_.particle_v_max;
+        sa.gx = cfg_.gravity[0]; sa.gy = cfg_.gravity[1]; sa.gz = cfg_.gravity[2];
+        sa.body = body_args();
+        sa.pmu = cfg_.soft_contact_mu; sa.ka = cfg_.particle_adhesion; sa.dt = sdt; sa.relax = relax_;
+        sa.dshape = dshape_.p; sa.owner = owner_.p;
+        sa.skip_plane = plane_fold ? 1 : 0;
+        return sa;
+    };
+    // on the side stream with only mesh contacts to solve (the plane's folded
+    // into the particle-pair solve, no boxes): the walk solves each contact it
+    // writes, so the first shape solve is not launched
+    const bool solve_in_walk = fork && !det_ && nbox == 0 && plane_fold && iterations > 0 && solve_in_walk_;
     if (collide && !meshes_.empty()) {
-        MeshArgs m;
+        MeshArgs m{};
         m.n = n; m.P = P_[s_].p; m.flags = flags_cur();
         m.verts = verts_d_.p; m.tris = tris_d_.p; m.nodes = nodes_d_.p; m.cnodes = cnodes_d_.p; m.lbox = lbox_d_.p; m.prim = prim_d_.p;
         m.pg_start = pg_start_.p; m.pg_soup = pg_soup_.p; m.tsoup = tsoup_.p; m.lb = lb_.p; m.lb_node = lb_node_.p;
@@ -814,6 +840,8 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
         m.err = reinterpret_cast<uint32_t*>(cc_.p + 2);
         m.stats = mesh_stats_on_ ? mesh_stats_.p : nullptr;
         m.plane = plane_in_mesh ? 1 : 0; m.altitude = cfg_.ground_plane_altitude;
+        m.solve_inline = solve_in_walk ? 1 : 0;
+        if (solve_in_walk) m.solve = shape_args(0, nullptr, nullptr);
         stage_begin(kMesh);
         launch_mesh_contacts(m, mesh_split_.data(), fork, fork ? side_ : stream_);
         stage_end(kMesh);
@@ -941,21 +969,8 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
         // iteration's output (both in sorted order)
         const float4* Xin = it == 0 ? Xs_[0].p : P_[cur].p;
         const float4* Vin = it == 0 ? Vs_[0].p : V_[cur].p;
-        ShapeArgs sa;
-        sa.cmax = int(cmax_); sa.cc = cc_solve;
+        const ShapeArgs sa = shape_args(it, Xin, Vin);
         const bool prev_order = it == 0 && n > 0 && !det_;
-        sa.cfd = det_ ? cfd_.p : nullptr;
-        sa.cbd = det_ ? cbd_.p : nullptr;
-        sa.cpart = cpart_.p; sa.cshape = cshape_.p; sa.cbpos = cbpos_.p; sa.cbvel = cbvel_.p; sa.cnorm = cnorm_.p;
-        sa.P = prev_order ? P_[s_].p : Xin; sa.V = prev_order ? V_[s_].p : Vin;
-        sa.flags = prev_order ? flags_prev : flags_cur();
-        sa.from_old = prev_order ? 1 : 0;
-        sa.g_dt = sdt; sa.g_vmax = cfg_.particle_v_max;
-        sa.gx = cfg_.gravity[0]; sa.gy = cfg_.gravity[1]; sa.gz = cfg_.gravity[2];
-        sa.body = body_args();
-        sa.pmu = cfg_.soft_contact_mu; sa.ka = cfg_.particle_adhesion; sa.dt = sdt; sa.relax = relax_;
-        sa.dshape = dshape_.p; sa.owner = owner_.p;
-        sa.skip_plane = plane_fold ? 1 : 0;
         // the first shape solve follows the mesh contacts on the side stream,
         // from the previous slot order, so it waits for nothing on the main
         // stream but the boxes; the particle-particle solve (which does not
@@ -963,10 +978,10 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
         const bool side = fork && it == 0 && !det_;
         if (side && nbox > 0) HIPCHECK(hipStreamWaitEvent(side_, sorted_ev_, 0));
         stage_begin(kShape);
-        if (shape_work) launch_shape_solve(sa, shape_grid, side ? side_ : stream_);
+        if (shape_work && !(it == 0 && solve_in_walk)) launch_shape_solve(sa, shape_grid, side ? side_ : stream_);
         stage_end(kShape);
         if (side) HIPCHECK(hipEventRecord(join_ev_, side_));
-        PPArgs pa;
+        PPArgs pa{};
         pa.n = n; pa.keys = keysA_.p; pa.vals = valsA_.p; pa.ids = ids_cur(); pa.range = range_.p;
         pa.Ps = P_[s_].p; pa.Xs = Xin; pa.Vs = Vin; pa.Fs = flags_cur();
         pa.delta = delta_.p; pa.Po = P_[nxt].p; pa.Vo = V_[nxt].p;
@@ -994,7 +1009,7 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
         } else if (!shape_work) {   // nothing to apply; the contacts still move to the sorted slots
             if (prev_order) launch_remap_contacts(cpart_.p, cc_solve, int(cmax_), rank_.p, shape_grid, stream_);
         } else {
-            ShapeApplyArgs aa;
+            ShapeApplyArgs aa{};
             aa.cmax = int(cmax_); aa.cc = cc_solve; aa.cpart = cpart_.p;
             aa.remap = prev_order ? rank_.p : nullptr;
             aa.owner = owner_.p; aa.dshape = dshape_.p; aa.delta = delta_.p;
@@ -1338,7 +1353,7 @@ void Solver::collide_only() {
         launch_particle_box_contacts(pb, stream_);
     }
     if (!meshes_.empty()) {
-        MeshArgs m;
+        MeshArgs m{};
         m.n = int(n_); m.P = P_[s_].p; m.flags = flags_cur();
         m.verts = verts_d_.p; m.tris = tris_d_.p; m.nodes = nodes_d_.p; m.cnodes = cnodes_d_.p; m.lbox = lbox_d_.p; m.prim = prim_d_.p;
         m.pg_start = pg_start_.p; m.pg_soup = pg_soup_.p; m.tsoup = tsoup_.p; m.lb = lb_.p; m.lb_node = lb_node_.p;
@@ -1350,6 +1365,8 @@ void Solver::collide_only() {
         m.cpart = cpart_.p; m.cshape = cshape_.p; m.cbpos = cbpos_.p; m.cbvel = cbvel_.p; m.cnorm = cnorm_.p;
         m.err = reinterpret_cast<uint32_t*>(cc_.p + 2);
         m.stats = mesh_stats_on_ ? mesh_stats_.p : nullptr;
+        m.plane = 0;          // the plane's contacts come from k_plane_contacts above
+        m.solve_inline = 0;   // contacts only, no solve
         launch_mesh_contacts(m, mesh_split_.data(), false, stream_);
     }
     HIPCHECK(hipGetLastError());

@@ -269,6 +269,7 @@ private:
     DevBuf<uint32_t> chunks_;          // chunk totals of the chunked radix passes, kSortMaxPasses arrays
     size_t sort_chunk_words_ = 0;      // words per pass array
     bool sort_chunked_ = false;        // chunked radix passes (VKXPBD_SORT_CHUNKED=1; else upsweep, binscan, scatter)
+    bool solve_in_walk_ = true;        // the walk solves its contacts (VKXPBD_SOLVE_IN_WALK=0: k_shape_solve)
     bool fuse_keys_ = true;            // substeps keyed by the previous one's apply (VKXPBD_FUSE_KEYS=0: off)
     DevBuf<float4> Xs_[1], Vs_[1];   // predicted state in sorted order (the first iteration's input)
     DevBuf<int32_t> range_;   // int2 per key (start, end); rows of the last build are reset before the next

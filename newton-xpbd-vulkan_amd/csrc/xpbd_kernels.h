@@ -144,6 +144,93 @@ struct ShapeArgs {
     float4* cfd; float4* cbd;
 };
 
+// One particle-shape contact of solve_particle_shape_contacts.comp:121-188:
+// the particle's state (P, Vv), the shape's body (-1: none) and friction mu,
+// the contact's body-local point and velocity and its normal.  Returns false
+// where the shader adds nothing (farther than the adhesion distance, or no
+// inverse mass on either side); else the particle's delta fd, the unweighted
+// delta dtot and the lever arm r.  The shape solve runs every contact
+// through it, and the particle-pair solve and the shape-delta apply run the
+// ground plane's through it (body -1, the contact a pure function of the
+// particle's substep-start position), so all three compute the same floats.
+__device__ __forceinline__ bool shape_contact(float4 P, float4 Vv, int body, const BodyArgs& B, float mu, f3 cbpos,
+                                              f3 cbvel, f3 n, float ka, float dt, float relax, f3& fd, f3& dtot, f3& r) {
+    f3 px = xyz(P), pv = xyz(Vv);
+    BodyFrame F = load_body(B, body);
+    f3 bx = F.pos + quat_rotate_sc(F.rot, cbpos);
+    r = bx - (F.pos + quat_rotate_sc(F.rot, F.com));
+    f3 d = px - bx;
+    float c = fma_(n.z, d.z, fma_(n.x, d.x, n.y * d.y)) - P.w;
+    if (c > ka) return false;
+    f3 bv = F.v + cross3(F.w, r) + quat_rotate_sc(F.rot, cbvel);
+    f3 v = pv - bv;
+    float lambda_n = c;
+    f3 delta_n = n * lambda_n;
+    float vn = n.y * v.y;
+    vn = fma_(n.x, v.x, vn);
+    vn = fma_(n.z, v.z, vn);
+    f3 vt = {fma_(-n.x, vn, v.x), fma_(-n.y, vn, v.y), fma_(-n.z, vn, v.z)};
+    float w1 = Vv.w;
+    float w2 = 0.0f;
+    if (body >= 0) {
+        f3 ang = cross3(r, n);
+        f3 ra = quat_rotate_inv_sc(F.rot, ang);
+        const float* I = B.invI + 9 * body;   // row-major, as this shader reads it
+        f3 Ira = {I[0] * ra.x + I[1] * ra.y + I[2] * ra.z, I[3] * ra.x + I[4] * ra.y + I[5] * ra.z,
+                  I[6] * ra.x + I[7] * ra.y + I[8] * ra.z};
+        w2 = B.invm[body] + dot3(ra, Ira);
+    }
+    float denom = w1 + w2;
+    if (denom == 0.0f) return false;
+    float l2 = vt.y * vt.y;
+    l2 = fma_(vt.x, vt.x, l2);
+    l2 = fma_(vt.z, vt.z, l2);
+    const RootInv vl = exact_sqrt_inv(l2);
+    const float vt_len = vl.d;
+    float lambda_f = gmax(mu * lambda_n, -vt_len * dt);
+    f3 vt_dir = vt_len > 0.0f ? exact_div_by(vt, vt_len, vl.inv) : f3{0.f, 0.f, 0.f};
+    f3 diff = {fma_(vt_dir.x, lambda_f, -delta_n.x), fma_(vt_dir.y, lambda_f, -delta_n.y),
+               fma_(vt_dir.z, lambda_f, -delta_n.z)};
+    dtot = exact_div(diff, denom) * relax;
+    fd = w1 * dtot;
+    return true;
+}
+
+// One contact of solve_particle_shape_contacts.comp:104-200 (t: its index,
+// pi: its particle's slot in a.P, s: its shape): the shader's early outs, the
+// particle state (integrated over g_dt when it is the substep-start state of
+// the previous slot order), shape_contact, then the float atomics into the
+// particle's shape sum and the body's delta; the particle is marked with its
+// largest contact index (owner), which k_shape_apply consumes and clears.
+// k_shape_solve runs it over the contact buffer; the mesh walk may run it on
+// each contact it writes, with the same values.
+__device__ __forceinline__ void shape_solve_one(const ShapeArgs& a, int t, int pi, int s, f3 bpos, f3 bvel, f3 nrm) {
+    const int body = a.body.shape_body[s];
+    const int pf = a.flags[pi];
+    if ((pf & 1) == 0) return;
+    if (pf & 2) {
+        if (body < 0) return;
+        if (a.body.flags[body] & 4) return;
+        if (a.body.invm[body] == 0.0f) return;
+    }
+    float4 P = a.P[pi], Vv = a.V[pi];
+    if (a.from_old) integrate_particle(P, Vv, pf, a.g_dt, a.g_vmax, a.gx, a.gy, a.gz);
+    const float mu = 0.5f * (a.pmu + a.body.mat_mu[s]);
+    f3 fd, dtot, r;
+    if (!shape_contact(P, Vv, body, a.body, mu, bpos, bvel, nrm, a.ka, a.dt, a.relax, fd, dtot, r)) return;
+    atomicMax(&a.owner[pi], t);
+    atomicAdd(&a.dshape[3 * pi + 0], fd.x);
+    atomicAdd(&a.dshape[3 * pi + 1], fd.y);
+    atomicAdd(&a.dshape[3 * pi + 2], fd.z);
+    if (body >= 0 && a.body.invm[body] > 0.0f) {
+        f3 dv = {dtot.x / a.dt, dtot.y / a.dt, dtot.z / a.dt};
+        f3 dw = cross3(r, dv);
+        float* bd = a.body.delta + 8 * body;
+        atomicAdd(bd + 0, -dv.x); atomicAdd(bd + 1, -dv.y); atomicAdd(bd + 2, -dv.z);
+        atomicAdd(bd + 4, -dw.x); atomicAdd(bd + 5, -dw.y); atomicAdd(bd + 6, -dw.z);
+    }
+}
+
 // Adds the shape deltas to the particles that received any, after k_pp_apply
 // wrote every particle with its particle-particle delta alone.
 struct ShapeApplyArgs {
@@ -241,6 +328,10 @@ struct MeshArgs {
     uint32_t* mcnt;             // per (substep parity, mesh): queue length and parity-queue length
     int slot;                   // substep parity: which counter pair this substep uses
     int2* mq;                   // candidate queue: (particle, search start node word)
+    // solve_inline: the walk solves each contact it writes (shape_solve_one
+    // with solve: the first shape solve, in the previous slot order)
+    int solve_inline = 0;
+    ShapeArgs solve{};
     // plane > 0: the first mesh's candidate pass also emits the ground-plane
     // contacts (generate_plane_contacts on the same substep-start positions)
     int plane = 0; float altitude = 0.0f;
