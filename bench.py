@@ -63,13 +63,16 @@ def survey_bytes(stage, N, C, Cp, Cm, U, U_prev, mesh_bytes):
     return None
 
 
-def stage_bytes(stage, N, C, Cm, U, tiles, mesh_bytes, passes, bins=128):
+def stage_bytes(stage, N, C, Cm, U, tiles, mesh_bytes, passes, bins=128, keyed=(SUBSTEPS - 1) / SUBSTEPS):
     Cp = C - Cm
     """This build's own count of the HBM bytes of ONE launch of a stage:
     every array the kernel must touch in its actual layout, counted once
     (neighbour re-reads are cache hits).  The state lives in sorted slot
     order; the sort runs `passes` LSD passes of `bins` digit values over
-    `tiles` tiles."""
+    `tiles` tiles.  `keyed`: the share of particle-pair solves and shape-delta
+    applies that also key the next substep (k_integrate runs once per frame):
+    the ids read and the (key, slot) pairs written by the first, the cell
+    rows reset from their keys by the second."""
     hist = tiles * bins * 4
     if stage == "integrate":      # P,V,flags,ids in; (key, slot) pair out;
         # reset of the last substep's cell rows (their keys, zeroed rows)
@@ -87,11 +90,12 @@ def stage_bytes(stage, N, C, Cm, U, tiles, mesh_bytes, passes, bins=128):
         # plane contacts: shape and slot read, skipped
         return Cm * (56 + 4 + 32 + 4 + 24 + 4) + Cp * 8
     if stage == "pp_solve_apply":  # vals, flags, Xs/Vs, x0, delta, P/V out; cell ranges; pair sums
-        # kept for particles with a plane contact
-        return N * (4 + 4 + 32 + 16 + 12 + 32) + 8 * U + 12 * Cp
+        # kept for particles with a plane contact; the next substep's keys
+        return N * (4 + 4 + 32 + 16 + 12 + 32) + 8 * U + 12 * Cp + keyed * N * (4 + 8)
     if stage == "shape_apply":     # per contact: slot, new slot, slot store, owner; per owner (the
-        # mesh contacts): owner clear, shape delta rw, delta rw, Xs/Vs, slot + x0, flags, pair sum, P/V out
-        return C * 16 + Cm * (4 + 24 + 24 + 32 + 4 + 16 + 4 + 12 + 32)
+        # mesh contacts): owner clear, shape delta rw, delta rw, Xs/Vs, slot + x0, flags, pair sum, P/V out;
+        # the next substep's cell-row reset
+        return C * 16 + Cm * (4 + 24 + 24 + 32 + 4 + 16 + 4 + 12 + 32) + keyed * (4 * N + 8 * U)
     if stage == "mesh_contacts":   # particles in, mesh and plane contacts out, mesh tables
         return N * (16 + 4) + 56 * C + mesh_bytes
     return 0
