@@ -135,9 +135,10 @@ __global__ __launch_bounds__(256) void k_plane_contacts(PlaneArgs a) {
 // grid_offsets.comp:16-35 into one interleaved (start, end) table.  Instead
 // of the reference's two 8 MiB table fills before every build
 // (scene.cpp:918-923), rkeys records the keys whose rows this build writes,
-// and the next build resets exactly those rows first (k_integrate does it
-// inside a substep, k_clear_ranges elsewhere), so every row not written by
-// the current build reads (0, 0).
+// and exactly those rows are reset before the next build (by k_integrate at
+// a frame's first substep, by the previous substep's k_shape_apply or the
+// first radix upsweep at the others, by k_clear_ranges outside a substep),
+// so every row not written by the current build reads (0, 0).
 __global__ __launch_bounds__(256) void k_cell_ranges(int n, const uint32_t* __restrict__ keys, int* __restrict__ range,
                                                      uint32_t* __restrict__ rkeys) {
     int t = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;

@@ -28,9 +28,10 @@ __host__ __device__ constexpr int sort_items_for(long long n) { return n >= kSor
 // zeroed by this pass's scatter.
 // chunk == nullptr: the three-launch pass (upsweep, binscan, scatter).
 // Work the first radix upsweep does for a substep keyed by the previous one
-// (k_integrate skipped): the rows of the last cell-range build are reset from
-// their keys (rkeys, as k_integrate resets them) and the next substep's
-// contact counter is zeroed.  rkeys == nullptr: nothing.
+// (k_integrate skipped): the next substep's contact counter is zeroed and,
+// unless the previous substep's k_shape_apply did it, the rows of the last
+// cell-range build are reset from their keys (rkeys, as k_integrate resets
+// them).  Null pointers: nothing.
 struct SortReset {
     const uint32_t* rkeys = nullptr;
     int* range = nullptr;
