@@ -418,6 +418,40 @@ def test_sort_and_keying_variants(vk, env, monkeypatch):
     assert_contacts_equal(gpu_contacts(g), o.get_contacts())
 
 
+@pytest.mark.parametrize("iterations", [1, 2])
+def test_keyed_substeps_inactive_proxy_iterations(vk, iterations, monkeypatch):
+    """Later substeps keyed by the previous substep's apply, with inactive and
+    proxy particles (the pass-through keys them too) and with two iterations
+    (only the last one keys the next substep and resets the cell rows): a pile
+    dropped onto an icosphere, against the oracle and against every substep
+    keyed by k_integrate."""
+    rng = np.random.default_rng(31)
+    r = 0.01
+    q = scenes.lattice(20, 20, 6, 2 * r, (-0.19, -0.19, 1.01), 0.1 * r, rng)
+    flags = np.ones(q.shape[0], np.int32)
+    flags[::7] = 0          # inactive
+    flags[3::11] = 1 | 2    # active proxy
+    verts, tris = scenes.icosphere(3)
+    sc = scene_from(q, r, meshes=[(verts, tris)], flags=flags)
+    g, o = make_pair(vk, sc)
+    monkeypatch.setenv("VKXPBD_FUSE_KEYS", "0")
+    h = scenes.build_solver(vk, sc)
+    monkeypatch.delenv("VKXPBD_FUSE_KEYS")
+    for f in range(8):
+        for s in (g, o, h):
+            s.step(1.0 / 60.0, 8, iterations)
+        assert g.last_contact_count == o.last_contact_count == h.last_contact_count
+        if f % 2 == 1:
+            compare_state(g, o, exact=True, what=f"keyed, {iterations} iteration(s), frame {f + 1}")
+    np.testing.assert_array_equal(g.get_positions(), h.get_positions())
+    np.testing.assert_array_equal(g.get_velocities(), h.get_velocities())
+    compare_grid(g, o)
+    assert_contacts_equal(gpu_contacts(g), o.get_contacts())
+    off = flags == 0
+    np.testing.assert_array_equal(g.get_positions()[off], q[off])
+    assert g.last_contact_count > 0
+
+
 def test_update_mesh_moving_collider(vk):
     rng = np.random.default_rng(9)
     r = 0.01
