@@ -456,13 +456,14 @@ __device__ __forceinline__ void pp_pair(const PPArgs& a, int t, int k, float4 Pj
 
 // The next substep's radix input for the particle finished at sorted slot t
 // with state (P, V) and flags f: what k_integrate computes at the start of
-// the next substep (integrate_particle, the cell key, the wrap plane above
-// it), stored as (key, t) at the particle's id.
+// the next substep (integrate_particle, the cell key, the wrap plane or a
+// batched solver's scene above it), stored as (key, t) at the particle's id.
 __device__ __forceinline__ void write_next_key(const NextKey& nk, int t, float4 P, float4 V, int f, float dt,
                                                float vmax) {
     integrate_particle(P, V, f, dt, vmax, nk.gx, nk.gy, nk.gz);
     uint32_t key = cell_of(P.x, P.y, P.z, nk.inv_cell);
-    if (nk.planes) key |= plane_of(P.x, P.y, P.z, nk.inv_cell, nk.planes) << kCellBits;
+    if (nk.keys != nullptr) key |= nk.keys[t] & ~((1u << kCellBits) - 1u);   // the particle's scene
+    else if (nk.planes) key |= plane_of(P.x, P.y, P.z, nk.inv_cell, nk.planes) << kCellBits;
     nk.kv[nk.ids[t]] = make_uint2(key, uint32_t(t));
 }
 

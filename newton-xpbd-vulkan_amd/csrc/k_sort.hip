@@ -53,6 +53,10 @@ __global__ __launch_bounds__(256) void k_sort_upsweep(int n, const uint32_t* __r
             const uint32_t c = rs.rkeys[idx];
             if (idx == 0 || rs.rkeys[idx - 1] != c) *reinterpret_cast<int2*>(rs.range + size_t(2) * c) = make_int2(0, 0);
         }
+        if (rs.rslot != nullptr && idx < n) {   // a batched solver's row table, as k_integrate empties it
+            const int r = rs.rslot[idx];
+            if (r >= 0) rs.rows[r] = make_uint2(0u, 0u);
+        }
     }
     __syncthreads();
     if (ch.chunk != nullptr) {   // padded rows, and the tile's share of its chunk's totals

@@ -759,19 +759,27 @@ BodyStepArgs Solver::body_step_args(float sdt) {
 
 // Substeps inside a frame may key each other (the last iteration's apply
 // writes the next substep's radix input, which k_integrate would compute from
-// the same state by the same code) where k_integrate has no other work: a
-// plain solver without boxes, restitution or deterministic mode, and no plane
-// contacts of its own to emit (a mesh's candidate pass emits them, or there is
-// no ground plane).  A frame's first substep is always keyed by k_integrate,
-// the host may have changed the state before it.
+// the same state by the same code) where k_integrate has no other work that
+// is read: solvers without boxes, restitution or deterministic mode whose
+// plane contacts come from a mesh's candidate pass, or that have no ground
+// plane, or whose plane contacts are solved inside the particle-pair solve
+// (then the contact buffer is read only after the frame, and k_integrate keys
+// the last substep to emit them).  A frame's first substep is always keyed by
+// k_integrate: the host may have changed the state before it.
 bool Solver::fuse_keys_ok(int iterations) const {
-    return fuse_keys_ && num_scenes_ == 1 && !det_ && boxes_.empty() && !cfg_.enable_restitution && iterations >= 1 &&
-           n_ > 0 && (!meshes_.empty() || !cfg_.has_ground_plane);
+    if (!fuse_keys_ || det_ || !boxes_.empty() || cfg_.enable_restitution || iterations < 1 || n_ == 0) return false;
+    if (!meshes_.empty() || !cfg_.has_ground_plane) return true;
+    // without a mesh k_integrate emits the plane contacts; with the plane
+    // folded into the particle-pair solve (every plane contact fits) nothing
+    // reads them before the frame ends, so only the last substep needs them
+    return cmax_ >= n_;
 }
 
 int Solver::frame_fuse(int sub, int substeps, int iterations) const {
     if (!fuse_keys_ok(iterations)) return 0;
-    return (sub > 0 ? kKeyed : 0) | (sub + 1 < substeps ? kKeyNext : 0);
+    const bool last_emits = meshes_.empty() && cfg_.has_ground_plane;   // k_integrate keys the last substep
+    auto keyed = [&](int k) { return k > 0 && !(last_emits && k == substeps - 1); };
+    return (keyed(sub) ? kKeyed : 0) | (sub + 1 < substeps && keyed(sub + 1) ? kKeyNext : 0);
 }
 
 void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zero_next, int fuse) {
@@ -908,6 +916,7 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
         nk.gx = cfg_.gravity[0]; nk.gy = cfg_.gravity[1]; nk.gz = cfg_.gravity[2];
         nk.inv_cell = inv_cell;
         nk.planes = planes_ ? plane_bits_ : 0;
+        nk.keys = num_scenes_ > 1 ? keysA_.p : nullptr;
     }
     if (n > 0) {
         // plain solvers move the state into the sorted order in the cell-range
@@ -916,7 +925,10 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
         const bool gather_in_ranges = num_scenes_ == 1;
         SortReset rs;
         if (keyed) {   // k_integrate's resets (the rows: the last shape-delta apply's, if it ran)
-            if (!shape_work) {
+            if (num_scenes_ > 1) {
+                rs.rslot = rslot_.p;
+                rs.rows = rows_.p;
+            } else if (!shape_work) {
                 rs.rkeys = rkeys_.p;
                 rs.range = range_.p;
             }

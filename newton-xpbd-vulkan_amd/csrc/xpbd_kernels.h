@@ -36,6 +36,8 @@ struct SortReset {
     const uint32_t* rkeys = nullptr;
     int* range = nullptr;
     int* cc_next = nullptr;
+    const int* rslot = nullptr;   // batched solvers: the row-table rows the last build filled (-1: none)
+    uint2* rows = nullptr;
 };
 constexpr int kSortChunkTiles = 32;
 constexpr int kSortMaxPasses = 4;   // 32-bit keys in 8-bit digits
@@ -81,6 +83,7 @@ struct NextKey {
     uint2* kv; const uint32_t* ids;   // sorted slot -> particle id
     float gx, gy, gz, inv_cell;
     int planes;
+    const uint32_t* keys;             // batched solvers: the sorted keys, whose scene bits the key keeps; else null
 };
 
 struct IntegrateArgs {
