@@ -14,7 +14,7 @@
 //               ballots per key give each lane its same-digit peers), per-wave
 //               bin counters in LDS, an LDS-staged reorder of the tile, then
 //               bin-run-contiguous global stores.
-// Chunked passes (SortChunks, at most 8-bit digits, two passes or more) have
+// Chunked passes (SortPass, at most 8-bit digits, two passes or more) have
 // no binscan: the upsweep writes hist[bin][tile] (rows padded to whole
 // chunks) and adds its histogram into its chunk of kSortChunkTiles tiles
 // (chunk[c][bin]: one device atomic per bin, a workgroup's atomics on
@@ -37,23 +37,29 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) { return uint32_t
 template <int BITS, int ITEMS>
 __global__ __launch_bounds__(256) void k_sort_upsweep(int n, const uint32_t* __restrict__ keys,
                                                       const uint2* __restrict__ pairs, int shift,
-                                                      uint32_t* __restrict__ hist, int tiles, SortChunks ch,
+                                                      uint32_t* __restrict__ hist, int tiles, SortPass ch,
                                                       SortReset rs) {
     constexpr int B = 1 << BITS;
     __shared__ uint32_t h[B];
     const int tile = xcd_tile(blockIdx.x, gridDim.x), w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     for (int b = threadIdx.x; b < B; b += 256) h[b] = 0;
     if (rs.cc_next != nullptr && blockIdx.x == 0 && threadIdx.x == 0) *rs.cc_next = 0;
+    int first = tile * (256 * ITEMS), lim = n;   // the tile's keys: [first, lim)
+    if (ch.seg != nullptr) {
+        const int4 sg = ch.seg[tile];
+        first = sg.x;
+        lim = sg.x + sg.y;
+    }
     __syncthreads();
 #pragma unroll
     for (int it = 0; it < ITEMS; it++) {
-        int idx = tile * (256 * ITEMS) + w * (ITEMS * kWave) + it * kWave + lane;
-        if (idx < n) atomicAdd(&h[((pairs ? pairs[idx].x : keys[idx]) >> shift) & (B - 1)], 1u);
-        if (rs.rkeys != nullptr && idx < n) {   // the last build's rows (k_integrate's reset)
+        int idx = first + w * (ITEMS * kWave) + it * kWave + lane;
+        if (idx < lim) atomicAdd(&h[((pairs ? pairs[idx].x : keys[idx]) >> shift) & (B - 1)], 1u);
+        if (rs.rkeys != nullptr && idx < lim) {   // the last build's rows (k_integrate's reset)
             const uint32_t c = rs.rkeys[idx];
             if (idx == 0 || rs.rkeys[idx - 1] != c) *reinterpret_cast<int2*>(rs.range + size_t(2) * c) = make_int2(0, 0);
         }
-        if (rs.rslot != nullptr && idx < n) {   // a batched solver's row table, as k_integrate empties it
+        if (rs.rslot != nullptr && idx < lim) {   // a batched solver's row table, as k_integrate empties it
             const int r = rs.rslot[idx];
             if (r >= 0) rs.rows[r] = make_uint2(0u, 0u);
         }
@@ -110,7 +116,7 @@ __global__ __launch_bounds__(256) void k_sort_scatter(int n, const uint32_t* __r
                                                       uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out,
                                                       int shift, const uint32_t* __restrict__ hist,
                                                       const uint32_t* __restrict__ bintot, int tiles, SortGather g,
-                                                      SortChunks ch) {
+                                                      SortPass ch) {
     constexpr int B = 1 << BITS;
     constexpr int TILE = 256 * ITEMS;
     constexpr int BPT = B > 256 ? B / 256 : 1;   // bins per thread in the tile-level scans
@@ -125,13 +131,20 @@ __global__ __launch_bounds__(256) void k_sort_scatter(int n, const uint32_t* __r
     for (int k = threadIdx.x; k < 4 * B; k += 256) (&s_cnt[0][0])[k] = 0;
     if (ch.chunk != nullptr)
         for (int e = blockIdx.x * 256 + threadIdx.x; e < ch.zero_count; e += gridDim.x * 256) ch.zero_next[e] = 0u;
+    int first = tile * TILE, lim = n;   // the tile's keys: [first, lim)
+    int4 sg = make_int4(0, 0, 0, 0);
+    if (ch.seg != nullptr) {
+        sg = ch.seg[tile];
+        first = sg.x;
+        lim = sg.x + sg.y;
+    }
     __syncthreads();
 
     uint32_t key[ITEMS], val[ITEMS], rank[ITEMS];
 #pragma unroll
     for (int it = 0; it < ITEMS; it++) {
-        const int idx = tile * TILE + w * (ITEMS * kWave) + it * kWave + lane;
-        const bool valid = idx < n;
+        const int idx = first + w * (ITEMS * kWave) + it * kWave + lane;
+        const bool valid = idx < lim;
         if (PAIRS) {
             const uint2 kv = valid ? pairs_in[idx] : make_uint2(0xFFFFFFFFu, 0u);
             key[it] = kv.x;
@@ -177,10 +190,26 @@ __global__ __launch_bounds__(256) void k_sort_scatter(int n, const uint32_t* __r
             pre[j] = p;
         }
     }
+    // segmented pass: each thread's bins' global prefixes at this tile, at
+    // the segment's first tile and at the next segment's first tile (the bin
+    // total past the last), requested behind the keys
+    uint32_t spt[BPT], sp0[BPT], sp1[BPT];
+    uint32_t segfirst = 0;
+    if (ch.seg != nullptr) {
+        segfirst = uint32_t(ch.seg[sg.z].x);
+#pragma unroll
+        for (int j = 0; j < BPT; j++) {
+            const int b = min(threadIdx.x * BPT + j, B - 1);
+            const uint32_t* row = hist + size_t(b) * tiles;
+            spt[j] = row[tile];
+            sp0[j] = row[sg.z];
+            sp1[j] = sg.w < tiles ? row[sg.w] : bintot[b];
+        }
+    }
 #pragma unroll
     for (int it = 0; it < ITEMS; it++) {
-        const int idx = tile * TILE + w * (ITEMS * kWave) + it * kWave + lane;
-        const bool valid = idx < n;
+        const int idx = first + w * (ITEMS * kWave) + it * kWave + lane;
+        const bool valid = idx < lim;
         const uint32_t d = (key[it] >> shift) & (B - 1);
         uint64_t peers = __ballot(valid);
 #pragma unroll
@@ -210,7 +239,7 @@ __global__ __launch_bounds__(256) void k_sort_scatter(int n, const uint32_t* __r
             const uint32_t c0 = s_cnt[0][b], c1 = s_cnt[1][b], c2 = s_cnt[2][b], c3 = s_cnt[3][b];
             s_cnt[0][b] = 0; s_cnt[1][b] = c0; s_cnt[2][b] = c0 + c1; s_cnt[3][b] = c0 + c1 + c2;
             tot[j] = c0 + c1 + c2 + c3;
-            bt[j] = ch.chunk != nullptr ? gt[j] : bintot[b];
+            bt[j] = ch.seg != nullptr ? sp1[j] - sp0[j] : (ch.chunk != nullptr ? gt[j] : bintot[b]);
         }
         tsum += tot[j];
         bsum += bt[j];
@@ -225,7 +254,8 @@ __global__ __launch_bounds__(256) void k_sort_scatter(int n, const uint32_t* __r
         const int b = threadIdx.x * BPT + j;
         if (b < B) {
             s_excl[b] = et;
-            s_gbase[b] = eb + (ch.chunk != nullptr ? pre[j] : hist[size_t(b) * tiles + tile]);
+            if (ch.seg != nullptr) s_gbase[b] = segfirst + eb + (spt[j] - sp0[j]);
+            else s_gbase[b] = eb + (ch.chunk != nullptr ? pre[j] : hist[size_t(b) * tiles + tile]);
         }
         et += tot[j];
         eb += bt[j];
@@ -233,8 +263,8 @@ __global__ __launch_bounds__(256) void k_sort_scatter(int n, const uint32_t* __r
     __syncthreads();
 #pragma unroll
     for (int it = 0; it < ITEMS; it++) {
-        const int idx = tile * TILE + w * (ITEMS * kWave) + it * kWave + lane;
-        if (idx < n) {
+        const int idx = first + w * (ITEMS * kWave) + it * kWave + lane;
+        if (idx < lim) {
             const uint32_t d = (key[it] >> shift) & (B - 1);
             const uint32_t pos = s_excl[d] + s_cnt[w][d] + rank[it];
             if (pos < uint32_t(TILE)) {
@@ -244,7 +274,7 @@ __global__ __launch_bounds__(256) void k_sort_scatter(int n, const uint32_t* __r
         }
     }
     __syncthreads();
-    const int cnt = min(TILE, n - tile * TILE);
+    const int cnt = min(TILE, lim - first);
     for (int e = threadIdx.x; e < cnt; e += 256) {
         const uint32_t k = s_keys[e];
         const uint32_t d = (k >> shift) & (B - 1);
@@ -270,14 +300,14 @@ __global__ __launch_bounds__(256) void k_sort_scatter(int n, const uint32_t* __r
 template <int BITS, bool PAIRS, bool GATHER, int ITEMS>
 static void scatter(int n, const uint32_t* keys_in, const uint32_t* vals_in, const uint2* pairs_in, uint32_t* keys_out,
                     uint32_t* vals_out, int shift, const uint32_t* hist, const uint32_t* bintot, int tiles,
-                    const SortGather& g, const SortChunks& ch, hipStream_t s) {
+                    const SortGather& g, const SortPass& ch, hipStream_t s) {
     hipLaunchKernelGGL((k_sort_scatter<BITS, PAIRS, GATHER, ITEMS>), dim3(tiles), dim3(256), 0, s, n, keys_in, vals_in,
                        pairs_in, keys_out, vals_out, shift, hist, bintot, tiles, g, ch);
 }
 template <int BITS, int ITEMS>
 static void scatter_bits(int n, const uint32_t* keys_in, const uint32_t* vals_in, const uint2* pairs_in,
                          uint32_t* keys_out, uint32_t* vals_out, int shift, const uint32_t* hist, const uint32_t* bintot,
-                         int tiles, const SortGather* gather, const SortChunks& ch, hipStream_t s) {
+                         int tiles, const SortGather* gather, const SortPass& ch, hipStream_t s) {
     const SortGather g = gather ? *gather : SortGather{};
     if (pairs_in) {
         if (gather) scatter<BITS, true, true, ITEMS>(n, keys_in, vals_in, pairs_in, keys_out, vals_out, shift, hist, bintot, tiles, g, ch, s);
@@ -290,7 +320,7 @@ static void scatter_bits(int n, const uint32_t* keys_in, const uint32_t* vals_in
 template <int ITEMS>
 static void scatter_items(int bits, int n, const uint32_t* keys_in, const uint32_t* vals_in, const uint2* pairs_in,
                           uint32_t* keys_out, uint32_t* vals_out, int shift, const uint32_t* hist, const uint32_t* bintot,
-                          int tiles, const SortGather* gather, const SortChunks& ch, hipStream_t s) {
+                          int tiles, const SortGather* gather, const SortPass& ch, hipStream_t s) {
     switch (bits) {
         case 7: scatter_bits<7, ITEMS>(n, keys_in, vals_in, pairs_in, keys_out, vals_out, shift, hist, bintot, tiles, gather, ch, s); break;
         case 8: scatter_bits<8, ITEMS>(n, keys_in, vals_in, pairs_in, keys_out, vals_out, shift, hist, bintot, tiles, gather, ch, s); break;
@@ -300,7 +330,7 @@ static void scatter_items(int bits, int n, const uint32_t* keys_in, const uint32
 }
 template <int ITEMS>
 static void upsweep_items(int bits, int n, const uint32_t* keys, const uint2* pairs, int shift, uint32_t* hist, int tiles,
-                          const SortChunks& ch, const SortReset& rs, hipStream_t s) {
+                          const SortPass& ch, const SortReset& rs, hipStream_t s) {
     switch (bits) {
         case 7: hipLaunchKernelGGL((k_sort_upsweep<7, ITEMS>), dim3(tiles), dim3(256), 0, s, n, keys, pairs, shift, hist, tiles, ch, rs); break;
         case 8: hipLaunchKernelGGL((k_sort_upsweep<8, ITEMS>), dim3(tiles), dim3(256), 0, s, n, keys, pairs, shift, hist, tiles, ch, rs); break;
@@ -310,7 +340,7 @@ static void upsweep_items(int bits, int n, const uint32_t* keys, const uint2* pa
 }
 
 void launch_sort_upsweep(int bits, int items, int n, const uint32_t* keys, const uint2* pairs, int shift, uint32_t* hist,
-                         int tiles, const SortChunks& ch, const SortReset& rs, hipStream_t s) {
+                         int tiles, const SortPass& ch, const SortReset& rs, hipStream_t s) {
     if (items == 8) upsweep_items<8>(bits, n, keys, pairs, shift, hist, tiles, ch, rs, s);
     else upsweep_items<4>(bits, n, keys, pairs, shift, hist, tiles, ch, rs, s);
 }
@@ -319,7 +349,7 @@ void launch_sort_binscan(int bits, uint32_t* hist, uint32_t* bintot, int tiles, 
 }
 void launch_sort_scatter(int bits, int items, int n, const uint32_t* keys_in, const uint32_t* vals_in, const uint2* pairs_in,
                          uint32_t* keys_out, uint32_t* vals_out, int shift, const uint32_t* hist, const uint32_t* bintot,
-                         int tiles, const SortGather* gather, const SortChunks& ch, hipStream_t s) {
+                         int tiles, const SortGather* gather, const SortPass& ch, hipStream_t s) {
     if (items == 8) scatter_items<8>(bits, n, keys_in, vals_in, pairs_in, keys_out, vals_out, shift, hist, bintot, tiles, gather, ch, s);
     else scatter_items<4>(bits, n, keys_in, vals_in, pairs_in, keys_out, vals_out, shift, hist, bintot, tiles, gather, ch, s);
 }

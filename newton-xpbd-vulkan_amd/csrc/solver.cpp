@@ -56,7 +56,7 @@ void Solver::require_init() const {
 void Solver::free_all() {
     for (int k = 0; k < 3; k++) { dfree(P_[k]); dfree(V_[k]); }
     for (int k = 0; k < 2; k++) { dfree(flags_[k]); dfree(ids_[k]); }
-    dfree(rank_); dfree(kv_); dfree(keysA_); dfree(valsA_); dfree(keysB_); dfree(valsB_); dfree(hist_); dfree(bintot_); dfree(chunks_);
+    dfree(rank_); dfree(kv_); dfree(keysA_); dfree(valsA_); dfree(keysB_); dfree(valsB_); dfree(hist_); dfree(bintot_); dfree(chunks_); dfree(seg_tab_[0]); dfree(seg_tab_[1]);
     dfree(Xs_[0]); dfree(Vs_[0]);
     dfree(body_half_); dfree(body_I_); dfree(body_invw_); dfree(mat_mu_t_); dfree(mat_mu_r_);
     dfree(rc_s0_); dfree(rc_s1_); dfree(rc_p0_); dfree(rc_p1_); dfree(rc_n_); dfree(rc_impulse_);
@@ -214,6 +214,35 @@ void Solver::init_batch(const xpbd_config& cfg, const size_t* counts, size_t num
         for (size_t i = scene_off_[k]; i < scene_off_[k + 1]; i++) sc[i] = int32_t(k);
     DeviceGuard g(device_);
     upload(scene_d_, sc.data(), sc.size());
+    // a scene's particles keep their key range (ids and slots are
+    // scene-major), so the passes may sort each scene by its cells alone
+    const char* senv = std::getenv("VKXPBD_SEG_SORT");
+    seg_sort_ = !(senv && std::strcmp(senv, "0") == 0);
+    if (seg_sort_) {
+        build_seg_tables();
+        set_sort_shape(kCellBits);
+    }
+}
+
+// Radix tiles of whole scenes for both tile sizes: each scene's keys
+// [off, off + count) split into tiles of at most 1,024 or 2,048 keys.
+void Solver::build_seg_tables() {
+    for (int k = 0; k < 2; k++) {
+        const int tile = 256 * (k ? 8 : 4);
+        std::vector<int4> tab;
+        for (size_t s = 0; s + 1 < scene_off_.size(); s++) {
+            const int a = int(scene_off_[s]), b = int(scene_off_[s + 1]);
+            const int t0 = int(tab.size());
+            for (int x = a; x < b; x += tile) tab.push_back(make_int4(x, std::min(tile, b - x), t0, 0));
+            for (size_t t = size_t(t0); t < tab.size(); t++) tab[t].w = int(tab.size());
+        }
+        seg_tiles_[k] = int(tab.size());
+        if (tab.empty()) tab.push_back(make_int4(0, 0, 0, 0));
+        upload(seg_tab_[k], tab.data(), tab.size());
+    }
+    // the tile histograms: room for the larger count of tiles
+    const size_t need = size_t(kSortMaxBins) * size_t(std::max(seg_tiles_[0], 1));
+    if (hist_.n < need) dalloc(hist_, need);
 }
 
 void Solver::add_mesh(const float* v, size_t nv, const uint32_t* t, size_t nt, float margin, const float* rot, int wt) {
@@ -699,14 +728,18 @@ void Solver::run_sort(int tiles, const SortGather* last, const SortReset& rs) {
     // chunked passes (no bin-scan launch) for digits of at most 8 bits; pass
     // p's scatter zeroes the totals pass (p + 1) mod P adds into
     const int chunks = (tiles + kSortChunkTiles - 1) / kSortChunkTiles;
-    const bool chunked = sort_chunked_ && sort_bits_ <= 8 && sort_passes_ >= 2 && sort_passes_ <= kSortMaxPasses &&
+    // batched solvers: whole-scene tiles, each scene sorted in its own range
+    const bool segmented = num_scenes_ > 1 && seg_sort_;
+    if (segmented) tiles = seg_tiles_[sort_items_ == 8 ? 1 : 0];
+    const bool chunked = !segmented && sort_chunked_ && sort_bits_ <= 8 && sort_passes_ >= 2 && sort_passes_ <= kSortMaxPasses &&
                          size_t(256) * size_t(chunks) <= sort_chunk_words_;
     for (int pass = 0; pass < sort_passes_; pass++) {
         const int shift = pass * sort_bits_;
         const uint32_t* kin = pass == 0 ? nullptr : kb[dst ^ 1];
         const uint32_t* vin = pass == 0 ? nullptr : vb[dst ^ 1];
         const uint2* pin = pass == 0 ? kv_.p : nullptr;
-        SortChunks ch;
+        SortPass ch;
+        if (segmented) ch.seg = seg_tab_[sort_items_ == 8 ? 1 : 0].p;
         if (chunked) {
             ch.chunk = chunks_.p + size_t(pass) * sort_chunk_words_;
             ch.chunks = chunks;

@@ -267,6 +267,13 @@ private:
     // grid and sort
     DevBuf<uint32_t> keysA_, valsA_, keysB_, valsB_, hist_, bintot_;
     DevBuf<uint32_t> chunks_;          // chunk totals of the chunked radix passes, kSortMaxPasses arrays
+    // batched solvers: radix tiles that never cross a scene (first key, count,
+    // the scene's first tile, the next scene's first tile), for 1,024- and
+    // 2,048-key tiles; the passes then sort each scene's key range by its cells
+    DevBuf<int4> seg_tab_[2];
+    int seg_tiles_[2] = {0, 0};
+    bool seg_sort_ = true;             // VKXPBD_SEG_SORT=0: the scene bits as a radix digit instead
+    void build_seg_tables();
     size_t sort_chunk_words_ = 0;      // words per pass array
     bool sort_chunked_ = false;        // chunked radix passes (VKXPBD_SORT_CHUNKED=1; else upsweep, binscan, scatter)
     bool solve_in_walk_ = true;        // the walk solves its contacts (VKXPBD_SOLVE_IN_WALK=0: k_shape_solve)

@@ -106,9 +106,9 @@ void sort_run(const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_o
         uint32_t* ko = to_out ? keys_out : tk;
         uint32_t* vo = to_out ? vals_out : tv;
         const int shift = pass * s.bits;
-        launch_sort_upsweep(s.bits, s.items, nn, kin, nullptr, shift, hist, tiles, SortChunks{}, SortReset{}, stream);
+        launch_sort_upsweep(s.bits, s.items, nn, kin, nullptr, shift, hist, tiles, SortPass{}, SortReset{}, stream);
         launch_sort_binscan(s.bits, hist, bintot, tiles, stream);
-        launch_sort_scatter(s.bits, s.items, nn, kin, vin, nullptr, ko, vo, shift, hist, bintot, tiles, nullptr, SortChunks{},
+        launch_sort_scatter(s.bits, s.items, nn, kin, vin, nullptr, ko, vo, shift, hist, bintot, tiles, nullptr, SortPass{},
                             stream);
         kin = ko;
         vin = vo;

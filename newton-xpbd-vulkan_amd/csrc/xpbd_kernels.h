@@ -20,13 +20,19 @@ constexpr int kCellBits = 21;       // batched solver: key = scene << 21 | cell
 constexpr int kSortLargeN = 1 << 18;
 __host__ __device__ constexpr int sort_items_for(long long n) { return n >= kSortLargeN ? 8 : 4; }
 
-// Chunked radix passes (no bin-scan launch): tiles are grouped in chunks of
-// kSortChunkTiles; chunk holds each chunk's per-bin totals, chunk[c][bin]
-// (chunks x 2^bits words), summed by the upsweep with device atomics; the tile
-// counts are hist[bin][tile] with rows of tstride words (a multiple of
-// kSortChunkTiles).  zero_next / zero_count: the totals of the next pass,
-// zeroed by this pass's scatter.
-// chunk == nullptr: the three-launch pass (upsweep, binscan, scatter).
+// Options of one radix pass.  Chunked passes (no bin-scan launch): tiles are
+// grouped in chunks of kSortChunkTiles; chunk holds each chunk's per-bin
+// totals, chunk[c][bin] (chunks x 2^bits words), summed by the upsweep with
+// device atomics; the tile counts are hist[bin][tile] with rows of tstride
+// words (a multiple of kSortChunkTiles).  zero_next / zero_count: the totals
+// of the next pass, zeroed by this pass's scatter.  chunk == nullptr: the
+// three-launch pass (upsweep, binscan, scatter).
+// Segments (seg non-null, a batched solver's scenes): tile k holds keys
+// [seg[k].x, seg[k].x + seg[k].y) of one segment whose tiles are
+// [seg[k].z, seg[k].w), and every segment is sorted within its own key range:
+// the scatter's base of a bin is the segment's first key, plus the bin's
+// earlier bins inside the segment, plus the tile's count before it inside the
+// segment, all differences of the global bin scan.
 // Work the first radix upsweep does for a substep keyed by the previous one
 // (k_integrate skipped): the next substep's contact counter is zeroed and,
 // unless the previous substep's k_shape_apply did it, the rows of the last
@@ -41,7 +47,8 @@ struct SortReset {
 };
 constexpr int kSortChunkTiles = 32;
 constexpr int kSortMaxPasses = 4;   // 32-bit keys in 8-bit digits
-struct SortChunks {
+struct SortPass {
+    const int4* seg = nullptr;
     uint32_t* chunk = nullptr;
     int chunks = 0;
     int tstride = 0;
@@ -525,11 +532,11 @@ void launch_mesh_contacts(const MeshArgs& a, const uint8_t* watertight, bool bes
 // Chunked passes (ch.chunk set, bits <= 8): upsweep -> scatter, hist
 // tile-major and no bin totals.
 void launch_sort_upsweep(int bits, int items, int n, const uint32_t* keys, const uint2* pairs, int shift, uint32_t* hist,
-                         int tiles, const SortChunks& ch, const SortReset& rs, hipStream_t s);
+                         int tiles, const SortPass& ch, const SortReset& rs, hipStream_t s);
 void launch_sort_binscan(int bits, uint32_t* hist, uint32_t* bintot, int tiles, hipStream_t s);
 void launch_sort_scatter(int bits, int items, int n, const uint32_t* keys_in, const uint32_t* vals_in, const uint2* pairs_in,
                          uint32_t* keys_out, uint32_t* vals_out, int shift, const uint32_t* hist, const uint32_t* bintot,
-                         int tiles, const SortGather* gather, const SortChunks& ch, hipStream_t s);
+                         int tiles, const SortGather* gather, const SortPass& ch, hipStream_t s);
 void launch_fill_zero(int* p, size_t count, hipStream_t s);
 
 }  // namespace xpbd
