@@ -244,9 +244,15 @@ __global__ __launch_bounds__(1024) void k_group_scan(int tiles, const uint32_t* 
 // position, which then stores the row's entry (first group, mask) and
 // records the row so the next build empties it.  Only rows that cross a tile
 // boundary meet in global memory, with atomics.
+// With g.Xs set, each position t also moves its particle into the sorted
+// order, as k_cell_ranges_gather does for a plain solver: the particle at
+// previous slot vals[t] is integrated and stored at t with its flags and id,
+// and rank records previous slot -> t (the last radix pass then moves only
+// keys and slots).
 __global__ __launch_bounds__(256) void k_group_build(int n, const uint32_t* __restrict__ keys,
                                                      const uint32_t* __restrict__ tileoff, uint2* __restrict__ rows,
-                                                     uint32_t* __restrict__ glb, int* __restrict__ rslot) {
+                                                     uint32_t* __restrict__ glb, int* __restrict__ rslot,
+                                                     const uint32_t* __restrict__ vals, SortGather g) {
     __shared__ uint32_t s_c[16];       // group starts per chunk (chunk = 4 i + w, in position order)
     __shared__ int s_rlast[16];        // per chunk: local index of its last row start, or -1
     __shared__ uint32_t s_mask[kGroupTile];
@@ -316,6 +322,23 @@ __global__ __launch_bounds__(256) void k_group_build(int n, const uint32_t* __re
             atomicOr(&rows[row].y, mask);
         } else {
             rows[row] = make_uint2(gi[i], mask);
+        }
+    }
+    if (g.Xs != nullptr) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int t = t0 + i * 256 + threadIdx.x;
+            if (t >= n) continue;
+            const uint32_t v = vals[t];
+            float4 P = g.P[v], V = g.V[v];
+            const int f = g.flags[v];
+            const uint32_t id = g.ids[v];
+            integrate_particle(P, V, f, g.dt, g.vmax, g.gx, g.gy, g.gz);
+            g.Xs[t] = P;
+            g.Vs[t] = V;
+            g.Fs[t] = f;
+            g.ids_out[t] = id;
+            g.rank[v] = uint32_t(t);
         }
     }
 }
@@ -1152,12 +1175,13 @@ void launch_gather_f3(float4* dst, const float* src, const uint32_t* ids, int n,
 }
 int group_tiles(int n) { return (n + kGroupTile - 1) / kGroupTile; }
 void launch_row_table(int n, const uint32_t* keys, uint32_t* tilecnt, uint32_t* tileoff, uint2* rows, uint32_t* glb,
-                      int* rslot, hipStream_t s) {
+                      int* rslot, const uint32_t* vals, const SortGather* gather, hipStream_t s) {
     if (n <= 0) return;
     const int tiles = group_tiles(n);
+    const SortGather g = gather ? *gather : SortGather{};
     hipLaunchKernelGGL(k_group_counts, dim3(tiles), dim3(256), 0, s, n, keys, tilecnt);
     hipLaunchKernelGGL(k_group_scan, dim3(1), dim3(1024), 0, s, tiles, tilecnt, tileoff);
-    hipLaunchKernelGGL(k_group_build, dim3(tiles), dim3(256), 0, s, n, keys, tileoff, rows, glb, rslot);
+    hipLaunchKernelGGL(k_group_build, dim3(tiles), dim3(256), 0, s, n, keys, tileoff, rows, glb, rslot, vals, g);
 }
 void launch_clear_rows(int n, const int* rslot, uint2* rows, hipStream_t s) {
     if (n > 0) hipLaunchKernelGGL(k_clear_rows, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, rslot, rows);

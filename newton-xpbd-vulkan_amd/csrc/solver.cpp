@@ -168,6 +168,8 @@ void Solver::init_common(const xpbd_config& cfg, size_t n, size_t max_contacts) 
     use_graphs_ = !(genv && std::strcmp(genv, "0") == 0);
     const char* cenv = std::getenv("VKXPBD_SORT_CHUNKED");
     sort_chunked_ = cenv && std::strcmp(cenv, "1") == 0;
+    const char* genv2 = std::getenv("VKXPBD_BATCH_GATHER_IN_ROWS");
+    batch_gather_rows_ = !(genv2 && std::strcmp(genv2, "0") == 0);
     const char* wenv = std::getenv("VKXPBD_SOLVE_IN_WALK");
     solve_in_walk_ = !(wenv && std::strcmp(wenv, "0") == 0);
     const char* fenv = std::getenv("VKXPBD_FUSE_KEYS");
@@ -691,11 +693,11 @@ void Solver::update_planes() {
 // wrap planes, 2^21 per batched scene) has the previous build's rows reset
 // first, from their keys (rkeys): a reset proportional to the particles, not
 // to the table.
-void Solver::build_ranges(const uint32_t* sorted_keys, bool cleared) {
+void Solver::build_ranges(const uint32_t* sorted_keys, bool cleared, const SortGather* gather) {
     const int n = int(n_);
     if (num_scenes_ > 1) {
         if (!cleared) launch_clear_rows(n, rslot_.p, rows_.p, stream_);
-        launch_row_table(n, sorted_keys, gtcnt_.p, gtoff_.p, rows_.p, glb_.p, rslot_.p, stream_);
+        launch_row_table(n, sorted_keys, gtcnt_.p, gtoff_.p, rows_.p, glb_.p, rslot_.p, valsA_.p, gather, stream_);
         return;
     }
     if (!cleared) launch_clear_ranges(n, rkeys_.p, range_.p, stream_);
@@ -952,10 +954,11 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
         nk.keys = num_scenes_ > 1 ? keysA_.p : nullptr;
     }
     if (n > 0) {
-        // plain solvers move the state into the sorted order in the cell-range
-        // pass (one thread per sorted slot); batched ones in the last radix pass
+        // the state moves into the sorted order in the cell-range pass (one
+        // thread per sorted slot): a plain solver's k_cell_ranges_gather, a
+        // batched solver's row-table build (or its last radix pass)
         const SortGather g = sort_gather(sdt);
-        const bool gather_in_ranges = num_scenes_ == 1;
+        const bool gather_in_ranges = num_scenes_ == 1 || batch_gather_rows_;
         SortReset rs;
         if (keyed) {   // k_integrate's resets (the rows: the last shape-delta apply's, if it ran)
             if (num_scenes_ > 1) {
@@ -969,8 +972,8 @@ void Solver::substep(float sdt, int iterations, bool collide, int slot, bool zer
         }
         run_sort(tiles, gather_in_ranges ? nullptr : &g, rs);
         stage_begin(kRanges);
-        if (gather_in_ranges) launch_cell_ranges_gather(n, keysA_.p, valsA_.p, range_.p, rkeys_.p, g, stream_);
-        else build_ranges(keysA_.p, true);
+        if (num_scenes_ == 1) launch_cell_ranges_gather(n, keysA_.p, valsA_.p, range_.p, rkeys_.p, g, stream_);
+        else build_ranges(keysA_.p, true, gather_in_ranges ? &g : nullptr);
         stage_end(kRanges);
         grid_valid_ = true;
         if (det_ && iterations > 0) {

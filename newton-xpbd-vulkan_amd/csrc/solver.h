@@ -165,7 +165,7 @@ private:
     std::vector<uint64_t> reference_sorted();
     // cell ranges of the sorted keys; cleared = the previous rows are already
     // zero (k_integrate clears them when wrap planes are on)
-    void build_ranges(const uint32_t* sorted_keys, bool cleared = false);
+    void build_ranges(const uint32_t* sorted_keys, bool cleared = false, const SortGather* gather = nullptr);
     void read_count(int slot);
     void ensure_det();   // deterministic mode's buffers (outside any graph capture)
     void stage_begin(Stage s);
@@ -276,6 +276,7 @@ private:
     void build_seg_tables();
     size_t sort_chunk_words_ = 0;      // words per pass array
     bool sort_chunked_ = false;        // chunked radix passes (VKXPBD_SORT_CHUNKED=1; else upsweep, binscan, scatter)
+    bool batch_gather_rows_ = true;    // batched: state moved by the row-table build (VKXPBD_BATCH_GATHER_IN_ROWS=0: last radix pass)
     bool solve_in_walk_ = true;        // the walk solves its contacts (VKXPBD_SOLVE_IN_WALK=0: k_shape_solve)
     bool fuse_keys_ = true;            // substeps keyed by the previous one's apply (VKXPBD_FUSE_KEYS=0: off)
     DevBuf<float4> Xs_[1], Vs_[1];   // predicted state in sorted order (the first iteration's input)

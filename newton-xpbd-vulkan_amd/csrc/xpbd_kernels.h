@@ -500,7 +500,7 @@ void launch_cell_ranges_gather(int n, const uint32_t* keys, const uint32_t* vals
 // filled; empty the last build's rows
 int group_tiles(int n);
 void launch_row_table(int n, const uint32_t* keys, uint32_t* tilecnt, uint32_t* tileoff, uint2* rows, uint32_t* glb,
-                      int* rslot, hipStream_t s);
+                      int* rslot, const uint32_t* vals, const SortGather* gather, hipStream_t s);
 void launch_clear_rows(int n, const int* rslot, uint2* rows, hipStream_t s);
 void launch_clear_ranges(int n, const uint32_t* rkeys, int* range, hipStream_t s);
 void launch_remap_contacts(int* cpart, const int* cc, int cmax, const uint32_t* map, int grid, hipStream_t s);
