@@ -452,6 +452,25 @@ def test_keyed_substeps_inactive_proxy_iterations(vk, iterations, monkeypatch):
     assert g.last_contact_count > 0
 
 
+def test_carried_keys_follow_the_substep_length(vk):
+    """A frame's first substep is keyed by the last frame's last substep only
+    when the substep length is the same: frames of changing dt and substep
+    count, with a mesh and without, against the oracle every frame."""
+    rng = np.random.default_rng(41)
+    r = 0.01
+    q = scenes.lattice(16, 16, 5, 2 * r, (-0.15, -0.15, 1.01), 0.1 * r, rng)
+    verts, tris = scenes.icosphere(3)
+    for meshes in ([(verts, tris)], []):
+        sc = scene_from(q if meshes else q - np.float32([0.0, 0.0, 0.99]), r, meshes=meshes)
+        g, o = make_pair(vk, sc)
+        for f, (dt, k) in enumerate([(1 / 60, 8), (1 / 60, 8), (1 / 120, 8), (1 / 60, 4), (1 / 60, 4), (1 / 60, 8)]):
+            g.step(dt, k, 1)
+            o.step(dt, k, 1)
+            assert g.last_contact_count == o.last_contact_count
+            compare_state(g, o, exact=True, what=f"{len(meshes)} mesh(es), frame {f + 1} (dt {dt:.4f}, {k} substeps)")
+        assert_contacts_equal(gpu_contacts(g), o.get_contacts())
+
+
 def test_update_mesh_moving_collider(vk):
     rng = np.random.default_rng(9)
     r = 0.01
