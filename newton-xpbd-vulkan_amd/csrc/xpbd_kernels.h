@@ -20,6 +20,19 @@ constexpr int kCellBits = 21;       // batched solver: key = scene << 21 | cell
 constexpr int kSortLargeN = 1 << 18;
 __host__ __device__ constexpr int sort_items_for(long long n) { return n >= kSortLargeN ? 8 : 4; }
 
+// Work the first radix upsweep does for a substep keyed by the previous one
+// (k_integrate skipped): the next substep's contact counter is zeroed and,
+// unless the previous substep's k_shape_apply did it, the rows of the last
+// cell-range build are reset from their keys (rkeys, as k_integrate resets
+// them).  Null pointers: nothing.
+struct SortReset {
+    const uint32_t* rkeys = nullptr;
+    int* range = nullptr;
+    int* cc_next = nullptr;
+    const int* rslot = nullptr;   // batched solvers: the row-table rows the last build filled (-1: none)
+    uint2* rows = nullptr;
+};
+
 // Options of one radix pass.  Chunked passes (no bin-scan launch): tiles are
 // grouped in chunks of kSortChunkTiles; chunk holds each chunk's per-bin
 // totals, chunk[c][bin] (chunks x 2^bits words), summed by the upsweep with
@@ -33,18 +46,6 @@ __host__ __device__ constexpr int sort_items_for(long long n) { return n >= kSor
 // the scatter's base of a bin is the segment's first key, plus the bin's
 // earlier bins inside the segment, plus the tile's count before it inside the
 // segment, all differences of the global bin scan.
-// Work the first radix upsweep does for a substep keyed by the previous one
-// (k_integrate skipped): the next substep's contact counter is zeroed and,
-// unless the previous substep's k_shape_apply did it, the rows of the last
-// cell-range build are reset from their keys (rkeys, as k_integrate resets
-// them).  Null pointers: nothing.
-struct SortReset {
-    const uint32_t* rkeys = nullptr;
-    int* range = nullptr;
-    int* cc_next = nullptr;
-    const int* rslot = nullptr;   // batched solvers: the row-table rows the last build filled (-1: none)
-    uint2* rows = nullptr;
-};
 constexpr int kSortChunkTiles = 32;
 constexpr int kSortMaxPasses = 4;   // 32-bit keys in 8-bit digits
 struct SortPass {
@@ -56,14 +57,12 @@ struct SortPass {
     int zero_count = 0;
 };
 
-// Sorted-order copies of the predicted particle state, written by the last
-// radix pass (positions with radius, velocities with inverse mass, flags).
-// Last radix pass: the particle state moves into the new sorted order.  The
-// sort values are slots of the previous order; ids follow their particles and
-// rank maps each previous slot to its new one.
-// The last radix pass moves each particle to its sorted slot: the
-// substep-start state (P, V, flags, id at the previous slot) is integrated
-// there (integrate_particle with dt, vmax, g) into Xs, Vs.
+// The particle state's move into the new sorted order (k_cell_ranges_gather
+// for a plain solver, k_group_build or the last radix pass for a batched
+// one): the sort's values are slots of the previous order; the particle at
+// slot v (P, V, flags, id) is integrated (integrate_particle with dt, vmax,
+// g) into Xs, Vs at its sorted slot, its flags and id follow it, and rank
+// maps each previous slot to its new one.
 struct SortGather {
     const float4* P; const float4* V; const int* flags; const uint32_t* ids;
     float4* Xs; float4* Vs; int* Fs; uint32_t* ids_out; uint32_t* rank;
@@ -84,8 +83,9 @@ struct BodyArgs {
 
 // The next substep's radix input written where a substep finishes its
 // particles (k_pp_apply, k_shape_apply): integrate_particle on the final
-// state, its cell key (with the wrap plane above it), (key, slot) at the
-// particle's id.  kv == nullptr: not written (k_integrate keys the substep).
+// state, its cell key (with the wrap plane or the scene above it), (key, slot)
+// at the particle's id.  kv == nullptr: not written (k_integrate keys the
+// substep).
 struct NextKey {
     uint2* kv; const uint32_t* ids;   // sorted slot -> particle id
     float gx, gy, gz, inv_cell;
