@@ -212,10 +212,11 @@ def reset(s, sc):
     s.set_particles(p["q"], p["qd"], p["inv_mass"], p["radii"], p["flags"])
 
 
-def bench_batched(vkxpbd, scenes, frames):
+def bench_batched(vkxpbd, scenes, frames, sequential=True):
     """256 scenes (log-uniform 1,024..65,536 particles) advanced by one batched
     solver (frame graphs, autotuned block size) against 256 plain solvers
-    stepped one after another, same frames; final states compared."""
+    stepped one after another, same frames; final states compared
+    (sequential=False: the batched solver alone)."""
     sc = scenes.batch_of_piles()
     counts = sc["counts"]
     total = int(counts.sum())
@@ -230,6 +231,10 @@ def bench_batched(vkxpbd, scenes, frames):
     tb = time.perf_counter() - t0
     xb = b.get_positions()
     del b
+    if not sequential:
+        return {"workload": sc["name"], "scenes": int(len(counts)), "particles": total, "frames_timed": frames,
+                "value": total * SUBSTEPS * frames / tb, "unit": "particle-substeps/s",
+                "ms_per_frame": 1e3 * tb / frames, "pp_block": int(block)}
     singles = [scenes.build_solver(vkxpbd, scenes.scene_slice(sc, k)) for k in range(len(counts))]
     for s in singles:                             # capture each solver's frame graph
         s.step(DT, SUBSTEPS, 1)
@@ -294,6 +299,9 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--batched-frames", type=int, default=10)
+    ap.add_argument("--batched-only", action="store_true",
+                    help="only the batched solver's frames (no plain step, no sequential comparison): "
+                         "the command profiles/ uses for the batched kernels")
     ap.add_argument("--summarize-pmc", nargs=4, metavar=("READ_DIR", "WRITE_DIR", "COPY_READ_DIR", "COPY_WRITE_DIR"),
                     help="print the per-kernel traffic summary from rocprofv3 --pmc output directories")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -320,6 +328,12 @@ def main():
 
     import vkxpbd
     from vkxpbd import scenes
+
+    if args.batched_only:
+        b = bench_batched(vkxpbd, scenes, args.batched_frames, sequential=False)
+        print(json.dumps({"metric": "particle-substeps/s (batched HIP)", "value": b["value"], "unit": b["unit"],
+                          "n_gpus": 1, "batched": b}), flush=True)
+        return
 
     sc = scenes.pile_1m_icosphere()
     N = sc["particles"]["q"].shape[0]
