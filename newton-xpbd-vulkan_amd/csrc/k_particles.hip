@@ -259,6 +259,31 @@ __global__ __launch_bounds__(256) void k_group_build(int n, const uint32_t* __re
     const int tile = blockIdx.x, w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int t0 = tile * kGroupTile;
     const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+    // the state's move: every load requested first, so that it is in flight
+    // during the row table's work
+    float4 gP[4], gV[4];
+    int gf[4];
+    uint32_t gv[4], gid[4];
+    if (g.Xs != nullptr) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int t = t0 + i * 256 + threadIdx.x;
+            gv[i] = t < n ? vals[t] : 0u;
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int t = t0 + i * 256 + threadIdx.x;
+            gP[i] = gV[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+            gf[i] = 0;
+            gid[i] = 0u;
+            if (t < n) {
+                gP[i] = g.P[gv[i]];
+                gV[i] = g.V[gv[i]];
+                gf[i] = g.flags[gv[i]];
+                gid[i] = g.ids[gv[i]];
+            }
+        }
+    }
     uint32_t k[4], kp[4], kn[4], gi[4];
     uint64_t fm[4], rm[4];
 #pragma unroll
@@ -329,16 +354,12 @@ __global__ __launch_bounds__(256) void k_group_build(int n, const uint32_t* __re
         for (int i = 0; i < 4; i++) {
             const int t = t0 + i * 256 + threadIdx.x;
             if (t >= n) continue;
-            const uint32_t v = vals[t];
-            float4 P = g.P[v], V = g.V[v];
-            const int f = g.flags[v];
-            const uint32_t id = g.ids[v];
-            integrate_particle(P, V, f, g.dt, g.vmax, g.gx, g.gy, g.gz);
-            g.Xs[t] = P;
-            g.Vs[t] = V;
-            g.Fs[t] = f;
-            g.ids_out[t] = id;
-            g.rank[v] = uint32_t(t);
+            integrate_particle(gP[i], gV[i], gf[i], g.dt, g.vmax, g.gx, g.gy, g.gz);
+            g.Xs[t] = gP[i];
+            g.Vs[t] = gV[i];
+            g.Fs[t] = gf[i];
+            g.ids_out[t] = gid[i];
+            g.rank[gv[i]] = uint32_t(t);
         }
     }
 }
