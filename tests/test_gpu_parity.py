@@ -264,6 +264,33 @@ def test_mesh_contacts_several_meshes_and_large_particles(vk):
     assert_contacts_equal(gc, oc)
 
 
+def test_two_meshes_stepped(vk):
+    """A pile dropped onto two intersecting spheres, away from the ground
+    plane: particles near the crease get one contact from each mesh, solved
+    by each mesh's walk (two float atomics into a zeroed sum, the same in
+    either order), so the state equals the oracle's bit for bit."""
+    rng = np.random.default_rng(51)
+    r = 0.01
+    v1, t1 = scenes.icosphere(3)
+    v2, t2 = scenes.icosphere(2, 0.3, (0.35, 0.0, 0.8))
+    q = scenes.lattice(16, 12, 4, 2 * r, (0.05, -0.11, 1.12), 0.1 * r, rng)
+    sc = scene_from(q, r, meshes=[(v1, t1), (v2, t2)])
+    g, o = make_pair(vk, sc)
+    both = 0
+    for f in range(12):
+        g.step(1.0 / 60.0, 8, 1)
+        o.step(1.0 / 60.0, 8, 1)
+        assert g.last_contact_count == o.last_contact_count
+        c = gpu_contacts(g)
+        ps, sh = np.asarray(c["particles"]), np.asarray(c["shapes"])
+        both = max(both, len(np.intersect1d(ps[sh == 1], ps[sh == 2])))
+        if f % 4 == 3:
+            compare_state(g, o, exact=True, what=f"two meshes, frame {f + 1}")
+    log(f"  particles touching both meshes in one substep: up to {both}")
+    assert both > 0
+    assert_contacts_equal(gpu_contacts(g), o.get_contacts())
+
+
 def test_profiled_frames_equal_graph_frames(vk):
     """Stage timing runs every kernel on one stream (mesh stage included, the
     walk on its full grid) with events between stages: the same bits as the
