@@ -370,10 +370,10 @@ __device__ __forceinline__ uint32_t* mesh_counters(const MeshArgs& a, int ci, in
 // Pass 1: candidates into the queue, one slot reservation per block (queue
 // order is arbitrary; every queued particle's result is its own).  Each
 // thread tests kEnqItems particles of its block's range (wave-striped), their
-// loads issued together.  Two per thread (512 blocks at 1,048,576 particles)
-// finish the pass sooner beside the radix passes than four, which the walk
-// behind it gains in the dense frames.
-constexpr int kEnqItems = 2;
+// loads issued together.  One per thread (1,024 blocks at 1,048,576
+// particles) finishes the pass sooner beside the radix passes than two or
+// four, which the walk behind it gains in the dense frames.
+constexpr int kEnqItems = 1;
 constexpr int kEnqBlock = 1024;   // fewer blocks, fewer same-address queue reservations
 template <bool STATS>
 __global__ __launch_bounds__(kEnqBlock) void k_mesh_enqueue(MeshArgs a, int ci) {
