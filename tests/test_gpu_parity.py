@@ -414,8 +414,10 @@ def test_mesh_pile_frames(vk):
 
 @pytest.mark.parametrize("env", [{"VKXPBD_SORT_CHUNKED": "1"},
                                  {"VKXPBD_SORT_CHUNKED": "1", "VKXPBD_WRAP_PLANES": "1"},
-                                 {"VKXPBD_FUSE_KEYS": "0"}],
-                         ids=["chunked_radix", "chunked_radix_wrap_planes", "integrate_every_substep"])
+                                 {"VKXPBD_FUSE_KEYS": "0"},
+                                 {"VKXPBD_SOLVE_IN_WALK": "0"}],
+                         ids=["chunked_radix", "chunked_radix_wrap_planes", "integrate_every_substep",
+                              "shape_solve_launched"])
 def test_sort_and_keying_variants(vk, env, monkeypatch):
     """The radix passes without a bin-scan launch (per-chunk totals, 65,536
     keys: two chunks of 32 tiles), with plain and wrap-plane keys, and
@@ -753,6 +755,46 @@ def test_batched_scenes_match_sequential(vk, num_scenes, lo, hi, frames):
     keys = bs.get_sorted_cells()
     assert np.all(np.diff(keys.astype(np.int64)) >= 0)
     assert int(keys.max()) >> 21 == num_scenes - 1
+
+
+@pytest.mark.parametrize("env", [{"VKXPBD_SEG_SORT": "0"},
+                                 {"VKXPBD_SEG_SORT": "0", "VKXPBD_SORT_CHUNKED": "1"},
+                                 {"VKXPBD_BATCH_GATHER_IN_ROWS": "0"}],
+                         ids=["whole_batch_radix", "whole_batch_chunked_radix", "gather_in_cell_ranges"])
+def test_batched_sort_variants(vk, env, monkeypatch):
+    """A batched solver whose radix passes sort the whole batch on
+    scene-tagged keys (plain and chunked) instead of each scene in its own
+    range, or whose state move runs in the cell-range pass instead of the
+    row-table build, equals the default batched solver and one plain solver
+    per scene, bit for bit."""
+    num_scenes, frames = 9, 6
+    sc = scenes.batch_of_piles(seed=17, num_scenes=num_scenes, lo=64, hi=6000)
+    sc["counts"][2] = 1
+    off = np.concatenate([[0], np.cumsum(sc["counts"])])
+    sc["particles"] = {k: np.concatenate([scenes.scene_slice(sc, j)["particles"][k] for j in range(num_scenes)])
+                       for k in sc["particles"]}
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    bs = scenes.build_batch_solver(vk, sc)
+    for k in env:
+        monkeypatch.delenv(k)
+    d = scenes.build_batch_solver(vk, sc)   # the defaults
+    singles = [scenes.build_solver(vk, scenes.scene_slice(sc, k)) for k in range(num_scenes)]
+    log(f"batched variant {env}: {int(off[-1])} particles")
+    for f in range(frames):
+        bs.step(1.0 / 60.0, 8, 1)
+        d.step(1.0 / 60.0, 8, 1)
+        for s in singles:
+            s.step(1.0 / 60.0, 8, 1)
+        gp, gv = bs.get_positions(), bs.get_velocities()
+        np.testing.assert_array_equal(gp, d.get_positions(), err_msg=f"frame {f}")
+        np.testing.assert_array_equal(gv, d.get_velocities(), err_msg=f"frame {f}")
+        assert bs.last_contact_count == d.last_contact_count == sum(s.last_contact_count for s in singles)
+        for k, s in enumerate(singles):
+            np.testing.assert_array_equal(gp[off[k]:off[k + 1]], s.get_positions(), err_msg=f"scene {k} frame {f}")
+            np.testing.assert_array_equal(gv[off[k]:off[k + 1]], s.get_velocities(), err_msg=f"scene {k} frame {f}")
+    keys = bs.get_sorted_cells()
+    assert np.all(np.diff(keys.astype(np.int64)) >= 0)
 
 
 def test_2048_scenes_use_every_scene_bit(vk):
